@@ -1,0 +1,87 @@
+// axh_internal.h -- shared declarations of the GPU time-loop library.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "axihip.h"
+
+namespace axh {
+
+constexpr int kNpol = 4;
+constexpr int kNpp = 25;           // GLL points per element
+constexpr int kElemPerTile = 10;   // elements per 256-lane workgroup
+constexpr int kTile = 256;         // slots per tile (250 used)
+constexpr int kMaxSls = 8;
+
+enum DType : uint32_t { kF32 = 1, kF64 = 2, kI32 = 3 };
+
+// One uploaded array, kept in the reference layout.
+struct Array {
+  DType dtype = kF32;
+  std::vector<uint64_t> dims;
+  std::vector<char> bytes;
+  size_t count() const {
+    size_t n = 1;
+    for (auto d : dims) n *= d;
+    return n;
+  }
+  template <class T> const T *as() const {
+    return reinterpret_cast<const T *>(bytes.data());
+  }
+};
+
+struct Device;  // device-side layout and buffers (axh_device.hip)
+struct DeviceDeleter {
+  void operator()(Device *d) const;
+};
+
+// Error raised inside the library; converted to a status code at the ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+
+}  // namespace axh
+
+struct axh_ctx {
+  axh_dims dims{};
+  std::map<std::string, axh::Array> arrays;
+  std::unique_ptr<axh::Device, axh::DeviceDeleter> dev;
+  int32_t iter_done = 0;  // last completed step
+  double last_run_ms = 0.0;
+  bool profile = false;
+
+  const axh::Array *get(const std::string &name) const {
+    auto it = arrays.find(name);
+    return it == arrays.end() ? nullptr : &it->second;
+  }
+  const axh::Array &need(const std::string &name) const;
+  void put(const std::string &name, axh::DType t, std::vector<uint64_t> dims,
+           const void *src);
+};
+
+namespace axh {
+// names of the solid 3-D terms in enum axh_solid_term order
+extern const char *const kSolid3dNames[AXH_N_SOLID_3D];
+extern const char *const kFluid3dNames[4];
+
+// device lifecycle (axh_device.hip)
+void device_build(axh_ctx *ctx);
+void device_destroy(Device *d);
+void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
+                     float *dchi, float *ddchi);
+void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
+                          const float *chi, const float *dchi,
+                          const float *ddchi);
+size_t device_bytes(const Device *d);
+const char *kernel_class_name(int k);
+bool kernel_stats(const Device *d, int k, double *ms, int64_t *n);
+int symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
+                            double *cf);
+}  // namespace axh

@@ -1,0 +1,558 @@
+!=========================================================================================
+!> Drop-in replacement of the reference module time_evol_wave
+!! (SOLVER/time_evol_wave.F90:26-41): same name, same two public entry points,
+!! prepare_waves and time_loop, no arguments.  prepare_waves follows the reference
+!! orchestration (SOLVER/time_evol_wave.F90:47-225) by calling the reference's own
+!! public setup routines, then hands every array the time loop reads to the GPU
+!! library through iso_c_binding (include/axihip.h).  time_loop runs the loop on the
+!! GPU and writes the receiver seismograms in the reference's ASCII format
+!! (SOLVER/seismograms.f90:742-780) and the generic epicentre/equator/antipode traces
+!! (SOLVER/seismograms.f90:688-735).
+!!
+!! Environment switches of the host:
+!!   AXH_SAVE_STATE=<path>  write the uploads to a state file (axh_save_state)
+!!   AXH_NO_RUN=1           stop after prepare/upload (records states without a GPU)
+module time_evol_wave
+
+  use global_parameters
+  use data_proc
+  use data_mesh
+  use data_source
+  use data_time
+  use seismograms
+  use rotations
+  use data_io,          only: verbose
+  use axh_iface
+
+
+  implicit none
+  public :: prepare_waves, time_loop
+  private
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer,     save :: nrec_all = 0          !< receivers + generic traces
+  integer,     save :: iepi = 0, ianti = 0, iequ = 0   !< rows of the generic traces
+  logical,     save :: no_run = .false.
+
+contains
+
+!-----------------------------------------------------------------------------------------
+!> Reference orchestration of SOLVER/time_evol_wave.F90:47-225, restricted to the outputs
+!! the GPU loop produces; requests for other outputs stop with a message.
+subroutine prepare_waves
+
+  use data_io
+  use parameters
+  use def_precomp_terms
+  use source
+  use clocks_mod
+  use commun
+
+  if (lpr) then
+     write(6,*)''
+     write(6,*)'++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++'
+     write(6,*)'++++++++   SEISMIC WAVE PROPAGATION ON THE GPU (gfx950)   +++++++'
+     write(6,*)'++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++'
+  endif
+
+  call check_supported_pre()
+
+  call read_sourceparams
+  if (rot_src) call def_rot_matrix
+  call compute_numerical_parameters
+  call read_model_compute_terms
+  call compute_stf
+  call compute_src
+  call create_fluid_free_surface_mask()
+  call check_absorbing_boundaries()
+
+  call prepare_seismograms
+  call open_hyp_epi_equ_anti
+  call prepare_from_recfile_seis
+
+  call write_parameters
+  call upload_all()
+
+  call barrier
+  if (lpr) write(6,*) 'done preparing waves (GPU time loop).'
+
+end subroutine prepare_waves
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Entry point of the loop (SOLVER/time_evol_wave.F90:231-245): runs all niter steps on
+!! the GPU in chunks and writes the seismogram samples after each chunk.
+subroutine time_loop
+
+  use clocks_mod, only: tick
+  use data_io,    only: datapath, lfdata
+
+  integer                          :: it0, it1, nsamp, k, ierr, chunk, isamp_iter
+  real(kind=realkind), allocatable :: seis(:)
+  real(kind=dp)                    :: tloc
+
+  if (no_run) then
+     if (lpr) write(6,*) 'AXH_NO_RUN set: skipping the time loop.'
+     return
+  endif
+
+  iclockold = tick()
+
+  if (lpr) write(6,*) '************ S T A R T I N G   T I M E   L O O P *************'
+
+  ! dump_stuff(0, ...) of the reference samples the zero initial state at t = 0
+  ! (SOLVER/time_evol_wave.F90:347,579)
+  if (nrec_all > 0) then
+     allocate(seis(3 * nrec_all))
+     seis = 0
+     call write_samples(seis, 1, 0.0_dp)
+     deallocate(seis)
+  endif
+
+  ! chunks of at most 512 seismogram samples
+  chunk = max(1, min(niter, 512 * seis_it))
+  allocate(seis(3 * max(nrec_all, 1) * (chunk / seis_it + 1)))
+  tloc = zero
+  isamp_iter = 0
+  it0 = 1
+  do while (it0 <= niter)
+     it1 = min(niter, it0 + chunk - 1)
+     ierr = axh_run(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis)
+     if (ierr == AXH_ERR_BLOWUP) then
+        write(6,*) procstrg, '!!!!!!!!!!!!!!! DISPLACEMENTS BLEW UP !!!!!!!!!!!!!!!'
+        write(6,*) procstrg, trim(axh_error_string())
+        stop
+     endif
+     call axh_check(ierr, 'axh_run')
+     ! replay the reference's time accumulation t = t + deltat for the sampled steps
+     nsamp = 0
+     do k = it0, it1
+        tloc = tloc + deltat
+        if (mod(k, seis_it) == 0 .and. nrec_all > 0) then
+           nsamp = nsamp + 1
+           call write_samples(seis(1 + 3 * nrec_all * (nsamp - 1):), 1, tloc)
+        endif
+     enddo
+     if (lpr) then
+        write(6,13) it1, tloc, real(it1) / real(niter) * 100.
+        call flush(6)
+     endif
+13   format('  time step:',i6,'; t=',f8.2,' s (',f5.1,'%)')
+     it0 = it1 + 1
+  enddo
+  t = tloc
+
+  iclockold = tick(id=idold, since=iclockold)
+
+  ierr = axh_destroy(ctx)
+  ctx = c_null_ptr
+
+end subroutine time_loop
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> One sample of compute_recfile_seis_bare (SOLVER/seismograms.f90:742-780) and of
+!! compute_hyp_epi_equ_anti (SOLVER/seismograms.f90:688-735), from the library's
+!! receiver rows s(3, nrec_all) in the same component order.
+subroutine write_samples(s, nsamp, time)
+
+  real(kind=realkind), intent(in) :: s(*)
+  integer, intent(in)             :: nsamp
+  real(kind=dp), intent(in)       :: time
+  integer                         :: i, b
+
+  if (nsamp < 1) return
+  do i = 1, num_rec
+     b = 3 * (i - 1)
+     if (src_type(1) == 'monopole') then
+        write(100000+i,*) s(b + 1), s(b + 2)
+     else
+        write(100000+i,*) s(b + 1), s(b + 2), s(b + 3)
+     endif
+  enddo
+
+  if (iepi > 0) then
+     b = 3 * (iepi - 1)
+     write(900,*) time, s(b + 1)
+     if (src_type(1) /= 'monopole') then
+        write(903,*) time, s(b + 2)
+        write(906,*) time, s(b + 3)
+     else
+        write(906,*) time, s(b + 2)
+     endif
+  endif
+  if (ianti > 0) then
+     b = 3 * (ianti - 1)
+     write(901,*) time, s(b + 1)
+     if (src_type(1) /= 'monopole') then
+        write(905,*) time, s(b + 2)
+        write(908,*) time, s(b + 3)
+     else
+        write(908,*) time, s(b + 2)
+     endif
+  endif
+  if (iequ > 0) then
+     b = 3 * (iequ - 1)
+     write(902,*) time, s(b + 1)
+     if (src_type(1) /= 'monopole') then
+        write(904,*) time, s(b + 2)
+        write(907,*) time, s(b + 3)
+     else
+        write(907,*) time, s(b + 2)
+     endif
+  endif
+
+end subroutine write_samples
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Outputs and options the GPU loop does not produce stop here, before any work.
+subroutine check_supported_pre()
+
+  use data_io
+  character(len=256) :: why
+
+  why = ' '
+  if (nproc > 1)        why = 'more than one mesh slice (NTHETA/NRADIAL_SLICES > 1)'
+  if (npol /= 4)        why = 'npol /= 4'
+  if (dump_energy)      why = 'SAVE_ENERGY'
+  if (dump_wavefields)  why = 'KERNEL_WAVEFIELDS'
+  if (dump_vtk .or. dump_xdmf) why = 'SAVE_SNAPSHOTS'
+  if (use_netcdf)       why = 'USE_NETCDF (ASCII seismograms only)'
+  if (why /= ' ') then
+     write(6,*) 'GPU time loop: unsupported option: ', trim(why)
+     stop 2
+  endif
+
+end subroutine check_supported_pre
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> create_fluid_free_surface_mask (SOLVER/time_evol_wave.F90:1615-1630): zero at fluid
+!! points on the outer surface (r > router - 1 m).
+subroutine create_fluid_free_surface_mask()
+
+  use utlity, only : rcoord
+  integer :: iel, ipol, jpol
+
+  allocate(fluid_free_surface_mask(0:npol, 0:npol, nel_fluid))
+  fluid_free_surface_mask = 1
+  do iel = 1, nel_fluid
+     do ipol = 0, npol
+        do jpol = 0, npol
+           if (rcoord(ipol, jpol, ielfluid(iel)) > router - 1) &
+              fluid_free_surface_mask(ipol, jpol, iel) = 0.
+        enddo
+     enddo
+  enddo
+
+end subroutine create_fluid_free_surface_mask
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> The absorbing boundary test of create_absorbing_gamma
+!! (SOLVER/time_evol_wave.F90:1633-1650): a mesh that does not reach the antipode has
+!! absorbing boundaries, which the GPU loop does not implement.
+subroutine check_absorbing_boundaries()
+
+  use commun, only : pmax
+  real(kind=dp) :: thetamax
+
+  thetamax = maxval(atan2(crd_nodes(:, 1), crd_nodes(:, 2)))
+  thetamax = pmax(thetamax)
+  have_absorbing_bc = thetamax < pi - 1e-7
+  if (have_absorbing_bc) then
+     write(6,*) 'GPU time loop: unsupported option: absorbing boundaries (LOCAL_MAX_COLAT)'
+     stop 2
+  endif
+
+end subroutine check_absorbing_boundaries
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> fast_correct (SOLVER/attenuation.f90:1139-1155): first-order correction of the
+!! linearised SLS weights for low Q, yp_j = y_j * delta_j.
+pure subroutine sls_fast_correct(y_j, yp_j)
+
+  real(kind=dp), intent(in)  :: y_j(:)
+  real(kind=dp), intent(out) :: yp_j(size(y_j))
+  real(kind=dp)              :: dy_j(size(y_j))
+  integer                    :: k
+
+  dy_j(1) = 1 + .5 * y_j(1)
+  do k = 2, size(y_j)
+     dy_j(k) = dy_j(k-1) + (dy_j(k-1) - .5) * y_j(k-1) + .5 * y_j(k)
+  enddo
+  yp_j = y_j * dy_j
+
+end subroutine sls_fast_correct
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Hands every array of the loop to the library (and optionally to a state file).
+subroutine upload_all()
+
+  use data_matr
+  use data_spec
+  use data_pointwise
+  use data_io,        only: datapath, lfdata
+  use attenuation,    only: n_sls_attenuation, att_coarse_grained
+  use source,         only: compute_stf_t
+
+  type(axh_dims_t)              :: dims
+  integer(c_int)                :: ierr
+  integer(c_int32_t), allocatable :: ax_s(:), ax_f(:), rec(:,:), isrc(:)
+  real(kind=realkind), allocatable :: stf_tab(:), empty(:)
+  real(kind=dp)                 :: coefd(8), coefv(8), coeff(8), subdt(8), stf_symp(8)
+  real(kind=dp)                 :: tacc
+  integer(c_int32_t)            :: nst
+  integer                       :: i, iter, nextra, iel
+  character(len=1024)           :: path
+  integer(c_int)                :: lpath
+
+  ! time scheme
+  select case (trim(time_scheme))
+  case ('newmark2');  dims%scheme = AXH_NEWMARK2
+  case ('symplec4');  dims%scheme = AXH_SYMPLEC4
+  case ('ML_SO4m5');  dims%scheme = AXH_ML_SO4M5
+  case ('ML_SO6m7');  dims%scheme = AXH_ML_SO6M7
+  case default
+     write(6,*) 'GPU time loop: unsupported time scheme ', trim(time_scheme)
+     stop 2
+  end select
+
+  select case (trim(src_type(1)))
+  case ('monopole'); dims%src_type = AXH_MONOPOLE
+  case ('dipole');   dims%src_type = AXH_DIPOLE
+  case ('quadpole'); dims%src_type = AXH_QUADPOLE
+  case default
+     write(6,*) 'GPU time loop: unknown source type ', trim(src_type(1))
+     stop 2
+  end select
+
+  dims%abi_version = AXH_ABI_VERSION
+  dims%npol        = npol
+  dims%nel_solid   = nel_solid
+  dims%nel_fluid   = nel_fluid
+  dims%nglob_solid = nglob_solid
+  dims%nglob_fluid = nglob_fluid
+  dims%nel_bdry    = nel_bdry
+  dims%anel        = merge(1, 0, anel_true)
+  dims%coarse_grained = 0
+  dims%n_sls       = 0
+  if (anel_true) then
+     if (.not. att_coarse_grained) then
+        write(6,*) 'GPU time loop: unsupported option: COARSE_GRAINED false'
+        stop 2
+     endif
+     dims%coarse_grained = 1
+     dims%n_sls = n_sls_attenuation
+  endif
+  dims%niter     = niter
+  dims%seis_it   = seis_it
+  dims%nelsrc    = nelsrc
+  dims%fluid_src = merge(1, 0, fluid_src)
+  dims%reserved  = 0
+  dims%deltat    = deltat
+  dims%magnitude = magnitude
+
+  ! receivers, then the generic epicentre / antipode / equator traces
+  nextra = 0
+  if (maxind > 0) then
+     if (have_epi)      nextra = nextra + 1
+     if (have_antipode) nextra = nextra + 1
+     if (have_equ)      nextra = nextra + 1
+  endif
+  nrec_all = num_rec + nextra
+  dims%num_rec = nrec_all
+  allocate(rec(max(nrec_all,1), 3))
+  do i = 1, num_rec
+     rec(i, :) = recfile_el(i, :)
+  enddo
+  i = num_rec
+  if (maxind > 0) then
+     if (have_epi) then
+        i = i + 1; iepi = i;  rec(i, :) = [ielepi, 0, npol]
+     endif
+     if (have_antipode) then
+        i = i + 1; ianti = i; rec(i, :) = [ielantipode, 0, 0]
+     endif
+     if (have_equ) then
+        i = i + 1; iequ = i;  rec(i, :) = [ielequ, npol, npol]
+     endif
+  endif
+
+  call axh_check(axh_create(dims, ctx), 'axh_create')
+
+  call axh_check(axh_upload_spectral(ctx, G0, G1, G1T, G2, G2T), 'spectral')
+
+  call up3d(AXH_M11S, M11s); call up3d(AXH_M21S, M21s); call up3d(AXH_M41S, M41s)
+  call up3d(AXH_M12S, M12s); call up3d(AXH_M22S, M22s); call up3d(AXH_M42S, M42s)
+  call up3d(AXH_M11Z, M11z); call up3d(AXH_M21Z, M21z); call up3d(AXH_M41Z, M41z)
+  call up3d(AXH_M32S, M32s); call up3d(AXH_M13S, M13s); call up3d(AXH_M33S, M33s)
+  call up3d(AXH_M43S, M43s); call up3d(AXH_M1PHI, M1phi); call up3d(AXH_M2PHI, M2phi)
+  call up3d(AXH_M4PHI, M4phi)
+  call up3d(AXH_M_1, M_1); call up3d(AXH_M_2, M_2); call up3d(AXH_M_3, M_3)
+  call up3d(AXH_M_4, M_4); call up3d(AXH_M_5, M_5); call up3d(AXH_M_6, M_6)
+  call up3d(AXH_M_7, M_7); call up3d(AXH_M_8, M_8)
+  call up3d(AXH_M_W1, M_w1); call up3d(AXH_M_W2, M_w2); call up3d(AXH_M_W3, M_w3)
+  call up3d(AXH_M_W4, M_w4); call up3d(AXH_M_W5, M_w5)
+  call upax(AXH_M0_W1 + 0, M0_w1); call upax(AXH_M0_W1 + 1, M0_w2)
+  call upax(AXH_M0_W1 + 2, M0_w3); call upax(AXH_M0_W1 + 3, M0_w4)
+  call upax(AXH_M0_W1 + 4, M0_w5); call upax(AXH_M0_W1 + 5, M0_w6)
+  call upax(AXH_M0_W1 + 6, M0_w7); call upax(AXH_M0_W1 + 7, M0_w8)
+  call upax(AXH_M0_W1 + 8, M0_w9); call upax(AXH_M0_W1 + 9, M0_w10)
+
+  allocate(empty(1))
+  empty = 0
+  if (nel_fluid > 0) then
+     if (allocated(M1chi_fl)) &
+        call axh_check(axh_upload_fluid_term(ctx, AXH_M1CHI_FL, M1chi_fl), 'M1chi_fl')
+     if (allocated(M2chi_fl)) &
+        call axh_check(axh_upload_fluid_term(ctx, AXH_M2CHI_FL, M2chi_fl), 'M2chi_fl')
+     if (allocated(M4chi_fl)) &
+        call axh_check(axh_upload_fluid_term(ctx, AXH_M4CHI_FL, M4chi_fl), 'M4chi_fl')
+     if (allocated(M_w_fl) .and. src_type(1) /= 'monopole') &
+        call axh_check(axh_upload_fluid_term(ctx, AXH_M_W_FL, M_w_fl), 'M_w_fl')
+     if (allocated(M0_w_fl) .and. src_type(1) /= 'monopole') &
+        call axh_check(axh_upload_fluid_term(ctx, AXH_M0_W_FL, M0_w_fl), 'M0_w_fl')
+     call axh_check(axh_upload_mass(ctx, inv_mass_rho, inv_mass_fluid), 'mass')
+     call axh_check(axh_upload_fluid_surface_mask(ctx, fluid_free_surface_mask), 'fsmask')
+  else
+     call axh_check(axh_upload_mass(ctx, inv_mass_rho, empty), 'mass')
+  endif
+
+  allocate(ax_s(nel_solid), ax_f(max(nel_fluid,1)))
+  ax_s = 0
+  ax_f = 0
+  do iel = 1, nel_solid
+     if (axis_solid(iel)) ax_s(iel) = 1
+  enddo
+  do iel = 1, nel_fluid
+     if (axis_fluid(iel)) ax_f(iel) = 1
+  enddo
+  if (nel_fluid > 0) then
+     call axh_check(axh_upload_mesh_maps(ctx, igloc_solid, igloc_fluid, ax_s, ax_f), 'maps')
+  else
+     call axh_check(axh_upload_mesh_maps(ctx, igloc_solid, ax_f, ax_s, ax_f), 'maps')
+  endif
+
+  if (nel_bdry > 0) &
+     call axh_check(axh_upload_boundary(ctx, bdry_solid_el, bdry_fluid_el, &
+                    bdry_jpol_solid, bdry_jpol_fluid, bdry_matr), 'boundary')
+
+  ! source time function table
+  if (dims%scheme == AXH_NEWMARK2) then
+     allocate(stf_tab(niter))
+     stf_tab = stf(1:niter)
+     nst = 1
+  else
+     call axh_check(axh_symplectic_coefficients(int(dims%scheme, c_int32_t), deltat, &
+                    coefd, coefv, coeff, nst), 'symplectic coefficients')
+     allocate(stf_tab(niter * nst))
+     ! t accumulation and sub-step times of SOLVER/time_evol_wave.F90:586-593
+     tacc = zero
+     do iter = 1, niter
+        tacc = tacc + deltat
+        subdt(1:nst) = tacc - deltat + coeff(1:nst)
+        call compute_stf_t(int(nst), subdt(1:nst), stf_symp(1:nst))
+        do i = 1, nst
+           stf_tab((iter - 1) * nst + i) = real(stf_symp(i), kind=realkind)
+        enddo
+     enddo
+  endif
+  allocate(isrc(max(nelsrc,1)))
+  isrc = 0
+  isrc(1:nelsrc) = ielsrc(1:nelsrc)
+  if (fluid_src) then
+     call axh_check(axh_upload_source(ctx, isrc, source_term_fl, stf_tab, &
+                    int(size(stf_tab), c_int32_t)), 'source')
+  else
+     call axh_check(axh_upload_source(ctx, isrc, source_term_el, stf_tab, &
+                    int(size(stf_tab), c_int32_t)), 'source')
+  endif
+
+  if (anel_true) call upload_attenuation()
+
+  if (nrec_all > 0) call axh_check(axh_upload_receivers(ctx, rec), 'receivers')
+
+  lpath = axh_glue_getenv('AXH_SAVE_STATE'//c_null_char, path, len(path))
+  if (lpath > 0) then
+     call axh_check(axh_save_state(ctx, trim(path(1:lpath))//c_null_char), 'save state')
+     if (lpr) write(6,*) '  wrote GPU state file ', trim(path(1:lpath))
+  endif
+  lpath = axh_glue_getenv('AXH_NO_RUN'//c_null_char, path, len(path))
+  no_run = lpath > 0 .and. path(1:1) /= '0'
+
+contains
+
+  subroutine up3d(term, a)
+    integer(c_int), intent(in) :: term
+    real(kind=realkind), allocatable, intent(in) :: a(:,:,:)
+    if (allocated(a)) call axh_check(axh_upload_solid_term(ctx, term, a), 'solid term')
+  end subroutine
+
+  subroutine upax(term, a)
+    integer(c_int), intent(in) :: term
+    real(kind=realkind), allocatable, intent(in) :: a(:,:)
+    if (allocated(a)) call axh_check(axh_upload_solid_term(ctx, term, a), 'axial term')
+  end subroutine
+
+end subroutine upload_all
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Coarse-grained attenuation terms.  The per-element SLS weights a_j are those of
+!! time_step_memvars_cg4 (SOLVER/attenuation.f90:105-127), evaluated for every element
+!! from the reference's own fitted y_j and Q values.
+subroutine upload_attenuation()
+
+  use data_matr
+  use data_pointwise
+  use attenuation,  only: n_sls_attenuation
+
+  real(kind=dp)       :: expw(64), tst(64), tstm1(64), yj(64)
+  integer(c_int)      :: docorr, n
+  real(kind=dp), allocatable :: ajmu(:,:), ajk(:,:), yp(:)
+  real(kind=realkind), allocatable :: invs(:,:)
+  integer             :: iel
+
+  n = axh_glue_reference_sls(64, expw, tst, tstm1, yj, docorr)
+  if (n /= n_sls_attenuation) then
+     write(6,*) 'GPU time loop: could not read the SLS coefficients of the attenuation module'
+     stop 2
+  endif
+  allocate(ajmu(n, nel_solid), ajk(n, nel_solid), yp(n))
+  do iel = 1, nel_solid
+     if (docorr /= 0) then
+        call sls_fast_correct(yj(1:n) / Q_mu(iel), yp)
+     else
+        yp = yj(1:n) / Q_mu(iel)
+     endif
+     ajmu(:, iel) = yp / sum(yp)
+     if (docorr /= 0) then
+        call sls_fast_correct(yj(1:n) / Q_kappa(iel), yp)
+     else
+        yp = yj(1:n) / Q_kappa(iel)
+     endif
+     ajk(:, iel) = yp / sum(yp)
+  enddo
+
+  ! f_over_s_solid_el_cg4 reads inv_s_solid at (1,1),(1,3),(3,1),(3,3)
+  ! (SOLVER/pointwise_derivatives.f90:66-69)
+  allocate(invs(4, nel_solid))
+  invs(1,:) = inv_s_solid(1,1,:)
+  invs(2,:) = inv_s_solid(1,3,:)
+  invs(3,:) = inv_s_solid(3,1,:)
+  invs(4,:) = inv_s_solid(3,3,:)
+
+  call axh_check(axh_glue_upload_att(ctx, Y_cg4, V_s_eta_cg4, V_s_xi_cg4, V_z_eta_cg4, &
+                 V_z_xi_cg4, DsDeta_over_J_sol_cg4, DzDeta_over_J_sol_cg4, &
+                 DsDxi_over_J_sol_cg4, DzDxi_over_J_sol_cg4, invs, delta_mu_cg4, &
+                 delta_kappa_cg4, expw, tst, tstm1, ajmu, ajk), 'attenuation')
+
+end subroutine upload_attenuation
+!-----------------------------------------------------------------------------------------
+
+end module time_evol_wave
+!=========================================================================================

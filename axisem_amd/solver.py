@@ -1,0 +1,266 @@
+"""Python host of the GPU time loop, over the C-ABI of include/axihip.h.
+
+Mirrors the reference's two-phase interface of module time_evol_wave
+(SOLVER/time_evol_wave.F90:26-41): a solver is *prepared* from the arrays the
+reference's setup produces (here: an axh state file written by the drop-in
+Fortran host, or the uploads done one by one) and the *time loop* is then run,
+in one call or in chunks, returning the receiver seismograms the reference
+writes (SOLVER/seismograms.f90:742-780).
+
+The library must be built (``axisem_amd/libaxihip.so``); there is no CPU
+fallback: without the library or without a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import state as _state
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libaxihip.so")
+
+AXH_OK, AXH_ERR_BLOWUP = 0, -6
+
+SOLID_TERMS = ["M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M11z", "M21z",
+               "M41z", "M32s", "M13s", "M33s", "M43s", "M1phi", "M2phi", "M4phi",
+               "M_1", "M_2", "M_3", "M_4", "M_5", "M_6", "M_7", "M_8",
+               "M_w1", "M_w2", "M_w3", "M_w4", "M_w5"]
+FLUID_TERMS = ["M1chi_fl", "M2chi_fl", "M4chi_fl", "M_w_fl"]
+
+
+class AxhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("axh error %d: %s" % (code, msg))
+        self.code = code
+
+
+class BlowUp(AxhError):
+    pass
+
+
+class Dims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in _state.DIMS_FIELDS] + \
+               [("reserved", C.c_int32 * 5), ("deltat", C.c_double), ("magnitude", C.c_double)]
+
+
+class Att(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "Y", "V_s_eta", "V_s_xi", "V_z_eta", "V_z_xi", "DsDeta_over_J", "DzDeta_over_J",
+        "DsDxi_over_J", "DzDxi_over_J", "inv_s", "delta_mu", "delta_kappa",
+        "exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa")]
+
+
+_lib = None
+
+
+def library():
+    """Load libaxihip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libaxihip.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P, I = C.c_void_p, C.c_int
+        L.axh_last_error.restype = C.c_char_p
+        L.axh_create.argtypes = [C.POINTER(Dims), C.POINTER(P)]
+        L.axh_destroy.argtypes = [P]
+        L.axh_load_state.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.axh_save_state.argtypes = [P, C.c_char_p]
+        L.axh_upload_spectral.argtypes = [P] * 6
+        L.axh_upload_solid_term.argtypes = [P, I, P]
+        L.axh_upload_fluid_term.argtypes = [P, I, P]
+        L.axh_upload_mass.argtypes = [P, P, P]
+        L.axh_upload_mesh_maps.argtypes = [P, P, P, P, P]
+        L.axh_upload_boundary.argtypes = [P, P, P, P, P, P]
+        L.axh_upload_fluid_surface_mask.argtypes = [P, P]
+        L.axh_upload_source.argtypes = [P, P, P, P, C.c_int32]
+        L.axh_upload_attenuation_cg4.argtypes = [P, C.POINTER(Att)]
+        L.axh_upload_receivers.argtypes = [P, P]
+        L.axh_run.argtypes = [P, C.c_int32, C.c_int32, P]
+        L.axh_seis_samples.argtypes = [P, C.c_int32, C.c_int32]
+        L.axh_download_fields.argtypes = [P, P, P, P, P, P]
+        L.axh_upload_fields.argtypes = [P, P, P, P, P, P]
+        L.axh_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
+        L.axh_device_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.axh_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.axh_kernel_name.restype = C.c_char_p
+        L.axh_kernel_name.argtypes = [C.c_int]
+        L.axh_set_profiling.argtypes = [P, C.c_int]
+        L.axh_symplectic_coefficients.argtypes = [C.c_int32, C.c_double, P, P, P,
+                                                  C.POINTER(C.c_int32)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != AXH_OK:
+        msg = library().axh_last_error().decode(errors="replace")
+        raise (BlowUp if rc == AXH_ERR_BLOWUP else AxhError)(rc, msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def symplectic_coefficients(scheme: int, dt: float):
+    cd, cv, cf = (np.zeros(16) for _ in range(3))
+    n = C.c_int32()
+    _check(library().axh_symplectic_coefficients(scheme, dt, _p(cd), _p(cv), _p(cf),
+                                                 C.byref(n)))
+    return cd[: n.value + 1], cv[: n.value], cf[: n.value]
+
+
+class Solver:
+    """One wavefield on one GPU: prepared once, then stepped by ``run``."""
+
+    def __init__(self, handle, dims: dict):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self.dims = dims
+        self.iter = 0
+
+    # --- construction -----------------------------------------------------
+    @classmethod
+    def from_state_file(cls, path: str) -> "Solver":
+        h = C.c_void_p()
+        _check(library().axh_load_state(path.encode(), C.byref(h)))
+        return cls(h, _state.read_dims(path))
+
+    @classmethod
+    def from_state(cls, st: _state.State) -> "Solver":
+        """Upload a State through the individual C-ABI upload calls, exactly as
+        the Fortran host does."""
+        L = library()
+        d = st.dims
+        dims = Dims()
+        for k in _state.DIMS_FIELDS:
+            setattr(dims, k, d[k])
+        dims.deltat, dims.magnitude = d["deltat"], d["magnitude"]
+        h = C.c_void_p()
+        _check(L.axh_create(C.byref(dims), C.byref(h)))
+        self = cls(h, dict(d))
+        f32 = lambda n: np.ascontiguousarray(st[n], np.float32)
+        i32 = lambda n: np.ascontiguousarray(st[n], np.int32)
+        keep = []
+
+        def K(a):
+            keep.append(a)
+            return _p(a)
+        _check(L.axh_upload_spectral(h, *[K(f32(n)) for n in ("G0", "G1", "G1T", "G2", "G2T")]))
+        for t, n in enumerate(SOLID_TERMS):
+            if n in st.arrays:
+                _check(L.axh_upload_solid_term(h, t, K(f32(n))))
+        for w in range(1, 11):
+            n = "M0_w%d" % w
+            if n in st.arrays:
+                _check(L.axh_upload_solid_term(h, 64 + w - 1, K(f32(n))))
+        for t, n in enumerate(FLUID_TERMS):
+            if n in st.arrays:
+                _check(L.axh_upload_fluid_term(h, t, K(f32(n))))
+        if "M0_w_fl" in st.arrays:
+            _check(L.axh_upload_fluid_term(h, 64, K(f32("M0_w_fl"))))
+        _check(L.axh_upload_mass(h, K(f32("inv_mass_rho")), K(f32("inv_mass_fluid"))))
+        _check(L.axh_upload_mesh_maps(h, K(i32("igloc_solid")), K(i32("igloc_fluid")),
+                                      K(i32("axis_solid")), K(i32("axis_fluid"))))
+        if d["nel_bdry"] > 0:
+            _check(L.axh_upload_boundary(h, *[K(i32(n)) for n in (
+                "bdry_solid_el", "bdry_fluid_el", "bdry_jpol_solid", "bdry_jpol_fluid")],
+                K(f32("bdry_matr"))))
+        if "fluid_free_surface_mask" in st.arrays:
+            _check(L.axh_upload_fluid_surface_mask(h, K(f32("fluid_free_surface_mask"))))
+        stf = f32("stf")
+        _check(L.axh_upload_source(h, K(i32("ielsrc")), K(f32("source_term")), K(stf),
+                                   int(stf.size)))
+        if d["anel"]:
+            a = Att()
+            for fld, n in (("Y", "Y_cg4"), ("V_s_eta", "V_s_eta_cg4"), ("V_s_xi", "V_s_xi_cg4"),
+                           ("V_z_eta", "V_z_eta_cg4"), ("V_z_xi", "V_z_xi_cg4"),
+                           ("DsDeta_over_J", "DsDeta_over_J_sol_cg4"),
+                           ("DzDeta_over_J", "DzDeta_over_J_sol_cg4"),
+                           ("DsDxi_over_J", "DsDxi_over_J_sol_cg4"),
+                           ("DzDxi_over_J", "DzDxi_over_J_sol_cg4"),
+                           ("inv_s", "inv_s_solid_cg4"), ("delta_mu", "delta_mu_cg4"),
+                           ("delta_kappa", "delta_kappa_cg4")):
+                setattr(a, fld, K(f32(n)))
+            for fld in ("exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa"):
+                setattr(a, fld, K(np.ascontiguousarray(st[fld], np.float64)))
+            _check(L.axh_upload_attenuation_cg4(h, C.byref(a)))
+        if d["num_rec"] > 0:
+            _check(L.axh_upload_receivers(h, K(i32("recfile_el"))))
+        return self
+
+    def close(self):
+        if self._h:
+            library().axh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- running ------------------------------------------------------------
+    def run(self, nsteps: int | None = None) -> np.ndarray:
+        """Advance nsteps (default: to niter); returns the receiver samples of
+        the sampled steps, shape (nsamples, num_rec, 3)."""
+        it0 = self.iter + 1
+        it1 = self.dims["niter"] if nsteps is None else min(self.dims["niter"], self.iter + nsteps)
+        n = library().axh_seis_samples(self._h, it0, it1)
+        seis = np.zeros((max(n, 1), max(self.dims["num_rec"], 1), 3), np.float32)
+        rc = library().axh_run(self._h, it0, it1, _p(seis))
+        self.iter = it1 if rc in (AXH_OK, AXH_ERR_BLOWUP) else self.iter
+        _check(rc)
+        return seis[:n]
+
+    def fields(self):
+        """(disp, velo, chi, dchi, ddchi) in the reference's element-local layout."""
+        d = self.dims
+        ns, nf = 25 * d["nel_solid"], 25 * d["nel_fluid"]
+        disp = np.zeros(3 * ns, np.float32)
+        velo = np.zeros(3 * ns, np.float32)
+        chi = np.zeros(max(nf, 1), np.float32)
+        dchi = np.zeros(max(nf, 1), np.float32)
+        ddchi = np.zeros(max(nf, 1), np.float32)
+        _check(library().axh_download_fields(self._h, _p(disp), _p(velo), _p(chi), _p(dchi),
+                                             _p(ddchi)))
+        return disp, velo, chi[:nf], dchi[:nf], ddchi[:nf]
+
+    def set_fields(self, disp=None, velo=None, chi=None, dchi=None, ddchi=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        arrs = [f(disp), f(velo), f(chi), f(dchi), f(ddchi)]
+        _check(library().axh_upload_fields(self._h, *[_p(a) for a in arrs]))
+
+    def last_run_ms(self) -> float:
+        v = C.c_double()
+        _check(library().axh_last_run_ms(self._h, C.byref(v)))
+        return v.value
+
+    def device_bytes(self) -> int:
+        v = C.c_size_t()
+        _check(library().axh_device_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def set_profiling(self, on: bool):
+        _check(library().axh_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        """Per kernel class: (total ms, launches) since profiling was enabled."""
+        out = {}
+        for k in range(16):
+            name = library().axh_kernel_name(k)
+            if not name:
+                break
+            ms, n = C.c_double(), C.c_int64()
+            _check(library().axh_kernel_stats(self._h, k, C.byref(ms), C.byref(n)))
+            out[name.decode()] = (ms.value, n.value)
+        return out
+

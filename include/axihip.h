@@ -1,0 +1,238 @@
+/*
+ * axihip.h -- C-ABI of the MI355X (gfx950) AxiSEM SOLVER time loop.
+ *
+ * This is the drop-in boundary for the reference's time loop: the reference
+ * program calls `time_loop` (SOLVER/main.f90:92) of module `time_evol_wave`
+ * (SOLVER/time_evol_wave.F90:26-41, public prepare_waves/time_loop, no
+ * arguments, all state in module globals).  A replacement `time_evol_wave`
+ * module written against this header hands those globals to the library
+ * through iso_c_binding and runs the loop on one GPU.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Every array is in the reference's Fortran
+ *    layout (column major, npol=4):
+ *      element fields/coefficients  (0:4,0:4,nel)   index i + 5 j + 25 (e-1)
+ *      solid fields with components (0:4,0:4,nel,3) index ... + 25 nel (c-1)
+ *      axial vectors                (0:4,nel)
+ *      coarse-grained (cg4) arrays  (1:4,nel)       points (1,1),(1,3),(3,1),(3,3)
+ *    Element and point indices passed in are 1-based, as in the reference.
+ *  - The library copies every array at upload and keeps no caller pointer.
+ *  - Functions return AXH_OK (0) or a negative AXH_ERR_* code; the message of
+ *    the last failure on the calling thread is returned by axh_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Nothing touches the GPU before the first axh_run / axh_download_fields /
+ *    axh_device_bytes call, so a context can be filled and written to a state
+ *    file on a host without a GPU (axh_save_state).
+ *  - One host thread per context; the library owns its HIP stream.
+ */
+#ifndef AXIHIP_H
+#define AXIHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXH_ABI_VERSION 1
+
+enum axh_status {
+  AXH_OK = 0,
+  AXH_ERR_ARG = -1,          /* bad argument / inconsistent sizes           */
+  AXH_ERR_STATE = -2,        /* call out of order, missing upload           */
+  AXH_ERR_HIP = -3,          /* HIP runtime failure                         */
+  AXH_ERR_IO = -4,           /* state file read/write failure               */
+  AXH_ERR_UNSUPPORTED = -5,  /* input the GPU path does not implement       */
+  AXH_ERR_BLOWUP = -6        /* |disp(1,1,:,:)| > 10 |magnitude|            */
+};
+
+/* src_type(1) of SOLVER/data_source.f90:31 */
+enum axh_src_type { AXH_MONOPOLE = 0, AXH_DIPOLE = 1, AXH_QUADPOLE = 2 };
+
+/* time_scheme of SOLVER/data_time.f90 (values of inparam_advanced TIME_SCHEME) */
+enum axh_scheme {
+  AXH_NEWMARK2 = 0,
+  AXH_SYMPLEC4 = 1,   /* PEFRL,            SOLVER/time_evol_wave.F90:763-791 */
+  AXH_ML_SO4M5 = 2,   /* McLachlan 4th,    SOLVER/time_evol_wave.F90:794-828 */
+  AXH_ML_SO6M7 = 3    /* McLachlan 6th,    SOLVER/time_evol_wave.F90:831-866 */
+};
+
+/* Problem dimensions and run constants (filled from data_mesh, data_time,
+ * data_source, attenuation by the Fortran host). */
+typedef struct axh_dims {
+  int32_t abi_version;      /* = AXH_ABI_VERSION                          */
+  int32_t npol;             /* must be 4                                  */
+  int32_t nel_solid, nel_fluid;
+  int32_t nglob_solid, nglob_fluid;   /* ranges of igloc_solid/igloc_fluid */
+  int32_t nel_bdry;         /* solid-fluid boundary element pairs         */
+  int32_t src_type;         /* enum axh_src_type                          */
+  int32_t scheme;           /* enum axh_scheme                            */
+  int32_t anel;             /* anel_true                                  */
+  int32_t coarse_grained;   /* att_coarse_grained (only 1 is supported)   */
+  int32_t n_sls;            /* n_sls_attenuation                          */
+  int32_t niter;            /* number of time steps                       */
+  int32_t seis_it;          /* seismogram sampling interval in steps      */
+  int32_t num_rec;          /* receivers (recfile_el rows)                */
+  int32_t nelsrc;           /* source elements (<= 8)                     */
+  int32_t fluid_src;        /* source in the fluid (explosion only)       */
+  int32_t reserved[5];
+  double deltat;            /* time step [s]                              */
+  double magnitude;         /* source magnitude, for the blow-up check    */
+} axh_dims;
+
+typedef struct axh_ctx axh_ctx;
+
+/* ---- lifetime ------------------------------------------------------------ */
+int axh_create(const axh_dims *dims, axh_ctx **out);
+int axh_destroy(axh_ctx *ctx);
+const char *axh_last_error(void);
+
+/* ---- uploads (SOLVER/data_spec.f90, data_matr.f90, data_mesh.f90) ------ */
+
+/* G0(0:4), G1/G1T/G2/G2T(0:4,0:4): data_spec (SOLVER/data_spec.f90:35-36). */
+int axh_upload_spectral(axh_ctx *ctx, const float *G0, const float *G1,
+                        const float *G1T, const float *G2, const float *G2T);
+
+/* Precomputed solid stiffness terms of data_matr (SOLVER/data_matr.f90:45-78),
+ * built by def_solid_stiffness_terms (SOLVER/def_precomp_terms.f90:1166-1626).
+ * 3-D terms are (0:4,0:4,nel_solid); AXH_M0_W* terms are (0:4,nel_solid). */
+enum axh_solid_term {
+  AXH_M11S = 0, AXH_M21S, AXH_M41S, AXH_M12S, AXH_M22S, AXH_M42S,
+  AXH_M11Z, AXH_M21Z, AXH_M41Z, AXH_M32S, AXH_M13S, AXH_M33S, AXH_M43S,
+  AXH_M1PHI, AXH_M2PHI, AXH_M4PHI,
+  AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_5, AXH_M_6, AXH_M_7, AXH_M_8,
+  AXH_M_W1, AXH_M_W2, AXH_M_W3, AXH_M_W4, AXH_M_W5,
+  AXH_N_SOLID_3D,
+  AXH_M0_W1 = 64, AXH_M0_W2, AXH_M0_W3, AXH_M0_W4, AXH_M0_W5, AXH_M0_W6,
+  AXH_M0_W7, AXH_M0_W8, AXH_M0_W9, AXH_M0_W10,
+  AXH_SOLID_TERM_END
+};
+int axh_upload_solid_term(axh_ctx *ctx, int term, const float *data);
+
+/* Fluid terms (SOLVER/data_matr.f90:80-84): M1chi_fl, M2chi_fl, M4chi_fl,
+ * M_w_fl are (0:4,0:4,nel_fluid); M0_w_fl is (0:4,nel_fluid). */
+enum axh_fluid_term {
+  AXH_M1CHI_FL = 0, AXH_M2CHI_FL, AXH_M4CHI_FL, AXH_M_W_FL,
+  AXH_M0_W_FL = 64
+};
+int axh_upload_fluid_term(axh_ctx *ctx, int term, const float *data);
+
+/* Assembled inverse mass matrices inv_mass_rho (0:4,0:4,nel_solid) and
+ * inv_mass_fluid (0:4,0:4,nel_fluid) (SOLVER/data_matr.f90:36-37). */
+int axh_upload_mass(axh_ctx *ctx, const float *inv_mass_rho,
+                    const float *inv_mass_fluid);
+
+/* Global numbering igloc_solid(25 nel_solid), igloc_fluid(25 nel_fluid)
+ * (SOLVER/data_mesh.f90:42-43, 1-based) and the axis flags axis_solid /
+ * axis_fluid (LOGICAL in SOLVER/data_mesh.f90:88-89, passed as 0/1 int32). */
+int axh_upload_mesh_maps(axh_ctx *ctx, const int32_t *igloc_solid,
+                         const int32_t *igloc_fluid, const int32_t *axis_solid,
+                         const int32_t *axis_fluid);
+
+/* Solid-fluid boundary (SOLVER/data_mesh.f90:93-95, data_matr.f90:90):
+ * bdry_solid_el, bdry_fluid_el, bdry_jpol_solid, bdry_jpol_fluid (nel_bdry,
+ * 1-based element indices) and bdry_matr(0:4,nel_bdry,2). */
+int axh_upload_boundary(axh_ctx *ctx, const int32_t *bdry_solid_el,
+                        const int32_t *bdry_fluid_el,
+                        const int32_t *bdry_jpol_solid,
+                        const int32_t *bdry_jpol_fluid, const float *bdry_matr);
+
+/* fluid_free_surface_mask(0:4,0:4,nel_fluid) (SOLVER/data_mesh.f90:181),
+ * built by create_fluid_free_surface_mask (SOLVER/time_evol_wave.F90:1615). */
+int axh_upload_fluid_surface_mask(axh_ctx *ctx, const float *mask);
+
+/* Source (SOLVER/data_source.f90:47-49): ielsrc(nelsrc) 1-based, the element
+ * source term source_term_el(0:4,0:4,nelsrc,3) (or source_term_fl
+ * (0:4,0:4,nelsrc) when fluid_src), and the source time function table:
+ * newmark2: stf(niter) as realkind (SOLVER/data_source.f90:37);
+ * symplectic: real(stf_symp(i), realkind) for every step and stage,
+ * stf[(iter-1)*nstages + (i-1)] (SOLVER/time_evol_wave.F90:590-593,689). */
+int axh_upload_source(axh_ctx *ctx, const int32_t *ielsrc,
+                      const float *source_term, const float *stf,
+                      int32_t nstf);
+
+/* Coarse-grained anelastic terms (SOLVER/data_matr.f90:103-117,
+ * data_pointwise.f90:40-43, attenuation.f90:31-44).  All float arrays are
+ * (1:4,nel_solid): Y_cg4, V_s_eta_cg4, V_s_xi_cg4, V_z_eta_cg4, V_z_xi_cg4,
+ * DsDeta/DzDeta/DsDxi/DzDxi_over_J_sol_cg4, inv_s_solid at the 4 cg points,
+ * delta_mu_cg4, delta_kappa_cg4.  exp_w_j_deltat, ts_fac_t, ts_fac_tm1 are
+ * (n_sls) doubles; a_j_mu, a_j_kappa are (n_sls, nel_solid) doubles: the
+ * normalised SLS weights of time_step_memvars_cg4
+ * (SOLVER/attenuation.f90:105-127) for every element. */
+typedef struct axh_attenuation_cg4 {
+  const float *Y, *V_s_eta, *V_s_xi, *V_z_eta, *V_z_xi;
+  const float *DsDeta_over_J, *DzDeta_over_J, *DsDxi_over_J, *DzDxi_over_J;
+  const float *inv_s;
+  const float *delta_mu, *delta_kappa;
+  const double *exp_w_j_deltat, *ts_fac_t, *ts_fac_tm1;
+  const double *a_j_mu, *a_j_kappa;
+} axh_attenuation_cg4;
+int axh_upload_attenuation_cg4(axh_ctx *ctx, const axh_attenuation_cg4 *att);
+
+/* Receivers: recfile_el(num_rec,3) = (element, ipol, jpol) in the solid,
+ * element 1-based, ipol/jpol 0-based (SOLVER/data_mesh.f90:135). */
+int axh_upload_receivers(axh_ctx *ctx, const int32_t *recfile_el);
+
+/* ---- running ------------------------------------------------------------- */
+
+/* Advance steps it_begin..it_end (1-based, inclusive; it_begin must be one
+ * past the last step run).  On return seis (may be NULL) holds, for every
+ * step with mod(iter, seis_it) == 0 inside the range, the receiver samples of
+ * compute_recfile_seis_bare (SOLVER/seismograms.f90:742-780) as
+ * seis[((k * num_rec) + r) * 3 + c], c in the written order
+ * (monopole: u_s, u_z, 0; dipole: u_1+u_2, u_1-u_2, u_3; quadpole: u_s,
+ * u_phi, u_z), k counting the sampled steps of this call from 0. */
+int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
+
+/* Number of seismogram samples axh_run(ctx, it_begin, it_end) will write. */
+int axh_seis_samples(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
+
+/* Copy the current fields back in the reference's element-local layout:
+ * disp/velo (0:4,0:4,nel_solid,3), chi/dchi/ddchi (0:4,0:4,nel_fluid).
+ * ddchi is ddchi0 (newmark2) or ddchi (symplectic).  Any pointer may be NULL. */
+int axh_download_fields(axh_ctx *ctx, float *disp, float *velo, float *chi,
+                        float *dchi, float *ddchi);
+
+/* Replace the current fields (same layout as axh_download_fields); copies of
+ * one global point must agree.  Any pointer may be NULL (left unchanged). */
+int axh_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
+                      const float *chi, const float *dchi, const float *ddchi);
+
+/* Time spent in the last axh_run on the device, from HIP events on the
+ * library stream [ms]. */
+int axh_last_run_ms(const axh_ctx *ctx, double *total_ms);
+
+/* Per-kernel timing.  When on, every kernel launch of axh_run is bracketed by
+ * HIP events on the library stream and its duration is accumulated per kernel
+ * class; axh_kernel_name(k) names class k (NULL past the last class). */
+int axh_set_profiling(axh_ctx *ctx, int on);
+const char *axh_kernel_name(int kclass);
+int axh_kernel_stats(const axh_ctx *ctx, int kclass, double *total_ms,
+                     int64_t *launches);
+
+/* Device memory held by the context [bytes] (0 before the first run). */
+int axh_device_bytes(axh_ctx *ctx, size_t *bytes);
+
+/* ---- state files --------------------------------------------------------- */
+
+/* Write every upload of ctx (plus dims) to a self-describing binary file, and
+ * rebuild a context from one.  Used to drive the library without the Fortran
+ * host (tests, benchmarks, the request loop). */
+int axh_save_state(const axh_ctx *ctx, const char *path);
+int axh_load_state(const char *path, axh_ctx **out);
+
+/* ---- helpers ------------------------------------------------------------- */
+
+/* coefd(nstages+1), coefv(nstages), coeff(nstages) scaled by deltat, as
+ * symplectic_coefficients (SOLVER/time_evol_wave.F90:749-967) computes them.
+ * Arrays must hold at least 8 entries; newmark2 returns nstages = 1 and
+ * fills nothing. */
+int axh_symplectic_coefficients(int32_t scheme, double deltat, double *coefd,
+                                double *coefv, double *coeff,
+                                int32_t *nstages);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXIHIP_H */
