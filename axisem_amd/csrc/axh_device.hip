@@ -50,6 +50,8 @@ constexpr uint8_t kFlagAxis = 1;      // on the symmetry axis (masked)
 constexpr uint8_t kFlagSource = 2;    // carries a source term (solid)
 constexpr uint8_t kFlagBlow = 4;      // GLL point (1,1) of an element
 constexpr uint8_t kFlagSurface = 8;   // fluid free surface (masked)
+constexpr uint8_t kFlagAxisBdry = 16; // axis point on the solid-fluid boundary
+constexpr int kMaxAxisBdry = 16;
 
 // kernel classes for timing
 enum KClass { KC_PRED = 0, KC_FLUID, KC_COUPLE_F, KC_SOLID, KC_ASM_F,
@@ -68,7 +70,7 @@ struct Device {
   hipStream_t stream = nullptr;
   int src = 0, scheme = 0, anel = 0, nsls = 0, nstages = 1;
   int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0;
-  int nglob_s = 0, nglob_f = 0, nbdry = 0, nrec = 0, nsrcpts = 0;
+  int nglob_s = 0, nglob_f = 0, nbdry = 0, nrec = 0, nsrcpts = 0, naxb = 0;
   double coefd[16] = {0}, coefv[16] = {0};
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
   std::vector<float> stf;  // host copy of the stf table
@@ -92,6 +94,9 @@ struct Device {
   // coupling
   int *cpl_ss = nullptr, *cpl_gs = nullptr, *cpl_sf = nullptr, *cpl_gf = nullptr;
   float *cpl_bm = nullptr;    // [nbdry*5][2]
+  int *cpl_axb = nullptr;     // side-buffer slot of a masked axis point or -1
+  int *axb_g = nullptr;       // global ids of the masked axis boundary points
+  float *axb_val = nullptr;   // their unmasked predicted displacement [naxb][3]
   // attenuation
   float *cg4 = nullptr, *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr,
         *devm1 = nullptr;
@@ -795,6 +800,9 @@ struct PredArgs {
   const uint8_t *flags;
   int stride;
   int *blow;
+  int naxb;
+  const int *axb_g;   // masked axis points on the solid-fluid boundary
+  float *axb_val;     // their unmasked prediction [naxb][3]
 };
 
 __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
@@ -811,8 +819,16 @@ __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
       un = (float)((double)u + A.dt * (double)v + A.hdt2 * (double)A.a0[p]);
     else
       un = (float)((double)u + (double)v * A.cd);
-    if (A.apply_mask && (fl & kFlagAxis) && c >= A.ncomp_mask0 && c <= A.ncomp_mask1)
+    if (A.apply_mask && (fl & kFlagAxis) && c >= A.ncomp_mask0 && c <= A.ncomp_mask1) {
+      if (fl & kFlagAxisBdry) {
+        for (int q = 0; q < A.naxb; q++)
+          if (A.axb_g[q] == g) A.axb_val[3 * q + c] = un;
+      }
       un = 0.0f;
+    } else if (fl & kFlagAxisBdry) {
+      for (int q = 0; q < A.naxb; q++)
+        if (A.axb_g[q] == g) A.axb_val[3 * q + c] = un;
+    }
     A.u[p] = un;
   }
 }
@@ -920,14 +936,24 @@ struct CoupleArgs {
   const float *ddchi;  // [nglob_f]
   float *force_f;      // [nslot_f]
   float *force_s;      // [3][nslot_s]
+  const int *axb;      // side-buffer slot per boundary point or -1
+  const float *axb_val;
 };
 
 __global__ void k_copy2fluid(CoupleArgs A) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= A.n) return;
-  const int g = A.gs[b];
-  const float s1 = A.disp[g], s2 = A.disp[(size_t)A.nglob_s + g],
-              s3 = A.disp[2 * (size_t)A.nglob_s + g];
+  const int g = A.gs[b], k = A.axb[b];
+  float s1, s2, s3;
+  if (k >= 0) {  // prediction before the axis mask
+    s1 = A.axb_val[3 * k];
+    s2 = A.axb_val[3 * k + 1];
+    s3 = A.axb_val[3 * k + 2];
+  } else {
+    s1 = A.disp[g];
+    s2 = A.disp[(size_t)A.nglob_s + g];
+    s3 = A.disp[2 * (size_t)A.nglob_s + g];
+  }
   const float bm1 = A.bm[2 * b], bm2 = A.bm[2 * b + 1];
   float f = A.force_f[A.sf[b]];
   if (A.dipole)
@@ -1050,8 +1076,295 @@ static void pack_terms(const axh_ctx *ctx, const std::vector<std::string> &names
   }
 }
 
+// Host-side layout of one context, computed without touching the GPU: global
+// numbering and gather lists, per-point values, flags, packed terms, coupling
+// and receiver maps.  Every consistency requirement of the per-point storage
+// is checked here.
+struct HostLayout {
+  int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0, nglob_s = 0, nglob_f = 0;
+  int nstages = 1, nsrcpts = 0, naxb = 0;
+  double coefd[16] = {0}, coefv[16] = {0};
+  std::vector<float> G, coef_s, m0_s, minv_s, src_val, coef_f, m0_f, minv_f, src_fl, cpl_bm,
+      cg4, stf;
+  std::vector<int> einfo_s, gid_s, gstart_s, gslot_s, src_g, einfo_f, gid_f, gstart_f,
+      gslot_f, srcel_f, cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, rec_g;
+  std::vector<uint8_t> flags_s, flags_f;
+  std::vector<double> aj_mu, aj_k;
+  double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
+};
+
+static void build_layout(const axh_ctx *ctx, HostLayout &H) {
+  const axh_dims &d = ctx->dims;
+  const int src = d.src_type;
+  const bool multipole = src != AXH_MONOPOLE;
+  H.nel_s = d.nel_solid;
+  H.nel_f = d.nel_fluid;
+  H.ntile_s = (H.nel_s + kElemPerTile - 1) / kElemPerTile;
+  H.ntile_f = (H.nel_f + kElemPerTile - 1) / kElemPerTile;
+  if (d.scheme != AXH_NEWMARK2) {
+    double cf[16];
+    H.nstages = symplectic_coefficients(d.scheme, d.deltat, H.coefd, H.coefv, cf);
+    if (H.nstages <= 0) fail(AXH_ERR_UNSUPPORTED, "time scheme");
+  }
+  // spectral constants
+  H.G.resize(105);
+  std::memcpy(&H.G[0], ctx->need("G0").as<float>(), 5 * 4);
+  std::memcpy(&H.G[5], ctx->need("G1").as<float>(), 25 * 4);
+  std::memcpy(&H.G[30], ctx->need("G1T").as<float>(), 25 * 4);
+  std::memcpy(&H.G[55], ctx->need("G2").as<float>(), 25 * 4);
+  std::memcpy(&H.G[80], ctx->need("G2T").as<float>(), 25 * 4);
+
+  // ---- solid ----
+  const int32_t *axis_s = ctx->need("axis_solid").as<int32_t>();
+  Numbering NS = build_numbering(ctx->need("igloc_solid").as<int32_t>(), H.nel_s,
+                                 d.nglob_solid, "solid");
+  H.nglob_s = NS.nglob;
+  H.minv_s = per_point(NS, ctx->need("inv_mass_rho").as<float>(), H.nel_s, "inv_mass_rho");
+  H.flags_s.assign(NS.nglob, 0);
+  std::vector<char> nonaxis_copy(NS.nglob, 0);
+  for (int e = 0; e < H.nel_s; e++)
+    for (int k = 0; k < kNpp; k++) {
+      int g = NS.gid[slot_of(e, k)];
+      if (axis_s[e] && (k % 5) == 0) H.flags_s[g] |= kFlagAxis;
+      else nonaxis_copy[g] = 1;
+      if (k == 1 + 5 * 1) H.flags_s[g] |= kFlagBlow;
+    }
+  for (int g = 0; g < NS.nglob; g++)
+    if ((H.flags_s[g] & kFlagAxis) && nonaxis_copy[g])
+      fail(AXH_ERR_UNSUPPORTED, "axis point shared with a non-axial copy");
+
+  // source terms (solid), added after assembly to every copy in a source
+  // element (SOLVER/time_evol_wave.F90:1082-1097): copies must agree
+  if (!d.fluid_src && d.nelsrc > 0) {
+    const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
+    const float *st = ctx->need("source_term").as<float>();
+    std::vector<int> srcidx(H.nel_s, -1);
+    for (int kk = 0; kk < d.nelsrc; kk++) {
+      if (iels[kk] < 1 || iels[kk] > H.nel_s) fail(AXH_ERR_ARG, "ielsrc out of range");
+      srcidx[iels[kk] - 1] = kk;
+    }
+    std::vector<float> val(3 * (size_t)NS.nglob, 0.0f);
+    std::vector<char> seen(NS.nglob, 0);
+    for (int e = 0; e < H.nel_s; e++)
+      for (int k = 0; k < kNpp; k++) {
+        int g = NS.gid[slot_of(e, k)];
+        float x[3];
+        for (int c = 0; c < 3; c++)
+          x[c] = srcidx[e] >= 0 ? st[k + kNpp * ((size_t)srcidx[e] + 8 * c)] : 0.0f;
+        if (!seen[g]) {
+          seen[g] = 1;
+          for (int c = 0; c < 3; c++) val[3 * g + c] = x[c];
+        } else {
+          for (int c = 0; c < 3; c++)
+            if (val[3 * g + c] != x[c])
+              fail(AXH_ERR_UNSUPPORTED, "source term differs between copies of a point");
+        }
+      }
+    for (int g = 0; g < NS.nglob; g++)
+      if (val[3 * g] != 0.0f || val[3 * g + 1] != 0.0f || val[3 * g + 2] != 0.0f) {
+        H.flags_s[g] |= kFlagSource;
+        H.src_g.push_back(g);
+        for (int c = 0; c < 3; c++) H.src_val.push_back(val[3 * g + c]);
+      }
+  }
+  H.nsrcpts = (int)H.src_g.size();
+
+  // packed stiffness terms
+  std::vector<std::string> names;
+  auto S = [&](int term) { names.push_back(kSolid3dNames[term]); };
+  if (src == AXH_MONOPOLE) {
+    for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_W1, AXH_M11S, AXH_M21S,
+                  AXH_M41S, AXH_M12S, AXH_M22S, AXH_M32S, AXH_M42S, AXH_M11Z, AXH_M21Z,
+                  AXH_M41Z})
+      S(t);
+  } else if (src == AXH_DIPOLE) {
+    for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_5, AXH_M_6, AXH_M_7, AXH_M_8,
+                  AXH_M_W1, AXH_M_W2, AXH_M_W3, AXH_M11S, AXH_M21S, AXH_M41S, AXH_M12S,
+                  AXH_M22S, AXH_M42S, AXH_M13S, AXH_M32S, AXH_M33S, AXH_M43S, AXH_M11Z,
+                  AXH_M21Z, AXH_M41Z})
+      S(t);
+  } else {
+    for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_5, AXH_M_6, AXH_M_7, AXH_M_8,
+                  AXH_M_W1, AXH_M_W2, AXH_M_W3, AXH_M_W4, AXH_M_W5, AXH_M11S, AXH_M21S,
+                  AXH_M41S, AXH_M12S, AXH_M22S, AXH_M32S, AXH_M42S, AXH_M11Z, AXH_M21Z,
+                  AXH_M41Z, AXH_M1PHI, AXH_M2PHI, AXH_M4PHI})
+      S(t);
+  }
+  pack_terms(ctx, names, H.nel_s, H.coef_s);
+
+  // axial terms M0_w1..10
+  H.einfo_s.assign(H.nel_s, -1);
+  int nax = 0;
+  for (int e = 0; e < H.nel_s; e++)
+    if (axis_s[e]) {
+      H.einfo_s[e] = nax++;
+      for (int w = 1; w <= 10; w++) {
+        const Array *a = ctx->get("M0_w" + std::to_string(w));
+        for (int j = 0; j < 5; j++)
+          H.m0_s.push_back(a ? a->as<float>()[(size_t)e * 5 + j] : 0.0f);
+      }
+    }
+  H.gid_s = std::move(NS.gid);
+  H.gstart_s = std::move(NS.gstart);
+  H.gslot_s = std::move(NS.gslot);
+
+  // ---- fluid ----
+  if (H.nel_f > 0) {
+    const int32_t *axis_f = ctx->need("axis_fluid").as<int32_t>();
+    Numbering NF = build_numbering(ctx->need("igloc_fluid").as<int32_t>(), H.nel_f,
+                                   d.nglob_fluid, "fluid");
+    H.nglob_f = NF.nglob;
+    H.minv_f = per_point(NF, ctx->need("inv_mass_fluid").as<float>(), H.nel_f,
+                         "inv_mass_fluid");
+    std::vector<float> surf = per_point(NF, ctx->need("fluid_free_surface_mask").as<float>(),
+                                        H.nel_f, "fluid_free_surface_mask");
+    H.flags_f.assign(NF.nglob, 0);
+    std::vector<char> nonaxis(NF.nglob, 0);
+    for (int e = 0; e < H.nel_f; e++)
+      for (int k = 0; k < kNpp; k++) {
+        int g = NF.gid[slot_of(e, k)];
+        if (axis_f[e] && (k % 5) == 0) H.flags_f[g] |= kFlagAxis;
+        else nonaxis[g] = 1;
+      }
+    for (int g = 0; g < NF.nglob; g++) {
+      if ((H.flags_f[g] & kFlagAxis) && nonaxis[g] && multipole)
+        fail(AXH_ERR_UNSUPPORTED, "fluid axis point shared with a non-axial copy");
+      if (surf[g] == 0.0f) H.flags_f[g] |= kFlagSurface;
+      else if (surf[g] != 1.0f) fail(AXH_ERR_UNSUPPORTED, "fluid surface mask must be 0 or 1");
+    }
+    std::vector<std::string> fn = {kFluid3dNames[0], kFluid3dNames[1], kFluid3dNames[2]};
+    if (multipole) fn.push_back(kFluid3dNames[3]);
+    pack_terms(ctx, fn, H.nel_f, H.coef_f);
+    H.einfo_f.assign(H.nel_f, -1);
+    int naxf = 0;
+    const Array *m0a = ctx->get("M0_w_fl");
+    for (int e = 0; e < H.nel_f; e++)
+      if (axis_f[e]) {
+        H.einfo_f[e] = naxf++;
+        for (int j = 0; j < 5; j++) H.m0_f.push_back(m0a ? m0a->as<float>()[(size_t)e * 5 + j] : 0.0f);
+      }
+    if (multipole && naxf > 0 && !m0a) fail(AXH_ERR_STATE, "missing upload: M0_w_fl");
+    H.srcel_f.assign(H.nel_f, -1);
+    H.src_fl.assign(25 * 8, 0.0f);
+    if (d.fluid_src && d.nelsrc > 0) {
+      const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
+      for (int kk = 0; kk < d.nelsrc; kk++) {
+        if (iels[kk] < 1 || iels[kk] > H.nel_f) fail(AXH_ERR_ARG, "ielsrc out of range");
+        H.srcel_f[iels[kk] - 1] = kk;
+      }
+      std::memcpy(H.src_fl.data(), ctx->need("source_term").as<float>(), 25 * 8 * 4);
+    }
+    H.gid_f = std::move(NF.gid);
+    H.gstart_f = std::move(NF.gstart);
+    H.gslot_f = std::move(NF.gslot);
+  }
+
+  // ---- coupling ----
+  if (d.nel_bdry > 0) {
+    const int32_t *bse = ctx->need("bdry_solid_el").as<int32_t>();
+    const int32_t *bfe = ctx->need("bdry_fluid_el").as<int32_t>();
+    const int32_t *bjs = ctx->need("bdry_jpol_solid").as<int32_t>();
+    const int32_t *bjf = ctx->need("bdry_jpol_fluid").as<int32_t>();
+    const float *bm = ctx->need("bdry_matr").as<float>();
+    std::vector<char> used_s((size_t)H.ntile_s * kTile, 0), used_f((size_t)H.ntile_f * kTile, 0);
+    for (int b = 0; b < d.nel_bdry; b++) {
+      int es = bse[b] - 1, ef = bfe[b] - 1;
+      if (es < 0 || es >= H.nel_s || ef < 0 || ef >= H.nel_f || bjs[b] < 0 || bjs[b] > 4 ||
+          bjf[b] < 0 || bjf[b] > 4)
+        fail(AXH_ERR_ARG, "boundary element index out of range");
+      for (int i = 0; i < 5; i++) {
+        size_t s1 = slot_of(es, i + 5 * bjs[b]), s2 = slot_of(ef, i + 5 * bjf[b]);
+        if (used_s[s1] || used_f[s2])
+          fail(AXH_ERR_UNSUPPORTED, "a GLL point appears twice on the boundary");
+        used_s[s1] = used_f[s2] = 1;
+        const int g = H.gid_s[s1];
+        // bdry_copy2fluid reads the predicted displacement before the axis
+        // mask (SOLVER/time_evol_wave.F90:377,389): masked boundary points
+        // keep their unmasked prediction in a side buffer
+        int k = -1;
+        if (H.flags_s[g] & kFlagAxis) {
+          for (size_t q = 0; q < H.axb_g.size(); q++)
+            if (H.axb_g[q] == g) k = (int)q;
+          if (k < 0) {
+            k = (int)H.axb_g.size();
+            H.axb_g.push_back(g);
+            H.flags_s[g] |= kFlagAxisBdry;
+          }
+        }
+        H.cpl_ss.push_back((int)s1);
+        H.cpl_gs.push_back(g);
+        H.cpl_sf.push_back((int)s2);
+        H.cpl_gf.push_back(H.gid_f[s2]);
+        H.cpl_axb.push_back(k);
+        H.cpl_bm.push_back(bm[i + 5 * (size_t)b]);
+        H.cpl_bm.push_back(bm[i + 5 * ((size_t)b + d.nel_bdry)]);
+      }
+    }
+    H.naxb = (int)H.axb_g.size();
+    if (H.naxb > kMaxAxisBdry) fail(AXH_ERR_UNSUPPORTED, "too many axis points on the boundary");
+  }
+
+  // ---- attenuation ----
+  if (d.anel) {
+    const char *cgn[kCg4Terms] = {"Y_cg4", "V_s_eta_cg4", "V_s_xi_cg4", "V_z_eta_cg4",
+                                  "V_z_xi_cg4", "DsDeta_over_J_sol_cg4",
+                                  "DzDeta_over_J_sol_cg4", "DsDxi_over_J_sol_cg4",
+                                  "DzDxi_over_J_sol_cg4", "inv_s_solid_cg4", "delta_mu_cg4",
+                                  "delta_kappa_cg4"};
+    H.cg4.resize((size_t)H.nel_s * kCg4Terms * 4);
+    for (int a = 0; a < kCg4Terms; a++) {
+      const float *p = ctx->need(cgn[a]).as<float>();
+      for (int e = 0; e < H.nel_s; e++)
+        for (int q = 0; q < 4; q++) H.cg4[((size_t)e * kCg4Terms + a) * 4 + q] = p[(size_t)e * 4 + q];
+    }
+    const double *ew = ctx->need("exp_w_j_deltat").as<double>();
+    const double *tt = ctx->need("ts_fac_t").as<double>();
+    const double *tm = ctx->need("ts_fac_tm1").as<double>();
+    for (int s = 0; s < d.n_sls; s++) {
+      H.expw[s] = ew[s];
+      H.tst[s] = tt[s];
+      H.tstm1[s] = tm[s];
+    }
+    const Array &amu = ctx->need("a_j_mu"), &ak = ctx->need("a_j_kappa");
+    H.aj_mu.assign(amu.as<double>(), amu.as<double>() + amu.count());
+    H.aj_k.assign(ak.as<double>(), ak.as<double>() + ak.count());
+  }
+
+  // ---- receivers ----
+  if (d.num_rec > 0) {
+    const int32_t *rf = ctx->need("recfile_el").as<int32_t>();
+    H.rec_g.resize(d.num_rec);
+    for (int r = 0; r < d.num_rec; r++) {
+      int e = rf[r] - 1, i = rf[r + d.num_rec], j = rf[r + 2 * d.num_rec];
+      if (e < 0 || e >= H.nel_s || i < 0 || i > 4 || j < 0 || j > 4)
+        fail(AXH_ERR_ARG, "receiver index out of range");
+      H.rec_g[r] = H.gid_s[slot_of(e, i + 5 * j)];
+    }
+  }
+  const Array &stf = ctx->need("stf");
+  H.stf.assign(stf.as<float>(), stf.as<float>() + stf.count());
+}
+
+void check_inputs(const axh_ctx *ctx) {
+  HostLayout H;
+  build_layout(ctx, H);
+}
+
+void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob) {
+  const bool solid = domain == 0;
+  const int nel = solid ? ctx->dims.nel_solid : ctx->dims.nel_fluid;
+  Numbering N = build_numbering(ctx->need(solid ? "igloc_solid" : "igloc_fluid").as<int32_t>(),
+                                nel, solid ? ctx->dims.nglob_solid : ctx->dims.nglob_fluid,
+                                solid ? "solid" : "fluid");
+  for (int e = 0; e < nel; e++)
+    for (int k = 0; k < kNpp; k++) gid[(size_t)e * kNpp + k] = N.gid[slot_of(e, k)];
+  *nglob = N.nglob;
+}
+
 void device_build(axh_ctx *ctx) {
   const axh_dims &d = ctx->dims;
+  HostLayout H;
+  build_layout(ctx, H);
   std::unique_ptr<Device> D(new Device);
   Device &V = *D;
   try {
@@ -1059,18 +1372,23 @@ void device_build(axh_ctx *ctx) {
     V.scheme = d.scheme;
     V.anel = d.anel;
     V.nsls = d.n_sls;
-    V.nel_s = d.nel_solid;
-    V.nel_f = d.nel_fluid;
-    V.ntile_s = (V.nel_s + kElemPerTile - 1) / kElemPerTile;
-    V.ntile_f = (V.nel_f + kElemPerTile - 1) / kElemPerTile;
+    V.nel_s = H.nel_s;
+    V.nel_f = H.nel_f;
+    V.ntile_s = H.ntile_s;
+    V.ntile_f = H.ntile_f;
     V.nbdry = d.nel_bdry;
     V.nrec = d.num_rec;
-    if (V.scheme != AXH_NEWMARK2) {
-      double cf[16];
-      V.nstages = symplectic_coefficients(V.scheme, d.deltat, V.coefd, V.coefv, cf);
-      if (V.nstages <= 0) fail(AXH_ERR_UNSUPPORTED, "time scheme");
-    }
-    const bool multipole = V.src != AXH_MONOPOLE;
+    V.nstages = H.nstages;
+    V.nglob_s = H.nglob_s;
+    V.nglob_f = H.nglob_f;
+    V.nsrcpts = H.nsrcpts;
+    V.naxb = H.naxb;
+    std::memcpy(V.coefd, H.coefd, sizeof V.coefd);
+    std::memcpy(V.coefv, H.coefv, sizeof V.coefv);
+    std::memcpy(V.expw, H.expw, sizeof V.expw);
+    std::memcpy(V.tst, H.tst, sizeof V.tst);
+    std::memcpy(V.tstm1, H.tstm1, sizeof V.tstm1);
+    V.stf = H.stf;
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -1079,284 +1397,58 @@ void device_build(axh_ctx *ctx) {
     HIPCHECK(hipEventCreate(&V.ev0));
     HIPCHECK(hipEventCreate(&V.ev1));
 
-    // spectral constants
-    std::vector<float> G(105);
-    std::memcpy(&G[0], ctx->need("G0").as<float>(), 5 * 4);
-    std::memcpy(&G[5], ctx->need("G1").as<float>(), 25 * 4);
-    std::memcpy(&G[30], ctx->need("G1T").as<float>(), 25 * 4);
-    std::memcpy(&G[55], ctx->need("G2").as<float>(), 25 * 4);
-    std::memcpy(&G[80], ctx->need("G2T").as<float>(), 25 * 4);
-    V.G = V.upload(G);
-
-    // ---- solid ----
-    const int32_t *axis_s = ctx->need("axis_solid").as<int32_t>();
-    Numbering NS = build_numbering(ctx->need("igloc_solid").as<int32_t>(), V.nel_s,
-                                   d.nglob_solid, "solid");
-    V.nglob_s = NS.nglob;
-    std::vector<float> minv_s =
-        per_point(NS, ctx->need("inv_mass_rho").as<float>(), V.nel_s, "inv_mass_rho");
-    std::vector<uint8_t> flags_s(NS.nglob, 0);
-    std::vector<char> nonaxis_copy(NS.nglob, 0);
-    for (int e = 0; e < V.nel_s; e++)
-      for (int k = 0; k < kNpp; k++) {
-        int g = NS.gid[slot_of(e, k)];
-        if (axis_s[e] && (k % 5) == 0) flags_s[g] |= kFlagAxis;
-        else nonaxis_copy[g] = 1;
-        if (k == 1 + 5 * 1) flags_s[g] |= kFlagBlow;
-      }
-    for (int g = 0; g < NS.nglob; g++)
-      if ((flags_s[g] & kFlagAxis) && nonaxis_copy[g])
-        fail(AXH_ERR_UNSUPPORTED, "axis point shared with a non-axial copy");
-
-    // source terms (solid), added after assembly to every copy in a source
-    // element (SOLVER/time_evol_wave.F90:1082-1097): copies must agree
-    std::vector<int> src_g;
-    std::vector<float> src_val;
-    if (!d.fluid_src && d.nelsrc > 0) {
-      const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
-      const float *st = ctx->need("source_term").as<float>();
-      std::vector<int> srcidx(V.nel_s, -1);
-      for (int kk = 0; kk < d.nelsrc; kk++) {
-        if (iels[kk] < 1 || iels[kk] > V.nel_s) fail(AXH_ERR_ARG, "ielsrc out of range");
-        srcidx[iels[kk] - 1] = kk;
-      }
-      std::vector<float> val(3 * (size_t)NS.nglob, 0.0f);
-      std::vector<char> seen(NS.nglob, 0);
-      for (int e = 0; e < V.nel_s; e++)
-        for (int k = 0; k < kNpp; k++) {
-          int g = NS.gid[slot_of(e, k)];
-          float x[3];
-          for (int c = 0; c < 3; c++)
-            x[c] = srcidx[e] >= 0 ? st[k + kNpp * ((size_t)srcidx[e] + 8 * c)] : 0.0f;
-          if (!seen[g]) {
-            seen[g] = 1;
-            for (int c = 0; c < 3; c++) val[3 * g + c] = x[c];
-          } else {
-            for (int c = 0; c < 3; c++)
-              if (val[3 * g + c] != x[c])
-                fail(AXH_ERR_UNSUPPORTED, "source term differs between copies of a point");
-          }
-        }
-      for (int g = 0; g < NS.nglob; g++)
-        if (val[3 * g] != 0.0f || val[3 * g + 1] != 0.0f || val[3 * g + 2] != 0.0f) {
-          flags_s[g] |= kFlagSource;
-          src_g.push_back(g);
-          for (int c = 0; c < 3; c++) src_val.push_back(val[3 * g + c]);
-        }
-    }
-    V.nsrcpts = (int)src_g.size();
-
-    // packed stiffness terms
-    std::vector<std::string> names;
-    auto S = [&](int term) { names.push_back(kSolid3dNames[term]); };
-    if (V.src == AXH_MONOPOLE) {
-      for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_W1, AXH_M11S, AXH_M21S,
-                    AXH_M41S, AXH_M12S, AXH_M22S, AXH_M32S, AXH_M42S, AXH_M11Z,
-                    AXH_M21Z, AXH_M41Z})
-        S(t);
-    } else if (V.src == AXH_DIPOLE) {
-      for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_5, AXH_M_6, AXH_M_7,
-                    AXH_M_8, AXH_M_W1, AXH_M_W2, AXH_M_W3, AXH_M11S, AXH_M21S,
-                    AXH_M41S, AXH_M12S, AXH_M22S, AXH_M42S, AXH_M13S, AXH_M32S,
-                    AXH_M33S, AXH_M43S, AXH_M11Z, AXH_M21Z, AXH_M41Z})
-        S(t);
-    } else {
-      for (int t : {AXH_M_1, AXH_M_2, AXH_M_3, AXH_M_4, AXH_M_5, AXH_M_6, AXH_M_7,
-                    AXH_M_8, AXH_M_W1, AXH_M_W2, AXH_M_W3, AXH_M_W4, AXH_M_W5,
-                    AXH_M11S, AXH_M21S, AXH_M41S, AXH_M12S, AXH_M22S, AXH_M32S,
-                    AXH_M42S, AXH_M11Z, AXH_M21Z, AXH_M41Z, AXH_M1PHI, AXH_M2PHI,
-                    AXH_M4PHI})
-        S(t);
-    }
-    std::vector<float> coef;
-    pack_terms(ctx, names, V.nel_s, coef);
-    V.coef_s = V.upload(coef);
-    coef.clear();
-    coef.shrink_to_fit();
-
-    // axial terms M0_w1..10
-    std::vector<int> einfo(V.nel_s, -1);
-    std::vector<float> m0;
-    int nax = 0;
-    for (int e = 0; e < V.nel_s; e++)
-      if (axis_s[e]) {
-        einfo[e] = nax++;
-        for (int w = 1; w <= 10; w++) {
-          const Array *a = ctx->get("M0_w" + std::to_string(w));
-          for (int j = 0; j < 5; j++) m0.push_back(a ? a->as<float>()[(size_t)e * 5 + j] : 0.0f);
-        }
-      }
-    V.einfo_s = V.upload(einfo);
-    V.m0_s = V.upload(m0);
-    V.gid_s = V.upload(NS.gid);
-    V.gstart_s = V.upload(NS.gstart);
-    V.gslot_s = V.upload(NS.gslot);
-    V.minv_s = V.upload(minv_s);
-    V.src_g = V.upload(src_g);
-    V.src_val = V.upload(src_val);
+    V.G = V.upload(H.G);
     const size_t nslot_s = (size_t)V.ntile_s * kTile;
-    V.disp = V.zeros<float>(3 * (size_t)NS.nglob);
-    V.velo = V.zeros<float>(3 * (size_t)NS.nglob);
-    V.acc0 = V.zeros<float>(3 * (size_t)NS.nglob);
+    V.coef_s = V.upload(H.coef_s);
+    V.einfo_s = V.upload(H.einfo_s);
+    V.m0_s = V.upload(H.m0_s);
+    V.gid_s = V.upload(H.gid_s);
+    V.gstart_s = V.upload(H.gstart_s);
+    V.gslot_s = V.upload(H.gslot_s);
+    V.minv_s = V.upload(H.minv_s);
+    V.flags_s = V.upload(H.flags_s);
+    V.src_g = V.upload(H.src_g);
+    V.src_val = V.upload(H.src_val);
+    V.disp = V.zeros<float>(3 * (size_t)V.nglob_s);
+    V.velo = V.zeros<float>(3 * (size_t)V.nglob_s);
+    V.acc0 = V.zeros<float>(3 * (size_t)V.nglob_s);
     V.force_s = V.zeros<float>(3 * nslot_s);
-
-    // ---- fluid ----
-    Numbering NF;
-    std::vector<uint8_t> flags_f;
     if (V.nel_f > 0) {
-      const int32_t *axis_f = ctx->need("axis_fluid").as<int32_t>();
-      NF = build_numbering(ctx->need("igloc_fluid").as<int32_t>(), V.nel_f,
-                           d.nglob_fluid, "fluid");
-      V.nglob_f = NF.nglob;
-      std::vector<float> minv_f = per_point(NF, ctx->need("inv_mass_fluid").as<float>(),
-                                            V.nel_f, "inv_mass_fluid");
-      std::vector<float> surf = per_point(
-          NF, ctx->need("fluid_free_surface_mask").as<float>(), V.nel_f,
-          "fluid_free_surface_mask");
-      flags_f.assign(NF.nglob, 0);
-      std::vector<char> nonaxis(NF.nglob, 0);
-      for (int e = 0; e < V.nel_f; e++)
-        for (int k = 0; k < kNpp; k++) {
-          int g = NF.gid[slot_of(e, k)];
-          if (axis_f[e] && (k % 5) == 0) flags_f[g] |= kFlagAxis;
-          else nonaxis[g] = 1;
-        }
-      for (int g = 0; g < NF.nglob; g++) {
-        if ((flags_f[g] & kFlagAxis) && nonaxis[g] && multipole)
-          fail(AXH_ERR_UNSUPPORTED, "fluid axis point shared with a non-axial copy");
-        if (surf[g] == 0.0f) flags_f[g] |= kFlagSurface;
-        else if (surf[g] != 1.0f)
-          fail(AXH_ERR_UNSUPPORTED, "fluid surface mask must be 0 or 1");
-      }
-      std::vector<std::string> fn = {kFluid3dNames[0], kFluid3dNames[1], kFluid3dNames[2]};
-      if (multipole) fn.push_back(kFluid3dNames[3]);
-      pack_terms(ctx, fn, V.nel_f, coef);
-      V.coef_f = V.upload(coef);
-      coef.clear();
-      std::vector<int> einfo_f(V.nel_f, -1);
-      std::vector<float> m0f;
-      int naxf = 0;
-      const Array *m0a = ctx->get("M0_w_fl");
-      for (int e = 0; e < V.nel_f; e++)
-        if (axis_f[e]) {
-          einfo_f[e] = naxf++;
-          for (int j = 0; j < 5; j++)
-            m0f.push_back(m0a ? m0a->as<float>()[(size_t)e * 5 + j] : 0.0f);
-        }
-      if (multipole && naxf > 0 && !m0a) fail(AXH_ERR_STATE, "missing upload: M0_w_fl");
-      V.einfo_f = V.upload(einfo_f);
-      V.m0_f = V.upload(m0f);
-      V.gid_f = V.upload(NF.gid);
-      V.gstart_f = V.upload(NF.gstart);
-      V.gslot_f = V.upload(NF.gslot);
-      V.minv_f = V.upload(minv_f);
-      V.flags_f = V.upload(flags_f);
-      std::vector<int> srcel(V.nel_f, -1);
-      std::vector<float> srcfl(25 * 8, 0.0f);
-      if (d.fluid_src && d.nelsrc > 0) {
-        const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
-        for (int kk = 0; kk < d.nelsrc; kk++) {
-          if (iels[kk] < 1 || iels[kk] > V.nel_f) fail(AXH_ERR_ARG, "ielsrc out of range");
-          srcel[iels[kk] - 1] = kk;
-        }
-        std::memcpy(srcfl.data(), ctx->need("source_term").as<float>(), 25 * 8 * 4);
-      }
-      V.srcel_f = V.upload(srcel);
-      V.src_fl = V.upload(srcfl);
-      const size_t nslot_f = (size_t)V.ntile_f * kTile;
-      V.chi = V.zeros<float>(NF.nglob);
-      V.dchi = V.zeros<float>(NF.nglob);
-      V.ddchi = V.zeros<float>(NF.nglob);
-      V.force_f = V.zeros<float>(nslot_f);
+      V.coef_f = V.upload(H.coef_f);
+      V.einfo_f = V.upload(H.einfo_f);
+      V.m0_f = V.upload(H.m0_f);
+      V.gid_f = V.upload(H.gid_f);
+      V.gstart_f = V.upload(H.gstart_f);
+      V.gslot_f = V.upload(H.gslot_f);
+      V.minv_f = V.upload(H.minv_f);
+      V.flags_f = V.upload(H.flags_f);
+      V.srcel_f = V.upload(H.srcel_f);
+      V.src_fl = V.upload(H.src_fl);
+      V.chi = V.zeros<float>(V.nglob_f);
+      V.dchi = V.zeros<float>(V.nglob_f);
+      V.ddchi = V.zeros<float>(V.nglob_f);
+      V.force_f = V.zeros<float>((size_t)V.ntile_f * kTile);
     }
-    V.flags_s = V.upload(flags_s);
-
-    // ---- coupling ----
     if (V.nbdry > 0) {
-      const int32_t *bse = ctx->need("bdry_solid_el").as<int32_t>();
-      const int32_t *bfe = ctx->need("bdry_fluid_el").as<int32_t>();
-      const int32_t *bjs = ctx->need("bdry_jpol_solid").as<int32_t>();
-      const int32_t *bjf = ctx->need("bdry_jpol_fluid").as<int32_t>();
-      const float *bm = ctx->need("bdry_matr").as<float>();
-      std::vector<int> ss, gs, sf, gf;
-      std::vector<float> bmp;
-      std::vector<char> used_s(nslot_s, 0), used_f((size_t)V.ntile_f * kTile, 0);
-      for (int b = 0; b < V.nbdry; b++) {
-        int es = bse[b] - 1, ef = bfe[b] - 1;
-        if (es < 0 || es >= V.nel_s || ef < 0 || ef >= V.nel_f || bjs[b] < 0 ||
-            bjs[b] > 4 || bjf[b] < 0 || bjf[b] > 4)
-          fail(AXH_ERR_ARG, "boundary element index out of range");
-        for (int i = 0; i < 5; i++) {
-          size_t s1 = slot_of(es, i + 5 * bjs[b]), s2 = slot_of(ef, i + 5 * bjf[b]);
-          if (used_s[s1] || used_f[s2])
-            fail(AXH_ERR_UNSUPPORTED, "a GLL point appears twice on the boundary");
-          used_s[s1] = used_f[s2] = 1;
-          int g = NS.gid[s1];
-          if ((flags_s[g] & kFlagAxis) && (flags_s[g] & kFlagSource))
-            fail(AXH_ERR_UNSUPPORTED, "source on the solid-fluid boundary at the axis");
-          ss.push_back((int)s1);
-          gs.push_back(g);
-          sf.push_back((int)s2);
-          gf.push_back(NF.gid[s2]);
-          bmp.push_back(bm[i + 5 * (size_t)b]);
-          bmp.push_back(bm[i + 5 * ((size_t)b + V.nbdry)]);
-        }
-      }
-      V.cpl_ss = V.upload(ss);
-      V.cpl_gs = V.upload(gs);
-      V.cpl_sf = V.upload(sf);
-      V.cpl_gf = V.upload(gf);
-      V.cpl_bm = V.upload(bmp);
+      V.cpl_ss = V.upload(H.cpl_ss);
+      V.cpl_gs = V.upload(H.cpl_gs);
+      V.cpl_sf = V.upload(H.cpl_sf);
+      V.cpl_gf = V.upload(H.cpl_gf);
+      V.cpl_axb = V.upload(H.cpl_axb);
+      V.cpl_bm = V.upload(H.cpl_bm);
+      V.axb_g = V.upload(H.axb_g);
+      V.axb_val = V.zeros<float>(3 * (size_t)std::max(V.naxb, 1));
     }
-
-    // ---- attenuation ----
     if (V.anel) {
-      const char *cgn[kCg4Terms] = {"Y_cg4", "V_s_eta_cg4", "V_s_xi_cg4", "V_z_eta_cg4",
-                                    "V_z_xi_cg4", "DsDeta_over_J_sol_cg4",
-                                    "DzDeta_over_J_sol_cg4", "DsDxi_over_J_sol_cg4",
-                                    "DzDxi_over_J_sol_cg4", "inv_s_solid_cg4",
-                                    "delta_mu_cg4", "delta_kappa_cg4"};
-      std::vector<float> cg((size_t)V.nel_s * kCg4Terms * 4);
-      for (int a = 0; a < kCg4Terms; a++) {
-        const float *p = ctx->need(cgn[a]).as<float>();
-        for (int e = 0; e < V.nel_s; e++)
-          for (int q = 0; q < 4; q++)
-            cg[((size_t)e * kCg4Terms + a) * 4 + q] = p[(size_t)e * 4 + q];
-      }
-      V.cg4 = V.upload(cg);
-      const double *ew = ctx->need("exp_w_j_deltat").as<double>();
-      const double *tt = ctx->need("ts_fac_t").as<double>();
-      const double *tm = ctx->need("ts_fac_tm1").as<double>();
-      for (int s = 0; s < V.nsls; s++) {
-        V.expw[s] = ew[s];
-        V.tst[s] = tt[s];
-        V.tstm1[s] = tm[s];
-      }
-      const Array &amu = ctx->need("a_j_mu"), &ak = ctx->need("a_j_kappa");
-      std::vector<double> vmu(amu.as<double>(), amu.as<double>() + amu.count());
-      std::vector<double> vk(ak.as<double>(), ak.as<double>() + ak.count());
-      V.aj_mu = V.upload(vmu);
-      V.aj_k = V.upload(vk);
+      V.cg4 = V.upload(H.cg4);
+      V.aj_mu = V.upload(H.aj_mu);
+      V.aj_k = V.upload(H.aj_k);
       V.memvar = V.zeros<float>((size_t)V.nel_s * 24 * V.nsls);
       V.rsum = V.zeros<float>((size_t)V.nel_s * 24);
       V.trm1 = V.zeros<float>((size_t)V.nel_s * 4);
       V.devm1 = V.zeros<float>((size_t)V.nel_s * 24);
     }
-
-    // ---- receivers ----
-    if (V.nrec > 0) {
-      const int32_t *rf = ctx->need("recfile_el").as<int32_t>();
-      std::vector<int> rg(V.nrec);
-      for (int r = 0; r < V.nrec; r++) {
-        int e = rf[r] - 1, i = rf[r + V.nrec], j = rf[r + 2 * V.nrec];
-        if (e < 0 || e >= V.nel_s || i < 0 || i > 4 || j < 0 || j > 4)
-          fail(AXH_ERR_ARG, "receiver index out of range");
-        rg[r] = NS.gid[slot_of(e, i + 5 * j)];
-      }
-      V.rec_g = V.upload(rg);
-    }
-    const Array &stf = ctx->need("stf");
-    V.stf.assign(stf.as<float>(), stf.as<float>() + stf.count());
+    if (V.nrec > 0) V.rec_g = V.upload(H.rec_g);
     V.blow = V.alloc<int>(1);
     HIPCHECK(hipDeviceSynchronize());
   } catch (...) {
@@ -1365,6 +1457,7 @@ void device_build(axh_ctx *ctx) {
   }
   ctx->dev.reset(D.release());
 }
+
 
 void DeviceDeleter::operator()(Device *d) const { device_destroy(d); }
 
@@ -1479,7 +1572,8 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   CA.n = 5 * V.nbdry; CA.dipole = V.src == AXH_DIPOLE; CA.nslot_s = (int)nslot_s;
   CA.nglob_s = V.nglob_s; CA.ss = V.cpl_ss; CA.gs = V.cpl_gs; CA.sf = V.cpl_sf;
   CA.gf = V.cpl_gf; CA.bm = V.cpl_bm; CA.disp = V.disp; CA.ddchi = V.ddchi;
-  CA.force_f = V.force_f; CA.force_s = V.force_s;
+  CA.force_f = V.force_f; CA.force_s = V.force_s; CA.axb = V.cpl_axb;
+  CA.axb_val = V.axb_val;
   // solid axis masks: monopole s; dipole +,z (2,3); quadrupole all
   const int mask0 = (V.src == AXH_DIPOLE) ? 1 : 0;
   const int mask1 = (V.src == AXH_MONOPOLE) ? 0 : 2;
@@ -1488,10 +1582,11 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   PS.n = V.nglob_s; PS.ncomp_mask0 = mask0; PS.ncomp_mask1 = mask1; PS.comps = comps;
   PS.dt = dt; PS.hdt2 = hdt2; PS.thr = 0.0; PS.u = V.disp; PS.v = V.velo;
   PS.a0 = V.acc0; PS.flags = V.flags_s; PS.stride = V.nglob_s; PS.blow = V.blow;
+  PS.naxb = V.naxb; PS.axb_g = V.axb_g; PS.axb_val = V.axb_val;
   PredArgs PF = PS;
   PF.n = V.nglob_f; PF.ncomp_mask0 = 0; PF.ncomp_mask1 = 0; PF.comps = 1;
   PF.u = V.chi; PF.v = V.dchi; PF.a0 = V.ddchi; PF.flags = V.flags_f;
-  PF.stride = V.nglob_f; PF.check_blow = 0; PF.apply_mask = multipole;
+  PF.stride = V.nglob_f; PF.check_blow = 0; PF.apply_mask = multipole; PF.naxb = 0;
   AsmSolidArgs AS{};
   AS.n = V.nglob_s; AS.src = V.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
   AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = V.gstart_s;

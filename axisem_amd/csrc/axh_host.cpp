@@ -359,6 +359,22 @@ int axh_last_run_ms(const axh_ctx *ctx, double *total_ms) {
   });
 }
 
+int axh_check_inputs(axh_ctx *ctx) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_inputs(ctx);
+  });
+}
+
+int axh_global_numbering(axh_ctx *ctx, int32_t domain, int32_t *gid, int32_t *nglob) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!gid || !nglob || (domain != 0 && domain != 1)) fail(AXH_ERR_ARG, "bad argument");
+    if (domain == 1 && ctx->dims.nel_fluid == 0) fail(AXH_ERR_ARG, "no fluid");
+    global_numbering(ctx, domain, gid, nglob);
+  });
+}
+
 int axh_set_profiling(axh_ctx *ctx, int on) {
   return guarded([&] {
     check_ctx(ctx);

@@ -81,6 +81,8 @@ void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
                           const float *ddchi);
 size_t device_bytes(const Device *d);
 const char *kernel_class_name(int k);
+void check_inputs(const axh_ctx *ctx);
+void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob);
 bool kernel_stats(const Device *d, int k, double *ms, int64_t *n);
 int symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
                             double *cf);

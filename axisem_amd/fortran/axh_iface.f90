@@ -127,6 +127,11 @@ module axh_iface
        real(c_float), intent(out) :: seis(*)
      end function
 
+     integer(c_int) function axh_check_inputs(ctx) bind(C, name='axh_check_inputs')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+
      integer(c_int) function axh_save_state(ctx, path) bind(C, name='axh_save_state')
        import :: c_int, c_ptr, c_char
        type(c_ptr), value :: ctx

@@ -476,6 +476,9 @@ subroutine upload_all()
 
   if (nrec_all > 0) call axh_check(axh_upload_receivers(ctx, rec), 'receivers')
 
+  ! layout checks of the library, before any device work
+  call axh_check(axh_check_inputs(ctx), 'input checks')
+
   lpath = axh_glue_getenv('AXH_SAVE_STATE'//c_null_char, path, len(path))
   if (lpath > 0) then
      call axh_check(axh_save_state(ctx, trim(path(1:lpath))//c_null_char), 'save state')

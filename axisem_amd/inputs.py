@@ -1,0 +1,134 @@
+"""Run directories for the reference's own input files.
+
+Writes ``inparam_basic``, ``inparam_advanced``, ``inparam_source`` and
+``STATIONS`` in the keyword format the SOLVER parses
+(SOLVER/parameters.F90:228-660, SOLVER/source.f90:40-140,
+SOLVER/seismograms.f90:305-377) and ``inparam_mesh`` for the MESHER
+(MESHER/input.f90:38).  The drop-in host and the reference read the same
+directory, so both see exactly the same inputs.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+from dataclasses import dataclass, field
+
+
+@dataclass
+class RunConfig:
+    source_type: str = "vertforce"      # mrr, explosion, vertforce, thetaforce, mtr, mtp, ...
+    source_depth_km: float = 0.0
+    source_amplitude: float = 1e20
+    time_scheme: str = "newmark2"       # newmark2, symplec4, ML_SO4m5, ML_SO6m7
+    attenuation: bool = False
+    seismogram_length_s: float = 1800.0
+    stations_deg: list = field(default_factory=lambda: [3.0 * k for k in range(61)])
+    source_function: str = "gauss_0"
+    n_sls: int = 5
+    coarse_grained: bool = True
+
+
+def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
+                       coarsening_layers: int = 3) -> None:
+    with open(path, "w") as f:
+        f.write("BACKGROUND_MODEL    %s\n" % model)
+        f.write("DOMINANT_PERIOD     %.1f\n" % period_s)
+        f.write("NTHETA_SLICES       1\nNRADIAL_SLICES      1\n")
+        f.write("ONLY_SUGGEST_NTHETA false\nWRITE_VTK           false\n")
+        f.write("WRITE_1DMODEL       false\n")
+        f.write("COARSENING_LAYERS   %d\n" % coarsening_layers)
+
+
+def run_mesher(xmesh: str, outdir: str, period_s: float, model: str = "prem_iso") -> str:
+    """Run the reference MESHER binary into outdir; returns the meshdb path."""
+    import subprocess
+    import resource
+
+    os.makedirs(os.path.join(outdir, "Diags"), exist_ok=True)
+    write_mesh_inparam(os.path.join(outdir, "inparam_mesh"), period_s, model)
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, OMP_STACKSIZE="512M")
+    env.setdefault("OMP_NUM_THREADS", "8")
+    with open(os.path.join(outdir, "OUTPUT_MESH"), "w") as log:
+        subprocess.run([os.path.abspath(xmesh)], cwd=outdir, stdout=log, stderr=subprocess.STDOUT,
+                       check=True, preexec_fn=unlimited_stack, env=env)
+    return os.path.join(outdir, "meshdb.dat0000")
+
+
+def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = False) -> str:
+    """Create a SOLVER run directory using mesh_dir (meshdb.dat0000)."""
+    os.makedirs(os.path.join(rundir, "Data"), exist_ok=True)
+    os.makedirs(os.path.join(rundir, "Info"), exist_ok=True)
+    mesh_link = os.path.join(rundir, "Mesh")
+    if os.path.lexists(mesh_link):
+        if os.path.islink(mesh_link):
+            os.remove(mesh_link)
+        else:
+            shutil.rmtree(mesh_link)
+    if copy_mesh:
+        os.makedirs(mesh_link)
+        for fn in ("meshdb.dat0000", "mesh_params.h"):
+            src = os.path.join(mesh_dir, fn)
+            if os.path.exists(src):
+                shutil.copy(src, mesh_link)
+    else:
+        os.symlink(os.path.abspath(mesh_dir), mesh_link)
+    with open(os.path.join(rundir, "inparam_basic"), "w") as f:
+        f.write("SIMULATION_TYPE   single\n")
+        f.write("SEISMOGRAM_LENGTH  %.1f\n" % cfg.seismogram_length_s)
+        f.write("RECFILE_TYPE      stations\nMESHNAME           Mesh\n")
+        f.write("LAT_HETEROGENEITY  false\n")
+        f.write("ATTENUATION         %s\n" % ("true" if cfg.attenuation else "false"))
+        f.write("SAVE_SNAPSHOTS         false\n")
+    with open(os.path.join(rundir, "inparam_advanced"), "w") as f:
+        f.write("SAMPLING_PERIOD     0.0\nTIME_STEP           0.0\nSOURCE_PERIOD       0.0\n")
+        f.write("SOURCE_FUNCTION     %s\n" % cfg.source_function)
+        f.write("TIME_SCHEME     %s\n" % cfg.time_scheme)
+        f.write('DATA_DIR        "./Data"\nINFO_DIR        "./Info"\n')
+        f.write("USE_NETCDF          false\nCHECKPOINTING       false\n")
+        f.write("VERBOSITY               1\nDIAGNOSTIC_FILE_OUTPUT false\n")
+        f.write("MESH_TEST           false\nKERNEL_WAVEFIELDS   false\n")
+        f.write("SAVE_ENERGY         .false.\n")
+        f.write("NR_LIN_SOLIDS       %d\n" % cfg.n_sls)
+        f.write("COARSE_GRAINED      %s\n" % ("true" if cfg.coarse_grained else "false"))
+    with open(os.path.join(rundir, "inparam_source"), "w") as f:
+        f.write("SOURCE_TYPE       %s\n" % cfg.source_type)
+        f.write("SOURCE_DEPTH      %.3f\n" % cfg.source_depth_km)
+        f.write("SOURCE_LAT         90.0\nSOURCE_LON          0.0\n")
+        f.write("SOURCE_AMPLITUDE   %.6E\n" % cfg.source_amplitude)
+    with open(os.path.join(rundir, "STATIONS"), "w") as f:
+        for k, dist in enumerate(cfg.stations_deg):
+            f.write("R%03d  AX  %9.4f  0.0000  0.0  0.0\n" % (k, 90.0 - dist))
+    return rundir
+
+
+def record_state(host_exe: str, rundir: str, state_path: str, timeout: float = 1800) -> str:
+    """Run the drop-in host in record mode (no GPU): prepare_waves + uploads,
+    written to state_path; the time loop is skipped."""
+    import subprocess
+    import resource
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, AXH_SAVE_STATE=os.path.abspath(state_path), AXH_NO_RUN="1")
+    with open(os.path.join(rundir, "OUTPUT_record"), "w") as log:
+        subprocess.run([os.path.abspath(host_exe)], cwd=rundir, stdout=log,
+                       stderr=subprocess.STDOUT, check=True, timeout=timeout,
+                       preexec_fn=unlimited_stack, env=env)
+    return state_path
+
+
+def read_seismograms(data_dir: str, nrec: int, ncomp: int):
+    """Receiver files R###_AX_disp.dat written by the reference or the drop-in
+    (first line = the t=0 sample).  Returns float32 array (nsamp, nrec, ncomp)."""
+    import numpy as np
+
+    cols = []
+    for k in range(nrec):
+        a = np.loadtxt(os.path.join(data_dir, "R%03d_AX_disp.dat" % k), dtype=np.float64, ndmin=2)
+        cols.append(a[:, :ncomp].astype(np.float32))
+    return np.stack(cols, axis=1)

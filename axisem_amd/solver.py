@@ -89,6 +89,8 @@ def library():
         L.axh_kernel_name.restype = C.c_char_p
         L.axh_kernel_name.argtypes = [C.c_int]
         L.axh_set_profiling.argtypes = [P, C.c_int]
+        L.axh_check_inputs.argtypes = [P]
+        L.axh_global_numbering.argtypes = [P, C.c_int32, P, C.POINTER(C.c_int32)]
         L.axh_symplectic_coefficients.argtypes = [C.c_int32, C.c_double, P, P, P,
                                                   C.POINTER(C.c_int32)]
         _lib = L

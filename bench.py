@@ -1,0 +1,222 @@
+"""Benchmark of the GPU time loop on its headline workload.
+
+Workload (one MI355X, whole mesh resident): prem_iso, DOMINANT_PERIOD 10 s,
+NPOL 4, anelastic with 5 standard linear solids (coarse-grained memory
+variables), symplectic 4th-order PEFRL, dipole source (horizontal force) at
+50 km, Gaussian STF, FP32, 3600 s seismograms at 181 receivers at 1 degree
+spacing.  The mesh is the reference MESHER's; the precomputed terms are those
+of the reference's own setup, recorded by the drop-in host.
+
+A "step" is one full time step (4 PEFRL stages) of the whole mesh; the metric
+is element-local GLL-point updates per second, 25 (nel_solid + nel_fluid) per
+step.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import resource
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from axisem_amd import inputs  # noqa: E402
+from axisem_amd.state import read_dims, read_state  # noqa: E402
+
+HOST = os.path.join(REPO, "axisem_amd", "fortran", "_build", "axisem_gpu")
+MESH = os.path.join(REPO, "data", "meshes", "prem_iso_10s")
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = inputs.RunConfig(
+    source_type="thetaforce", source_depth_km=50.0, source_amplitude=1e20,
+    time_scheme="symplec4", attenuation=True, seismogram_length_s=3600.0,
+    stations_deg=[float(k) for k in range(181)])
+WORKLOAD_NAME = ("prem_iso 10 s, NPOL 4, anelastic 5 SLS (cg4), symplec4, dipole "
+                 "thetaforce 50 km, 3600 s, 181 receivers")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def _stack():
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def prepare_state(workdir: str) -> str:
+    """Record (once) the uploads of the drop-in host for the workload."""
+    if not os.path.exists(os.path.join(MESH, "meshdb.dat0000")):
+        raise SystemExit("missing %s: run __graft_entry__.build() where the reference "
+                         "tree exists" % MESH)
+    if not os.path.exists(HOST):
+        raise SystemExit("missing drop-in host %s" % HOST)
+    rundir = os.path.join(workdir, "run_gpu")
+    state = os.path.join(workdir, "state.bin")
+    stamp = os.path.join(workdir, "stamp.json")
+    key = json.dumps({"cfg": repr(WORKLOAD), "mesh": os.path.getsize(os.path.join(MESH, "meshdb.dat0000")),
+                      "host": os.path.getmtime(HOST)})
+    if os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key:
+        return state
+    inputs.write_run_dir(rundir, MESH, WORKLOAD)
+    t = time.time()
+    inputs.record_state(HOST, rundir, state)
+    log("recorded the workload state in %.1fs" % (time.time() - t))
+    with open(stamp, "w") as f:
+        f.write(key)
+    return state
+
+
+def solid_kernel_bytes(st) -> int:
+    """Algorithmic HBM bytes of one launch of the solid stiffness kernel in a
+    symplectic stage: every precomputed term of every element-local point
+    once, the displacement of every global point once, the element-local
+    force written once, the element->global index, the SLS sums and the cg4
+    terms it reads, and the axial terms."""
+    d = st.dims
+    nel, nglob = d["nel_solid"], d["nglob_solid"]
+    ncomp = 2 if d["src_type"] == 0 else 3
+    nterms = {0: 15, 1: 24, 2: 26}[d["src_type"]]
+    naxel = int(st["axis_solid"].sum())
+    b = 4 * 25 * nel * nterms          # stiffness terms
+    b += 4 * 25 * nel                  # element -> global index
+    b += 4 * nel                       # axial flag / index
+    b += 4 * ncomp * nglob             # displacement gather
+    b += 4 * ncomp * 25 * nel          # force
+    b += 4 * 50 * naxel                # M0_w terms of axial elements
+    if d["anel"]:
+        b += nel * (24 * 4 + 5 * 4 * 4)  # SLS sums + Y, V_{s,z}_{eta,xi}
+    return b
+
+
+def cpu_baseline(seconds: float, dt: float):
+    """The reference program (oracle/_ref/axisem, serial, realkind 4) on a
+    bounded sample of the same workload: the same mesh and inputs with the
+    seismogram shortened to a few steps, timed by the reference's own
+    'Time loop routine' clock."""
+    exe = os.path.join(REPO, "oracle", "_ref", "axisem")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        # one step costs ~0.5 s on one core; aim at `seconds` of loop time
+        steps = max(5, int(seconds / 0.5))
+        cfg = inputs.RunConfig(**{**WORKLOAD.__dict__})
+        cfg.seismogram_length_s = steps * dt
+        inputs.write_run_dir(tmp, MESH, cfg)
+        r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, preexec_fn=_stack,
+                           timeout=1800)
+        if r.returncode != 0:
+            log("reference run failed: " + r.stdout[-500:])
+            return None
+        m = re.search(r"Time loop routine\s+\d+\s+([\d.]+)\s+([\d.]+)", r.stdout)
+        n = re.search(r"# iterations\s*:\s*(\d+)", r.stdout)
+        ne = re.search(r"nelem\s*=\s*(\d+)", open(os.path.join(MESH, "mesh_params.h")).read())
+        if not (m and n and ne):
+            return None
+        t_loop, niter, nelem = float(m.group(2)), int(n.group(1)), int(ne.group(1))
+        return {"value": 25.0 * nelem * niter / t_loop, "unit": "GLL-point updates/s",
+                "cores": 1, "kind": "reference",
+                "sample": "reference SOLVER (amdflang -O2, serial, FP32) on the same mesh and "
+                          "inputs, %d time steps, 'Time loop routine' clock %.1f s" % (niter, t_loop)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workdir", default=os.path.join(REPO, "data", "bench_10s"))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the time loop keeps the whole mesh resident on "
+              "one MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from axisem_amd.solver import Solver
+
+    os.makedirs(a.workdir, exist_ok=True)
+    state = prepare_state(a.workdir)
+    st = read_state(state)
+    d = st.dims
+    gll = 25 * (d["nel_solid"] + d["nel_fluid"])
+    if a.warmup + a.steps + a.profile_steps > d["niter"]:
+        raise SystemExit("warmup + steps exceed the %d steps of the workload" % d["niter"])
+    solver = Solver.from_state_file(state)
+    log("warmup %d steps" % a.warmup)
+    solver.run(a.warmup)
+    log("timed %d steps" % a.steps)
+    t0 = time.perf_counter()
+    solver.run(a.steps)           # returns after the library stream is synchronized
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    dev_ms = solver.last_run_ms()
+    value = gll * a.steps / wall
+
+    # per-kernel durations (HIP events around every launch, same stream)
+    solver.set_profiling(True)
+    solver.run(a.profile_steps)
+    stats = solver.kernel_stats()
+    solver.set_profiling(False)
+    sk_ms, sk_n = stats["solid_stiffness"]
+    sk_avg_s = sk_ms / sk_n * 1e-3
+    sk_bytes = solid_kernel_bytes(st)
+    achieved = sk_bytes / sk_avg_s / 1e9
+    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
+               for k, v in stats.items() if v[1]}
+    log("kernels: " + json.dumps(kernels))
+    solver.close()
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline (reference program, ~%.0f s loop)" % a.cpu_seconds)
+        cpu = cpu_baseline(a.cpu_seconds, d["deltat"])
+
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "solid_stiffness_prem10s_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except ValueError:
+            traffic = None
+
+    out = {
+        "metric": "GLL-point updates/s per GPU",
+        "value": value,
+        "unit": "GLL-point updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "device_ms_per_step": dev_ms / a.steps,
+        "time_steps_per_s": a.steps / wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "reference MESHER mesh and reference setup terms (prem_iso 10 s), deterministic",
+        "config": {"workload": WORKLOAD_NAME, "nel_solid": d["nel_solid"],
+                   "nel_fluid": d["nel_fluid"], "gll_points_per_step": gll,
+                   "niter_full_run": d["niter"], "deltat": d["deltat"],
+                   "receivers": len(WORKLOAD.stations_deg)},
+        "roofline": {"bound": "hbm", "kernel": "solid_stiffness (k_solid, symplectic stage)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": sk_bytes,
+                     "avg_launch_us": sk_avg_s * 1e6},
+        "kernels_us": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -143,8 +143,9 @@ int axh_upload_boundary(axh_ctx *ctx, const int32_t *bdry_solid_el,
 int axh_upload_fluid_surface_mask(axh_ctx *ctx, const float *mask);
 
 /* Source (SOLVER/data_source.f90:47-49): ielsrc(nelsrc) 1-based, the element
- * source term source_term_el(0:4,0:4,nelsrc,3) (or source_term_fl
- * (0:4,0:4,nelsrc) when fluid_src), and the source time function table:
+ * source term source_term_el(0:4,0:4,8,3) as compute_src allocates it
+ * (SOLVER/source.f90:376; source_term_fl(0:4,0:4,8) when fluid_src), and the
+ * source time function table:
  * newmark2: stf(niter) as realkind (SOLVER/data_source.f90:37);
  * symplectic: real(stf_symp(i), realkind) for every step and stage,
  * stf[(iter-1)*nstages + (i-1)] (SOLVER/time_evol_wave.F90:590-593,689). */
@@ -173,6 +174,19 @@ int axh_upload_attenuation_cg4(axh_ctx *ctx, const axh_attenuation_cg4 *att);
 /* Receivers: recfile_el(num_rec,3) = (element, ipol, jpol) in the solid,
  * element 1-based, ipol/jpol 0-based (SOLVER/data_mesh.f90:135). */
 int axh_upload_receivers(axh_ctx *ctx, const int32_t *recfile_el);
+
+/* Build the library's layout from the uploads without touching the GPU and
+ * report inputs the GPU loop cannot represent (AXH_ERR_UNSUPPORTED: e.g.
+ * copies of one GLL point with different inverse mass).  axh_run does the
+ * same checks; this entry point lets a host fail before any device work. */
+int axh_check_inputs(axh_ctx *ctx);
+
+/* The library's numbering of the GLL points of a domain (0 solid, 1 fluid):
+ * gid(25 nel) in the reference's element-local order, 0-based, and the number
+ * of distinct points.  It is a relabelling of igloc_solid / igloc_fluid
+ * (SOLVER/data_mesh.f90:42-43): gid(a) == gid(b) iff igloc(a) == igloc(b). */
+int axh_global_numbering(axh_ctx *ctx, int32_t domain, int32_t *gid,
+                         int32_t *nglob);
 
 /* ---- running ------------------------------------------------------------- */
 

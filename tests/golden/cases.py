@@ -1,0 +1,50 @@
+"""Golden cases: small runs of the reference SOLVER on the 200 s prem_iso mesh.
+
+Each case fixes the inputs (run directory) of one reference run; the recorded
+outputs live in tests/golden/<name>/ref_seis.npz (written by make_golden.py).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from axisem_amd.inputs import RunConfig  # noqa: E402
+
+MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s")
+MESH_PERIOD = 200.0
+STATIONS = [10.0 * k for k in range(19)]
+SAMPLE_STRIDE = 5          # every 5th seismogram sample is stored
+
+CASES = {
+    # monopole, surface vertical force, Newmark, elastic
+    "mono_vertforce_newmark": RunConfig("vertforce", 0.0, 1e20, "newmark2", False, 900.0, STATIONS),
+    # monopole explosion at depth, symplectic PEFRL, anelastic (cg4 memory variables)
+    "mono_explosion_symplec4_anel": RunConfig("explosion", 2000.0, 1e20, "symplec4", True, 600.0,
+                                              STATIONS),
+    # monopole Mtt+Mpp at 300 km, Newmark, anelastic
+    "mono_mttpmpp_newmark_anel": RunConfig("mtt_p_mpp", 300.0, 1e20, "newmark2", True, 900.0,
+                                           STATIONS),
+    # dipole horizontal force at 50 km, Newmark, anelastic
+    "dipole_thetaforce_newmark_anel": RunConfig("thetaforce", 50.0, 1e20, "newmark2", True, 900.0,
+                                                STATIONS),
+    # dipole phi-force at 1000 km, symplectic, anelastic
+    "dipole_phiforce_symplec4_anel": RunConfig("phiforce", 1000.0, 1e20, "symplec4", True, 600.0,
+                                               STATIONS),
+    # dipole Mtr just above the core-mantle boundary, symplectic, elastic
+    "dipole_mtr_symplec4": RunConfig("mtr", 2800.0, 1e20, "symplec4", False, 600.0, STATIONS),
+    # quadrupole Mtp, Newmark, elastic
+    "quad_mtp_newmark": RunConfig("mtp", 100.0, 1e20, "newmark2", False, 900.0, STATIONS),
+    # quadrupole Mtt-Mpp at depth, symplectic, anelastic
+    "quad_mttmmpp_symplec4_anel": RunConfig("mtt_m_mpp", 600.0, 1e20, "symplec4", True, 600.0,
+                                            STATIONS),
+}
+
+NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
+         "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}
+
+
+def case_dir(name: str) -> str:
+    return os.path.join(HERE, name)

@@ -1,0 +1,82 @@
+"""Write the golden fixtures: reference seismograms of the cases in cases.py.
+
+Runs in the build container only (needs /root/reference for the reference
+programs built by oracle/build_ref.py):
+
+1. the reference MESHER -> tests/golden/mesh_prem_iso_200s/ (meshdb.dat0000);
+2. for every case, the unmodified reference SOLVER (oracle/_ref/axisem) on the
+   case's inputs -> receiver seismograms, stored as float32 (the reference
+   prints them with round-trip precision) every SAMPLE_STRIDE samples in
+   tests/golden/<case>/ref_seis.npz, with niter, deltat and the seismogram
+   sampling of the run.
+
+Usage: python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import resource
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, HERE)
+
+from axisem_amd import inputs  # noqa: E402
+import cases  # noqa: E402
+
+XMESH = os.path.join(REPO, "oracle", "_ref", "xmesh")
+AXISEM = os.path.join(REPO, "oracle", "_ref", "axisem")
+
+
+def _unlimited_stack():
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def make_mesh():
+    os.makedirs(cases.MESH_DIR, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        inputs.run_mesher(XMESH, tmp, cases.MESH_PERIOD)
+        for fn in ("meshdb.dat0000", "mesh_params.h"):
+            shutil.copy(os.path.join(tmp, fn), cases.MESH_DIR)
+
+
+def run_reference(name: str):
+    cfg = cases.CASES[name]
+    out = cases.case_dir(name)
+    os.makedirs(out, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        inputs.write_run_dir(tmp, cases.MESH_DIR, cfg)
+        with open(os.path.join(tmp, "OUTPUT"), "w") as log:
+            subprocess.run([AXISEM], cwd=tmp, stdout=log, stderr=subprocess.STDOUT, check=True,
+                           preexec_fn=_unlimited_stack, timeout=3600)
+        ncomp = cases.NCOMP[cfg.source_type]
+        seis = inputs.read_seismograms(os.path.join(tmp, "Data"), len(cfg.stations_deg), ncomp)
+        text = open(os.path.join(tmp, "OUTPUT"), errors="replace").read()
+    # the first row is the t = 0 sample written before the loop
+    nsamp = seis.shape[0] - 1
+    np.savez_compressed(os.path.join(out, "ref_seis.npz"),
+                        seis=seis[1::cases.SAMPLE_STRIDE],
+                        nsamples=np.int64(nsamp), stride=np.int64(cases.SAMPLE_STRIDE))
+    for line in text.splitlines():
+        if "Time loop routine" in line:
+            print(name, line.strip())
+    print(name, "samples", nsamp, "receivers", seis.shape[1], "components", ncomp)
+
+
+def main(argv):
+    if not os.path.exists(os.path.join(cases.MESH_DIR, "meshdb.dat0000")):
+        make_mesh()
+    names = argv or list(cases.CASES)
+    for n in names:
+        run_reference(n)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,123 @@
+"""The C-ABI library without a GPU: symbols, argument checking, state files.
+
+Nothing here launches device work (the library touches the GPU only from the
+first axh_run / axh_download_fields / axh_device_bytes call).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from axisem_amd import solver as S
+from axisem_amd.state import read_state, write_state
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    text = open(os.path.join(REPO, "include", "axihip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(axh_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = S.library()
+    names = _declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_bad_dims_are_rejected():
+    lib = S.library()
+    d = S.Dims()
+    d.abi_version, d.npol, d.nel_solid, d.nglob_solid = 1, 5, 10, 100
+    d.seis_it, d.deltat = 1, 0.1
+    h = C.c_void_p()
+    rc = lib.axh_create(C.byref(d), C.byref(h))
+    assert rc == -5 and b"npol" in lib.axh_last_error()
+    d.npol = 4
+    d.abi_version = 99
+    assert lib.axh_create(C.byref(d), C.byref(h)) == -1
+    d.abi_version = 1
+    d.deltat = -1.0
+    assert lib.axh_create(C.byref(d), C.byref(h)) == -1
+    d.deltat, d.nelsrc = 0.1, 9
+    assert lib.axh_create(C.byref(d), C.byref(h)) == -1
+    assert lib.axh_run(None, 1, 1, None) == -1
+
+
+def test_state_file_roundtrip(state_cache, tmp_path):
+    """Uploads through the C-ABI, written by axh_save_state, equal the file
+    the Fortran host wrote; the python reader/writer round-trips it too."""
+    _, path = state_cache("dipole_thetaforce_newmark_anel")
+    st = read_state(path)
+    s = S.Solver.from_state(st)
+    out = str(tmp_path / "again.bin")
+    S._check(S.library().axh_save_state(s._h, out.encode()))
+    st2 = read_state(out)
+    assert st.dims == st2.dims
+    assert set(st.arrays) == set(st2.arrays)
+    for k in st.arrays:
+        np.testing.assert_array_equal(st[k], st2[k], err_msg=k)
+        assert st.shape(k) == st2.shape(k)
+    out2 = str(tmp_path / "py.bin")
+    write_state(st, out2)
+    st3 = read_state(out2)
+    for k in st.arrays:
+        np.testing.assert_array_equal(st[k], st3[k])
+    s.close()
+
+
+def test_upload_size_checks(state_cache):
+    _, path = state_cache("mono_vertforce_newmark")
+    st = read_state(path)
+    lib = S.library()
+    d = S.Dims()
+    for k in ("abi_version", "npol", "nel_solid", "nel_fluid", "nglob_solid", "nglob_fluid",
+              "nel_bdry", "src_type", "scheme", "niter", "seis_it", "num_rec", "nelsrc"):
+        setattr(d, k, st.dims[k])
+    d.deltat, d.magnitude = st.dims["deltat"], st.dims["magnitude"]
+    h = C.c_void_p()
+    assert lib.axh_create(C.byref(d), C.byref(h)) == 0
+    stf = np.zeros(5, np.float32)
+    src = np.zeros(8 * 75, np.float32)
+    iel = np.ones(8, np.int32)
+    rc = lib.axh_upload_source(h, iel.ctypes.data_as(C.c_void_p), src.ctypes.data_as(C.c_void_p),
+                               stf.ctypes.data_as(C.c_void_p), 5)
+    assert rc == -1 and b"stf table" in lib.axh_last_error()
+    assert lib.axh_upload_solid_term(h, 33, src.ctypes.data_as(C.c_void_p)) == -1
+    assert lib.axh_destroy(h) == 0
+
+
+def test_symbols_have_reference_citations():
+    """Every declaration in include/axihip.h documents what it replaces."""
+    text = open(os.path.join(REPO, "include", "axihip.h")).read()
+    assert text.count("SOLVER/") >= 15
+
+
+def test_inputs_pass_layout_checks_and_numbering_is_igloc(state_cache):
+    """The library's gather maps are a relabelling of the reference's igloc:
+    two element-local points share a library index iff they share igloc."""
+    import cases
+    for name in sorted(cases.CASES):
+        _, path = state_cache(name)
+        st = read_state(path)
+        s = S.Solver.from_state(st)
+        S._check(S.library().axh_check_inputs(s._h))
+        for dom, key, nk in ((0, "igloc_solid", "nglob_solid"), (1, "igloc_fluid", "nglob_fluid")):
+            ig = st[key]
+            gid = np.zeros(ig.size, np.int32)
+            n = C.c_int32()
+            S._check(S.library().axh_global_numbering(s._h, dom, gid.ctypes.data_as(C.c_void_p),
+                                                      C.byref(n)))
+            assert n.value == len(np.unique(ig)) == st.dims[nk]
+            # bijection between the two labellings
+            pairs = np.unique(np.stack([ig, gid]), axis=1)
+            assert pairs.shape[1] == n.value
+            # numbering by first appearance in element order
+            first = np.unique(gid, return_index=True)[1]
+            assert np.all(np.diff(first) > 0)
+        s.close()
