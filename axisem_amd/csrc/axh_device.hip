@@ -169,48 +169,53 @@ struct FluidArgs {
 // ---------------------------------------------------------------------------
 // 5x5 products on LDS tiles (mxm_4 / vxm_4, SOLVER/unrolled_loops.f90:164-206)
 // ---------------------------------------------------------------------------
-// (A M)(i,j) = sum_k A(i,k) M(k,j)   with A(i,:) held in registers
-__device__ __forceinline__ float mcol(const float *Arow, const float *M, int j) {
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 5; k++) s = s + Arow[k] * M[k + 5 * j];
-  return s;
-}
-// (M B)(i,j) = sum_k M(i,k) B(k,j)   with B(:,j) held in registers
-__device__ __forceinline__ float mrow(const float *M, const float *Bcol, int i) {
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 5; k++) s = s + M[i + 5 * k] * Bcol[k];
-  return s;
-}
-
-// sum_k v(k) B(k,j) for a 5-vector v in LDS (vxm_4 with a vector operand)
-__device__ __forceinline__ float vdot(const float *v, const float *Bcol) {
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 5; k++) s = s + v[k] * Bcol[k];
-  return s;
-}
-
+// Spectral operators in LDS (105 floats, copied once per workgroup); lanes
+// index them with their own (i, j), which keeps the register footprint small.
 struct GRegs {
-  float gT[5];    // (axial ? G1T : G2T)(i, :)
-  float g[5];     // (axial ? G1 : G2)(i, :)
-  float g2c[5];   // G2(:, j)
-  float g2Tc[5];  // G2T(:, j)
-  float g0[5];    // G0(:)
+  const float *gT;   // (axial ? G1T : G2T), column major
+  const float *g;    // (axial ? G1 : G2)
+  const float *g2;   // G2
+  const float *g2T;  // G2T
+  const float *g0;   // G0
+  int i, j;
+  float g0i;         // G0(i)
 };
 
-__device__ __forceinline__ void load_g(const float *G, bool axial, int i, int j,
-                                       GRegs &r) {
-  const float *G0 = G, *G1 = G + 5, *G1T = G + 30, *G2 = G + 55, *G2T = G + 80;
+__device__ __forceinline__ void load_g(const float *sG, bool axial, int i, int j, GRegs &r) {
+  r.g0 = sG;
+  r.gT = axial ? sG + 30 : sG + 80;
+  r.g = axial ? sG + 5 : sG + 55;
+  r.g2 = sG + 55;
+  r.g2T = sG + 80;
+  r.i = i;
+  r.j = j;
+  r.g0i = sG[i];
+}
+
+__device__ __forceinline__ void stage_g(const float *G, float *sG) {
+  for (int t = threadIdx.x; t < 105; t += blockDim.x) sG[t] = G[t];
+}
+
+// (A M)(i,j) = sum_k A(i,k) M(k,j)  for a 5x5 operator A in LDS
+__device__ __forceinline__ float amul(const float *A, const float *M, int i, int j) {
+  float s = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 5; k++) {
-    r.gT[k] = axial ? G1T[i + 5 * k] : G2T[i + 5 * k];
-    r.g[k] = axial ? G1[i + 5 * k] : G2[i + 5 * k];
-    r.g2c[k] = G2[k + 5 * j];
-    r.g2Tc[k] = G2T[k + 5 * j];
-    r.g0[k] = G0[k];
-  }
+  for (int k = 0; k < 5; k++) s = s + A[i + 5 * k] * M[k + 5 * j];
+  return s;
+}
+// (M B)(i,j) = sum_k M(i,k) B(k,j)  for a 5x5 operator B in LDS
+__device__ __forceinline__ float mulb(const float *M, const float *B, int i, int j) {
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) s = s + M[i + 5 * k] * B[k + 5 * j];
+  return s;
+}
+// sum_k v(k) B(k,j)
+__device__ __forceinline__ float vmul(const float *v, const float *B, int j) {
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) s = s + v[k] * B[k + 5 * j];
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -349,6 +354,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   __shared__ float sSa[ANEL ? kElemPerTile * 24 : 1];
   __shared__ float sSrc[ANEL ? kElemPerTile * 28 : 1];
   __shared__ float sOld[ANEL ? kElemPerTile * 28 : 1];
+  __shared__ float sG[105];
 
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
@@ -359,6 +365,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
 
+  stage_g(A.G, sG);
   // gather the (predicted, masked) displacement of the element
   if (valid) {
     const int g = A.gid[slot];
@@ -382,15 +389,15 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   __syncthreads();
 
   GRegs G;
-  load_g(A.G, axial, i, j, G);
+  load_g(sG, axial, i, j, G);
   const float *coef = A.coef + (size_t)tile * NC * 256 + t;
   float acc[3];
 
   if (SRC == AXH_MONOPOLE) {
     const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25];
     const float us = U0[k], uz = U1[k];
-    const float X1 = mcol(G.gT, U0, j), X2 = mcol(G.gT, U1, j);
-    const float X3 = mrow(U0, G.g2c, i), X4 = mrow(U1, G.g2c, i);
+    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j);
+    const float X3 = mulb(U0, G.g2, i, j), X4 = mulb(U1, G.g2, i, j);
     const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
                 m4 = coef[3 * 256], mw1 = coef[4 * 256], m11s = coef[5 * 256],
                 m21s = coef[6 * 256], m41s = coef[7 * 256], m12s = coef[8 * 256],
@@ -404,8 +411,8 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
     __syncthreads();
     const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
                 *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25];
-    const float Y2 = mrow(S2s, G.g2Tc, i), Y4 = mrow(S2z, G.g2Tc, i);
-    const float Y1 = mcol(G.g, S1s, j), Y3 = mcol(G.g, S1z, j);
+    const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2z, G.g2T, i, j);
+    const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1z, i, j);
     ls = ls + Y1 + Y2;
     float lz = Y3 + Y4;
     // axial terms with M0_w1..3
@@ -413,19 +420,19 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
     if (axial) {
       const float *m0 = A.m0 + (size_t)ax * 50;
       w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
-      V1 = mcol(G.g0, U0, j);        // vxm(G0, us)
-      V2 = mrow(U1, G.g2c, 0);       // vxm(uz(0,:), G2)
-      V3 = mcol(G.g0, U1, j);        // vxm(G0, uz)
+      V1 = vmul(G.g0, U0, j);        // vxm(G0, us)
+      V2 = mulb(U1, G.g2, 0, j);       // vxm(uz(0,:), G2)
+      V3 = vmul(G.g0, U1, j);        // vxm(G0, uz)
       if (i == 0) sV[le][j] = w3 * V1;
     }
     __syncthreads();
     if (axial) {
       float V4 = w1 * V1 + w3 * V2;
       V4 = V4 + w2 * V3;
-      float X2a = G.g0[i] * (w2 * V1);
-      const float V1b = vdot(&sV[le][0], G.g2Tc);  // vxm(w3*V1, G2T)
+      float X2a = G.g0i * (w2 * V1);
+      const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(w3*V1, G2T)
       if (i == 0) X2a = X2a + V1b;
-      ls = ls + G.g0[i] * V4;
+      ls = ls + G.g0i * V4;
       lz = X2a + lz;
     }
     acc[0] = ls;
@@ -434,10 +441,10 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   } else if (SRC == AXH_DIPOLE) {
     const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
     const float u1 = U0[k], u2 = U1[k], u3 = U2[k];
-    const float X4 = mrow(U0, G.g2c, i), X5 = mrow(U1, G.g2c, i),
-                X6 = mrow(U2, G.g2c, i);
-    const float X1 = mcol(G.gT, U0, j), X2 = mcol(G.gT, U1, j),
-                X3 = mcol(G.gT, U2, j);
+    const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
+                X6 = mulb(U2, G.g2, i, j);
+    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
+                X3 = amul(G.gT, U2, i, j);
     const float X7 = X1 + X2, X8 = X4 + X5;
     const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
                 m4 = coef[3 * 256], m5 = coef[4 * 256], m6 = coef[5 * 256],
@@ -464,9 +471,9 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
     const float *S1p = &sS[0][le * 25], *S1m = &sS[1][le * 25],
                 *S2p = &sS[2][le * 25], *S2m = &sS[3][le * 25],
                 *S1z = &sS[4][le * 25], *S2z = &sS[5][le * 25];
-    const float Y1 = mcol(G.g, S1p, j), Y3 = mcol(G.g, S1m, j), Y5 = mcol(G.g, S1z, j);
-    const float Y2 = mrow(S2p, G.g2Tc, i), Y4 = mrow(S2m, G.g2Tc, i),
-                Y6 = mrow(S2z, G.g2Tc, i);
+    const float Y1 = amul(G.g, S1p, i, j), Y3 = amul(G.g, S1m, i, j), Y5 = amul(G.g, S1z, i, j);
+    const float Y2 = mulb(S2p, G.g2T, i, j), Y4 = mulb(S2m, G.g2T, i, j),
+                Y6 = mulb(S2z, G.g2T, i, j);
     float l1 = Y1 + Y2;
     float l2 = Y3 + Y4 + ls2;
     float l3 = Y5 + Y6 + ls3;
@@ -476,10 +483,10 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
       const float w1 = m0[0 * 5 + j], w3 = m0[2 * 5 + j], w7 = m0[6 * 5 + j],
                   w9 = m0[8 * 5 + j], w10 = m0[9 * 5 + j];
       w2 = m0[1 * 5 + j]; w4 = m0[3 * 5 + j]; w6 = m0[5 * 5 + j]; w8 = m0[7 * 5 + j];
-      const float V1 = mcol(G.g0, U0, j);
-      V2 = mcol(G.g0, U1, j);
-      V3 = mcol(G.g0, U2, j);
-      const float V4 = mrow(U0, G.g2c, 0);   // vxm(u1(0,:), G2)
+      const float V1 = vmul(G.g0, U0, j);
+      V2 = vmul(G.g0, U1, j);
+      V3 = vmul(G.g0, U2, j);
+      const float V4 = mulb(U0, G.g2, 0, j);   // vxm(u1(0,:), G2)
       a = w1 * V2 + w3 * V3;
       b = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
       c = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
@@ -487,9 +494,9 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
     }
     __syncthreads();
     if (axial) {
-      float S1pa = G.g0[i] * a;
-      const float S1ma = G.g0[i] * b, S1za = G.g0[i] * c;
-      const float V1b = vdot(&sV[le][0], G.g2Tc);  // vxm(V4, G2T)
+      float S1pa = G.g0i * a;
+      const float S1ma = G.g0i * b, S1za = G.g0i * c;
+      const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(V4, G2T)
       if (i == 0) S1pa = S1pa + V1b;
       l1 = l1 + S1pa;
       l2 = l2 + S1ma;
@@ -501,10 +508,10 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   } else {
     const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
     const float us = U0[k], up = U1[k], uz = U2[k];
-    const float X1 = mcol(G.gT, U0, j), X2 = mcol(G.gT, U1, j),
-                X3 = mcol(G.gT, U2, j);
-    const float X4 = mrow(U0, G.g2c, i), X5 = mrow(U1, G.g2c, i),
-                X6 = mrow(U2, G.g2c, i);
+    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
+                X3 = amul(G.gT, U2, i, j);
+    const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
+                X6 = mulb(U2, G.g2, i, j);
     const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
                 m4 = coef[3 * 256], m5 = coef[4 * 256], m6 = coef[5 * 256],
                 m7 = coef[6 * 256], m8 = coef[7 * 256], mw1 = coef[8 * 256],
@@ -532,9 +539,9 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
     const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
                 *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25],
                 *S1p = &sS[4][le * 25], *S2p = &sS[5][le * 25];
-    const float Y2 = mrow(S2s, G.g2Tc, i), Y4 = mrow(S2p, G.g2Tc, i),
-                Y6 = mrow(S2z, G.g2Tc, i);
-    const float Y1 = mcol(G.g, S1s, j), Y3 = mcol(G.g, S1p, j), Y5 = mcol(G.g, S1z, j);
+    const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2p, G.g2T, i, j),
+                Y6 = mulb(S2z, G.g2T, i, j);
+    const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1p, i, j), Y5 = amul(G.g, S1z, i, j);
     ls = ls + Y1 + Y2;
     lp = lp + Y3 + Y4;
     lz = lz + Y5 + Y6;
@@ -543,14 +550,14 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
       const float *m0 = A.m0 + (size_t)ax * 50;
       const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
                   w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
-      const float V1 = mcol(G.g0, U0, j), V2 = mcol(G.g0, U1, j),
-                  V3 = mcol(G.g0, U2, j);
+      const float V1 = vmul(G.g0, U0, j), V2 = vmul(G.g0, U1, j),
+                  V3 = vmul(G.g0, U2, j);
       const float a = w1 * V1 + w2 * V2 + w3 * V3;
       const float b = w2 * V1 + w4 * V2 + w5 * V3;
       const float c = w3 * V1 + w5 * V2 + w6 * V3;
-      ls = ls + G.g0[i] * a;
-      lp = lp + G.g0[i] * b;
-      lz = lz + G.g0[i] * c;
+      ls = ls + G.g0i * a;
+      lp = lp + G.g0i * b;
+      lz = lz + G.g0i * c;
     }
     acc[0] = ls;
     acc[1] = lp;
@@ -605,13 +612,13 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
       // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
       // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
       auto spb = [&](int s1) -> float {
-        if (j == 1) return G.g[1] * Sa[s1 * 4 + 0] + G.g[3] * Sa[s1 * 4 + 2];
-        if (j == 3) return G.g[1] * Sa[s1 * 4 + 1] + G.g[3] * Sa[s1 * 4 + 3];
+        if (j == 1) return G.g[i + 5] * Sa[s1 * 4 + 0] + G.g[i + 15] * Sa[s1 * 4 + 2];
+        if (j == 3) return G.g[i + 5] * Sa[s1 * 4 + 1] + G.g[i + 15] * Sa[s1 * 4 + 3];
         return 0.0f;
       };
       auto spa = [&](int s2) -> float {
-        if (i == 1) return Sa[s2 * 4 + 0] * G.g2Tc[1] + Sa[s2 * 4 + 1] * G.g2Tc[3];
-        if (i == 3) return Sa[s2 * 4 + 2] * G.g2Tc[1] + Sa[s2 * 4 + 3] * G.g2Tc[3];
+        if (i == 1) return Sa[s2 * 4 + 0] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 1] * G.g2T[3 + 5 * j];
+        if (i == 3) return Sa[s2 * 4 + 2] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 3] * G.g2T[3 + 5 * j];
         return 0.0f;
       };
       const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
@@ -744,6 +751,7 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
 __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
   __shared__ float sC[256];
   __shared__ float sS[2][256];
+  __shared__ float sG[105];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int i = k % 5, j = k / 5;
@@ -752,28 +760,29 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
+  stage_g(A.G, sG);
   sC[t] = valid ? A.chi[A.gid[slot]] : 0.0f;
   __syncthreads();
   GRegs G;
-  load_g(A.G, axial, i, j, G);
+  load_g(sG, axial, i, j, G);
   const int nc = A.multipole ? kNcFluidMulti : kNcFluidMono;
   const float *coef = A.coef + (size_t)tile * nc * 256 + t;
   const float *C = &sC[le * 25];
   const float c0 = C[k];
-  const float X1 = mcol(G.gT, C, j), X2 = mrow(C, G.g2c, i);
+  const float X1 = amul(G.gT, C, i, j), X2 = mulb(C, G.g2, i, j);
   const float m1 = coef[0], m2 = coef[256], m4 = coef[512];
   sS[0][t] = m1 * X2 + m2 * X1;
   sS[1][t] = m1 * X1 + m4 * X2;
   __syncthreads();
-  const float Y1 = mcol(G.g, &sS[0][le * 25], j);
-  const float Y2 = mrow(&sS[1][le * 25], G.g2Tc, i);
+  const float Y1 = amul(G.g, &sS[0][le * 25], i, j);
+  const float Y2 = mulb(&sS[1][le * 25], G.g2T, i, j);
   float loc = Y1 + Y2;
   if (A.multipole) {
     loc = loc + coef[768] * c0;
     if (axial) {
       const float w = A.m0[(size_t)ax * 5 + j];
-      const float V1 = mcol(G.g0, C, j);
-      loc = loc + G.g0[i] * (w * V1);
+      const float V1 = vmul(G.g0, C, j);
+      loc = loc + G.g0i * (w * V1);
     }
   }
   if (valid) {

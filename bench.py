@@ -104,8 +104,9 @@ def cpu_baseline(seconds: float, dt: float):
     if not os.path.exists(exe):
         return None
     with tempfile.TemporaryDirectory() as tmp:
-        # one step costs ~0.5 s on one core; aim at `seconds` of loop time
-        steps = max(5, int(seconds / 0.5))
+        # the reference needs at least 100 steps (runtime_info takes
+        # mod(iter, floor(niter/100.)), SOLVER/time_evol_wave.F90:1017-1024)
+        steps = max(100, int(seconds / 0.35))
         cfg = inputs.RunConfig(**{**WORKLOAD.__dict__})
         cfg.seismogram_length_s = steps * dt
         inputs.write_run_dir(tmp, MESH, cfg)
@@ -126,6 +127,24 @@ def cpu_baseline(seconds: float, dt: float):
                           "inputs, %d time steps, 'Time loop routine' clock %.1f s" % (niter, t_loop)}
 
 
+def copy_calibration(nbytes: int, reps: int = 5) -> None:
+    """Device-to-device copies of nbytes (the HIP runtime's plain copy
+    kernel): the reference point for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled run to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +154,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=os.path.join(REPO, "data", "bench_10s"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of BYTES (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the time loop keeps the whole mesh resident on "
               "one MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
@@ -189,7 +213,7 @@ def main():
             traffic = None
 
     out = {
-        "metric": "GLL-point updates/s per GPU",
+        "metric": "GLL-point updates/s per GPU, batched vs plain at \u22641e-4 misfit; p50/p99 latency",
         "value": value,
         "unit": "GLL-point updates/s",
         "n_gpus": 1,
