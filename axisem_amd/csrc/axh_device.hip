@@ -15,15 +15,22 @@
 //    point sums its element contributions through a CSR list ordered by
 //    element number, which is the reference's summation order, so the FP32
 //    sums are the reference's bit for bit.  No atomics.
+//  * The mesh part (stiffness terms, gather lists, masses, coupling maps,
+//    attenuation terms) lives once per mesh and is shared by every wavefield
+//    stepped on it.  A kernel advances up to kMaxBatch wavefields: it reads
+//    each mesh term once into registers and loops over the wavefields, so a
+//    batch of sources on one mesh pays the mesh traffic once per step.
 //  * Arithmetic order follows the reference expression by expression; the
 //    file is compiled with -ffp-contract=off (no FMA), and dp constants enter
-//    in double precision where the Fortran promotes to dp.
+//    in double precision where the Fortran promotes to dp.  A wavefield's
+//    arithmetic does not depend on the other members of its batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -43,11 +50,14 @@ namespace axh {
 // number of packed element terms per source type (order: see pack_terms)
 constexpr int kNcMono = 15, kNcDi = 24, kNcQuad = 26;
 constexpr int kNcFluidMono = 3, kNcFluidMulti = 4;
-constexpr int kCg4Terms = 12;  // packed cg4 coefficients per element
+// coarse-grained terms per element, 4 cg points each.  Read by every stage:
+// Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi.  Read by the memory-variable update:
+// DsDeta, DzDeta, DsDxi, DzDxi over J, inv_s, delta_mu, delta_kappa, pad.
+constexpr int kCgStage = 5, kCgUpd = 8;
 
 // flag bits per global point
 constexpr uint8_t kFlagAxis = 1;      // on the symmetry axis (masked)
-constexpr uint8_t kFlagSource = 2;    // carries a source term (solid)
+constexpr uint8_t kFlagSource = 2;    // source term of some wavefield on the mesh
 constexpr uint8_t kFlagBlow = 4;      // GLL point (1,1) of an element
 constexpr uint8_t kFlagSurface = 8;   // fluid free surface (masked)
 constexpr uint8_t kFlagAxisBdry = 16; // axis point on the solid-fluid boundary
@@ -61,65 +71,18 @@ static const char *const kClassName[KC_N] = {
     "fluid_assemble_update", "copy2solid", "solid_assemble_update",
     "final_drift", "memvar_update", "receivers"};
 
-template <class T> struct DBuf {
-  T *p = nullptr;
-  size_t n = 0;
-};
-
-struct Device {
-  hipStream_t stream = nullptr;
-  int src = 0, scheme = 0, anel = 0, nsls = 0, nstages = 1;
-  int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0;
-  int nglob_s = 0, nglob_f = 0, nbdry = 0, nrec = 0, nsrcpts = 0, naxb = 0;
-  double coefd[16] = {0}, coefv[16] = {0};
-  double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
-  std::vector<float> stf;  // host copy of the stf table
+// Device allocations owned by one object and freed with it.
+struct DevAlloc {
+  std::vector<void *> ptrs;
   size_t bytes = 0;
-  // constants
-  float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
-  // solid
-  float *coef_s = nullptr, *m0_s = nullptr, *minv_s = nullptr;
-  int *einfo_s = nullptr, *gid_s = nullptr, *gstart_s = nullptr, *gslot_s = nullptr;
-  uint8_t *flags_s = nullptr;
-  float *disp = nullptr, *velo = nullptr, *acc0 = nullptr, *force_s = nullptr;
-  int *src_g = nullptr;
-  float *src_val = nullptr;
-  // fluid
-  float *coef_f = nullptr, *m0_f = nullptr, *minv_f = nullptr;
-  int *einfo_f = nullptr, *gid_f = nullptr, *gstart_f = nullptr, *gslot_f = nullptr;
-  uint8_t *flags_f = nullptr;
-  float *chi = nullptr, *dchi = nullptr, *ddchi = nullptr, *force_f = nullptr;
-  int *srcel_f = nullptr;     // per fluid element: source slot k or -1
-  float *src_fl = nullptr;    // source_term_fl (25 x 8)
-  // coupling
-  int *cpl_ss = nullptr, *cpl_gs = nullptr, *cpl_sf = nullptr, *cpl_gf = nullptr;
-  float *cpl_bm = nullptr;    // [nbdry*5][2]
-  int *cpl_axb = nullptr;     // side-buffer slot of a masked axis point or -1
-  int *axb_g = nullptr;       // global ids of the masked axis boundary points
-  float *axb_val = nullptr;   // their unmasked predicted displacement [naxb][3]
-  // attenuation
-  float *cg4 = nullptr, *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr,
-        *devm1 = nullptr;
-  double *aj_mu = nullptr, *aj_k = nullptr;
-  // receivers
-  int *rec_g = nullptr;
-  float *seis_d = nullptr;
-  size_t seis_cap = 0;
-  int *blow = nullptr;
-  // timing
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool profile = false;
-  std::vector<hipEvent_t> kev;      // pairs
-  std::vector<int> kev_class;
-  double kms[KC_N] = {0};
-  long kcount[KC_N] = {0};
-  std::vector<void *> allocs;
-
+  DevAlloc() = default;
+  DevAlloc(const DevAlloc &) = delete;
+  DevAlloc &operator=(const DevAlloc &) = delete;
   template <class T> T *alloc(size_t n) {
     if (n == 0) n = 1;
     void *p = nullptr;
     HIPCHECK(hipMalloc(&p, n * sizeof(T)));
-    allocs.push_back(p);
+    ptrs.push_back(p);
     bytes += n * sizeof(T);
     return (T *)p;
   }
@@ -134,36 +97,125 @@ struct Device {
     HIPCHECK(hipMemset(p, 0, (n ? n : 1) * sizeof(T)));
     return p;
   }
+  ~DevAlloc() {
+    for (void *p : ptrs) (void)hipFree(p);  // teardown: not actionable
+  }
+};
+
+// Mesh part: depends on the mesh, the model, the source type and the time
+// scheme only.
+struct MeshDev {
+  DevAlloc mem;
+  int src = 0, scheme = 0, anel = 0, nsls = 0, nstages = 1;
+  int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0;
+  int nglob_s = 0, nglob_f = 0, nbdry = 0, naxb = 0;
+  double coefd[16] = {0}, coefv[16] = {0};
+  double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
+  // host copies used by wavefield builds and field transfers
+  std::vector<int> gid_s_h, gid_f_h;
+  std::vector<uint8_t> flags_s_h;
+  float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
+  // solid
+  float *coef_s = nullptr, *m0_s = nullptr, *minv_s = nullptr;
+  int *einfo_s = nullptr, *gid_s = nullptr, *gstart_s = nullptr, *gslot_s = nullptr;
+  uint8_t *flags_s = nullptr;
+  // fluid
+  float *coef_f = nullptr, *m0_f = nullptr, *minv_f = nullptr;
+  int *einfo_f = nullptr, *gid_f = nullptr, *gstart_f = nullptr, *gslot_f = nullptr;
+  uint8_t *flags_f = nullptr;
+  // coupling
+  int *cpl_ss = nullptr, *cpl_gs = nullptr, *cpl_sf = nullptr, *cpl_gf = nullptr;
+  float *cpl_bm = nullptr;    // [nbdry*5][2]
+  int *cpl_axb = nullptr;     // side-buffer slot of a masked axis point or -1
+  int *axb_g = nullptr;       // global ids of the masked axis boundary points
+  // attenuation
+  float *cg_stage = nullptr, *cg_upd = nullptr;
+  double *aj_mu = nullptr, *aj_k = nullptr;
+};
+
+// Wavefield part: fields, element forces, memory variables, source,
+// receivers.
+struct WaveDev {
+  DevAlloc mem;
+  float *disp = nullptr, *velo = nullptr, *acc0 = nullptr, *force_s = nullptr;
+  float *chi = nullptr, *dchi = nullptr, *ddchi = nullptr, *force_f = nullptr;
+  float *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr, *devm1 = nullptr;
+  float *axb_val = nullptr;   // unmasked prediction of the axis boundary points
+  int nsrc = 0;               // solid source points
+  std::vector<int> src_g_h;
+  int *src_g = nullptr;
+  float *src_val = nullptr;   // [nsrc][3]
+  int fluid_src = 0;
+  int *srcel_f = nullptr;     // per fluid element: source slot k or -1
+  float *src_fl = nullptr;    // source_term_fl (25 x 8)
+  int nrec = 0, seis_it = 1;
+  int *rec_g = nullptr;
+  float *seis_d = nullptr;
+  size_t seis_cap = 0;
+  int *blow = nullptr;
+  double blow_thr = 0.0;
+  std::vector<float> stf;     // [niter][nstages]
+};
+
+// Stream, events and per-kernel timing of the runs of a context or a batch.
+struct Runner {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool profile = false;
+  std::vector<hipEvent_t> kev;  // pairs
+  std::vector<int> kev_class;
+  double kms[KC_N] = {0};
+  long kcount[KC_N] = {0};
+  void init() {
+    if (stream) return;
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+  }
+  ~Runner() {
+    // teardown: errors are not actionable here
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : kev) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct Device {
+  std::shared_ptr<MeshDev> mesh;
+  WaveDev wave;
+  Runner run;  // declared last: its stream is drained before buffers are freed
 };
 
 // ---------------------------------------------------------------------------
-// kernel argument blocks
+// kernel argument blocks: mesh part once, wavefield parts per batch member
 // ---------------------------------------------------------------------------
 struct SolidArgs {
-  int nel, nsls, iter_unused;
+  int nel, nsls, nglob, nslot, nb;
   const float *G, *coef, *m0;
   const int *einfo, *gid;
-  const float *disp;      // [3][nglob]
-  int nglob;
-  float *force;           // [3][nslot]
-  int nslot;
-  // anelastic
-  const float *cg4;       // [nel][12][4]
-  float *memvar;          // [nel][nsls][6][4]
-  float *rsum;            // [nel][6][4]
-  float *trm1, *devm1;    // [nel][4], [nel][6][4]
+  const float *cg_stage;  // [nel][5][4]
+  const float *cg_upd;    // [nel][8][4]
   const double *aj_mu, *aj_k;  // [nel][nsls]
   double expw[kMaxSls], tst[kMaxSls], tstm1[kMaxSls];
+  const float *disp[kMaxBatch];   // [3][nglob]
+  float *force[kMaxBatch];        // [3][nslot]
+  float *memvar[kMaxBatch];       // [nel][nsls][6][4]
+  float *rsum[kMaxBatch];         // [nel][6][4]
+  float *trm1[kMaxBatch];         // [nel][4]
+  float *devm1[kMaxBatch];        // [nel][6][4]
 };
 
 struct FluidArgs {
-  int nel, multipole, fluid_src;
+  int nel, multipole, nb;
   const float *G, *coef, *m0;
-  const int *einfo, *gid, *srcel;
-  const float *src_fl;
-  float stf;
-  const float *chi;
-  float *force;
+  const int *einfo, *gid;
+  const float *chi[kMaxBatch];
+  float *force[kMaxBatch];
+  const int *srcel[kMaxBatch];
+  const float *src_fl[kMaxBatch];
+  float stf[kMaxBatch];  // 0 when the wavefield has no fluid source this stage
 };
 
 // ---------------------------------------------------------------------------
@@ -225,11 +277,11 @@ __device__ __forceinline__ float vmul(const float *v, const float *B, int j) {
 __device__ __forceinline__ int cg_r(int q) { return (q < 2) ? 1 : 3; }
 __device__ __forceinline__ int cg_c(int q) { return (q & 1) ? 3 : 1; }
 
-// gradient of f at cg point q: returns (dsdf, dzdf)
+// gradient of f at cg point q: returns (dsdf, dzdf); cu = the element's
+// update terms [8][4]: 0 DsDeta, 1 DzDeta, 2 DsDxi, 3 DzDxi (over J)
 template <class F>
-__device__ __forceinline__ void grad_cg4(const float *G, bool axial,
-                                         const float *cg, int q, F f,
-                                         float &gs, float &gz) {
+__device__ __forceinline__ void grad_cg4(const float *G, bool axial, const float *cu, int q,
+                                         F f, float &gs, float &gz) {
   const float *GT = axial ? (G + 30) : (G + 80);
   const float *G2 = G + 55;
   const int r = cg_r(q), c = cg_c(q);
@@ -240,26 +292,25 @@ __device__ __forceinline__ void grad_cg4(const float *G, bool axial,
   // mxm_cg4_sparse_c(f, G2): row r of f times column c of G2
 #pragma unroll
   for (int k = 0; k < 5; k++) m2 = m2 + f(r + 5 * k) * G2[k + 5 * c];
-  // cg layout [12][4]: 5 DsDeta, 6 DzDeta, 7 DsDxi, 8 DzDxi
-  gs = cg[6 * 4 + q] * m1 + cg[8 * 4 + q] * m2;
-  gz = cg[5 * 4 + q] * m1 + cg[7 * 4 + q] * m2;
+  gs = cu[1 * 4 + q] * m1 + cu[3 * 4 + q] * m2;
+  gz = cu[0 * 4 + q] * m1 + cu[2 * 4 + q] * m2;
 }
 
 // Voigt strain E1..E6 at cg point q from the element's displacement in LDS.
 template <int SRC>
-__device__ void strain_cg4(const float *G, bool axial, const float *cg, int q,
+__device__ void strain_cg4(const float *G, bool axial, const float *cu, int q,
                            const float *U1, const float *U2, const float *U3,
                            float g[6]) {
-  const float *invs = cg + 9 * 4;
+  const float *invs = cu + 4 * 4;
   const int p = cg_r(q) + 5 * cg_c(q);
   float b1s, b1z, b2s, b2z;
 #pragma unroll
   for (int c = 0; c < 6; c++) g[c] = 0.0f;
   if (SRC == AXH_DIPOLE)
-    grad_cg4(G, axial, cg, q, [&](int k) { return U1[k] + U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k] + U2[k]; }, b1s, b1z);
   else
-    grad_cg4(G, axial, cg, q, [&](int k) { return U1[k]; }, b1s, b1z);
-  grad_cg4(G, axial, cg, q, [&](int k) { return U3[k]; }, b2s, b2z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k]; }, b1s, b1z);
+  grad_cg4(G, axial, cu, q, [&](int k) { return U3[k]; }, b2s, b2z);
   g[0] = b1s;
   g[2] = b2z;
   g[4] = b1z + b2s;
@@ -267,13 +318,13 @@ __device__ void strain_cg4(const float *G, bool axial, const float *cg, int q,
     g[1] = invs[q] * U1[p];
   } else if (SRC == AXH_DIPOLE) {
     g[1] = 2.0f * (invs[q] * U2[p]);
-    grad_cg4(G, axial, cg, q, [&](int k) { return U1[k] - U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k] - U2[k]; }, b1s, b1z);
     float f3 = invs[q] * U3[p];
     g[3] = -f3 - b1z;
     g[5] = -g[1] - b1s;
   } else {
     g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
-    grad_cg4(G, axial, cg, q, [&](int k) { return U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U2[k]; }, b1s, b1z);
     float f3 = invs[q] * U3[p];
     float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
     g[3] = -(2.0f * f3) - b1z;
@@ -283,11 +334,10 @@ __device__ void strain_cg4(const float *G, bool axial, const float *cg, int q,
 
 // source terms of the memory-variable update at cg point q
 template <int SRC>
-__device__ __forceinline__ void memvar_sources(const float *cg, int q,
-                                               const float g[6], float &str,
-                                               float sdev[6]) {
+__device__ __forceinline__ void memvar_sources(const float *cu, int q, const float g[6],
+                                               float &str, float sdev[6]) {
   const double third = 1.0 / 3.0;
-  const float dm = cg[10 * 4 + q], dk = cg[11 * 4 + q];
+  const float dm = cu[5 * 4 + q], dk = cu[6 * 4 + q];
   const float tr = g[0] + g[1] + g[2];
   str = dk * tr;
 #pragma unroll
@@ -302,15 +352,41 @@ __device__ __forceinline__ void memvar_sources(const float *cg, int q,
   }
 }
 
-// Update of the memory variables of one element.  Work of the element is
-// spread over its lanes: lane k handles entries idx = k, k+25, ... of
-// (q, c, j) = (idx%4, (idx/4)%6, idx/24).  mv/old in LDS, src_* in LDS.
+// Strain sources of the memory-variable update at cg point q of element e
+// for wavefield b (lanes k < 4 of the element, one cg point each).  The
+// sources of the previous update go to old, the new ones to src and to the
+// wavefield's t-1 store.
 template <int SRC>
-__device__ __forceinline__ void memvar_update_elem(
-    const SolidArgs &A, int e, int k, const float *mv_lds, const float *src_lds,
-    const float *old_lds, float *rs_lds) {
+__device__ __forceinline__ void memvar_prepare(const SolidArgs &A, int b, int e, int q,
+                                               bool axial, const float *U1, const float *U2,
+                                               const float *U3, float *src, float *old) {
+  const float *cu = A.cg_upd + (size_t)e * (kCgUpd * 4);
+  float g[6], str, sdev[6];
+  strain_cg4<SRC>(A.G, axial, cu, q, U1, U2, U3, g);
+  memvar_sources<SRC>(cu, q, g, str, sdev);
+  float *trm1 = A.trm1[b] + (size_t)e * 4;
+  float *devm1 = A.devm1[b] + (size_t)e * 24;
+  old[q] = trm1[q];
+#pragma unroll
+  for (int c = 0; c < 6; c++) old[4 + 4 * c + q] = devm1[4 * c + q];
+  src[q] = str;
+#pragma unroll
+  for (int c = 0; c < 6; c++) src[4 + 4 * c + q] = sdev[c];
+  trm1[q] = str;
+#pragma unroll
+  for (int c = 0; c < 6; c++) devm1[4 * c + q] = sdev[c];
+}
+
+// Update of the memory variables of element e of wavefield b.  The work of
+// the element is spread over its lanes: lane k handles entries idx = k, k+25,
+// ... of (q, c, j) = (idx%4, (idx/4)%6, idx/24).  mv/src/old in LDS.
+template <int SRC>
+__device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int b, int e, int k,
+                                                   const float *mv_lds, const float *src_lds,
+                                                   const float *old_lds, float *rs_lds) {
   const int nsls = A.nsls;
   const int n = 24 * nsls;
+  float *memvar = A.memvar[b] + (size_t)e * n;
   for (int idx = k; idx < n; idx += 25) {
     const int q = idx & 3, c = (idx >> 2) % 6, jj = idx / 24;
     if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
@@ -322,13 +398,12 @@ __device__ __forceinline__ void memvar_update_elem(
     float nv;
     if (c < 3) {
       const double ak = A.aj_k[(size_t)e * nsls + jj];
-      const float trb = (float)(Tt * ak * (double)src_lds[q] +
-                                Tm * ak * (double)old_lds[q]);
+      const float trb = (float)(Tt * ak * (double)src_lds[q] + Tm * ak * (double)old_lds[q]);
       nv = (float)((E * m + (double)db) + (double)trb);
     } else {
       nv = (float)(E * m + (double)db);
     }
-    A.memvar[(size_t)e * n + idx] = nv;
+    memvar[idx] = nv;
     if (rs_lds) rs_lds[idx] = nv;
   }
 }
@@ -339,6 +414,8 @@ __device__ __forceinline__ void memvar_update_elem(
 // glob_anel_stiffness_*_cg4 and, for newmark2, time_step_memvars_cg4.
 // MODE 0: newmark2 (memory variables read, used, updated)
 // MODE 1: symplectic stage (sums of memory variables read)
+// One workgroup = one tile of 10 elements; the tile's stiffness terms are read
+// once and applied to the A.nb wavefields in turn.
 // ---------------------------------------------------------------------------
 template <int SRC, int MODE, bool ANEL>
 __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
@@ -364,328 +441,322 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
+  const int gp = valid ? A.gid[slot] : 0;
 
   stage_g(A.G, sG);
-  // gather the (predicted, masked) displacement of the element
-  if (valid) {
-    const int g = A.gid[slot];
-    if (SRC == AXH_MONOPOLE) {
-      sU[0][t] = A.disp[g];
-      sU[1][t] = A.disp[2 * (size_t)A.nglob + g];
+  // the lane's stiffness terms, read once for every wavefield (the packed
+  // rows cover all 256 lanes of a tile)
+  float cf[NC];
+  {
+    const float *coef = A.coef + (size_t)tile * NC * 256 + t;
+#pragma unroll
+    for (int a = 0; a < NC; a++) cf[a] = coef[a * 256];
+  }
+
+  for (int b = 0; b < A.nb; b++) {
+    __syncthreads();  // LDS of the previous wavefield consumed; sG staged
+    // gather the (predicted, masked) displacement of the element
+    const float *disp = A.disp[b];
+    if (valid) {
+      if (SRC == AXH_MONOPOLE) {
+        sU[0][t] = disp[gp];
+        sU[1][t] = disp[2 * (size_t)A.nglob + gp];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) sU[c][t] = disp[c * (size_t)A.nglob + gp];
+      }
     } else {
 #pragma unroll
-      for (int c = 0; c < 3; c++) sU[c][t] = A.disp[c * (size_t)A.nglob + g];
+      for (int c = 0; c < NCOMP; c++) sU[c][t] = 0.0f;
     }
-  } else {
-#pragma unroll
-    for (int c = 0; c < NCOMP; c++) sU[c][t] = 0.0f;
-  }
-  if (ANEL && MODE == 0) {
-    // memory variables of the tile (contiguous per element)
-    const int n = 24 * A.nsls;
-    for (int idx = k; idx < n && valid; idx += 25)
-      sMv[le * 24 * kMaxSls + idx] = A.memvar[(size_t)e * n + idx];
-  }
-  __syncthreads();
+    if (ANEL && MODE == 0) {
+      // memory variables of the tile (contiguous per element)
+      const int n = 24 * A.nsls;
+      for (int idx = k; idx < n && valid; idx += 25)
+        sMv[le * 24 * kMaxSls + idx] = A.memvar[b][(size_t)e * n + idx];
+    }
+    __syncthreads();
 
-  GRegs G;
-  load_g(sG, axial, i, j, G);
-  const float *coef = A.coef + (size_t)tile * NC * 256 + t;
-  float acc[3];
+    GRegs G;
+    load_g(sG, axial, i, j, G);
+    float acc[3];
 
-  if (SRC == AXH_MONOPOLE) {
-    const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25];
-    const float us = U0[k], uz = U1[k];
-    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j);
-    const float X3 = mulb(U0, G.g2, i, j), X4 = mulb(U1, G.g2, i, j);
-    const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
-                m4 = coef[3 * 256], mw1 = coef[4 * 256], m11s = coef[5 * 256],
-                m21s = coef[6 * 256], m41s = coef[7 * 256], m12s = coef[8 * 256],
-                m22s = coef[9 * 256], m32s = coef[10 * 256], m42s = coef[11 * 256],
-                m11z = coef[12 * 256], m21z = coef[13 * 256], m41z = coef[14 * 256];
-    float ls = m4 * X4 + m2 * X3 + m1 * X1 + m3 * X2 + us * mw1;
-    sS[0][t] = m11s * X3 + m21s * X1 + m12s * X4 + m22s * X2 + m1 * us;
-    sS[1][t] = m11s * X1 + m41s * X3 + m32s * X2 + m42s * X4 + m2 * us;
-    sS[2][t] = m11z * X4 + m21z * X2 + m32s * X3 + m22s * X1 + m3 * us;
-    sS[3][t] = m11z * X2 + m41z * X4 + m12s * X1 + m42s * X3 + m4 * us;
-    __syncthreads();
-    const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
-                *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25];
-    const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2z, G.g2T, i, j);
-    const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1z, i, j);
-    ls = ls + Y1 + Y2;
-    float lz = Y3 + Y4;
-    // axial terms with M0_w1..3
-    float w1 = 0, w2 = 0, w3 = 0, V1 = 0, V2 = 0, V3 = 0;
-    if (axial) {
-      const float *m0 = A.m0 + (size_t)ax * 50;
-      w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
-      V1 = vmul(G.g0, U0, j);        // vxm(G0, us)
-      V2 = mulb(U1, G.g2, 0, j);       // vxm(uz(0,:), G2)
-      V3 = vmul(G.g0, U1, j);        // vxm(G0, uz)
-      if (i == 0) sV[le][j] = w3 * V1;
-    }
-    __syncthreads();
-    if (axial) {
-      float V4 = w1 * V1 + w3 * V2;
-      V4 = V4 + w2 * V3;
-      float X2a = G.g0i * (w2 * V1);
-      const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(w3*V1, G2T)
-      if (i == 0) X2a = X2a + V1b;
-      ls = ls + G.g0i * V4;
-      lz = X2a + lz;
-    }
-    acc[0] = ls;
-    acc[1] = 0.0f;
-    acc[2] = lz;
-  } else if (SRC == AXH_DIPOLE) {
-    const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
-    const float u1 = U0[k], u2 = U1[k], u3 = U2[k];
-    const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
-                X6 = mulb(U2, G.g2, i, j);
-    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
-                X3 = amul(G.gT, U2, i, j);
-    const float X7 = X1 + X2, X8 = X4 + X5;
-    const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
-                m4 = coef[3 * 256], m5 = coef[4 * 256], m6 = coef[5 * 256],
-                m7 = coef[6 * 256], m8 = coef[7 * 256], mw1 = coef[8 * 256],
-                mw2 = coef[9 * 256], mw3 = coef[10 * 256], m11s = coef[11 * 256],
-                m21s = coef[12 * 256], m41s = coef[13 * 256], m12s = coef[14 * 256],
-                m22s = coef[15 * 256], m42s = coef[16 * 256], m13s = coef[17 * 256],
-                m23s = coef[18 * 256], m33s = coef[19 * 256], m43s = coef[20 * 256],
-                m11z = coef[21 * 256], m21z = coef[22 * 256], m41z = coef[23 * 256];
-    const float ls2 = m8 * X6 + m7 * X3 + m1 * X1 + m5 * X2 + m2 * X4 + m6 * X5 +
-                      mw1 * u2 + mw2 * u3;
-    const float ls3 = m4 * X4 - m4 * X5 + m3 * X1 - m3 * X2 + mw2 * u2 + mw3 * u3;
-    float c1 = m13s * X6, c2 = m23s * X3, c3 = m3 * u3;
-    sS[0][t] = c1 + c2 + c3 + m11s * X4 + m21s * X1 + m12s * X5 + m22s * X2 + m1 * u2;
-    sS[1][t] = c1 + c2 - c3 + m11s * X5 + m21s * X2 + m12s * X4 + m22s * X1 + m5 * u2;
-    c1 = m33s * X3;
-    c2 = m43s * X6;
-    c3 = m4 * u3;
-    sS[2][t] = c1 + c2 + c3 + m11s * X1 + m41s * X4 + m12s * X2 + m42s * X5 + m2 * u2;
-    sS[3][t] = c1 + c2 - c3 + m11s * X2 + m41s * X5 + m12s * X1 + m42s * X4 + m6 * u2;
-    sS[4][t] = m33s * X8 + m23s * X7 + m11z * X6 + m21z * X3 + m7 * u2;
-    sS[5][t] = m13s * X7 + m43s * X8 + m11z * X3 + m41z * X6 + m8 * u2;
-    __syncthreads();
-    const float *S1p = &sS[0][le * 25], *S1m = &sS[1][le * 25],
-                *S2p = &sS[2][le * 25], *S2m = &sS[3][le * 25],
-                *S1z = &sS[4][le * 25], *S2z = &sS[5][le * 25];
-    const float Y1 = amul(G.g, S1p, i, j), Y3 = amul(G.g, S1m, i, j), Y5 = amul(G.g, S1z, i, j);
-    const float Y2 = mulb(S2p, G.g2T, i, j), Y4 = mulb(S2m, G.g2T, i, j),
-                Y6 = mulb(S2z, G.g2T, i, j);
-    float l1 = Y1 + Y2;
-    float l2 = Y3 + Y4 + ls2;
-    float l3 = Y5 + Y6 + ls3;
-    float a = 0, b = 0, c = 0, w2 = 0, w4 = 0, w6 = 0, w8 = 0, V2 = 0, V3 = 0;
-    if (axial) {
-      const float *m0 = A.m0 + (size_t)ax * 50;
-      const float w1 = m0[0 * 5 + j], w3 = m0[2 * 5 + j], w7 = m0[6 * 5 + j],
-                  w9 = m0[8 * 5 + j], w10 = m0[9 * 5 + j];
-      w2 = m0[1 * 5 + j]; w4 = m0[3 * 5 + j]; w6 = m0[5 * 5 + j]; w8 = m0[7 * 5 + j];
-      const float V1 = vmul(G.g0, U0, j);
-      V2 = vmul(G.g0, U1, j);
-      V3 = vmul(G.g0, U2, j);
-      const float V4 = mulb(U0, G.g2, 0, j);   // vxm(u1(0,:), G2)
-      a = w1 * V2 + w3 * V3;
-      b = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
-      c = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
-      if (i == 0) sV[le][j] = (w2 + w6) * V2 + (w4 + w8) * V3;
-    }
-    __syncthreads();
-    if (axial) {
-      float S1pa = G.g0i * a;
-      const float S1ma = G.g0i * b, S1za = G.g0i * c;
-      const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(V4, G2T)
-      if (i == 0) S1pa = S1pa + V1b;
-      l1 = l1 + S1pa;
-      l2 = l2 + S1ma;
-      l3 = l3 + S1za;
-    }
-    acc[0] = l1;
-    acc[1] = l2;
-    acc[2] = l3;
-  } else {
-    const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
-    const float us = U0[k], up = U1[k], uz = U2[k];
-    const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
-                X3 = amul(G.gT, U2, i, j);
-    const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
-                X6 = mulb(U2, G.g2, i, j);
-    const float m1 = coef[0 * 256], m2 = coef[1 * 256], m3 = coef[2 * 256],
-                m4 = coef[3 * 256], m5 = coef[4 * 256], m6 = coef[5 * 256],
-                m7 = coef[6 * 256], m8 = coef[7 * 256], mw1 = coef[8 * 256],
-                mw2 = coef[9 * 256], mw3 = coef[10 * 256], mw4 = coef[11 * 256],
-                mw5 = coef[12 * 256], m11s = coef[13 * 256], m21s = coef[14 * 256],
-                m41s = coef[15 * 256], m12s = coef[16 * 256], m22s = coef[17 * 256],
-                m32s = coef[18 * 256], m42s = coef[19 * 256], m11z = coef[20 * 256],
-                m21z = coef[21 * 256], m41z = coef[22 * 256], m1phi = coef[23 * 256],
-                m2phi = coef[24 * 256], m4phi = coef[25 * 256];
-    const float c1 = m2 * X4, c2 = m1 * X1, c3 = m6 * X5, c4 = m5 * X2,
-                c5 = m4 * X6, c6 = m3 * X3;
-    float ls = c1 + c2 + 2.0f * (c3 + c4) + c5 + c6 + mw1 * us + mw2 * up +
-               2.0f * mw3 * uz;
-    float lp = -2.0f * (c1 + c2 + c5 + c6) - (c3 + c4) + mw2 * us + mw4 * up -
-               mw3 * uz;
-    float lz = 2.0f * (m8 * X5 + m7 * X2) + mw3 * (2.0f * us - up) + mw5 * uz;
-    const float usp = us - 2.0f * up, tus = 2.0f * us - up;
-    sS[0][t] = m11s * X4 + m21s * X1 + m12s * X6 + m22s * X3 + m1 * usp;   // S1s
-    sS[1][t] = m11s * X1 + m41s * X4 + m32s * X3 + m42s * X6 + m2 * usp;   // S2s
-    sS[2][t] = m11z * X6 + m21z * X3 + m32s * X4 + m22s * X1 + m3 * usp;   // S1z
-    sS[3][t] = m11z * X3 + m41z * X6 + m12s * X1 + m42s * X4 + m4 * usp;   // S2z
-    sS[4][t] = m1phi * X5 + m2phi * X2 + m5 * tus + 2.0f * m7 * uz;        // S1phi
-    sS[5][t] = m1phi * X2 + m4phi * X5 + m6 * tus + 2.0f * m8 * uz;        // S2phi
-    __syncthreads();
-    const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
-                *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25],
-                *S1p = &sS[4][le * 25], *S2p = &sS[5][le * 25];
-    const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2p, G.g2T, i, j),
-                Y6 = mulb(S2z, G.g2T, i, j);
-    const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1p, i, j), Y5 = amul(G.g, S1z, i, j);
-    ls = ls + Y1 + Y2;
-    lp = lp + Y3 + Y4;
-    lz = lz + Y5 + Y6;
-    __syncthreads();  // uniform barrier count with the other source types
-    if (axial) {
-      const float *m0 = A.m0 + (size_t)ax * 50;
-      const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
-                  w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
-      const float V1 = vmul(G.g0, U0, j), V2 = vmul(G.g0, U1, j),
-                  V3 = vmul(G.g0, U2, j);
-      const float a = w1 * V1 + w2 * V2 + w3 * V3;
-      const float b = w2 * V1 + w4 * V2 + w5 * V3;
-      const float c = w3 * V1 + w5 * V2 + w6 * V3;
-      ls = ls + G.g0i * a;
-      lp = lp + G.g0i * b;
-      lz = lz + G.g0i * c;
-    }
-    acc[0] = ls;
-    acc[1] = lp;
-    acc[2] = lz;
-  }
-
-  if (ANEL) {
-    // sums of the memory variables over the SLS, (c, q) per lane k < 24
-    const int n = 24 * A.nsls;
-    float *R = &sR[le * 24];
-    if (valid && k < 24) {
-      if (MODE == 0) {
-        float s = 0.0f;
-        for (int jj = 0; jj < A.nsls; jj++) s = s + sMv[le * 24 * kMaxSls + jj * 24 + k];
-        R[k] = s;
-      } else {
-        R[k] = A.rsum[(size_t)e * 24 + k];
-      }
-    }
-    const float *cg = A.cg4 + (size_t)(valid ? e : 0) * (kCg4Terms * 4);
-    __syncthreads();
-    float *Sa = &sSa[le * 24];
-    if (valid && k < 4) {
-      const int q = k;
-      const float vse = cg[1 * 4 + q], vsx = cg[2 * 4 + q], vze = cg[3 * 4 + q],
-                  vzx = cg[4 * 4 + q];
-      const float r1 = R[0 * 4 + q], r3 = R[2 * 4 + q], r4 = R[3 * 4 + q],
-                  r5 = R[4 * 4 + q], r6 = R[5 * 4 + q];
-      if (SRC == AXH_MONOPOLE) {
-        Sa[0 * 4 + q] = vze * r1 + vse * r5;
-        Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
-        Sa[4 * 4 + q] = vze * r5 + vse * r3;
-        Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
-      } else if (SRC == AXH_DIPOLE) {
-        Sa[0 * 4 + q] = vze * (r1 - r6) + vse * (r5 - r4);
-        Sa[1 * 4 + q] = vzx * (r1 - r6) + vsx * (r5 - r4);
-        Sa[2 * 4 + q] = vze * (r1 + r6) + vse * (r5 + r4);
-        Sa[3 * 4 + q] = vzx * (r1 + r6) + vsx * (r5 + r4);
-        Sa[4 * 4 + q] = vze * r5 + vse * r3;
-        Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
-      } else {
-        Sa[0 * 4 + q] = vze * r1 + vse * r5;
-        Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
-        Sa[2 * 4 + q] = vze * r6 + vse * r4;
-        Sa[3 * 4 + q] = vzx * r6 + vsx * r4;
-        Sa[4 * 4 + q] = vze * r5 + vse * r3;
-        Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
-      }
-    }
-    __syncthreads();
-    if (valid) {
-      // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
-      // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
-      auto spb = [&](int s1) -> float {
-        if (j == 1) return G.g[i + 5] * Sa[s1 * 4 + 0] + G.g[i + 15] * Sa[s1 * 4 + 2];
-        if (j == 3) return G.g[i + 5] * Sa[s1 * 4 + 1] + G.g[i + 15] * Sa[s1 * 4 + 3];
-        return 0.0f;
-      };
-      auto spa = [&](int s2) -> float {
-        if (i == 1) return Sa[s2 * 4 + 0] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 1] * G.g2T[3 + 5 * j];
-        if (i == 3) return Sa[s2 * 4 + 2] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 3] * G.g2T[3 + 5 * j];
-        return 0.0f;
-      };
-      const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
-      const int q = cgp ? ((i == 1 ? 0 : 2) + (j == 1 ? 0 : 1)) : 0;
-      const float yl = cg[0 * 4 + q];
-      const float r2 = R[1 * 4 + q], r4 = R[3 * 4 + q], r6 = R[5 * 4 + q];
-      if (SRC == AXH_MONOPOLE) {
-        float l1 = spb(0) + spa(1);
-        if (cgp) l1 = l1 + yl * r2;
-        const float l3 = spb(4) + spa(5);
-        acc[0] = acc[0] - l1;
-        acc[2] = acc[2] - l3;
-      } else if (SRC == AXH_DIPOLE) {
-        const float lp = spb(0) + spa(1);
-        float lm = spb(2) + spa(3);
-        float lz = spb(4) + spa(5);
-        if (cgp) {
-          lm = lm + 2.0f * yl * (r2 - r6);
-          lz = lz - yl * r4;
-        }
-        acc[0] = acc[0] - lp;
-        acc[1] = acc[1] - lm;
-        acc[2] = acc[2] - lz;
-      } else {
-        float ls = spb(0) + spa(1);
-        float lp = -spb(2) - spa(3);
-        float lz = spb(4) + spa(5);
-        if (cgp) {
-          ls = ls + yl * (r2 - 2.0f * r6);
-          lp = lp + yl * (r6 - 2.0f * r2);
-          lz = lz - 2.0f * yl * r4;
-        }
-        acc[0] = acc[0] - ls;
-        acc[1] = acc[1] - lp;
-        acc[2] = acc[2] - lz;
-      }
-    }
-    if (MODE == 0) {
-      // time_step_memvars_cg4 with the masked predicted displacement
-      float *src = &sSrc[le * 28], *old = &sOld[le * 28];
-      if (valid && k < 4) {
-        const int q = k;
-        float g[6], str, sdev[6];
-        strain_cg4<SRC>(A.G, axial, cg, q, &sU[0][le * 25],
-                        &sU[NCOMP == 2 ? 0 : 1][le * 25], &sU[NCOMP - 1][le * 25], g);
-        memvar_sources<SRC>(cg, q, g, str, sdev);
-        old[q] = A.trm1[(size_t)e * 4 + q];
-#pragma unroll
-        for (int c = 0; c < 6; c++) old[4 + 4 * c + q] = A.devm1[(size_t)e * 24 + 4 * c + q];
-        src[q] = str;
-#pragma unroll
-        for (int c = 0; c < 6; c++) src[4 + 4 * c + q] = sdev[c];
-        A.trm1[(size_t)e * 4 + q] = str;
-#pragma unroll
-        for (int c = 0; c < 6; c++) A.devm1[(size_t)e * 24 + 4 * c + q] = sdev[c];
+    if (SRC == AXH_MONOPOLE) {
+      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25];
+      const float us = U0[k], uz = U1[k];
+      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j);
+      const float X3 = mulb(U0, G.g2, i, j), X4 = mulb(U1, G.g2, i, j);
+      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], mw1 = cf[4],
+                  m11s = cf[5], m21s = cf[6], m41s = cf[7], m12s = cf[8], m22s = cf[9],
+                  m32s = cf[10], m42s = cf[11], m11z = cf[12], m21z = cf[13], m41z = cf[14];
+      float ls = m4 * X4 + m2 * X3 + m1 * X1 + m3 * X2 + us * mw1;
+      sS[0][t] = m11s * X3 + m21s * X1 + m12s * X4 + m22s * X2 + m1 * us;
+      sS[1][t] = m11s * X1 + m41s * X3 + m32s * X2 + m42s * X4 + m2 * us;
+      sS[2][t] = m11z * X4 + m21z * X2 + m32s * X3 + m22s * X1 + m3 * us;
+      sS[3][t] = m11z * X2 + m41z * X4 + m12s * X1 + m42s * X3 + m4 * us;
+      __syncthreads();
+      const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
+                  *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25];
+      const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2z, G.g2T, i, j);
+      const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1z, i, j);
+      ls = ls + Y1 + Y2;
+      float lz = Y3 + Y4;
+      // axial terms with M0_w1..3
+      float w1 = 0, w2 = 0, w3 = 0, V1 = 0, V2 = 0, V3 = 0;
+      if (axial) {
+        const float *m0 = A.m0 + (size_t)ax * 50;
+        w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
+        V1 = vmul(G.g0, U0, j);        // vxm(G0, us)
+        V2 = mulb(U1, G.g2, 0, j);     // vxm(uz(0,:), G2)
+        V3 = vmul(G.g0, U1, j);        // vxm(G0, uz)
+        if (i == 0) sV[le][j] = w3 * V1;
       }
       __syncthreads();
-      if (valid)
-        memvar_update_elem<SRC>(A, e, k, &sMv[le * 24 * kMaxSls], src, old, nullptr);
-      (void)n;
+      if (axial) {
+        float V4 = w1 * V1 + w3 * V2;
+        V4 = V4 + w2 * V3;
+        float X2a = G.g0i * (w2 * V1);
+        const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(w3*V1, G2T)
+        if (i == 0) X2a = X2a + V1b;
+        ls = ls + G.g0i * V4;
+        lz = X2a + lz;
+      }
+      acc[0] = ls;
+      acc[1] = 0.0f;
+      acc[2] = lz;
+    } else if (SRC == AXH_DIPOLE) {
+      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
+      const float u1 = U0[k], u2 = U1[k], u3 = U2[k];
+      const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
+                  X6 = mulb(U2, G.g2, i, j);
+      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
+                  X3 = amul(G.gT, U2, i, j);
+      const float X7 = X1 + X2, X8 = X4 + X5;
+      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
+                  m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
+                  m11s = cf[11], m21s = cf[12], m41s = cf[13], m12s = cf[14],
+                  m22s = cf[15], m42s = cf[16], m13s = cf[17], m23s = cf[18],
+                  m33s = cf[19], m43s = cf[20], m11z = cf[21], m21z = cf[22],
+                  m41z = cf[23];
+      const float ls2 = m8 * X6 + m7 * X3 + m1 * X1 + m5 * X2 + m2 * X4 + m6 * X5 +
+                        mw1 * u2 + mw2 * u3;
+      const float ls3 = m4 * X4 - m4 * X5 + m3 * X1 - m3 * X2 + mw2 * u2 + mw3 * u3;
+      float c1 = m13s * X6, c2 = m23s * X3, c3 = m3 * u3;
+      sS[0][t] = c1 + c2 + c3 + m11s * X4 + m21s * X1 + m12s * X5 + m22s * X2 + m1 * u2;
+      sS[1][t] = c1 + c2 - c3 + m11s * X5 + m21s * X2 + m12s * X4 + m22s * X1 + m5 * u2;
+      c1 = m33s * X3;
+      c2 = m43s * X6;
+      c3 = m4 * u3;
+      sS[2][t] = c1 + c2 + c3 + m11s * X1 + m41s * X4 + m12s * X2 + m42s * X5 + m2 * u2;
+      sS[3][t] = c1 + c2 - c3 + m11s * X2 + m41s * X5 + m12s * X1 + m42s * X4 + m6 * u2;
+      sS[4][t] = m33s * X8 + m23s * X7 + m11z * X6 + m21z * X3 + m7 * u2;
+      sS[5][t] = m13s * X7 + m43s * X8 + m11z * X3 + m41z * X6 + m8 * u2;
+      __syncthreads();
+      const float *S1p = &sS[0][le * 25], *S1m = &sS[1][le * 25],
+                  *S2p = &sS[2][le * 25], *S2m = &sS[3][le * 25],
+                  *S1z = &sS[4][le * 25], *S2z = &sS[5][le * 25];
+      const float Y1 = amul(G.g, S1p, i, j), Y3 = amul(G.g, S1m, i, j),
+                  Y5 = amul(G.g, S1z, i, j);
+      const float Y2 = mulb(S2p, G.g2T, i, j), Y4 = mulb(S2m, G.g2T, i, j),
+                  Y6 = mulb(S2z, G.g2T, i, j);
+      float l1 = Y1 + Y2;
+      float l2 = Y3 + Y4 + ls2;
+      float l3 = Y5 + Y6 + ls3;
+      float a = 0, bb = 0, c = 0;
+      if (axial) {
+        const float *m0 = A.m0 + (size_t)ax * 50;
+        const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
+                    w4 = m0[3 * 5 + j], w6 = m0[5 * 5 + j], w7 = m0[6 * 5 + j],
+                    w8 = m0[7 * 5 + j], w9 = m0[8 * 5 + j], w10 = m0[9 * 5 + j];
+        const float V1 = vmul(G.g0, U0, j);
+        const float V2 = vmul(G.g0, U1, j);
+        const float V3 = vmul(G.g0, U2, j);
+        const float V4 = mulb(U0, G.g2, 0, j);   // vxm(u1(0,:), G2)
+        a = w1 * V2 + w3 * V3;
+        bb = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
+        c = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
+        if (i == 0) sV[le][j] = (w2 + w6) * V2 + (w4 + w8) * V3;
+      }
+      __syncthreads();
+      if (axial) {
+        float S1pa = G.g0i * a;
+        const float S1ma = G.g0i * bb, S1za = G.g0i * c;
+        const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(V4, G2T)
+        if (i == 0) S1pa = S1pa + V1b;
+        l1 = l1 + S1pa;
+        l2 = l2 + S1ma;
+        l3 = l3 + S1za;
+      }
+      acc[0] = l1;
+      acc[1] = l2;
+      acc[2] = l3;
+    } else {
+      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
+      const float us = U0[k], up = U1[k], uz = U2[k];
+      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
+                  X3 = amul(G.gT, U2, i, j);
+      const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
+                  X6 = mulb(U2, G.g2, i, j);
+      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
+                  m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
+                  mw4 = cf[11], mw5 = cf[12], m11s = cf[13], m21s = cf[14],
+                  m41s = cf[15], m12s = cf[16], m22s = cf[17], m32s = cf[18],
+                  m42s = cf[19], m11z = cf[20], m21z = cf[21], m41z = cf[22],
+                  m1phi = cf[23], m2phi = cf[24], m4phi = cf[25];
+      const float c1 = m2 * X4, c2 = m1 * X1, c3 = m6 * X5, c4 = m5 * X2,
+                  c5 = m4 * X6, c6 = m3 * X3;
+      float ls = c1 + c2 + 2.0f * (c3 + c4) + c5 + c6 + mw1 * us + mw2 * up +
+                 2.0f * mw3 * uz;
+      float lp = -2.0f * (c1 + c2 + c5 + c6) - (c3 + c4) + mw2 * us + mw4 * up -
+                 mw3 * uz;
+      float lz = 2.0f * (m8 * X5 + m7 * X2) + mw3 * (2.0f * us - up) + mw5 * uz;
+      const float usp = us - 2.0f * up, tus = 2.0f * us - up;
+      sS[0][t] = m11s * X4 + m21s * X1 + m12s * X6 + m22s * X3 + m1 * usp;   // S1s
+      sS[1][t] = m11s * X1 + m41s * X4 + m32s * X3 + m42s * X6 + m2 * usp;   // S2s
+      sS[2][t] = m11z * X6 + m21z * X3 + m32s * X4 + m22s * X1 + m3 * usp;   // S1z
+      sS[3][t] = m11z * X3 + m41z * X6 + m12s * X1 + m42s * X4 + m4 * usp;   // S2z
+      sS[4][t] = m1phi * X5 + m2phi * X2 + m5 * tus + 2.0f * m7 * uz;        // S1phi
+      sS[5][t] = m1phi * X2 + m4phi * X5 + m6 * tus + 2.0f * m8 * uz;        // S2phi
+      __syncthreads();
+      const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
+                  *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25],
+                  *S1p = &sS[4][le * 25], *S2p = &sS[5][le * 25];
+      const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2p, G.g2T, i, j),
+                  Y6 = mulb(S2z, G.g2T, i, j);
+      const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1p, i, j),
+                  Y5 = amul(G.g, S1z, i, j);
+      ls = ls + Y1 + Y2;
+      lp = lp + Y3 + Y4;
+      lz = lz + Y5 + Y6;
+      __syncthreads();  // uniform barrier count with the other source types
+      if (axial) {
+        const float *m0 = A.m0 + (size_t)ax * 50;
+        const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
+                    w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
+        const float V1 = vmul(G.g0, U0, j), V2 = vmul(G.g0, U1, j),
+                    V3 = vmul(G.g0, U2, j);
+        const float a = w1 * V1 + w2 * V2 + w3 * V3;
+        const float bq = w2 * V1 + w4 * V2 + w5 * V3;
+        const float c = w3 * V1 + w5 * V2 + w6 * V3;
+        ls = ls + G.g0i * a;
+        lp = lp + G.g0i * bq;
+        lz = lz + G.g0i * c;
+      }
+      acc[0] = ls;
+      acc[1] = lp;
+      acc[2] = lz;
     }
-  }
 
-  if (valid) {
-    A.force[slot] = acc[0];
-    if (SRC != AXH_MONOPOLE) A.force[(size_t)A.nslot + slot] = acc[1];
-    A.force[2 * (size_t)A.nslot + slot] = acc[2];
+    if (ANEL) {
+      // sums of the memory variables over the SLS, (c, q) per lane k < 24
+      float *R = &sR[le * 24];
+      if (valid && k < 24) {
+        if (MODE == 0) {
+          float s = 0.0f;
+          for (int jj = 0; jj < A.nsls; jj++) s = s + sMv[le * 24 * kMaxSls + jj * 24 + k];
+          R[k] = s;
+        } else {
+          R[k] = A.rsum[b][(size_t)e * 24 + k];
+        }
+      }
+      const float *cs = A.cg_stage + (size_t)(valid ? e : 0) * (kCgStage * 4);
+      __syncthreads();
+      float *Sa = &sSa[le * 24];
+      if (valid && k < 4) {
+        const int q = k;
+        const float vse = cs[1 * 4 + q], vsx = cs[2 * 4 + q], vze = cs[3 * 4 + q],
+                    vzx = cs[4 * 4 + q];
+        const float r1 = R[0 * 4 + q], r3 = R[2 * 4 + q], r4 = R[3 * 4 + q],
+                    r5 = R[4 * 4 + q], r6 = R[5 * 4 + q];
+        if (SRC == AXH_MONOPOLE) {
+          Sa[0 * 4 + q] = vze * r1 + vse * r5;
+          Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
+          Sa[4 * 4 + q] = vze * r5 + vse * r3;
+          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+        } else if (SRC == AXH_DIPOLE) {
+          Sa[0 * 4 + q] = vze * (r1 - r6) + vse * (r5 - r4);
+          Sa[1 * 4 + q] = vzx * (r1 - r6) + vsx * (r5 - r4);
+          Sa[2 * 4 + q] = vze * (r1 + r6) + vse * (r5 + r4);
+          Sa[3 * 4 + q] = vzx * (r1 + r6) + vsx * (r5 + r4);
+          Sa[4 * 4 + q] = vze * r5 + vse * r3;
+          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+        } else {
+          Sa[0 * 4 + q] = vze * r1 + vse * r5;
+          Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
+          Sa[2 * 4 + q] = vze * r6 + vse * r4;
+          Sa[3 * 4 + q] = vzx * r6 + vsx * r4;
+          Sa[4 * 4 + q] = vze * r5 + vse * r3;
+          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+        }
+      }
+      __syncthreads();
+      if (valid) {
+        // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
+        // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
+        auto spb = [&](int s1) -> float {
+          if (j == 1) return G.g[i + 5] * Sa[s1 * 4 + 0] + G.g[i + 15] * Sa[s1 * 4 + 2];
+          if (j == 3) return G.g[i + 5] * Sa[s1 * 4 + 1] + G.g[i + 15] * Sa[s1 * 4 + 3];
+          return 0.0f;
+        };
+        auto spa = [&](int s2) -> float {
+          if (i == 1) return Sa[s2 * 4 + 0] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 1] * G.g2T[3 + 5 * j];
+          if (i == 3) return Sa[s2 * 4 + 2] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 3] * G.g2T[3 + 5 * j];
+          return 0.0f;
+        };
+        const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
+        const int q = cgp ? ((i == 1 ? 0 : 2) + (j == 1 ? 0 : 1)) : 0;
+        const float yl = cs[0 * 4 + q];
+        const float r2 = R[1 * 4 + q], r4 = R[3 * 4 + q], r6 = R[5 * 4 + q];
+        if (SRC == AXH_MONOPOLE) {
+          float l1 = spb(0) + spa(1);
+          if (cgp) l1 = l1 + yl * r2;
+          const float l3 = spb(4) + spa(5);
+          acc[0] = acc[0] - l1;
+          acc[2] = acc[2] - l3;
+        } else if (SRC == AXH_DIPOLE) {
+          const float lp = spb(0) + spa(1);
+          float lm = spb(2) + spa(3);
+          float lz = spb(4) + spa(5);
+          if (cgp) {
+            lm = lm + 2.0f * yl * (r2 - r6);
+            lz = lz - yl * r4;
+          }
+          acc[0] = acc[0] - lp;
+          acc[1] = acc[1] - lm;
+          acc[2] = acc[2] - lz;
+        } else {
+          float ls = spb(0) + spa(1);
+          float lp = -spb(2) - spa(3);
+          float lz = spb(4) + spa(5);
+          if (cgp) {
+            ls = ls + yl * (r2 - 2.0f * r6);
+            lp = lp + yl * (r6 - 2.0f * r2);
+            lz = lz - 2.0f * yl * r4;
+          }
+          acc[0] = acc[0] - ls;
+          acc[1] = acc[1] - lp;
+          acc[2] = acc[2] - lz;
+        }
+      }
+      if (MODE == 0) {
+        // time_step_memvars_cg4 with the masked predicted displacement
+        float *src = &sSrc[le * 28], *old = &sOld[le * 28];
+        if (valid && k < 4)
+          memvar_prepare<SRC>(A, b, e, k, axial, &sU[0][le * 25],
+                              &sU[NCOMP == 2 ? 0 : 1][le * 25], &sU[NCOMP - 1][le * 25],
+                              src, old);
+        __syncthreads();
+        if (valid)
+          memvar_update_elem<SRC>(A, b, e, k, &sMv[le * 24 * kMaxSls], src, old, nullptr);
+      }
+    }
+
+    if (valid) {
+      float *force = A.force[b];
+      force[slot] = acc[0];
+      if (SRC != AXH_MONOPOLE) force[(size_t)A.nslot + slot] = acc[1];
+      force[2 * (size_t)A.nslot + slot] = acc[2];
+    }
   }
 }
 
@@ -705,42 +776,34 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   const bool valid = (t < 250) && (e < A.nel);
   const size_t slot = (size_t)tile * 256 + t;
   const int n = 24 * A.nsls;
-  if (valid) {
-    const int g = A.gid[slot];
-    if (SRC == AXH_MONOPOLE) {
-      sU[0][t] = A.disp[g];
-      sU[1][t] = A.disp[2 * (size_t)A.nglob + g];
-    } else {
-      for (int c = 0; c < 3; c++) sU[c][t] = A.disp[c * (size_t)A.nglob + g];
-    }
-    for (int idx = k; idx < n; idx += 25)
-      sMv[le * 24 * kMaxSls + idx] = A.memvar[(size_t)e * n + idx];
-  }
-  __syncthreads();
+  const int gp = valid ? A.gid[slot] : 0;
   const bool axial = valid && A.einfo[e] >= 0;
-  const float *cg = A.cg4 + (size_t)(valid ? e : 0) * (kCg4Terms * 4);
   float *src = &sSrc[le * 28], *old = &sOld[le * 28];
-  if (valid && k < 4) {
-    const int q = k;
-    float g[6], str, sdev[6];
-    strain_cg4<SRC>(A.G, axial, cg, q, &sU[0][le * 25],
-                    &sU[NCOMP == 2 ? 0 : 1][le * 25], &sU[NCOMP - 1][le * 25], g);
-    memvar_sources<SRC>(cg, q, g, str, sdev);
-    old[q] = A.trm1[(size_t)e * 4 + q];
-    for (int c = 0; c < 6; c++) old[4 + 4 * c + q] = A.devm1[(size_t)e * 24 + 4 * c + q];
-    src[q] = str;
-    for (int c = 0; c < 6; c++) src[4 + 4 * c + q] = sdev[c];
-    A.trm1[(size_t)e * 4 + q] = str;
-    for (int c = 0; c < 6; c++) A.devm1[(size_t)e * 24 + 4 * c + q] = sdev[c];
-  }
-  __syncthreads();
   float *mv = &sMv[le * 24 * kMaxSls];
-  if (valid) memvar_update_elem<SRC>(A, e, k, mv, src, old, mv);
-  __syncthreads();
-  if (valid && k < 24) {
-    float s = 0.0f;
-    for (int jj = 0; jj < A.nsls; jj++) s = s + mv[jj * 24 + k];
-    A.rsum[(size_t)e * 24 + k] = s;
+  for (int b = 0; b < A.nb; b++) {
+    __syncthreads();  // LDS of the previous wavefield consumed
+    if (valid) {
+      const float *disp = A.disp[b];
+      if (SRC == AXH_MONOPOLE) {
+        sU[0][t] = disp[gp];
+        sU[1][t] = disp[2 * (size_t)A.nglob + gp];
+      } else {
+        for (int c = 0; c < 3; c++) sU[c][t] = disp[c * (size_t)A.nglob + gp];
+      }
+      for (int idx = k; idx < n; idx += 25) mv[idx] = A.memvar[b][(size_t)e * n + idx];
+    }
+    __syncthreads();
+    if (valid && k < 4)
+      memvar_prepare<SRC>(A, b, e, k, axial, &sU[0][le * 25], &sU[NCOMP == 2 ? 0 : 1][le * 25],
+                          &sU[NCOMP - 1][le * 25], src, old);
+    __syncthreads();
+    if (valid) memvar_update_elem<SRC>(A, b, e, k, mv, src, old, mv);
+    __syncthreads();
+    if (valid && k < 24) {
+      float s = 0.0f;
+      for (int jj = 0; jj < A.nsls; jj++) s = s + mv[jj * 24 + k];
+      A.rsum[b][(size_t)e * 24 + k] = s;
+    }
   }
 }
 
@@ -760,240 +823,273 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
+  const int gp = valid ? A.gid[slot] : 0;
   stage_g(A.G, sG);
-  sC[t] = valid ? A.chi[A.gid[slot]] : 0.0f;
-  __syncthreads();
-  GRegs G;
-  load_g(sG, axial, i, j, G);
   const int nc = A.multipole ? kNcFluidMulti : kNcFluidMono;
   const float *coef = A.coef + (size_t)tile * nc * 256 + t;
-  const float *C = &sC[le * 25];
-  const float c0 = C[k];
-  const float X1 = amul(G.gT, C, i, j), X2 = mulb(C, G.g2, i, j);
   const float m1 = coef[0], m2 = coef[256], m4 = coef[512];
-  sS[0][t] = m1 * X2 + m2 * X1;
-  sS[1][t] = m1 * X1 + m4 * X2;
-  __syncthreads();
-  const float Y1 = amul(G.g, &sS[0][le * 25], i, j);
-  const float Y2 = mulb(&sS[1][le * 25], G.g2T, i, j);
-  float loc = Y1 + Y2;
-  if (A.multipole) {
-    loc = loc + coef[768] * c0;
-    if (axial) {
-      const float w = A.m0[(size_t)ax * 5 + j];
-      const float V1 = vmul(G.g0, C, j);
-      loc = loc + G.g0i * (w * V1);
+  const float mw = A.multipole ? coef[768] : 0.0f;
+  const float w = (A.multipole && axial) ? A.m0[(size_t)ax * 5 + j] : 0.0f;
+  for (int b = 0; b < A.nb; b++) {
+    __syncthreads();  // LDS of the previous wavefield consumed; sG staged
+    sC[t] = valid ? A.chi[b][gp] : 0.0f;
+    __syncthreads();
+    GRegs G;
+    load_g(sG, axial, i, j, G);
+    const float *C = &sC[le * 25];
+    const float c0 = C[k];
+    const float X1 = amul(G.gT, C, i, j), X2 = mulb(C, G.g2, i, j);
+    sS[0][t] = m1 * X2 + m2 * X1;
+    sS[1][t] = m1 * X1 + m4 * X2;
+    __syncthreads();
+    const float Y1 = amul(G.g, &sS[0][le * 25], i, j);
+    const float Y2 = mulb(&sS[1][le * 25], G.g2T, i, j);
+    float loc = Y1 + Y2;
+    if (A.multipole) {
+      loc = loc + mw * c0;
+      if (axial) {
+        const float V1 = vmul(G.g0, C, j);
+        loc = loc + G.g0i * (w * V1);
+      }
     }
-  }
-  if (valid) {
-    if (A.fluid_src && A.stf != 0.0f) {
-      const int ks = A.srcel[e];
-      if (ks >= 0) loc = loc - A.src_fl[ks * 25 + k] * A.stf;
+    if (valid) {
+      const float stf = A.stf[b];
+      if (stf != 0.0f) {
+        const int ks = A.srcel[b][e];
+        if (ks >= 0) loc = loc - A.src_fl[b][ks * 25 + k] * stf;
+      }
+      A.force[b][slot] = loc;
     }
-    A.force[slot] = loc;
   }
 }
 
 // ---------------------------------------------------------------------------
 // predictor / drift per global point (SOLVER/time_evol_wave.F90:357-367,
 // 597-605, 720-725) with the axis masks (SOLVER/apply_masks.f90) and the
-// blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042)
+// blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042).
+// blockIdx.y = batch member.
 // ---------------------------------------------------------------------------
 struct PredArgs {
-  int n, ncomp_mask0, ncomp_mask1;  // masked component range [m0, m1]
-  int comps;                         // bit c set: component c is stepped
-  int newmark, apply_mask, check_blow, iter;
-  double dt, hdt2, cd, thr;
-  float *u, *v;
-  const float *a0;
+  int n, mask0, mask1;   // masked component range [mask0, mask1]
+  int comps;             // bit c set: component c is stepped
+  int newmark, apply_mask, check_blow, iter, stride, naxb;
+  double dt, hdt2, cd;
   const uint8_t *flags;
-  int stride;
-  int *blow;
-  int naxb;
-  const int *axb_g;   // masked axis points on the solid-fluid boundary
-  float *axb_val;     // their unmasked prediction [naxb][3]
+  const int *axb_g;      // masked axis points on the solid-fluid boundary
+  float *u[kMaxBatch], *v[kMaxBatch];
+  const float *a0[kMaxBatch];
+  int *blow[kMaxBatch];
+  double thr[kMaxBatch];
+  float *axb_val[kMaxBatch];  // unmasked prediction of the axb points [naxb][3]
 };
 
 __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
+  float *U = A.u[b];
+  const float *Vv = A.v[b];
   for (int c = 0; c < 3; c++) {
     if (!(A.comps & (1 << c))) continue;
     const size_t p = (size_t)c * A.stride + g;
-    const float u = A.u[p], v = A.v[p];
-    if (A.check_blow && (fl & kFlagBlow) && (double)fabsf(u) > A.thr) atomicMin(A.blow, A.iter);
+    const float u = U[p], v = Vv[p];
+    if (A.check_blow && (fl & kFlagBlow) && (double)fabsf(u) > A.thr[b])
+      atomicMin(A.blow[b], A.iter);
     float un;
     if (A.newmark)
-      un = (float)((double)u + A.dt * (double)v + A.hdt2 * (double)A.a0[p]);
+      un = (float)((double)u + A.dt * (double)v + A.hdt2 * (double)A.a0[b][p]);
     else
       un = (float)((double)u + (double)v * A.cd);
-    if (A.apply_mask && (fl & kFlagAxis) && c >= A.ncomp_mask0 && c <= A.ncomp_mask1) {
-      if (fl & kFlagAxisBdry) {
-        for (int q = 0; q < A.naxb; q++)
-          if (A.axb_g[q] == g) A.axb_val[3 * q + c] = un;
-      }
-      un = 0.0f;
-    } else if (fl & kFlagAxisBdry) {
+    if (fl & kFlagAxisBdry) {
       for (int q = 0; q < A.naxb; q++)
-        if (A.axb_g[q] == g) A.axb_val[3 * q + c] = un;
+        if (A.axb_g[q] == g) A.axb_val[b][3 * q + c] = un;
     }
-    A.u[p] = un;
+    if (A.apply_mask && (fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) un = 0.0f;
+    U[p] = un;
   }
 }
 
 // ---------------------------------------------------------------------------
-// assembly + mass + update, per global point
+// assembly + mass + update, per global point, for every batch member
 // ---------------------------------------------------------------------------
 struct AsmSolidArgs {
-  int n, src, newmark, nslot, comps, mask0, mask1;
+  int n, src, newmark, nslot, comps, mask0, mask1, nb;
   const int *gstart, *gslot;
   const uint8_t *flags;
-  const float *force, *minv;
-  float *velo, *acc0;
+  const float *minv;
   double hdt, cv;
-  // source
-  int nsrc;
-  const int *src_g;
-  const float *src_val;
-  float stf;
+  const float *force[kMaxBatch];
+  float *velo[kMaxBatch], *acc0[kMaxBatch];
+  int nsrc[kMaxBatch];
+  const int *src_g[kMaxBatch];
+  const float *src_val[kMaxBatch];
+  float stf[kMaxBatch];
 };
 
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
-  const int b = A.gstart[g], en = A.gstart[g + 1];
+  const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
-  int si = -1;
-  if ((fl & kFlagSource) && A.stf != 0.0f) {
-    for (int s = 0; s < A.nsrc; s++)
-      if (A.src_g[s] == g) { si = s; break; }
-  }
-  const int order[3] = {0, 2, 1};
-  for (int kk = 0; kk < 3; kk++) {
-    const int c = order[kk];
-    if (!(A.comps & (1 << c))) continue;
-    float s = 0.0f;
-    if ((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
-      s = 0.0f;  // masked copies are zero before the gather
-    } else {
-      const float *f = A.force + (size_t)c * A.nslot;
-      for (int q = b; q < en; q++) s = s + f[A.gslot[q]];
+  for (int b = 0; b < A.nb; b++) {
+    int si = -1;
+    const float stf = A.stf[b];
+    if ((fl & kFlagSource) && stf != 0.0f) {
+      const int *sg = A.src_g[b];
+      for (int s = 0; s < A.nsrc[b]; s++)
+        if (sg[s] == g) { si = s; break; }
     }
-    if (si >= 0) s = s - A.src_val[3 * si + c] * A.stf;
-    const size_t p = (size_t)c * A.n + g;
-    if (A.newmark) {
-      float a;
-      if (c == 2 && A.src == AXH_DIPOLE)
-        a = (float)(-2.0 * (double)mi * (double)s);
-      else
-        a = -mi * s;
-      const float a0 = A.acc0[p];
-      A.velo[p] = (float)((double)A.velo[p] + A.hdt * (double)(a0 + a));
-      A.acc0[p] = a;
-    } else {
-      const float a = -mi * s;
-      if (c == 2 && A.src == AXH_DIPOLE)
-        A.velo[p] = (float)((double)A.velo[p] + 2.0 * (double)a * A.cv);
-      else
-        A.velo[p] = (float)((double)A.velo[p] + (double)a * A.cv);
+    const int order[3] = {0, 2, 1};
+    for (int kk = 0; kk < 3; kk++) {
+      const int c = order[kk];
+      if (!(A.comps & (1 << c))) continue;
+      float s = 0.0f;
+      if ((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
+        s = 0.0f;  // masked copies are zero before the gather
+      } else {
+        const float *f = A.force[b] + (size_t)c * A.nslot;
+        for (int q = q0; q < q1; q++) s = s + f[A.gslot[q]];
+      }
+      if (si >= 0) s = s - A.src_val[b][3 * si + c] * stf;
+      const size_t p = (size_t)c * A.n + g;
+      float *velo = A.velo[b];
+      if (A.newmark) {
+        float a;
+        if (c == 2 && A.src == AXH_DIPOLE)
+          a = (float)(-2.0 * (double)mi * (double)s);
+        else
+          a = -mi * s;
+        float *acc0 = A.acc0[b];
+        const float a0 = acc0[p];
+        velo[p] = (float)((double)velo[p] + A.hdt * (double)(a0 + a));
+        acc0[p] = a;
+      } else {
+        const float a = -mi * s;
+        if (c == 2 && A.src == AXH_DIPOLE)
+          velo[p] = (float)((double)velo[p] + 2.0 * (double)a * A.cv);
+        else
+          velo[p] = (float)((double)velo[p] + (double)a * A.cv);
+      }
     }
   }
 }
 
 struct AsmFluidArgs {
-  int n, newmark, mask_axis, mask_surface;
+  int n, newmark, mask_axis, mask_surface, nb;
   const int *gstart, *gslot;
   const uint8_t *flags;
-  const float *force, *minv;
-  float *dchi, *ddchi;
+  const float *minv;
   double hdt, cv;
+  const float *force[kMaxBatch];
+  float *dchi[kMaxBatch], *ddchi[kMaxBatch];
 };
 
 __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
-  float s = 0.0f;
   const bool masked = (A.mask_axis && (fl & kFlagAxis)) ||
                       (A.mask_surface && (fl & kFlagSurface));
-  if (!masked)
-    for (int q = A.gstart[g]; q < A.gstart[g + 1]; q++) s = s + A.force[A.gslot[q]];
+  const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
-  if (A.newmark) {
-    const float d1 = -mi * s;
-    const float d0 = A.ddchi[g];
-    A.dchi[g] = (float)((double)A.dchi[g] + A.hdt * (double)(d0 + d1));
-    A.ddchi[g] = d1;
-  } else {
-    const float d = -s * mi;
-    A.ddchi[g] = d;
-    A.dchi[g] = (float)((double)A.dchi[g] + A.cv * (double)d);
+  for (int b = 0; b < A.nb; b++) {
+    float s = 0.0f;
+    if (!masked) {
+      const float *f = A.force[b];
+      for (int q = q0; q < q1; q++) s = s + f[A.gslot[q]];
+    }
+    float *dchi = A.dchi[b], *ddchi = A.ddchi[b];
+    if (A.newmark) {
+      const float d1 = -mi * s;
+      const float d0 = ddchi[g];
+      dchi[g] = (float)((double)dchi[g] + A.hdt * (double)(d0 + d1));
+      ddchi[g] = d1;
+    } else {
+      const float d = -s * mi;
+      ddchi[g] = d;
+      dchi[g] = (float)((double)dchi[g] + A.cv * (double)d);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // solid-fluid coupling (SOLVER/time_evol_wave.F90:1532-1611), one lane per
-// boundary GLL point
+// boundary GLL point, blockIdx.y = batch member
 // ---------------------------------------------------------------------------
 struct CoupleArgs {
   int n, dipole, nslot_s, nglob_s;
   const int *ss, *gs, *sf, *gf;
   const float *bm;     // [n][2]
-  const float *disp;   // [3][nglob_s]
-  const float *ddchi;  // [nglob_f]
-  float *force_f;      // [nslot_f]
-  float *force_s;      // [3][nslot_s]
   const int *axb;      // side-buffer slot per boundary point or -1
-  const float *axb_val;
+  const float *disp[kMaxBatch];     // [3][nglob_s]
+  const float *ddchi[kMaxBatch];    // [nglob_f]
+  float *force_f[kMaxBatch];        // [nslot_f]
+  float *force_s[kMaxBatch];        // [3][nslot_s]
+  const float *axb_val[kMaxBatch];
 };
 
-__global__ void k_copy2fluid(CoupleArgs A) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= A.n) return;
-  const int g = A.gs[b], k = A.axb[b];
+__global__ __launch_bounds__(256) void k_copy2fluid(CoupleArgs A) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (p >= A.n) return;
+  const int g = A.gs[p], k = A.axb[p];
   float s1, s2, s3;
   if (k >= 0) {  // prediction before the axis mask
-    s1 = A.axb_val[3 * k];
-    s2 = A.axb_val[3 * k + 1];
-    s3 = A.axb_val[3 * k + 2];
+    const float *av = A.axb_val[b];
+    s1 = av[3 * k];
+    s2 = av[3 * k + 1];
+    s3 = av[3 * k + 2];
   } else {
-    s1 = A.disp[g];
-    s2 = A.disp[(size_t)A.nglob_s + g];
-    s3 = A.disp[2 * (size_t)A.nglob_s + g];
+    const float *disp = A.disp[b];
+    s1 = disp[g];
+    s2 = disp[(size_t)A.nglob_s + g];
+    s3 = disp[2 * (size_t)A.nglob_s + g];
   }
-  const float bm1 = A.bm[2 * b], bm2 = A.bm[2 * b + 1];
-  float f = A.force_f[A.sf[b]];
+  const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
+  float *ff = A.force_f[b];
+  float f = ff[A.sf[p]];
   if (A.dipole)
     f = f - bm1 * (s1 + s2) - bm2 * s3;
   else
     f = f - bm1 * s1 - bm2 * s3;
-  A.force_f[A.sf[b]] = f;
+  ff[A.sf[p]] = f;
 }
 
-__global__ void k_copy2solid(CoupleArgs A) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= A.n) return;
-  const float fl = A.ddchi[A.gf[b]];
-  const float bm1 = A.bm[2 * b], bm2 = A.bm[2 * b + 1];
-  const size_t s = A.ss[b];
-  A.force_s[s] = A.force_s[s] + bm1 * fl;
-  if (A.dipole)
-    A.force_s[(size_t)A.nslot_s + s] = A.force_s[(size_t)A.nslot_s + s] + bm1 * fl;
-  A.force_s[2 * (size_t)A.nslot_s + s] = A.force_s[2 * (size_t)A.nslot_s + s] + bm2 * fl;
+__global__ __launch_bounds__(256) void k_copy2solid(CoupleArgs A) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (p >= A.n) return;
+  const float fl = A.ddchi[b][A.gf[p]];
+  const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
+  const size_t s = A.ss[p];
+  float *fs = A.force_s[b];
+  fs[s] = fs[s] + bm1 * fl;
+  if (A.dipole) fs[(size_t)A.nslot_s + s] = fs[(size_t)A.nslot_s + s] + bm1 * fl;
+  fs[2 * (size_t)A.nslot_s + s] = fs[2 * (size_t)A.nslot_s + s] + bm2 * fl;
 }
 
-// receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742)
-__global__ void k_receivers(int n, int src, const int *rec_g, const float *disp,
-                            int nglob, float *out) {
+// receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
+// blockIdx.y = batch member; members without a sample this step have out NULL
+struct RecArgs {
+  int src, nglob;
+  int nrec[kMaxBatch];
+  const int *rec_g[kMaxBatch];
+  const float *disp[kMaxBatch];
+  float *out[kMaxBatch];
+};
+
+__global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
   const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  const int g = rec_g[r];
-  const float d1 = disp[g], d2 = disp[(size_t)nglob + g], d3 = disp[2 * (size_t)nglob + g];
-  float *o = out + 3 * (size_t)r;
-  if (src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
-  else if (src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
+  const int b = blockIdx.y;
+  if (!A.out[b] || r >= A.nrec[b]) return;
+  const int g = A.rec_g[b][r];
+  const float *disp = A.disp[b];
+  const float d1 = disp[g], d2 = disp[(size_t)A.nglob + g], d3 = disp[2 * (size_t)A.nglob + g];
+  float *o = A.out[b] + 3 * (size_t)r;
+  if (A.src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
+  else if (A.src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
   else { o[0] = d1; o[1] = d2; o[2] = d3; }
 }
 
@@ -1008,7 +1104,6 @@ struct Numbering {
   std::vector<int> gid;      // per slot, -1 for padding
   std::vector<int> gstart;   // CSR over global points
   std::vector<int> gslot;
-  std::vector<int> first;    // per global point: first (element, k) as e*25+k
   int nglob = 0;
 };
 
@@ -1031,7 +1126,6 @@ static Numbering build_numbering(const int32_t *igloc, int nel, int nglob_ref,
       if (m < 0) {
         m = N.nglob++;
         count.push_back(0);
-        N.first.push_back(e * kNpp + k);
       }
       count[m]++;
       N.gid[slot_of(e, k)] = m;
@@ -1085,24 +1179,30 @@ static void pack_terms(const axh_ctx *ctx, const std::vector<std::string> &names
   }
 }
 
-// Host-side layout of one context, computed without touching the GPU: global
-// numbering and gather lists, per-point values, flags, packed terms, coupling
-// and receiver maps.  Every consistency requirement of the per-point storage
-// is checked here.
-struct HostLayout {
+// Host-side layout of the mesh part, computed without touching the GPU:
+// global numbering and gather lists, per-point values, flags, packed terms,
+// coupling maps.  Every consistency requirement of the per-point storage is
+// checked here.
+struct MeshHost {
   int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0, nglob_s = 0, nglob_f = 0;
-  int nstages = 1, nsrcpts = 0, naxb = 0;
+  int nstages = 1, naxb = 0;
   double coefd[16] = {0}, coefv[16] = {0};
-  std::vector<float> G, coef_s, m0_s, minv_s, src_val, coef_f, m0_f, minv_f, src_fl, cpl_bm,
-      cg4, stf;
-  std::vector<int> einfo_s, gid_s, gstart_s, gslot_s, src_g, einfo_f, gid_f, gstart_f,
-      gslot_f, srcel_f, cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, rec_g;
+  std::vector<float> G, coef_s, m0_s, minv_s, coef_f, m0_f, minv_f, cpl_bm, cg_stage, cg_upd;
+  std::vector<int> einfo_s, gid_s, gstart_s, gslot_s, einfo_f, gid_f, gstart_f, gslot_f,
+      cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g;
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
 };
 
-static void build_layout(const axh_ctx *ctx, HostLayout &H) {
+// Host-side layout of the wavefield part: source points, fluid source,
+// receivers, source time function.
+struct WaveHost {
+  std::vector<int> src_g, srcel_f, rec_g;
+  std::vector<float> src_val, src_fl, stf;
+};
+
+static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
   const axh_dims &d = ctx->dims;
   const int src = d.src_type;
   const bool multipole = src != AXH_MONOPOLE;
@@ -1142,42 +1242,6 @@ static void build_layout(const axh_ctx *ctx, HostLayout &H) {
     if ((H.flags_s[g] & kFlagAxis) && nonaxis_copy[g])
       fail(AXH_ERR_UNSUPPORTED, "axis point shared with a non-axial copy");
 
-  // source terms (solid), added after assembly to every copy in a source
-  // element (SOLVER/time_evol_wave.F90:1082-1097): copies must agree
-  if (!d.fluid_src && d.nelsrc > 0) {
-    const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
-    const float *st = ctx->need("source_term").as<float>();
-    std::vector<int> srcidx(H.nel_s, -1);
-    for (int kk = 0; kk < d.nelsrc; kk++) {
-      if (iels[kk] < 1 || iels[kk] > H.nel_s) fail(AXH_ERR_ARG, "ielsrc out of range");
-      srcidx[iels[kk] - 1] = kk;
-    }
-    std::vector<float> val(3 * (size_t)NS.nglob, 0.0f);
-    std::vector<char> seen(NS.nglob, 0);
-    for (int e = 0; e < H.nel_s; e++)
-      for (int k = 0; k < kNpp; k++) {
-        int g = NS.gid[slot_of(e, k)];
-        float x[3];
-        for (int c = 0; c < 3; c++)
-          x[c] = srcidx[e] >= 0 ? st[k + kNpp * ((size_t)srcidx[e] + 8 * c)] : 0.0f;
-        if (!seen[g]) {
-          seen[g] = 1;
-          for (int c = 0; c < 3; c++) val[3 * g + c] = x[c];
-        } else {
-          for (int c = 0; c < 3; c++)
-            if (val[3 * g + c] != x[c])
-              fail(AXH_ERR_UNSUPPORTED, "source term differs between copies of a point");
-        }
-      }
-    for (int g = 0; g < NS.nglob; g++)
-      if (val[3 * g] != 0.0f || val[3 * g + 1] != 0.0f || val[3 * g + 2] != 0.0f) {
-        H.flags_s[g] |= kFlagSource;
-        H.src_g.push_back(g);
-        for (int c = 0; c < 3; c++) H.src_val.push_back(val[3 * g + c]);
-      }
-  }
-  H.nsrcpts = (int)H.src_g.size();
-
   // packed stiffness terms
   std::vector<std::string> names;
   auto S = [&](int term) { names.push_back(kSolid3dNames[term]); };
@@ -1201,14 +1265,21 @@ static void build_layout(const axh_ctx *ctx, HostLayout &H) {
   }
   pack_terms(ctx, names, H.nel_s, H.coef_s);
 
-  // axial terms M0_w1..10
+  // axial terms M0_w1..10 of the axial elements, the ones the source type
+  // reads (monopole w1-3, dipole w1-4 and w6-10, quadrupole w1-6); the rest
+  // stays zero
+  auto m0_used = [&](int w) {
+    if (src == AXH_MONOPOLE) return w <= 3;
+    if (src == AXH_DIPOLE) return w != 5;
+    return w <= 6;
+  };
   H.einfo_s.assign(H.nel_s, -1);
   int nax = 0;
   for (int e = 0; e < H.nel_s; e++)
     if (axis_s[e]) {
       H.einfo_s[e] = nax++;
       for (int w = 1; w <= 10; w++) {
-        const Array *a = ctx->get("M0_w" + std::to_string(w));
+        const Array *a = m0_used(w) ? ctx->get("M0_w" + std::to_string(w)) : nullptr;
         for (int j = 0; j < 5; j++)
           H.m0_s.push_back(a ? a->as<float>()[(size_t)e * 5 + j] : 0.0f);
       }
@@ -1250,19 +1321,10 @@ static void build_layout(const axh_ctx *ctx, HostLayout &H) {
     for (int e = 0; e < H.nel_f; e++)
       if (axis_f[e]) {
         H.einfo_f[e] = naxf++;
-        for (int j = 0; j < 5; j++) H.m0_f.push_back(m0a ? m0a->as<float>()[(size_t)e * 5 + j] : 0.0f);
+        for (int j = 0; j < 5; j++)
+          H.m0_f.push_back(m0a ? m0a->as<float>()[(size_t)e * 5 + j] : 0.0f);
       }
     if (multipole && naxf > 0 && !m0a) fail(AXH_ERR_STATE, "missing upload: M0_w_fl");
-    H.srcel_f.assign(H.nel_f, -1);
-    H.src_fl.assign(25 * 8, 0.0f);
-    if (d.fluid_src && d.nelsrc > 0) {
-      const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
-      for (int kk = 0; kk < d.nelsrc; kk++) {
-        if (iels[kk] < 1 || iels[kk] > H.nel_f) fail(AXH_ERR_ARG, "ielsrc out of range");
-        H.srcel_f[iels[kk] - 1] = kk;
-      }
-      std::memcpy(H.src_fl.data(), ctx->need("source_term").as<float>(), 25 * 8 * 4);
-    }
     H.gid_f = std::move(NF.gid);
     H.gstart_f = std::move(NF.gstart);
     H.gslot_f = std::move(NF.gslot);
@@ -1315,16 +1377,24 @@ static void build_layout(const axh_ctx *ctx, HostLayout &H) {
 
   // ---- attenuation ----
   if (d.anel) {
-    const char *cgn[kCg4Terms] = {"Y_cg4", "V_s_eta_cg4", "V_s_xi_cg4", "V_z_eta_cg4",
-                                  "V_z_xi_cg4", "DsDeta_over_J_sol_cg4",
-                                  "DzDeta_over_J_sol_cg4", "DsDxi_over_J_sol_cg4",
-                                  "DzDxi_over_J_sol_cg4", "inv_s_solid_cg4", "delta_mu_cg4",
-                                  "delta_kappa_cg4"};
-    H.cg4.resize((size_t)H.nel_s * kCg4Terms * 4);
-    for (int a = 0; a < kCg4Terms; a++) {
-      const float *p = ctx->need(cgn[a]).as<float>();
+    const char *stage_names[kCgStage] = {"Y_cg4", "V_s_eta_cg4", "V_s_xi_cg4", "V_z_eta_cg4",
+                                         "V_z_xi_cg4"};
+    const char *upd_names[kCgUpd - 1] = {"DsDeta_over_J_sol_cg4", "DzDeta_over_J_sol_cg4",
+                                         "DsDxi_over_J_sol_cg4", "DzDxi_over_J_sol_cg4",
+                                         "inv_s_solid_cg4", "delta_mu_cg4", "delta_kappa_cg4"};
+    H.cg_stage.assign((size_t)H.nel_s * kCgStage * 4, 0.0f);
+    H.cg_upd.assign((size_t)H.nel_s * kCgUpd * 4, 0.0f);
+    for (int a = 0; a < kCgStage; a++) {
+      const float *p = ctx->need(stage_names[a]).as<float>();
       for (int e = 0; e < H.nel_s; e++)
-        for (int q = 0; q < 4; q++) H.cg4[((size_t)e * kCg4Terms + a) * 4 + q] = p[(size_t)e * 4 + q];
+        for (int q = 0; q < 4; q++)
+          H.cg_stage[((size_t)e * kCgStage + a) * 4 + q] = p[(size_t)e * 4 + q];
+    }
+    for (int a = 0; a < kCgUpd - 1; a++) {
+      const float *p = ctx->need(upd_names[a]).as<float>();
+      for (int e = 0; e < H.nel_s; e++)
+        for (int q = 0; q < 4; q++)
+          H.cg_upd[((size_t)e * kCgUpd + a) * 4 + q] = p[(size_t)e * 4 + q];
     }
     const double *ew = ctx->need("exp_w_j_deltat").as<double>();
     const double *tt = ctx->need("ts_fac_t").as<double>();
@@ -1338,25 +1408,151 @@ static void build_layout(const axh_ctx *ctx, HostLayout &H) {
     H.aj_mu.assign(amu.as<double>(), amu.as<double>() + amu.count());
     H.aj_k.assign(ak.as<double>(), ak.as<double>() + ak.count());
   }
+}
 
+static void build_wave_layout(const axh_ctx *ctx, const std::vector<int> &gid_s, int nglob_s,
+                              WaveHost &W) {
+  const axh_dims &d = ctx->dims;
+  const int nel_s = d.nel_solid, nel_f = d.nel_fluid;
+  // source terms (solid), added after assembly to every copy in a source
+  // element (SOLVER/time_evol_wave.F90:1082-1097): copies must agree
+  if (!d.fluid_src && d.nelsrc > 0) {
+    const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
+    const float *st = ctx->need("source_term").as<float>();
+    std::vector<int> srcidx(nel_s, -1);
+    for (int kk = 0; kk < d.nelsrc; kk++) {
+      if (iels[kk] < 1 || iels[kk] > nel_s) fail(AXH_ERR_ARG, "ielsrc out of range");
+      srcidx[iels[kk] - 1] = kk;
+    }
+    std::vector<int> at(nglob_s, -1);  // global point -> index in pts
+    std::vector<int> pts;
+    std::vector<float> val;
+    auto nonzero = [&](int q) {
+      return val[3 * q] != 0.0f || val[3 * q + 1] != 0.0f || val[3 * q + 2] != 0.0f;
+    };
+    for (int e = 0; e < nel_s; e++) {
+      if (srcidx[e] < 0) continue;
+      for (int k = 0; k < kNpp; k++) {
+        const int g = gid_s[slot_of(e, k)];
+        float x[3];
+        for (int c = 0; c < 3; c++) x[c] = st[k + kNpp * ((size_t)srcidx[e] + 8 * c)];
+        if (at[g] < 0) {
+          at[g] = (int)pts.size();
+          pts.push_back(g);
+          for (int c = 0; c < 3; c++) val.push_back(x[c]);
+        } else {
+          for (int c = 0; c < 3; c++)
+            if (val[3 * at[g] + c] != x[c])
+              fail(AXH_ERR_UNSUPPORTED, "source term differs between copies of a point");
+        }
+      }
+    }
+    // a copy in an element without source holds zero
+    for (int e = 0; e < nel_s; e++) {
+      if (srcidx[e] >= 0) continue;
+      for (int k = 0; k < kNpp; k++) {
+        const int q = at[gid_s[slot_of(e, k)]];
+        if (q >= 0 && nonzero(q))
+          fail(AXH_ERR_UNSUPPORTED, "source term differs between copies of a point");
+      }
+    }
+    // the points with a nonzero term, in ascending global index
+    for (int g = 0; g < nglob_s; g++) {
+      const int q = at[g];
+      if (q >= 0 && nonzero(q)) {
+        W.src_g.push_back(g);
+        for (int c = 0; c < 3; c++) W.src_val.push_back(val[3 * q + c]);
+      }
+    }
+  }
+  if (nel_f > 0) {
+    W.srcel_f.assign(nel_f, -1);
+    W.src_fl.assign(25 * 8, 0.0f);
+    if (d.fluid_src && d.nelsrc > 0) {
+      const int32_t *iels = ctx->need("ielsrc").as<int32_t>();
+      for (int kk = 0; kk < d.nelsrc; kk++) {
+        if (iels[kk] < 1 || iels[kk] > nel_f) fail(AXH_ERR_ARG, "ielsrc out of range");
+        W.srcel_f[iels[kk] - 1] = kk;
+      }
+      std::memcpy(W.src_fl.data(), ctx->need("source_term").as<float>(), 25 * 8 * 4);
+    }
+  }
   // ---- receivers ----
   if (d.num_rec > 0) {
     const int32_t *rf = ctx->need("recfile_el").as<int32_t>();
-    H.rec_g.resize(d.num_rec);
+    W.rec_g.resize(d.num_rec);
     for (int r = 0; r < d.num_rec; r++) {
       int e = rf[r] - 1, i = rf[r + d.num_rec], j = rf[r + 2 * d.num_rec];
-      if (e < 0 || e >= H.nel_s || i < 0 || i > 4 || j < 0 || j > 4)
+      if (e < 0 || e >= nel_s || i < 0 || i > 4 || j < 0 || j > 4)
         fail(AXH_ERR_ARG, "receiver index out of range");
-      H.rec_g[r] = H.gid_s[slot_of(e, i + 5 * j)];
+      W.rec_g[r] = gid_s[slot_of(e, i + 5 * j)];
     }
   }
   const Array &stf = ctx->need("stf");
-  H.stf.assign(stf.as<float>(), stf.as<float>() + stf.count());
+  W.stf.assign(stf.as<float>(), stf.as<float>() + stf.count());
 }
 
 void check_inputs(const axh_ctx *ctx) {
-  HostLayout H;
-  build_layout(ctx, H);
+  MeshHost H;
+  build_mesh_layout(ctx, H);
+  WaveHost W;
+  build_wave_layout(ctx, H.gid_s, H.nglob_s, W);
+}
+
+// 128-bit fingerprint of the mesh part the device would hold for ctx (every
+// array of MeshHost and its sizes).  Two contexts with equal fingerprints
+// step on the same device mesh, whatever their unused upload entries hold.
+struct FpHash {
+  uint64_t a = 0x9e3779b97f4a7c15ull, b = 0xc2b2ae3d27d4eb4full;
+  void word(uint64_t w) {
+    a = (a ^ w) * 0xff51afd7ed558ccdull;
+    a ^= a >> 32;
+    b = (b + w) * 0xc4ceb9fe1a85ec53ull;
+    b ^= b >> 29;
+  }
+  void bytes(const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    word(n);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, c + i, 8);
+      word(w);
+    }
+    uint64_t t = 0;
+    std::memcpy(&t, c + i, n - i);
+    word(t);
+  }
+  template <class T> void vec(const std::vector<T> &v) { bytes(v.data(), v.size() * sizeof(T)); }
+};
+
+void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]) {
+  MeshHost H;
+  build_mesh_layout(ctx, H);
+  FpHash h;
+  const int ints[] = {ctx->dims.src_type, ctx->dims.scheme, ctx->dims.anel, ctx->dims.n_sls,
+                      ctx->dims.nel_bdry, H.nel_s, H.nel_f, H.nglob_s, H.nglob_f,
+                      H.nstages, H.naxb};
+  h.bytes(ints, sizeof ints);
+  h.bytes(&ctx->dims.deltat, sizeof(double));
+  h.bytes(H.coefd, sizeof H.coefd);
+  h.bytes(H.coefv, sizeof H.coefv);
+  h.bytes(H.expw, sizeof H.expw);
+  h.bytes(H.tst, sizeof H.tst);
+  h.bytes(H.tstm1, sizeof H.tstm1);
+  for (const auto *v : {&H.G, &H.coef_s, &H.m0_s, &H.minv_s, &H.coef_f, &H.m0_f, &H.minv_f,
+                        &H.cpl_bm, &H.cg_stage, &H.cg_upd})
+    h.vec(*v);
+  for (const auto *v : {&H.einfo_s, &H.gid_s, &H.gstart_s, &H.gslot_s, &H.einfo_f, &H.gid_f,
+                        &H.gstart_f, &H.gslot_f, &H.cpl_ss, &H.cpl_gs, &H.cpl_sf, &H.cpl_gf,
+                        &H.cpl_axb, &H.axb_g})
+    h.vec(*v);
+  h.vec(H.flags_s);
+  h.vec(H.flags_f);
+  h.vec(H.aj_mu);
+  h.vec(H.aj_k);
+  fp[0] = h.a;
+  fp[1] = h.b;
 }
 
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob) {
@@ -1370,147 +1566,204 @@ void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *ngl
   *nglob = N.nglob;
 }
 
-void device_build(axh_ctx *ctx) {
+static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   const axh_dims &d = ctx->dims;
-  HostLayout H;
-  build_layout(ctx, H);
-  std::unique_ptr<Device> D(new Device);
-  Device &V = *D;
-  try {
-    V.src = d.src_type;
-    V.scheme = d.scheme;
-    V.anel = d.anel;
-    V.nsls = d.n_sls;
-    V.nel_s = H.nel_s;
-    V.nel_f = H.nel_f;
-    V.ntile_s = H.ntile_s;
-    V.ntile_f = H.ntile_f;
-    V.nbdry = d.nel_bdry;
-    V.nrec = d.num_rec;
-    V.nstages = H.nstages;
-    V.nglob_s = H.nglob_s;
-    V.nglob_f = H.nglob_f;
-    V.nsrcpts = H.nsrcpts;
-    V.naxb = H.naxb;
-    std::memcpy(V.coefd, H.coefd, sizeof V.coefd);
-    std::memcpy(V.coefv, H.coefv, sizeof V.coefv);
-    std::memcpy(V.expw, H.expw, sizeof V.expw);
-    std::memcpy(V.tst, H.tst, sizeof V.tst);
-    std::memcpy(V.tstm1, H.tstm1, sizeof V.tstm1);
-    V.stf = H.stf;
-
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-      fail(AXH_ERR_HIP, "no HIP device available");
-    HIPCHECK(hipStreamCreateWithFlags(&V.stream, hipStreamNonBlocking));
-    HIPCHECK(hipEventCreate(&V.ev0));
-    HIPCHECK(hipEventCreate(&V.ev1));
-
-    V.G = V.upload(H.G);
-    const size_t nslot_s = (size_t)V.ntile_s * kTile;
-    V.coef_s = V.upload(H.coef_s);
-    V.einfo_s = V.upload(H.einfo_s);
-    V.m0_s = V.upload(H.m0_s);
-    V.gid_s = V.upload(H.gid_s);
-    V.gstart_s = V.upload(H.gstart_s);
-    V.gslot_s = V.upload(H.gslot_s);
-    V.minv_s = V.upload(H.minv_s);
-    V.flags_s = V.upload(H.flags_s);
-    V.src_g = V.upload(H.src_g);
-    V.src_val = V.upload(H.src_val);
-    V.disp = V.zeros<float>(3 * (size_t)V.nglob_s);
-    V.velo = V.zeros<float>(3 * (size_t)V.nglob_s);
-    V.acc0 = V.zeros<float>(3 * (size_t)V.nglob_s);
-    V.force_s = V.zeros<float>(3 * nslot_s);
-    if (V.nel_f > 0) {
-      V.coef_f = V.upload(H.coef_f);
-      V.einfo_f = V.upload(H.einfo_f);
-      V.m0_f = V.upload(H.m0_f);
-      V.gid_f = V.upload(H.gid_f);
-      V.gstart_f = V.upload(H.gstart_f);
-      V.gslot_f = V.upload(H.gslot_f);
-      V.minv_f = V.upload(H.minv_f);
-      V.flags_f = V.upload(H.flags_f);
-      V.srcel_f = V.upload(H.srcel_f);
-      V.src_fl = V.upload(H.src_fl);
-      V.chi = V.zeros<float>(V.nglob_f);
-      V.dchi = V.zeros<float>(V.nglob_f);
-      V.ddchi = V.zeros<float>(V.nglob_f);
-      V.force_f = V.zeros<float>((size_t)V.ntile_f * kTile);
-    }
-    if (V.nbdry > 0) {
-      V.cpl_ss = V.upload(H.cpl_ss);
-      V.cpl_gs = V.upload(H.cpl_gs);
-      V.cpl_sf = V.upload(H.cpl_sf);
-      V.cpl_gf = V.upload(H.cpl_gf);
-      V.cpl_axb = V.upload(H.cpl_axb);
-      V.cpl_bm = V.upload(H.cpl_bm);
-      V.axb_g = V.upload(H.axb_g);
-      V.axb_val = V.zeros<float>(3 * (size_t)std::max(V.naxb, 1));
-    }
-    if (V.anel) {
-      V.cg4 = V.upload(H.cg4);
-      V.aj_mu = V.upload(H.aj_mu);
-      V.aj_k = V.upload(H.aj_k);
-      V.memvar = V.zeros<float>((size_t)V.nel_s * 24 * V.nsls);
-      V.rsum = V.zeros<float>((size_t)V.nel_s * 24);
-      V.trm1 = V.zeros<float>((size_t)V.nel_s * 4);
-      V.devm1 = V.zeros<float>((size_t)V.nel_s * 24);
-    }
-    if (V.nrec > 0) V.rec_g = V.upload(H.rec_g);
-    V.blow = V.alloc<int>(1);
-    HIPCHECK(hipDeviceSynchronize());
-  } catch (...) {
-    device_destroy(D.release());
-    throw;
+  MeshHost H;
+  build_mesh_layout(ctx, H);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    fail(AXH_ERR_HIP, "no HIP device available");
+  auto M = std::make_shared<MeshDev>();
+  MeshDev &V = *M;
+  V.src = d.src_type;
+  V.scheme = d.scheme;
+  V.anel = d.anel;
+  V.nsls = d.n_sls;
+  V.nel_s = H.nel_s;
+  V.nel_f = H.nel_f;
+  V.ntile_s = H.ntile_s;
+  V.ntile_f = H.ntile_f;
+  V.nbdry = d.nel_bdry;
+  V.nstages = H.nstages;
+  V.nglob_s = H.nglob_s;
+  V.nglob_f = H.nglob_f;
+  V.naxb = H.naxb;
+  std::memcpy(V.coefd, H.coefd, sizeof V.coefd);
+  std::memcpy(V.coefv, H.coefv, sizeof V.coefv);
+  std::memcpy(V.expw, H.expw, sizeof V.expw);
+  std::memcpy(V.tst, H.tst, sizeof V.tst);
+  std::memcpy(V.tstm1, H.tstm1, sizeof V.tstm1);
+  DevAlloc &m = V.mem;
+  V.G = m.upload(H.G);
+  V.coef_s = m.upload(H.coef_s);
+  V.einfo_s = m.upload(H.einfo_s);
+  V.m0_s = m.upload(H.m0_s);
+  V.gid_s = m.upload(H.gid_s);
+  V.gstart_s = m.upload(H.gstart_s);
+  V.gslot_s = m.upload(H.gslot_s);
+  V.minv_s = m.upload(H.minv_s);
+  V.flags_s = m.upload(H.flags_s);
+  if (V.nel_f > 0) {
+    V.coef_f = m.upload(H.coef_f);
+    V.einfo_f = m.upload(H.einfo_f);
+    V.m0_f = m.upload(H.m0_f);
+    V.gid_f = m.upload(H.gid_f);
+    V.gstart_f = m.upload(H.gstart_f);
+    V.gslot_f = m.upload(H.gslot_f);
+    V.minv_f = m.upload(H.minv_f);
+    V.flags_f = m.upload(H.flags_f);
   }
+  if (V.nbdry > 0) {
+    V.cpl_ss = m.upload(H.cpl_ss);
+    V.cpl_gs = m.upload(H.cpl_gs);
+    V.cpl_sf = m.upload(H.cpl_sf);
+    V.cpl_gf = m.upload(H.cpl_gf);
+    V.cpl_axb = m.upload(H.cpl_axb);
+    V.cpl_bm = m.upload(H.cpl_bm);
+    V.axb_g = m.upload(H.axb_g);
+  }
+  if (V.anel) {
+    V.cg_stage = m.upload(H.cg_stage);
+    V.cg_upd = m.upload(H.cg_upd);
+    V.aj_mu = m.upload(H.aj_mu);
+    V.aj_k = m.upload(H.aj_k);
+  }
+  V.gid_s_h = std::move(H.gid_s);
+  V.gid_f_h = std::move(H.gid_f);
+  V.flags_s_h = std::move(H.flags_s);
+  return M;
+}
+
+// Mark the wavefield's source points in the mesh flags (the flag is the union
+// over the wavefields stepped on the mesh; the assembly looks a flagged point
+// up in its own wavefield's source list).
+static void mesh_add_sources(MeshDev &M, const std::vector<int> &src_g) {
+  bool changed = false;
+  for (int g : src_g)
+    if (!(M.flags_s_h[g] & kFlagSource)) {
+      M.flags_s_h[g] |= kFlagSource;
+      changed = true;
+    }
+  if (changed)
+    HIPCHECK(hipMemcpy(M.flags_s, M.flags_s_h.data(), M.flags_s_h.size(),
+                       hipMemcpyHostToDevice));
+}
+
+static void wave_build(const axh_ctx *ctx, MeshDev &M, WaveDev &W) {
+  const axh_dims &d = ctx->dims;
+  WaveHost H;
+  build_wave_layout(ctx, M.gid_s_h, M.nglob_s, H);
+  DevAlloc &m = W.mem;
+  const size_t nslot_s = (size_t)M.ntile_s * kTile;
+  W.disp = m.zeros<float>(3 * (size_t)M.nglob_s);
+  W.velo = m.zeros<float>(3 * (size_t)M.nglob_s);
+  W.acc0 = m.zeros<float>(3 * (size_t)M.nglob_s);
+  W.force_s = m.zeros<float>(3 * nslot_s);
+  W.nsrc = (int)H.src_g.size();
+  W.src_g = m.upload(H.src_g);
+  W.src_val = m.upload(H.src_val);
+  W.fluid_src = d.fluid_src;
+  if (M.nel_f > 0) {
+    W.chi = m.zeros<float>(M.nglob_f);
+    W.dchi = m.zeros<float>(M.nglob_f);
+    W.ddchi = m.zeros<float>(M.nglob_f);
+    W.force_f = m.zeros<float>((size_t)M.ntile_f * kTile);
+    W.srcel_f = m.upload(H.srcel_f);
+    W.src_fl = m.upload(H.src_fl);
+  }
+  W.axb_val = m.zeros<float>(3 * (size_t)std::max(M.naxb, 1));
+  if (M.anel) {
+    W.memvar = m.zeros<float>((size_t)M.nel_s * 24 * M.nsls);
+    W.rsum = m.zeros<float>((size_t)M.nel_s * 24);
+    W.trm1 = m.zeros<float>((size_t)M.nel_s * 4);
+    W.devm1 = m.zeros<float>((size_t)M.nel_s * 24);
+  }
+  W.nrec = d.num_rec;
+  W.seis_it = d.seis_it;
+  if (W.nrec > 0) W.rec_g = m.upload(H.rec_g);
+  W.blow = m.alloc<int>(1);
+  // blow-up threshold of runtime_info: maxval(abs(disp(1,1,:,:))) >
+  // 10*abs(magnitude), the real(4) maximum promoted to double
+  W.blow_thr = 10.0 * std::fabs(d.magnitude);
+  W.stf = std::move(H.stf);
+  W.src_g_h = std::move(H.src_g);
+  mesh_add_sources(M, W.src_g_h);
+}
+
+void device_build(axh_ctx *ctx) {
+  std::unique_ptr<Device> D(new Device);
+  D->mesh = mesh_build(ctx);
+  wave_build(ctx, *D->mesh, D->wave);
+  HIPCHECK(hipDeviceSynchronize());
   ctx->dev.reset(D.release());
 }
 
-
-void DeviceDeleter::operator()(Device *d) const { device_destroy(d); }
-
-void device_destroy(Device *d) {
-  if (!d) return;
-  // teardown: errors are not actionable here
-  if (d->stream) (void)hipStreamSynchronize(d->stream);
-  for (auto e : d->kev) (void)hipEventDestroy(e);
-  if (d->ev0) (void)hipEventDestroy(d->ev0);
-  if (d->ev1) (void)hipEventDestroy(d->ev1);
-  for (void *p : d->allocs) (void)hipFree(p);
-  if (d->stream) (void)hipStreamDestroy(d->stream);
-  delete d;
+void batch_attach(const std::vector<axh_ctx *> &members) {
+  axh_ctx *first = members[0];
+  if (!first->dev) device_build(first);
+  std::shared_ptr<MeshDev> mesh = first->dev->mesh;
+  for (size_t b = 1; b < members.size(); b++) {
+    axh_ctx *c = members[b];
+    if (!c->dev) {
+      std::unique_ptr<Device> D(new Device);
+      D->mesh = mesh;
+      wave_build(c, *mesh, D->wave);
+      c->dev.reset(D.release());
+    } else if (c->dev->mesh != mesh) {
+      // same mesh uploads (checked by the caller): the global numbering is
+      // the same, so the wavefield part carries over unchanged
+      if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
+      mesh_add_sources(*mesh, c->dev->wave.src_g_h);
+      c->dev->mesh = mesh;
+    }
+  }
+  HIPCHECK(hipDeviceSynchronize());
 }
 
-size_t device_bytes(const Device *d) { return d ? d->bytes : 0; }
+void DeviceDeleter::operator()(Device *d) const { device_destroy(d); }
+void RunnerDeleter::operator()(Runner *r) const { delete r; }
+
+void device_destroy(Device *d) { delete d; }
+
+size_t device_bytes(const Device *d) {
+  return d ? d->mesh->mem.bytes + d->wave.mem.bytes : 0;
+}
 
 const char *kernel_class_name(int k) { return (k >= 0 && k < KC_N) ? kClassName[k] : nullptr; }
 
-bool kernel_stats(const Device *d, int k, double *ms, int64_t *n) {
+static bool runner_stats(const Runner *r, int k, double *ms, int64_t *n) {
   if (k < 0 || k >= KC_N) return false;
-  *ms = d ? d->kms[k] : 0.0;
-  *n = d ? d->kcount[k] : 0;
+  *ms = r ? r->kms[k] : 0.0;
+  *n = r ? r->kcount[k] : 0;
   return true;
 }
 
+bool kernel_stats(const Device *d, int k, double *ms, int64_t *n) {
+  return runner_stats(d ? &d->run : nullptr, k, ms, n);
+}
+
+bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n) {
+  return runner_stats(b ? b->run.get() : nullptr, k, ms, n);
+}
+
 // ---------------------------------------------------------------------------
-// the time loop
+// the time loop, for one wavefield or a batch on one mesh
 // ---------------------------------------------------------------------------
 static inline int nblk(long n) { return (int)((n + 255) / 256); }
 
 struct Launcher {
-  Device &V;
+  Runner &R;
   template <class F> void operator()(int kc, F &&launch) {
-    if (V.profile) {
+    if (R.profile) {
       hipEvent_t a, b;
       HIPCHECK(hipEventCreate(&a));
       HIPCHECK(hipEventCreate(&b));
-      HIPCHECK(hipEventRecord(a, V.stream));
+      HIPCHECK(hipEventRecord(a, R.stream));
       launch();
-      HIPCHECK(hipEventRecord(b, V.stream));
-      V.kev.push_back(a);
-      V.kev.push_back(b);
-      V.kev_class.push_back(kc);
+      HIPCHECK(hipEventRecord(b, R.stream));
+      R.kev.push_back(a);
+      R.kev.push_back(b);
+      R.kev_class.push_back(kc);
     } else {
       launch();
     }
@@ -1518,266 +1771,317 @@ struct Launcher {
 };
 
 template <int SRC, int MODE, bool ANEL>
-static void launch_solid(Device &V, const SolidArgs &A) {
-  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL>), dim3(V.ntile_s), dim3(256), 0,
-                     V.stream, A);
+static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
+  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL>), dim3(M.ntile_s), dim3(256), 0, s, A);
 }
 
-static void dispatch_solid(Device &V, const SolidArgs &A, int mode) {
-  const bool an = V.anel != 0;
-#define AXH_SOLID_CASE(S)                                                  \
-  if (mode == 0) {                                                         \
-    if (an) launch_solid<S, 0, true>(V, A); else launch_solid<S, 0, false>(V, A); \
-  } else {                                                                 \
-    if (an) launch_solid<S, 1, true>(V, A); else launch_solid<S, 1, false>(V, A); \
+static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A, int mode) {
+  const bool an = M.anel != 0;
+#define AXH_SOLID_CASE(S)                                                     \
+  if (mode == 0) {                                                            \
+    if (an) launch_solid<S, 0, true>(M, s, A); else launch_solid<S, 0, false>(M, s, A); \
+  } else {                                                                    \
+    if (an) launch_solid<S, 1, true>(M, s, A); else launch_solid<S, 1, false>(M, s, A); \
   }
-  if (V.src == AXH_MONOPOLE) { AXH_SOLID_CASE(AXH_MONOPOLE) }
-  else if (V.src == AXH_DIPOLE) { AXH_SOLID_CASE(AXH_DIPOLE) }
+  if (M.src == AXH_MONOPOLE) { AXH_SOLID_CASE(AXH_MONOPOLE) }
+  else if (M.src == AXH_DIPOLE) { AXH_SOLID_CASE(AXH_DIPOLE) }
   else { AXH_SOLID_CASE(AXH_QUADPOLE) }
 #undef AXH_SOLID_CASE
 }
 
-static void dispatch_memvar(Device &V, const SolidArgs &A) {
-  if (V.src == AXH_MONOPOLE)
-    hipLaunchKernelGGL(k_memvar<AXH_MONOPOLE>, dim3(V.ntile_s), dim3(256), 0, V.stream, A);
-  else if (V.src == AXH_DIPOLE)
-    hipLaunchKernelGGL(k_memvar<AXH_DIPOLE>, dim3(V.ntile_s), dim3(256), 0, V.stream, A);
+static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
+  if (M.src == AXH_MONOPOLE)
+    hipLaunchKernelGGL(k_memvar<AXH_MONOPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
+  else if (M.src == AXH_DIPOLE)
+    hipLaunchKernelGGL(k_memvar<AXH_DIPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
   else
-    hipLaunchKernelGGL(k_memvar<AXH_QUADPOLE>, dim3(V.ntile_s), dim3(256), 0, V.stream, A);
+    hipLaunchKernelGGL(k_memvar<AXH_QUADPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
 }
 
-void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
-  Device &V = *ctx->dev;
-  const axh_dims &d = ctx->dims;
-  const bool multipole = V.src != AXH_MONOPOLE;
-  const bool newmark = V.scheme == AXH_NEWMARK2;
-  const double dt = d.deltat, hdt = 0.5 * dt, hdt2 = 0.5 * (dt * dt);
-  const size_t nslot_s = (size_t)V.ntile_s * kTile;
-  V.profile = ctx->profile;
-  Launcher L{V};
-
-  const int nsamp = axh_seis_samples(ctx, it0, it1);
-  if (nsamp > 0 && (size_t)nsamp * 3 * V.nrec > V.seis_cap) {
-    V.seis_cap = (size_t)nsamp * 3 * V.nrec;
-    V.seis_d = V.alloc<float>(V.seis_cap);
+// Steps it0..it1 of the wavefields of members (all on one mesh, at most
+// kMaxBatch) on the stream of R; seis[b] (may be NULL) receives member b's
+// samples.  Returns the device time of the run [ms].
+static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int32_t it0,
+                          int32_t it1, float *const *seis) {
+  const int nb = (int)members.size();
+  if (nb < 1 || nb > kMaxBatch) fail(AXH_ERR_ARG, "batch size must be 1..8");
+  const MeshDev &M = *members[0]->dev->mesh;
+  std::vector<WaveDev *> W(nb);
+  for (int b = 0; b < nb; b++) {
+    if (members[b]->dev->mesh.get() != &M) fail(AXH_ERR_STATE, "batch members on different meshes");
+    W[b] = &members[b]->dev->wave;
   }
-  const int intmax = INT_MAX;
-  HIPCHECK(hipMemcpyAsync(V.blow, &intmax, sizeof(int), hipMemcpyHostToDevice, V.stream));
+  const axh_dims &d = members[0]->dims;
+  const bool multipole = M.src != AXH_MONOPOLE;
+  const bool newmark = M.scheme == AXH_NEWMARK2;
+  const double dt = d.deltat, hdt = 0.5 * dt, hdt2 = 0.5 * (dt * dt);
+  const size_t nslot_s = (size_t)M.ntile_s * kTile;
+  R.init();
+  Launcher L{R};
+  hipStream_t st_ = R.stream;
+
+  for (int b = 0; b < nb; b++) {
+    WaveDev &w = *W[b];
+    const int nsamp = axh_seis_samples(members[b], it0, it1);
+    if (nsamp > 0 && (size_t)nsamp * 3 * w.nrec > w.seis_cap) {
+      w.seis_cap = (size_t)nsamp * 3 * w.nrec;
+      w.seis_d = w.mem.alloc<float>(w.seis_cap);
+    }
+    HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)w.blow, INT_MAX, 1, st_));
+  }
 
   SolidArgs SA{};
-  SA.nel = V.nel_s; SA.nsls = V.nsls; SA.G = V.G; SA.coef = V.coef_s; SA.m0 = V.m0_s;
-  SA.einfo = V.einfo_s; SA.gid = V.gid_s; SA.disp = V.disp; SA.nglob = V.nglob_s;
-  SA.force = V.force_s; SA.nslot = (int)nslot_s; SA.cg4 = V.cg4; SA.memvar = V.memvar;
-  SA.rsum = V.rsum; SA.trm1 = V.trm1; SA.devm1 = V.devm1; SA.aj_mu = V.aj_mu;
-  SA.aj_k = V.aj_k;
+  SA.nel = M.nel_s; SA.nsls = M.nsls; SA.nglob = M.nglob_s; SA.nslot = (int)nslot_s;
+  SA.nb = nb; SA.G = M.G; SA.coef = M.coef_s; SA.m0 = M.m0_s; SA.einfo = M.einfo_s;
+  SA.gid = M.gid_s; SA.cg_stage = M.cg_stage; SA.cg_upd = M.cg_upd; SA.aj_mu = M.aj_mu;
+  SA.aj_k = M.aj_k;
   for (int s = 0; s < kMaxSls; s++) {
-    SA.expw[s] = V.expw[s]; SA.tst[s] = V.tst[s]; SA.tstm1[s] = V.tstm1[s];
+    SA.expw[s] = M.expw[s]; SA.tst[s] = M.tst[s]; SA.tstm1[s] = M.tstm1[s];
   }
   FluidArgs FA{};
-  FA.nel = V.nel_f; FA.multipole = multipole; FA.fluid_src = d.fluid_src; FA.G = V.G;
-  FA.coef = V.coef_f; FA.m0 = V.m0_f; FA.einfo = V.einfo_f; FA.gid = V.gid_f;
-  FA.srcel = V.srcel_f; FA.src_fl = V.src_fl; FA.chi = V.chi; FA.force = V.force_f;
+  FA.nel = M.nel_f; FA.multipole = multipole; FA.nb = nb; FA.G = M.G; FA.coef = M.coef_f;
+  FA.m0 = M.m0_f; FA.einfo = M.einfo_f; FA.gid = M.gid_f;
   CoupleArgs CA{};
-  CA.n = 5 * V.nbdry; CA.dipole = V.src == AXH_DIPOLE; CA.nslot_s = (int)nslot_s;
-  CA.nglob_s = V.nglob_s; CA.ss = V.cpl_ss; CA.gs = V.cpl_gs; CA.sf = V.cpl_sf;
-  CA.gf = V.cpl_gf; CA.bm = V.cpl_bm; CA.disp = V.disp; CA.ddchi = V.ddchi;
-  CA.force_f = V.force_f; CA.force_s = V.force_s; CA.axb = V.cpl_axb;
-  CA.axb_val = V.axb_val;
+  CA.n = 5 * M.nbdry; CA.dipole = M.src == AXH_DIPOLE; CA.nslot_s = (int)nslot_s;
+  CA.nglob_s = M.nglob_s; CA.ss = M.cpl_ss; CA.gs = M.cpl_gs; CA.sf = M.cpl_sf;
+  CA.gf = M.cpl_gf; CA.bm = M.cpl_bm; CA.axb = M.cpl_axb;
   // solid axis masks: monopole s; dipole +,z (2,3); quadrupole all
-  const int mask0 = (V.src == AXH_DIPOLE) ? 1 : 0;
-  const int mask1 = (V.src == AXH_MONOPOLE) ? 0 : 2;
+  const int mask0 = (M.src == AXH_DIPOLE) ? 1 : 0;
+  const int mask1 = (M.src == AXH_MONOPOLE) ? 0 : 2;
   const int comps = multipole ? 7 : 5;
   PredArgs PS{};
-  PS.n = V.nglob_s; PS.ncomp_mask0 = mask0; PS.ncomp_mask1 = mask1; PS.comps = comps;
-  PS.dt = dt; PS.hdt2 = hdt2; PS.thr = 0.0; PS.u = V.disp; PS.v = V.velo;
-  PS.a0 = V.acc0; PS.flags = V.flags_s; PS.stride = V.nglob_s; PS.blow = V.blow;
-  PS.naxb = V.naxb; PS.axb_g = V.axb_g; PS.axb_val = V.axb_val;
-  PredArgs PF = PS;
-  PF.n = V.nglob_f; PF.ncomp_mask0 = 0; PF.ncomp_mask1 = 0; PF.comps = 1;
-  PF.u = V.chi; PF.v = V.dchi; PF.a0 = V.ddchi; PF.flags = V.flags_f;
-  PF.stride = V.nglob_f; PF.check_blow = 0; PF.apply_mask = multipole; PF.naxb = 0;
+  PS.n = M.nglob_s; PS.mask0 = mask0; PS.mask1 = mask1; PS.comps = comps;
+  PS.dt = dt; PS.hdt2 = hdt2; PS.flags = M.flags_s; PS.stride = M.nglob_s;
+  PS.naxb = M.naxb; PS.axb_g = M.axb_g;
+  PredArgs PF{};
+  PF.n = M.nglob_f; PF.mask0 = 0; PF.mask1 = 0; PF.comps = 1; PF.dt = dt; PF.hdt2 = hdt2;
+  PF.flags = M.flags_f; PF.stride = M.nglob_f; PF.naxb = 0; PF.axb_g = nullptr;
   AsmSolidArgs AS{};
-  AS.n = V.nglob_s; AS.src = V.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
-  AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = V.gstart_s;
-  AS.gslot = V.gslot_s; AS.flags = V.flags_s; AS.force = V.force_s; AS.minv = V.minv_s;
-  AS.velo = V.velo; AS.acc0 = V.acc0; AS.hdt = hdt; AS.nsrc = V.nsrcpts;
-  AS.src_g = V.src_g; AS.src_val = V.src_val;
+  AS.n = M.nglob_s; AS.src = M.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
+  AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.nb = nb; AS.gstart = M.gstart_s;
+  AS.gslot = M.gslot_s; AS.flags = M.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
   AsmFluidArgs AF{};
-  AF.n = V.nglob_f; AF.newmark = newmark; AF.mask_axis = multipole;
-  AF.mask_surface = newmark; AF.gstart = V.gstart_f; AF.gslot = V.gslot_f;
-  AF.flags = V.flags_f; AF.force = V.force_f; AF.minv = V.minv_f; AF.dchi = V.dchi;
-  AF.ddchi = V.ddchi; AF.hdt = hdt;
-  // blow-up threshold of runtime_info: maxval(abs(disp(1,1,:,:))) >
-  // 10*abs(magnitude), the real(4) maximum promoted to double
-  PS.thr = 10.0 * std::fabs(d.magnitude);
+  AF.n = M.nglob_f; AF.newmark = newmark; AF.mask_axis = multipole;
+  AF.mask_surface = newmark; AF.nb = nb; AF.gstart = M.gstart_f; AF.gslot = M.gslot_f;
+  AF.flags = M.flags_f; AF.minv = M.minv_f; AF.hdt = hdt;
+  RecArgs RA{};
+  RA.src = M.src; RA.nglob = M.nglob_s;
+  int maxrec = 0;
+  for (int b = 0; b < nb; b++) {
+    WaveDev &w = *W[b];
+    SA.disp[b] = w.disp; SA.force[b] = w.force_s; SA.memvar[b] = w.memvar;
+    SA.rsum[b] = w.rsum; SA.trm1[b] = w.trm1; SA.devm1[b] = w.devm1;
+    FA.chi[b] = w.chi; FA.force[b] = w.force_f; FA.srcel[b] = w.srcel_f;
+    FA.src_fl[b] = w.src_fl;
+    CA.disp[b] = w.disp; CA.ddchi[b] = w.ddchi; CA.force_f[b] = w.force_f;
+    CA.force_s[b] = w.force_s; CA.axb_val[b] = w.axb_val;
+    PS.u[b] = w.disp; PS.v[b] = w.velo; PS.a0[b] = w.acc0; PS.blow[b] = w.blow;
+    PS.thr[b] = w.blow_thr; PS.axb_val[b] = w.axb_val;
+    PF.u[b] = w.chi; PF.v[b] = w.dchi; PF.a0[b] = w.ddchi; PF.blow[b] = w.blow;
+    PF.axb_val[b] = w.axb_val;
+    AS.force[b] = w.force_s; AS.velo[b] = w.velo; AS.acc0[b] = w.acc0;
+    AS.nsrc[b] = w.nsrc; AS.src_g[b] = w.src_g; AS.src_val[b] = w.src_val;
+    AF.force[b] = w.force_f; AF.dchi[b] = w.dchi; AF.ddchi[b] = w.ddchi;
+    RA.nrec[b] = w.nrec; RA.rec_g[b] = w.rec_g; RA.disp[b] = w.disp;
+    maxrec = std::max(maxrec, w.nrec);
+  }
+  const dim3 gS(nblk(M.nglob_s), nb), gF(nblk(M.nglob_f), nb), gC(nblk(CA.n), nb);
 
-  HIPCHECK(hipEventRecord(V.ev0, V.stream));
-  int isamp = 0;
+  HIPCHECK(hipEventRecord(R.ev0, st_));
+  std::vector<int> isamp(nb, 0);
   for (int it = it0; it <= it1; it++) {
-    for (int st = 0; st < V.nstages; st++) {
-      const float stf = V.stf[(size_t)(it - 1) * V.nstages + st];
+    for (int st = 0; st < M.nstages; st++) {
       // predictor / drift
       PS.newmark = PF.newmark = newmark;
-      PS.cd = PF.cd = newmark ? 0.0 : V.coefd[st];
+      PS.cd = PF.cd = newmark ? 0.0 : M.coefd[st];
       PS.apply_mask = 1;
+      PF.apply_mask = multipole;
       PS.check_blow = (st == 0);
-      PS.iter = it;
+      PF.check_blow = 0;
+      PS.iter = PF.iter = it;
       L(KC_PRED, [&] {
-        hipLaunchKernelGGL(k_predict, dim3(nblk(PS.n)), dim3(256), 0, V.stream, PS);
-        if (V.nel_f > 0)
-          hipLaunchKernelGGL(k_predict, dim3(nblk(PF.n)), dim3(256), 0, V.stream, PF);
+        hipLaunchKernelGGL(k_predict, gS, dim3(256), 0, st_, PS);
+        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict, gF, dim3(256), 0, st_, PF);
       });
-      if (V.nel_f > 0) {
-        FA.stf = newmark ? stf : 0.0f;
+      for (int b = 0; b < nb; b++) {
+        const float stf = W[b]->stf[(size_t)(it - 1) * M.nstages + st];
+        FA.stf[b] = (newmark && W[b]->fluid_src) ? stf : 0.0f;
+        AS.stf[b] = (newmark && W[b]->fluid_src) ? 0.0f : stf;
+      }
+      if (M.nel_f > 0) {
         L(KC_FLUID, [&] {
-          hipLaunchKernelGGL(k_fluid, dim3(V.ntile_f), dim3(256), 0, V.stream, FA);
+          hipLaunchKernelGGL(k_fluid, dim3(M.ntile_f), dim3(256), 0, st_, FA);
         });
-        if (V.nbdry > 0)
+        if (M.nbdry > 0)
           L(KC_COUPLE_F, [&] {
-            hipLaunchKernelGGL(k_copy2fluid, dim3(nblk(CA.n)), dim3(256), 0, V.stream, CA);
+            hipLaunchKernelGGL(k_copy2fluid, gC, dim3(256), 0, st_, CA);
           });
       }
-      L(KC_SOLID, [&] { dispatch_solid(V, SA, newmark ? 0 : 1); });
-      if (V.nel_f > 0) {
-        AF.cv = newmark ? 0.0 : V.coefv[st];
+      L(KC_SOLID, [&] { dispatch_solid(M, st_, SA, newmark ? 0 : 1); });
+      if (M.nel_f > 0) {
+        AF.cv = newmark ? 0.0 : M.coefv[st];
         L(KC_ASM_F, [&] {
-          hipLaunchKernelGGL(k_asm_fluid, dim3(nblk(AF.n)), dim3(256), 0, V.stream, AF);
+          hipLaunchKernelGGL(k_asm_fluid, dim3(nblk(AF.n)), dim3(256), 0, st_, AF);
         });
-        if (V.nbdry > 0)
+        if (M.nbdry > 0)
           L(KC_COUPLE_S, [&] {
-            hipLaunchKernelGGL(k_copy2solid, dim3(nblk(CA.n)), dim3(256), 0, V.stream, CA);
+            hipLaunchKernelGGL(k_copy2solid, gC, dim3(256), 0, st_, CA);
           });
       }
-      AS.cv = newmark ? 0.0 : V.coefv[st];
-      AS.stf = (newmark && d.fluid_src) ? 0.0f : stf;
+      AS.cv = newmark ? 0.0 : M.coefv[st];
       L(KC_ASM_S, [&] {
-        hipLaunchKernelGGL(k_asm_solid, dim3(nblk(AS.n)), dim3(256), 0, V.stream, AS);
+        hipLaunchKernelGGL(k_asm_solid, dim3(nblk(AS.n)), dim3(256), 0, st_, AS);
       });
     }
     if (!newmark) {
       // final drift with coefd(nstages+1), no mask (SOLVER/time_evol_wave.F90:720-725)
       PredArgs PD = PS, PDF = PF;
-      PD.cd = PDF.cd = V.coefd[V.nstages];
+      PD.cd = PDF.cd = M.coefd[M.nstages];
       PD.newmark = PDF.newmark = 0;
       PD.apply_mask = PDF.apply_mask = 0;
-      PD.check_blow = 0;
+      PD.check_blow = PDF.check_blow = 0;
       L(KC_DRIFT, [&] {
-        hipLaunchKernelGGL(k_predict, dim3(nblk(PD.n)), dim3(256), 0, V.stream, PD);
-        if (V.nel_f > 0)
-          hipLaunchKernelGGL(k_predict, dim3(nblk(PDF.n)), dim3(256), 0, V.stream, PDF);
+        hipLaunchKernelGGL(k_predict, gS, dim3(256), 0, st_, PD);
+        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict, gF, dim3(256), 0, st_, PDF);
       });
-      if (V.anel) L(KC_MEMVAR, [&] { dispatch_memvar(V, SA); });
+      if (M.anel) L(KC_MEMVAR, [&] { dispatch_memvar(M, st_, SA); });
     }
-    if (V.nrec > 0 && it % d.seis_it == 0) {
-      float *out = V.seis_d + (size_t)isamp * 3 * V.nrec;
+    bool any = false;
+    for (int b = 0; b < nb; b++) {
+      WaveDev &w = *W[b];
+      RA.out[b] = nullptr;
+      if (w.nrec > 0 && it % w.seis_it == 0) {
+        RA.out[b] = w.seis_d + (size_t)isamp[b] * 3 * w.nrec;
+        isamp[b]++;
+        any = true;
+      }
+    }
+    if (any)
       L(KC_REC, [&] {
-        hipLaunchKernelGGL(k_receivers, dim3(nblk(V.nrec)), dim3(256), 0, V.stream,
-                           V.nrec, V.src, (const int *)V.rec_g, (const float *)V.disp,
-                           V.nglob_s, out);
+        hipLaunchKernelGGL(k_receivers, dim3(nblk(maxrec), nb), dim3(256), 0, st_, RA);
       });
-      isamp++;
-    }
   }
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipEventRecord(V.ev1, V.stream));
-  int blow = INT_MAX;
-  HIPCHECK(hipMemcpyAsync(&blow, V.blow, sizeof(int), hipMemcpyDeviceToHost, V.stream));
-  if (seis && isamp > 0)
-    HIPCHECK(hipMemcpyAsync(seis, V.seis_d, (size_t)isamp * 3 * V.nrec * sizeof(float),
-                            hipMemcpyDeviceToHost, V.stream));
-  HIPCHECK(hipStreamSynchronize(V.stream));
-  float ms = 0.0f;
-  HIPCHECK(hipEventElapsedTime(&ms, V.ev0, V.ev1));
-  ctx->last_run_ms = ms;
-  if (V.profile) {
-    for (size_t q = 0; q < V.kev_class.size(); q++) {
-      float kms = 0.0f;
-      HIPCHECK(hipEventElapsedTime(&kms, V.kev[2 * q], V.kev[2 * q + 1]));
-      V.kms[V.kev_class[q]] += kms;
-      V.kcount[V.kev_class[q]] += 1;
-    }
-    for (auto e : V.kev) (void)hipEventDestroy(e);
-    V.kev.clear();
-    V.kev_class.clear();
+  HIPCHECK(hipEventRecord(R.ev1, st_));
+  std::vector<int> blow(nb, INT_MAX);
+  for (int b = 0; b < nb; b++) {
+    HIPCHECK(hipMemcpyAsync(&blow[b], W[b]->blow, sizeof(int), hipMemcpyDeviceToHost, st_));
+    if (seis && seis[b] && isamp[b] > 0)
+      HIPCHECK(hipMemcpyAsync(seis[b], W[b]->seis_d,
+                              (size_t)isamp[b] * 3 * W[b]->nrec * sizeof(float),
+                              hipMemcpyDeviceToHost, st_));
   }
-  ctx->iter_done = it1;
-  if (blow != INT_MAX)
-    fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
+  HIPCHECK(hipStreamSynchronize(st_));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, R.ev0, R.ev1));
+  if (R.profile) {
+    for (size_t q = 0; q < R.kev_class.size(); q++) {
+      float kms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&kms, R.kev[2 * q], R.kev[2 * q + 1]));
+      R.kms[R.kev_class[q]] += kms;
+      R.kcount[R.kev_class[q]] += 1;
+    }
+    for (auto e : R.kev) (void)hipEventDestroy(e);
+    R.kev.clear();
+    R.kev_class.clear();
+  }
+  for (int b = 0; b < nb; b++) {
+    members[b]->iter_done = it1;
+    members[b]->last_run_ms = ms;
+  }
+  for (int b = 0; b < nb; b++)
+    if (blow[b] != INT_MAX)
+      fail(AXH_ERR_BLOWUP, (nb > 1 ? "batch member " + std::to_string(b) + ": " : std::string()) +
+                               "displacements blew up at time step " + std::to_string(blow[b]));
+  return ms;
+}
+
+void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
+  Device &V = *ctx->dev;
+  V.run.profile = ctx->profile;
+  float *out[1] = {seis};
+  run_members(V.run, {ctx}, it0, it1, out);
+}
+
+void batch_run(axh_batch *bt, int32_t it0, int32_t it1, float *const *seis) {
+  if (!bt->run) {
+    std::unique_ptr<Runner, RunnerDeleter> r(new Runner);
+    bt->run = std::move(r);
+  }
+  bt->run->profile = bt->profile;
+  bt->last_run_ms = run_members(*bt->run, bt->members, it0, it1, seis);
+}
+
+static void sync_member(Device &V) {
+  if (V.run.stream) HIPCHECK(hipStreamSynchronize(V.run.stream));
 }
 
 // element-local fields from the per-point device arrays
 void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
                      float *dchi, float *ddchi) {
   Device &V = *ctx->dev;
-  HIPCHECK(hipStreamSynchronize(V.stream));
-  std::vector<int> gid_s((size_t)V.ntile_s * kTile);
-  HIPCHECK(hipMemcpy(gid_s.data(), V.gid_s, gid_s.size() * 4, hipMemcpyDeviceToHost));
+  const MeshDev &M = *V.mesh;
+  sync_member(V);
   auto scatter_s = [&](const float *dsrc, float *out) {
-    std::vector<float> h(3 * (size_t)V.nglob_s);
+    std::vector<float> h(3 * (size_t)M.nglob_s);
     HIPCHECK(hipMemcpy(h.data(), dsrc, h.size() * 4, hipMemcpyDeviceToHost));
     for (int c = 0; c < 3; c++)
-      for (int e = 0; e < V.nel_s; e++)
+      for (int e = 0; e < M.nel_s; e++)
         for (int k = 0; k < kNpp; k++)
-          out[((size_t)c * V.nel_s + e) * kNpp + k] =
-              h[(size_t)c * V.nglob_s + gid_s[slot_of(e, k)]];
+          out[((size_t)c * M.nel_s + e) * kNpp + k] =
+              h[(size_t)c * M.nglob_s + M.gid_s_h[slot_of(e, k)]];
   };
-  if (disp) scatter_s(V.disp, disp);
-  if (velo) scatter_s(V.velo, velo);
-  if (V.nel_f > 0 && (chi || dchi || ddchi)) {
-    std::vector<int> gid_f((size_t)V.ntile_f * kTile);
-    HIPCHECK(hipMemcpy(gid_f.data(), V.gid_f, gid_f.size() * 4, hipMemcpyDeviceToHost));
+  if (disp) scatter_s(V.wave.disp, disp);
+  if (velo) scatter_s(V.wave.velo, velo);
+  if (M.nel_f > 0 && (chi || dchi || ddchi)) {
     auto scatter_f = [&](const float *dsrc, float *out) {
-      std::vector<float> h(V.nglob_f);
+      std::vector<float> h(M.nglob_f);
       HIPCHECK(hipMemcpy(h.data(), dsrc, h.size() * 4, hipMemcpyDeviceToHost));
-      for (int e = 0; e < V.nel_f; e++)
-        for (int k = 0; k < kNpp; k++)
-          out[(size_t)e * kNpp + k] = h[gid_f[slot_of(e, k)]];
+      for (int e = 0; e < M.nel_f; e++)
+        for (int k = 0; k < kNpp; k++) out[(size_t)e * kNpp + k] = h[M.gid_f_h[slot_of(e, k)]];
     };
-    if (chi) scatter_f(V.chi, chi);
-    if (dchi) scatter_f(V.dchi, dchi);
-    if (ddchi) scatter_f(V.ddchi, ddchi);
+    if (chi) scatter_f(V.wave.chi, chi);
+    if (dchi) scatter_f(V.wave.dchi, dchi);
+    if (ddchi) scatter_f(V.wave.ddchi, ddchi);
   }
 }
 
 void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
                           const float *chi, const float *dchi, const float *ddchi) {
   Device &V = *ctx->dev;
-  HIPCHECK(hipStreamSynchronize(V.stream));
-  std::vector<int> gid_s((size_t)V.ntile_s * kTile);
-  HIPCHECK(hipMemcpy(gid_s.data(), V.gid_s, gid_s.size() * 4, hipMemcpyDeviceToHost));
+  const MeshDev &M = *V.mesh;
+  sync_member(V);
   auto gather_s = [&](const float *in, float *ddst, const char *what) {
-    std::vector<float> h(3 * (size_t)V.nglob_s, 0.0f);
+    std::vector<float> h(3 * (size_t)M.nglob_s, 0.0f);
     std::vector<char> seen(h.size(), 0);
     for (int c = 0; c < 3; c++)
-      for (int e = 0; e < V.nel_s; e++)
+      for (int e = 0; e < M.nel_s; e++)
         for (int k = 0; k < kNpp; k++) {
-          size_t g = (size_t)c * V.nglob_s + gid_s[slot_of(e, k)];
-          float x = in[((size_t)c * V.nel_s + e) * kNpp + k];
+          size_t g = (size_t)c * M.nglob_s + M.gid_s_h[slot_of(e, k)];
+          float x = in[((size_t)c * M.nel_s + e) * kNpp + k];
           if (!seen[g]) { h[g] = x; seen[g] = 1; }
           else if (h[g] != x) fail(AXH_ERR_ARG, std::string(what) + ": copies of a point differ");
         }
     HIPCHECK(hipMemcpy(ddst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   };
-  if (disp) gather_s(disp, V.disp, "disp");
-  if (velo) gather_s(velo, V.velo, "velo");
-  if (V.nel_f > 0 && (chi || dchi || ddchi)) {
-    std::vector<int> gid_f((size_t)V.ntile_f * kTile);
-    HIPCHECK(hipMemcpy(gid_f.data(), V.gid_f, gid_f.size() * 4, hipMemcpyDeviceToHost));
+  if (disp) gather_s(disp, V.wave.disp, "disp");
+  if (velo) gather_s(velo, V.wave.velo, "velo");
+  if (M.nel_f > 0 && (chi || dchi || ddchi)) {
     auto gather_f = [&](const float *in, float *ddst, const char *what) {
-      std::vector<float> h(V.nglob_f, 0.0f);
+      std::vector<float> h(M.nglob_f, 0.0f);
       std::vector<char> seen(h.size(), 0);
-      for (int e = 0; e < V.nel_f; e++)
+      for (int e = 0; e < M.nel_f; e++)
         for (int k = 0; k < kNpp; k++) {
-          size_t g = gid_f[slot_of(e, k)];
+          size_t g = M.gid_f_h[slot_of(e, k)];
           float x = in[(size_t)e * kNpp + k];
           if (!seen[g]) { h[g] = x; seen[g] = 1; }
           else if (h[g] != x) fail(AXH_ERR_ARG, std::string(what) + ": copies of a point differ");
         }
       HIPCHECK(hipMemcpy(ddst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     };
-    if (chi) gather_f(chi, V.chi, "chi");
-    if (dchi) gather_f(dchi, V.dchi, "dchi");
-    if (ddchi) gather_f(ddchi, V.ddchi, "ddchi");
+    if (chi) gather_f(chi, V.wave.chi, "chi");
+    if (dchi) gather_f(dchi, V.wave.dchi, "dchi");
+    if (ddchi) gather_f(ddchi, V.wave.ddchi, "ddchi");
   }
 }
 

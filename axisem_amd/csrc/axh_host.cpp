@@ -109,6 +109,7 @@ const axh::Array &axh_ctx::need(const std::string &name) const {
 void axh_ctx::put(const std::string &name, DType t, std::vector<uint64_t> dims_,
                   const void *src) {
   if (dev) fail(AXH_ERR_STATE, "uploads are closed after the first run: " + name);
+  if (batch_refs > 0) fail(AXH_ERR_STATE, "uploads are closed while in a batch: " + name);
   Array a;
   a.dtype = t;
   a.dims = std::move(dims_);
@@ -168,6 +169,7 @@ int axh_create(const axh_dims *dims, axh_ctx **out) {
 int axh_destroy(axh_ctx *ctx) {
   return guarded([&] {
     if (!ctx) return;
+    if (ctx->batch_refs > 0) fail(AXH_ERR_STATE, "context is held by a batch");
     delete ctx;
   });
 }
@@ -400,6 +402,111 @@ int axh_device_bytes(axh_ctx *ctx, size_t *bytes) {
     if (!bytes) fail(AXH_ERR_ARG, "null pointer");
     if (!ctx->dev) device_build(ctx);
     *bytes = device_bytes(ctx->dev.get());
+  });
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+// Members of a batch step on member 0's device mesh: dims and the mesh part
+// the device would build (every term the kernels read; not the per-source
+// uploads ielsrc, source_term, stf, recfile_el) must agree.  The reference
+// leaves entries it never reads unset (e.g. M0_w_fl of non-axial elements,
+// SOLVER/def_precomp_terms.f90:2355-2455), so the comparison is on the built
+// layout, not on the raw uploads.
+static void check_same_mesh(const axh_ctx *a, const uint64_t fa[2], const axh_ctx *b, int idx) {
+  const std::string who = "batch member " + std::to_string(idx) + ": ";
+  const axh_dims &x = a->dims, &y = b->dims;
+  if (x.abi_version != y.abi_version || x.npol != y.npol || x.nel_solid != y.nel_solid ||
+      x.nel_fluid != y.nel_fluid || x.nglob_solid != y.nglob_solid ||
+      x.nglob_fluid != y.nglob_fluid || x.nel_bdry != y.nel_bdry || x.src_type != y.src_type ||
+      x.scheme != y.scheme || x.anel != y.anel || x.coarse_grained != y.coarse_grained ||
+      x.n_sls != y.n_sls || std::memcmp(&x.deltat, &y.deltat, sizeof(double)) != 0)
+    fail(AXH_ERR_ARG, who + "mesh, source type or time scheme differs from member 0");
+  uint64_t fb[2];
+  mesh_fingerprint(b, fb);
+  if (fa[0] != fb[0] || fa[1] != fb[1])
+    fail(AXH_ERR_ARG, who + "mesh terms differ from member 0");
+}
+
+static void check_batch(const axh_batch *b) {
+  if (!b) fail(AXH_ERR_ARG, "null batch");
+}
+
+extern "C" {
+
+int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
+  return guarded([&] {
+    if (!members || !out) fail(AXH_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (n < 1 || n > AXH_MAX_BATCH)
+      fail(AXH_ERR_ARG, "batch size must be 1.." + std::to_string(AXH_MAX_BATCH));
+    for (int i = 0; i < n; i++) {
+      if (!members[i]) fail(AXH_ERR_ARG, "null batch member");
+      for (int j = 0; j < i; j++)
+        if (members[j] == members[i]) fail(AXH_ERR_ARG, "a context appears twice in a batch");
+    }
+    uint64_t f0[2] = {0, 0};
+    if (n > 1) mesh_fingerprint(members[0], f0);
+    for (int i = 1; i < n; i++) {
+      check_same_mesh(members[0], f0, members[i], i);
+      if (members[i]->iter_done != members[0]->iter_done)
+        fail(AXH_ERR_ARG, "batch members are at different time steps");
+    }
+    auto *b = new axh_batch;
+    b->members.assign(members, members + n);
+    for (auto *c : b->members) c->batch_refs++;
+    *out = b;
+  });
+}
+
+int axh_batch_destroy(axh_batch *batch) {
+  return guarded([&] {
+    if (!batch) return;
+    for (auto *c : batch->members) c->batch_refs--;
+    delete batch;
+  });
+}
+
+int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end, float *const *seis) {
+  return guarded([&] {
+    check_batch(batch);
+    for (auto *c : batch->members) {
+      if (it_begin != c->iter_done + 1)
+        fail(AXH_ERR_ARG, "axh_batch_run must continue at step " +
+                              std::to_string(c->iter_done + 1));
+      if (it_end < it_begin - 1 || it_end > c->dims.niter)
+        fail(AXH_ERR_ARG, "step range outside 1..niter of a member");
+    }
+    batch_attach(batch->members);
+    batch_run(batch, it_begin, it_end, seis);
+  });
+}
+
+int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms) {
+  return guarded([&] {
+    check_batch(batch);
+    if (!total_ms) fail(AXH_ERR_ARG, "null pointer");
+    *total_ms = batch->last_run_ms;
+  });
+}
+
+int axh_batch_set_profiling(axh_batch *batch, int on) {
+  return guarded([&] {
+    check_batch(batch);
+    batch->profile = on != 0;
+  });
+}
+
+int axh_batch_kernel_stats(const axh_batch *batch, int kclass, double *total_ms,
+                           int64_t *launches) {
+  return guarded([&] {
+    check_batch(batch);
+    if (!total_ms || !launches) fail(AXH_ERR_ARG, "null pointer");
+    if (!batch_kernel_stats(batch, kclass, total_ms, launches))
+      fail(AXH_ERR_ARG, "unknown kernel class");
   });
 }
 

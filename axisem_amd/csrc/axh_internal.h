@@ -16,6 +16,7 @@ constexpr int kNpp = 25;           // GLL points per element
 constexpr int kElemPerTile = 10;   // elements per 256-lane workgroup
 constexpr int kTile = 256;         // slots per tile (250 used)
 constexpr int kMaxSls = 8;
+constexpr int kMaxBatch = AXH_MAX_BATCH;  // wavefields per kernel launch
 
 enum DType : uint32_t { kF32 = 1, kF64 = 2, kI32 = 3 };
 
@@ -38,6 +39,10 @@ struct Device;  // device-side layout and buffers (axh_device.hip)
 struct DeviceDeleter {
   void operator()(Device *d) const;
 };
+struct Runner;  // stream, events and kernel timing of runs (axh_device.hip)
+struct RunnerDeleter {
+  void operator()(Runner *r) const;
+};
 
 // Error raised inside the library; converted to a status code at the ABI.
 struct Error {
@@ -55,6 +60,7 @@ struct axh_ctx {
   int32_t iter_done = 0;  // last completed step
   double last_run_ms = 0.0;
   bool profile = false;
+  int batch_refs = 0;     // batches holding this context
 
   const axh::Array *get(const std::string &name) const {
     auto it = arrays.find(name);
@@ -63,6 +69,16 @@ struct axh_ctx {
   const axh::Array &need(const std::string &name) const;
   void put(const std::string &name, axh::DType t, std::vector<uint64_t> dims,
            const void *src);
+};
+
+// Wavefields on one mesh stepped together (axh_batch_*).  The members are not
+// owned; while a batch holds a context, the context takes no uploads and
+// cannot be destroyed.
+struct axh_batch {
+  std::vector<axh_ctx *> members;
+  std::unique_ptr<axh::Runner, axh::RunnerDeleter> run;
+  double last_run_ms = 0.0;
+  bool profile = false;
 };
 
 namespace axh {
@@ -84,6 +100,12 @@ const char *kernel_class_name(int k);
 void check_inputs(const axh_ctx *ctx);
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob);
 bool kernel_stats(const Device *d, int k, double *ms, int64_t *n);
+// batches: put every member on the first member's device mesh (building what
+// is missing), then step them together
+void batch_attach(const std::vector<axh_ctx *> &members);
+void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]);
+void batch_run(axh_batch *b, int32_t it0, int32_t it1, float *const *seis);
+bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n);
 int symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
                             double *cf);
 }  // namespace axh

@@ -23,6 +23,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libaxihip.so")
 
 AXH_OK, AXH_ERR_BLOWUP = 0, -6
+MAX_BATCH = 8
 
 SOLID_TERMS = ["M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M11z", "M21z",
                "M41z", "M32s", "M13s", "M33s", "M43s", "M1phi", "M2phi", "M4phi",
@@ -93,6 +94,13 @@ def library():
         L.axh_global_numbering.argtypes = [P, C.c_int32, P, C.POINTER(C.c_int32)]
         L.axh_symplectic_coefficients.argtypes = [C.c_int32, C.c_double, P, P, P,
                                                   C.POINTER(C.c_int32)]
+        L.axh_batch_create.argtypes = [C.POINTER(P), C.c_int32, C.POINTER(P)]
+        L.axh_batch_destroy.argtypes = [P]
+        L.axh_batch_run.argtypes = [P, C.c_int32, C.c_int32, C.POINTER(P)]
+        L.axh_batch_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
+        L.axh_batch_set_profiling.argtypes = [P, C.c_int]
+        L.axh_batch_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -256,13 +264,84 @@ class Solver:
 
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""
-        out = {}
-        for k in range(16):
-            name = library().axh_kernel_name(k)
-            if not name:
-                break
-            ms, n = C.c_double(), C.c_int64()
-            _check(library().axh_kernel_stats(self._h, k, C.byref(ms), C.byref(n)))
-            out[name.decode()] = (ms.value, n.value)
-        return out
+        return _kernel_stats(library().axh_kernel_stats, self._h)
 
+
+
+def _kernel_stats(fn, handle) -> dict:
+    out = {}
+    for k in range(16):
+        name = library().axh_kernel_name(k)
+        if not name:
+            break
+        ms, n = C.c_double(), C.c_int64()
+        _check(fn(handle, k, C.byref(ms), C.byref(n)))
+        out[name.decode()] = (ms.value, n.value)
+    return out
+
+
+class Batch:
+    """Several wavefields on one mesh (one Solver each, differing in source and
+    receivers), stepped together: every kernel launch advances all of them and
+    reads the mesh terms once.  Each member's seismograms and fields are the
+    ones it gets when run alone."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        n = len(self.solvers)
+        arr = (C.c_void_p * n)(*[s._h for s in self.solvers])
+        self._h = C.c_void_p()
+        _check(library().axh_batch_create(arr, n, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            library().axh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def iter(self) -> int:
+        return self.solvers[0].iter
+
+    def run(self, nsteps: int | None = None) -> list:
+        """Advance every member nsteps (default: to the smallest niter);
+        returns one (nsamples, num_rec, 3) array per member."""
+        it0 = self.iter + 1
+        top = min(s.dims["niter"] for s in self.solvers)
+        it1 = top if nsteps is None else min(top, self.iter + nsteps)
+        outs, ptrs = [], []
+        for s in self.solvers:
+            n = library().axh_seis_samples(s._h, it0, it1)
+            a = np.zeros((max(n, 1), max(s.dims["num_rec"], 1), 3), np.float32)
+            outs.append((a, n))
+            ptrs.append(a.ctypes.data_as(C.c_void_p))
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        rc = library().axh_batch_run(self._h, it0, it1, arr)
+        if rc in (AXH_OK, AXH_ERR_BLOWUP):
+            for s in self.solvers:
+                s.iter = it1
+        _check(rc)
+        return [a[:n] for a, n in outs]
+
+    def last_run_ms(self) -> float:
+        v = C.c_double()
+        _check(library().axh_batch_last_run_ms(self._h, C.byref(v)))
+        return v.value
+
+    def set_profiling(self, on: bool):
+        _check(library().axh_batch_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        """Per kernel class: (total ms, launches) since profiling was enabled."""
+        return _kernel_stats(library().axh_batch_kernel_stats, self._h)

@@ -9,7 +9,10 @@ of the reference's own setup, recorded by the drop-in host.
 
 A "step" is one full time step (4 PEFRL stages) of the whole mesh; the metric
 is element-local GLL-point updates per second, 25 (nel_solid + nel_fluid) per
-step.  Prints one JSON line.
+step.  ``value`` is the plain path (one wavefield).  The batched path steps 4
+wavefields on the same mesh (the workload's source at 50, 150, 300 and 600 km)
+in one launch per kernel and is reported beside it, per wavefield and in
+total, with the step latency of both.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -23,6 +26,7 @@ import subprocess
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
@@ -40,6 +44,7 @@ WORKLOAD = inputs.RunConfig(
     stations_deg=[float(k) for k in range(181)])
 WORKLOAD_NAME = ("prem_iso 10 s, NPOL 4, anelastic 5 SLS (cg4), symplec4, dipole "
                  "thetaforce 50 km, 3600 s, 181 receivers")
+BATCH_DEPTHS_KM = (50.0, 150.0, 300.0, 600.0)   # member 0 is the plain workload
 
 
 def log(msg):
@@ -50,49 +55,85 @@ def _stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def prepare_state(workdir: str) -> str:
-    """Record (once) the uploads of the drop-in host for the workload."""
+def member_config(depth_km: float) -> inputs.RunConfig:
+    return inputs.RunConfig(**{**WORKLOAD.__dict__, "source_depth_km": depth_km})
+
+
+def prepare_state(workdir: str, cfg: inputs.RunConfig = WORKLOAD) -> str:
+    """Record (once) the uploads of the drop-in host for one source of the
+    workload."""
     if not os.path.exists(os.path.join(MESH, "meshdb.dat0000")):
         raise SystemExit("missing %s: run __graft_entry__.build() where the reference "
                          "tree exists" % MESH)
     if not os.path.exists(HOST):
         raise SystemExit("missing drop-in host %s" % HOST)
+    os.makedirs(workdir, exist_ok=True)
     rundir = os.path.join(workdir, "run_gpu")
     state = os.path.join(workdir, "state.bin")
     stamp = os.path.join(workdir, "stamp.json")
-    key = json.dumps({"cfg": repr(WORKLOAD), "mesh": os.path.getsize(os.path.join(MESH, "meshdb.dat0000")),
+    key = json.dumps({"cfg": repr(cfg), "mesh": os.path.getsize(os.path.join(MESH, "meshdb.dat0000")),
                       "host": os.path.getmtime(HOST)})
     if os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key:
         return state
-    inputs.write_run_dir(rundir, MESH, WORKLOAD)
+    inputs.write_run_dir(rundir, MESH, cfg)
     t = time.time()
     inputs.record_state(HOST, rundir, state)
-    log("recorded the workload state in %.1fs" % (time.time() - t))
+    log("recorded the state of the %.0f km source in %.1fs" % (cfg.source_depth_km,
+                                                                 time.time() - t))
     with open(stamp, "w") as f:
         f.write(key)
     return state
 
 
-def solid_kernel_bytes(st) -> int:
+def prepare_states(workdir: str, nbatch: int) -> list:
+    """States of the plain workload (in workdir) and of the other batch
+    members (workdir/member<k>), recorded in parallel."""
+    jobs = [(workdir, WORKLOAD)] + [
+        (os.path.join(workdir, "member%d" % k), member_config(BATCH_DEPTHS_KM[k]))
+        for k in range(1, nbatch)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        return list(ex.map(lambda j: prepare_state(*j), jobs))
+
+
+def solid_kernel_bytes(st, nwave: int = 1) -> int:
     """Algorithmic HBM bytes of one launch of the solid stiffness kernel in a
-    symplectic stage: every precomputed term of every element-local point
-    once, the displacement of every global point once, the element-local
-    force written once, the element->global index, the SLS sums and the cg4
-    terms it reads, and the axial terms."""
+    symplectic stage advancing nwave wavefields: every precomputed term of
+    every element-local point once, the element->global index, the axial
+    index, the axial terms and the stage cg4 terms once (shared by the
+    wavefields); per wavefield the displacement of every global point once,
+    the element-local force written once and the SLS sums."""
     d = st.dims
     nel, nglob = d["nel_solid"], d["nglob_solid"]
     ncomp = 2 if d["src_type"] == 0 else 3
     nterms = {0: 15, 1: 24, 2: 26}[d["src_type"]]
     naxel = int(st["axis_solid"].sum())
-    b = 4 * 25 * nel * nterms          # stiffness terms
-    b += 4 * 25 * nel                  # element -> global index
-    b += 4 * nel                       # axial flag / index
-    b += 4 * ncomp * nglob             # displacement gather
-    b += 4 * ncomp * 25 * nel          # force
-    b += 4 * 50 * naxel                # M0_w terms of axial elements
+    mesh = 4 * 25 * nel * nterms       # stiffness terms
+    mesh += 4 * 25 * nel               # element -> global index
+    mesh += 4 * nel                    # axial flag / index
+    mesh += 4 * 50 * naxel             # M0_w terms of axial elements
+    wave = 4 * ncomp * nglob           # displacement gather
+    wave += 4 * ncomp * 25 * nel       # force
     if d["anel"]:
-        b += nel * (24 * 4 + 5 * 4 * 4)  # SLS sums + Y, V_{s,z}_{eta,xi}
-    return b
+        mesh += nel * 5 * 4 * 4        # Y, V_{s,z}_{eta,xi} at the cg points
+        wave += nel * 24 * 4           # SLS sums
+    return mesh + nwave * wave
+
+
+def kernel_table(stats: dict) -> dict:
+    return {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
+            for k, v in stats.items() if v[1]}
+
+
+def step_latency(runner, nsteps: int) -> dict:
+    """Wall time of single-step calls (launch, run and synchronize)."""
+    import numpy as np
+    t = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        runner.run(1)
+        t.append((time.perf_counter() - t0) * 1e3)
+    return {"p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99)),
+            "samples": nsteps}
 
 
 def cpu_baseline(seconds: float, dt: float):
@@ -151,6 +192,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--latency-steps", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=len(BATCH_DEPTHS_KM),
+                    help="wavefields of the batched leg (1 = plain only)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=os.path.join(REPO, "data", "bench_10s"))
@@ -165,25 +209,30 @@ def main():
               "one MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from axisem_amd.solver import Solver
+    from axisem_amd.solver import Batch, Solver
 
-    os.makedirs(a.workdir, exist_ok=True)
-    state = prepare_state(a.workdir)
+    nbatch = max(1, min(a.batch, len(BATCH_DEPTHS_KM)))
+    states = prepare_states(a.workdir, nbatch if a.batch > 1 else 1)
+    state = states[0]
     st = read_state(state)
     d = st.dims
     gll = 25 * (d["nel_solid"] + d["nel_fluid"])
-    if a.warmup + a.steps + a.profile_steps > d["niter"]:
+    need = a.warmup + a.steps + a.profile_steps + a.latency_steps
+    if need > d["niter"]:
         raise SystemExit("warmup + steps exceed the %d steps of the workload" % d["niter"])
+
+    # ---- plain path: one wavefield ----
     solver = Solver.from_state_file(state)
-    log("warmup %d steps" % a.warmup)
+    log("plain: warmup %d steps" % a.warmup)
     solver.run(a.warmup)
-    log("timed %d steps" % a.steps)
+    log("plain: timed %d steps" % a.steps)
     t0 = time.perf_counter()
     solver.run(a.steps)           # returns after the library stream is synchronized
     t1 = time.perf_counter()
     wall = t1 - t0
     dev_ms = solver.last_run_ms()
     value = gll * a.steps / wall
+    latency = step_latency(solver, a.latency_steps)
 
     # per-kernel durations (HIP events around every launch, same stream)
     solver.set_profiling(True)
@@ -194,10 +243,50 @@ def main():
     sk_avg_s = sk_ms / sk_n * 1e-3
     sk_bytes = solid_kernel_bytes(st)
     achieved = sk_bytes / sk_avg_s / 1e9
-    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
-               for k, v in stats.items() if v[1]}
-    log("kernels: " + json.dumps(kernels))
+    kernels = kernel_table(stats)
+    log("plain kernels: " + json.dumps(kernels))
     solver.close()
+
+    # ---- batched path: nbatch wavefields on the same mesh ----
+    batched = None
+    if a.batch > 1:
+        members = [Solver.from_state_file(p) for p in states]
+        with Batch(members) as b:
+            log("batched x%d: warmup %d steps" % (nbatch, a.warmup))
+            b.run(a.warmup)
+            log("batched x%d: timed %d steps" % (nbatch, a.steps))
+            t0 = time.perf_counter()
+            b.run(a.steps)
+            bwall = time.perf_counter() - t0
+            blat = step_latency(b, a.latency_steps)
+            b.set_profiling(True)
+            b.run(a.profile_steps)
+            bstats = b.kernel_stats()
+            b.set_profiling(False)
+        for m in members:
+            m.close()
+        bsk_ms, bsk_n = bstats["solid_stiffness"]
+        bsk_avg_s = bsk_ms / bsk_n * 1e-3
+        bsk_bytes = solid_kernel_bytes(st, nbatch)
+        btotal = nbatch * gll * a.steps / bwall
+        batched = {
+            "wavefields": nbatch,
+            "sources": "dipole thetaforce at %s km" % "/".join("%g" % x for x in
+                                                               BATCH_DEPTHS_KM[:nbatch]),
+            # GLL-point updates of all member wavefields per second, against
+            # the plain path's updates per second
+            "value": btotal,
+            "vs_plain": btotal / value,
+            "ms_per_step": bwall / a.steps * 1e3,
+            "latency": blat,
+            "roofline": {"kernel": "solid_stiffness, %d wavefields per launch" % nbatch,
+                         "achieved": bsk_bytes / bsk_avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": bsk_bytes / bsk_avg_s / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": bsk_bytes,
+                         "avg_launch_us": bsk_avg_s * 1e6},
+            "kernels_us": kernel_table(bstats),
+        }
+        log("batched kernels: " + json.dumps(batched["kernels_us"]))
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -236,7 +325,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": sk_bytes,
                      "avg_launch_us": sk_avg_s * 1e6},
+        "latency": latency,
         "kernels_us": kernels,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -36,6 +36,7 @@ extern "C" {
 #endif
 
 #define AXH_ABI_VERSION 1
+#define AXH_MAX_BATCH 8     /* wavefields per batch */
 
 enum axh_status {
   AXH_OK = 0,
@@ -227,6 +228,36 @@ int axh_kernel_stats(const axh_ctx *ctx, int kclass, double *total_ms,
 
 /* Device memory held by the context [bytes] (0 before the first run). */
 int axh_device_bytes(axh_ctx *ctx, size_t *bytes);
+
+/* ---- batches of wavefields ------------------------------------------------ */
+
+/* A batch advances several contexts that share one mesh, in lock step, with
+ * one launch per kernel for all of them.  The reference runs one source per
+ * SOLVER process (SOLVER/main.f90:92, time_loop of
+ * SOLVER/time_evol_wave.F90:26-41) and a multi-source study runs the program
+ * once per source; a batch is that loop over sources with the mesh terms read
+ * from HBM once per step for all members.
+ *
+ * Members must have equal dims except nelsrc, fluid_src, num_rec, seis_it,
+ * niter and magnitude, and byte-equal uploads except ielsrc, source_term, stf
+ * and recfile_el (the per-source inputs).  Each member's fields, seismograms
+ * and blow-up test are its own and bit-identical to stepping it alone with
+ * axh_run.  1 <= n <= AXH_MAX_BATCH.  The batch does not own the members; a
+ * member in a batch takes no uploads and axh_destroy refuses it until
+ * axh_batch_destroy.  Like axh_create, creating a batch touches no GPU. */
+typedef struct axh_batch axh_batch;
+int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out);
+int axh_batch_destroy(axh_batch *batch);
+
+/* axh_run for every member: all members must be at step it_begin - 1 and
+ * have it_end <= their niter.  seis[b] (seis or seis[b] may be NULL) receives
+ * member b's samples in the layout of axh_run. */
+int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end,
+                  float *const *seis);
+int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms);
+int axh_batch_set_profiling(axh_batch *batch, int on);
+int axh_batch_kernel_stats(const axh_batch *batch, int kclass,
+                           double *total_ms, int64_t *launches);
 
 /* ---- state files --------------------------------------------------------- */
 

@@ -25,8 +25,9 @@ def host_exe():
 
 @pytest.fixture(scope="session")
 def state_cache(tmp_path_factory, host_exe):
-    """Record (once per session) the state of a golden case with the drop-in
-    host in record mode; returns a function name -> (rundir, state path)."""
+    """Record (once per session) the state of a golden case or batch member
+    with the drop-in host in record mode; returns a function name -> (rundir,
+    state path)."""
     import cases
     from axisem_amd import inputs
 
@@ -36,7 +37,7 @@ def state_cache(tmp_path_factory, host_exe):
     def get(name):
         if name not in done:
             rundir = str(base / name)
-            inputs.write_run_dir(rundir, cases.MESH_DIR, cases.CASES[name])
+            inputs.write_run_dir(rundir, cases.MESH_DIR, cases.ALL_CONFIGS[name])
             st = os.path.join(rundir, "state.bin")
             inputs.record_state(host_exe, rundir, st)
             done[name] = (rundir, st)

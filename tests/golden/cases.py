@@ -42,6 +42,31 @@ CASES = {
                                             STATIONS),
 }
 
+# Source batches: several sources on the mesh above with one source type and
+# time scheme, stepped together by the batched time loop.  There is no
+# reference output for them; each member is checked against its own solo run
+# and against the oracle.
+BATCH_MEMBERS = {
+    "b_dipole_thetaforce_50": RunConfig("thetaforce", 50.0, 1e20, "symplec4", True, 600.0,
+                                        STATIONS),
+    "b_dipole_phiforce_1000": RunConfig("phiforce", 1000.0, 1e20, "symplec4", True, 600.0,
+                                        STATIONS),
+    "b_dipole_mtr_700": RunConfig("mtr", 700.0, 3e20, "symplec4", True, 600.0, STATIONS),
+    "b_dipole_mpr_2000": RunConfig("mpr", 2000.0, 1e20, "symplec4", True, 600.0,
+                                   STATIONS[::2]),
+    "b_mono_mtt_p_mpp_300": RunConfig("mtt_p_mpp", 300.0, 1e20, "newmark2", True, 900.0,
+                                      STATIONS),
+    "b_mono_explosion_100": RunConfig("explosion", 100.0, 1e20, "newmark2", True, 900.0,
+                                      STATIONS),
+    "b_mono_vertforce_0": RunConfig("vertforce", 0.0, 2e20, "newmark2", True, 700.0, STATIONS),
+}
+BATCHES = {
+    "dipole_symplec4_anel": ["b_dipole_thetaforce_50", "b_dipole_phiforce_1000",
+                             "b_dipole_mtr_700", "b_dipole_mpr_2000"],
+    "mono_newmark_anel": ["b_mono_mtt_p_mpp_300", "b_mono_explosion_100", "b_mono_vertforce_0"],
+}
+ALL_CONFIGS = {**CASES, **BATCH_MEMBERS}
+
 NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
          "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}
 

@@ -121,3 +121,48 @@ def test_inputs_pass_layout_checks_and_numbering_is_igloc(state_cache):
             first = np.unique(gid, return_index=True)[1]
             assert np.all(np.diff(first) > 0)
         s.close()
+
+
+def _ctx_array(solvers):
+    return (C.c_void_p * len(solvers))(*[s._h for s in solvers])
+
+
+def test_batch_members_share_the_mesh(state_cache):
+    """Sources of one type at different depths on one mesh form a batch: every
+    upload except the per-source ones is byte-equal (checked by
+    axh_batch_create, which touches no GPU)."""
+    import cases
+    lib = S.library()
+    for batch, names in sorted(cases.BATCHES.items()):
+        solvers = [S.Solver.from_state_file(state_cache(n)[1]) for n in names]
+        h = C.c_void_p()
+        S._check(lib.axh_batch_create(_ctx_array(solvers), len(solvers), C.byref(h)))
+        # members are held: no destroy, no uploads
+        assert lib.axh_destroy(solvers[0]._h) == -2
+        assert b"batch" in lib.axh_last_error()
+        rec = np.ones(3 * solvers[1].dims["num_rec"], np.int32)
+        assert lib.axh_upload_receivers(solvers[1]._h, rec.ctypes.data_as(C.c_void_p)) == -2
+        S._check(lib.axh_batch_destroy(h))
+        for s in solvers:
+            s.close()
+
+
+def test_batch_rejects_other_meshes_and_bad_sizes(state_cache):
+    import cases
+    lib = S.library()
+    mono = S.Solver.from_state_file(state_cache(cases.BATCHES["mono_newmark_anel"][0])[1])
+    di = [S.Solver.from_state_file(state_cache(n)[1])
+          for n in cases.BATCHES["dipole_symplec4_anel"][:2]]
+    h = C.c_void_p()
+    assert lib.axh_batch_create(_ctx_array([di[0], mono]), 2, C.byref(h)) == -1
+    assert b"differs" in lib.axh_last_error()
+    assert lib.axh_batch_create(_ctx_array([di[0], di[0]]), 2, C.byref(h)) == -1
+    assert lib.axh_batch_create(_ctx_array(di), 0, C.byref(h)) == -1
+    nine = [di[0]] * 9
+    assert lib.axh_batch_create(_ctx_array(nine), 9, C.byref(h)) == -1
+    assert lib.axh_batch_run(None, 1, 1, None) == -1
+    # a different elastic/anelastic setting is a different mesh part
+    el = S.Solver.from_state_file(state_cache("dipole_mtr_symplec4")[1])
+    assert lib.axh_batch_create(_ctx_array([di[0], el]), 2, C.byref(h)) == -1
+    for s in (mono, el, *di):
+        s.close()

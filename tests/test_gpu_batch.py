@@ -1,0 +1,117 @@
+"""Batched wavefields on one mesh (needs an MI355X).
+
+A batch steps several sources of one type on one mesh with one launch per
+kernel.  Every member must produce, bit for bit, the seismograms and fields of
+its own solo run, and the oracle's.
+"""
+import numpy as np
+import pytest
+
+import cases
+from axisem_amd.state import read_dims, read_state
+from axisem_amd.solver import Batch, BlowUp, Solver
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("disp", "velo", "chi", "dchi", "ddchi")
+
+
+def _same_fields(a, b, what):
+    for nm, x, y in zip(NAMES, a, b):
+        np.testing.assert_array_equal(x, y, err_msg="%s %s" % (what, nm))
+
+
+def _solo(path, n):
+    with Solver.from_state_file(path) as s:
+        seis = s.run(n)
+        return seis, s.fields()
+
+
+@pytest.mark.parametrize("batch", sorted(cases.BATCHES))
+def test_batch_members_match_solo_and_oracle(batch, state_cache):
+    names = cases.BATCHES[batch]
+    print("batch vs solo and oracle:", batch, names, flush=True)
+    paths = [state_cache(n)[1] for n in names]
+    n = 300
+    solo = [_solo(p, n) for p in paths]
+    solvers = [Solver.from_state_file(p) for p in paths]
+    try:
+        with Batch(solvers) as b:
+            first = b.run(117)
+            second = b.run(n - 117)
+        for k, nm in enumerate(names):
+            seis = np.concatenate([first[k], second[k]])
+            np.testing.assert_array_equal(seis, solo[k][0], err_msg=nm)
+            _same_fields(solvers[k].fields(), solo[k][1], nm)
+            # the sources differ: so must the members' seismograms
+            if k:
+                assert not np.array_equal(solo[k][0], solo[0][0])
+    finally:
+        for s in solvers:
+            s.close()
+    for k, p in enumerate(paths):
+        d = read_dims(p)
+        o = Oracle(p)
+        oseis = o.run(1, n, d["num_rec"], d["seis_it"])
+        np.testing.assert_array_equal(solo[k][0], oseis, err_msg=names[k])
+        _same_fields(solo[k][1], o.fields(), names[k] + " oracle")
+
+
+def test_batch_continues_solo_runs(state_cache):
+    """Members that ran alone first (each on its own device mesh) join one
+    batch and continue exactly."""
+    names = cases.BATCHES["dipole_symplec4_anel"][:3]
+    paths = [state_cache(n)[1] for n in names]
+    print("solo then batch:", names, flush=True)
+    ref = [_solo(p, 100) for p in paths]
+    solvers = [Solver.from_state_file(p) for p in paths]
+    try:
+        head = [s.run(40) for s in solvers]
+        with Batch(solvers) as b:
+            mid = b.run(50)
+        # after the batch is gone the members run alone again; members 1 and 2
+        # keep stepping on member 0's device mesh after member 0 is closed
+        tail = [solvers[0].run(10)]
+        f0 = solvers[0].fields()
+        solvers[0].close()
+        tail += [s.run(10) for s in solvers[1:]]
+        for k in range(len(solvers)):
+            seis = np.concatenate([head[k], mid[k], tail[k]])
+            np.testing.assert_array_equal(seis, ref[k][0], err_msg=names[k])
+            _same_fields(f0 if k == 0 else solvers[k].fields(), ref[k][1], names[k])
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_batch_of_one_equals_solo(state_cache):
+    path = state_cache(cases.BATCHES["mono_newmark_anel"][0])[1]
+    ref = _solo(path, 120)
+    with Solver.from_state_file(path) as s:
+        with Batch([s]) as b:
+            seis = b.run(120)[0]
+            assert b.last_run_ms() > 0
+        np.testing.assert_array_equal(seis, ref[0])
+        _same_fields(s.fields(), ref[1], "batch of one")
+
+
+def test_batch_member_blowup_is_reported(state_cache):
+    """The blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042) is per
+    member: forced from the host in member 1 only."""
+    names = cases.BATCHES["mono_newmark_anel"][:2]
+    paths = [state_cache(n)[1] for n in names]
+    st = read_state(paths[1])
+    disp = np.zeros(75 * st.dims["nel_solid"], np.float32)
+    g = st["igloc_solid"][6] - 1              # point (1,1) of element 1
+    mask = (st["igloc_solid"] - 1) == g
+    disp[:25 * st.dims["nel_solid"]][mask] = 1e22
+    solvers = [Solver.from_state_file(p) for p in paths]
+    try:
+        solvers[1].set_fields(disp=disp)
+        with Batch(solvers) as b:
+            with pytest.raises(BlowUp, match="batch member 1"):
+                b.run(3)
+    finally:
+        for s in solvers:
+            s.close()
