@@ -17,13 +17,16 @@
 //    sums are the reference's bit for bit.  No atomics.
 //  * The mesh part (stiffness terms, gather lists, masses, coupling maps,
 //    attenuation terms) lives once per mesh and is shared by every wavefield
-//    stepped on it.  A kernel advances up to kMaxBatch wavefields: it reads
-//    each mesh term once into registers and loops over the wavefields, so a
-//    batch of sources on one mesh pays the mesh traffic once per step.
+//    stepped on it.  Wavefields are advanced in groups of B = 1..4: every
+//    per-wavefield array of a group is interleaved with the wavefield index
+//    innermost (element x of wavefield b at x*B + b), so one lane loads,
+//    computes and stores the B wavefields of its point together, every mesh
+//    term is read once for the group and every barrier is shared by it.
+//    B = 1 is the plain layout.
 //  * Arithmetic order follows the reference expression by expression; the
 //    file is compiled with -ffp-contract=off (no FMA), and dp constants enter
 //    in double precision where the Fortran promotes to dp.  A wavefield's
-//    arithmetic does not depend on the other members of its batch.
+//    arithmetic does not depend on the other members of its group.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +57,8 @@ constexpr int kNcFluidMono = 3, kNcFluidMulti = 4;
 // Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi.  Read by the memory-variable update:
 // DsDeta, DzDeta, DsDxi, DzDxi over J, inv_s, delta_mu, delta_kappa, pad.
 constexpr int kCgStage = 5, kCgUpd = 8;
+// wavefields per kernel instance (interleaved group width)
+constexpr int kGroup = AXH_MAX_GROUP;
 
 // flag bits per global point
 constexpr uint8_t kFlagAxis = 1;      // on the symmetry axis (masked)
@@ -133,89 +138,63 @@ struct MeshDev {
   double *aj_mu = nullptr, *aj_k = nullptr;
 };
 
-// Wavefield part: fields, element forces, memory variables, source,
-// receivers.
-struct WaveDev {
-  DevAlloc mem;
-  float *disp = nullptr, *velo = nullptr, *acc0 = nullptr, *force_s = nullptr;
-  float *chi = nullptr, *dchi = nullptr, *ddchi = nullptr, *force_f = nullptr;
-  float *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr, *devm1 = nullptr;
-  float *axb_val = nullptr;   // unmasked prediction of the axis boundary points
-  int nsrc = 0;               // solid source points
-  std::vector<int> src_g_h;
-  int *src_g = nullptr;
-  float *src_val = nullptr;   // [nsrc][3]
-  int fluid_src = 0;
-  int *srcel_f = nullptr;     // per fluid element: source slot k or -1
-  float *src_fl = nullptr;    // source_term_fl (25 x 8)
-  int nrec = 0, seis_it = 1;
-  int *rec_g = nullptr;
-  float *seis_d = nullptr;
-  size_t seis_cap = 0;
-  int *blow = nullptr;
-  double blow_thr = 0.0;
-  std::vector<float> stf;     // [niter][nstages]
-};
-
-// Stream, events and per-kernel timing of the runs of a context or a batch.
-struct Runner {
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool profile = false;
-  std::vector<hipEvent_t> kev;  // pairs
-  std::vector<int> kev_class;
-  double kms[KC_N] = {0};
-  long kcount[KC_N] = {0};
-  void init() {
-    if (stream) return;
-    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHECK(hipEventCreate(&ev0));
-    HIPCHECK(hipEventCreate(&ev1));
+// ---------------------------------------------------------------------------
+// B consecutive floats (one point, the wavefields of a group)
+// ---------------------------------------------------------------------------
+template <int B>
+__device__ __forceinline__ void ldv(const float *p, float (&o)[B]) {
+  if constexpr (B == 4) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else if constexpr (B == 2) {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = p[b];
   }
-  ~Runner() {
-    // teardown: errors are not actionable here
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (auto e : kev) (void)hipEventDestroy(e);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+}
+template <int B>
+__device__ __forceinline__ void stv(float *p, const float (&o)[B]) {
+  if constexpr (B == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+  } else if constexpr (B == 2) {
+    *reinterpret_cast<float2 *>(p) = make_float2(o[0], o[1]);
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) p[b] = o[b];
   }
-};
-
-struct Device {
-  std::shared_ptr<MeshDev> mesh;
-  WaveDev wave;
-  Runner run;  // declared last: its stream is drained before buffers are freed
-};
+}
 
 // ---------------------------------------------------------------------------
-// kernel argument blocks: mesh part once, wavefield parts per batch member
+// kernel argument blocks: mesh part, the group's interleaved fields, and the
+// per-wavefield source / receiver data of its members
 // ---------------------------------------------------------------------------
 struct SolidArgs {
-  int nel, nsls, nglob, nslot, nb;
+  int nel, nsls, nglob, nslot;
   const float *G, *coef, *m0;
   const int *einfo, *gid;
   const float *cg_stage;  // [nel][5][4]
   const float *cg_upd;    // [nel][8][4]
   const double *aj_mu, *aj_k;  // [nel][nsls]
   double expw[kMaxSls], tst[kMaxSls], tstm1[kMaxSls];
-  const float *disp[kMaxBatch];   // [3][nglob]
-  float *force[kMaxBatch];        // [3][nslot]
-  float *memvar[kMaxBatch];       // [nel][nsls][6][4]
-  float *rsum[kMaxBatch];         // [nel][6][4]
-  float *trm1[kMaxBatch];         // [nel][4]
-  float *devm1[kMaxBatch];        // [nel][6][4]
+  const float *disp;      // [3][nglob][B]
+  float *force;           // [3][nslot][B]
+  float *memvar;          // [nel][nsls][6][4][B]
+  float *rsum;            // [nel][6][4][B]
+  float *trm1;            // [nel][4][B]
+  float *devm1;           // [nel][6][4][B]
 };
 
 struct FluidArgs {
-  int nel, multipole, nb;
+  int nel, multipole;
   const float *G, *coef, *m0;
   const int *einfo, *gid;
-  const float *chi[kMaxBatch];
-  float *force[kMaxBatch];
-  const int *srcel[kMaxBatch];
-  const float *src_fl[kMaxBatch];
-  float stf[kMaxBatch];  // 0 when the wavefield has no fluid source this stage
+  const float *chi;       // [nglob_f][B]
+  float *force;           // [nslot_f][B]
+  const int *srcel[kGroup];
+  const float *src_fl[kGroup];
+  float stf[kGroup];      // 0 when the wavefield has no fluid source this stage
 };
 
 // ---------------------------------------------------------------------------
@@ -223,6 +202,8 @@ struct FluidArgs {
 // ---------------------------------------------------------------------------
 // Spectral operators in LDS (105 floats, copied once per workgroup); lanes
 // index them with their own (i, j), which keeps the register footprint small.
+// Field tiles in LDS hold the B wavefields of a point together: entry k of a
+// wavefield's 5x5 tile is at k*S (S = B).
 struct GRegs {
   const float *gT;   // (axial ? G1T : G2T), column major
   const float *g;    // (axial ? G1 : G2)
@@ -248,25 +229,36 @@ __device__ __forceinline__ void stage_g(const float *G, float *sG) {
   for (int t = threadIdx.x; t < 105; t += blockDim.x) sG[t] = G[t];
 }
 
-// (A M)(i,j) = sum_k A(i,k) M(k,j)  for a 5x5 operator A in LDS
+// (A M)(i,j) = sum_k A(i,k) M(k,j)  for a 5x5 operator A
+template <int S>
 __device__ __forceinline__ float amul(const float *A, const float *M, int i, int j) {
   float s = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 5; k++) s = s + A[i + 5 * k] * M[k + 5 * j];
+  for (int k = 0; k < 5; k++) s = s + A[i + 5 * k] * M[(k + 5 * j) * S];
   return s;
 }
-// (M B)(i,j) = sum_k M(i,k) B(k,j)  for a 5x5 operator B in LDS
-__device__ __forceinline__ float mulb(const float *M, const float *B, int i, int j) {
+// (M B)(i,j) = sum_k M(i,k) B(k,j)  for a 5x5 operator B
+template <int S>
+__device__ __forceinline__ float mulb(const float *M, const float *Bop, int i, int j) {
   float s = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 5; k++) s = s + M[i + 5 * k] * B[k + 5 * j];
+  for (int k = 0; k < 5; k++) s = s + M[(i + 5 * k) * S] * Bop[k + 5 * j];
   return s;
 }
-// sum_k v(k) B(k,j)
-__device__ __forceinline__ float vmul(const float *v, const float *B, int j) {
+// sum_k g(k) M(k,j) for an operator vector g
+template <int S>
+__device__ __forceinline__ float vgm(const float *g, const float *M, int j) {
   float s = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 5; k++) s = s + v[k] * B[k + 5 * j];
+  for (int k = 0; k < 5; k++) s = s + g[k] * M[(k + 5 * j) * S];
+  return s;
+}
+// sum_k v(k) B(k,j) for a field vector v and an operator B
+template <int S>
+__device__ __forceinline__ float vmb(const float *v, const float *Bop, int j) {
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) s = s + v[k * S] * Bop[k + 5 * j];
   return s;
 }
 
@@ -296,21 +288,22 @@ __device__ __forceinline__ void grad_cg4(const float *G, bool axial, const float
   gz = cu[0 * 4 + q] * m1 + cu[2 * 4 + q] * m2;
 }
 
-// Voigt strain E1..E6 at cg point q from the element's displacement in LDS.
-template <int SRC>
+// Voigt strain E1..E6 at cg point q from a wavefield's displacement tile in
+// LDS (entry k at k*S).
+template <int SRC, int S>
 __device__ void strain_cg4(const float *G, bool axial, const float *cu, int q,
                            const float *U1, const float *U2, const float *U3,
                            float g[6]) {
   const float *invs = cu + 4 * 4;
-  const int p = cg_r(q) + 5 * cg_c(q);
+  const int p = (cg_r(q) + 5 * cg_c(q)) * S;
   float b1s, b1z, b2s, b2z;
 #pragma unroll
   for (int c = 0; c < 6; c++) g[c] = 0.0f;
   if (SRC == AXH_DIPOLE)
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k] + U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] + U2[k * S]; }, b1s, b1z);
   else
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k]; }, b1s, b1z);
-  grad_cg4(G, axial, cu, q, [&](int k) { return U3[k]; }, b2s, b2z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S]; }, b1s, b1z);
+  grad_cg4(G, axial, cu, q, [&](int k) { return U3[k * S]; }, b2s, b2z);
   g[0] = b1s;
   g[2] = b2z;
   g[4] = b1z + b2s;
@@ -318,13 +311,13 @@ __device__ void strain_cg4(const float *G, bool axial, const float *cu, int q,
     g[1] = invs[q] * U1[p];
   } else if (SRC == AXH_DIPOLE) {
     g[1] = 2.0f * (invs[q] * U2[p]);
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k] - U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] - U2[k * S]; }, b1s, b1z);
     float f3 = invs[q] * U3[p];
     g[3] = -f3 - b1z;
     g[5] = -g[1] - b1s;
   } else {
     g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
-    grad_cg4(G, axial, cu, q, [&](int k) { return U2[k]; }, b1s, b1z);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U2[k * S]; }, b1s, b1z);
     float f3 = invs[q] * U3[p];
     float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
     g[3] = -(2.0f * f3) - b1z;
@@ -353,58 +346,87 @@ __device__ __forceinline__ void memvar_sources(const float *cu, int q, const flo
 }
 
 // Strain sources of the memory-variable update at cg point q of element e
-// for wavefield b (lanes k < 4 of the element, one cg point each).  The
-// sources of the previous update go to old, the new ones to src and to the
-// wavefield's t-1 store.
-template <int SRC>
+// for wavefield b of the group (lanes k < 4 of the element, one cg point
+// each).  The sources of the previous update go to old, the new ones to src
+// and to the t-1 store; src/old are [28][B] per element.
+template <int SRC, int B>
 __device__ __forceinline__ void memvar_prepare(const SolidArgs &A, int b, int e, int q,
                                                bool axial, const float *U1, const float *U2,
                                                const float *U3, float *src, float *old) {
   const float *cu = A.cg_upd + (size_t)e * (kCgUpd * 4);
   float g[6], str, sdev[6];
-  strain_cg4<SRC>(A.G, axial, cu, q, U1, U2, U3, g);
+  strain_cg4<SRC, B>(A.G, axial, cu, q, U1, U2, U3, g);
   memvar_sources<SRC>(cu, q, g, str, sdev);
-  float *trm1 = A.trm1[b] + (size_t)e * 4;
-  float *devm1 = A.devm1[b] + (size_t)e * 24;
-  old[q] = trm1[q];
+  float *trm1 = A.trm1 + (size_t)e * 4 * B;
+  float *devm1 = A.devm1 + (size_t)e * 24 * B;
+  old[q * B + b] = trm1[q * B + b];
 #pragma unroll
-  for (int c = 0; c < 6; c++) old[4 + 4 * c + q] = devm1[4 * c + q];
-  src[q] = str;
+  for (int c = 0; c < 6; c++) old[(4 + 4 * c + q) * B + b] = devm1[(4 * c + q) * B + b];
+  src[q * B + b] = str;
 #pragma unroll
-  for (int c = 0; c < 6; c++) src[4 + 4 * c + q] = sdev[c];
-  trm1[q] = str;
+  for (int c = 0; c < 6; c++) src[(4 + 4 * c + q) * B + b] = sdev[c];
+  trm1[q * B + b] = str;
 #pragma unroll
-  for (int c = 0; c < 6; c++) devm1[4 * c + q] = sdev[c];
+  for (int c = 0; c < 6; c++) devm1[(4 * c + q) * B + b] = sdev[c];
 }
 
-// Update of the memory variables of element e of wavefield b.  The work of
-// the element is spread over its lanes: lane k handles entries idx = k, k+25,
-// ... of (q, c, j) = (idx%4, (idx/4)%6, idx/24).  mv/src/old in LDS.
-template <int SRC>
-__device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int b, int e, int k,
-                                                   const float *mv_lds, const float *src_lds,
+// Update of the memory variables of element e for the B wavefields of the
+// group.  For each SLS jj (uniform across the element's lanes) the 24 * B
+// entries r = (c*4 + q)*B + b of that SLS are spread over the element's 25
+// lanes.  mv_old holds the element's current memory variables (LDS or its own
+// place in HBM: every entry is read before the same lane overwrites it);
+// src/old in LDS; rs_lds (may be NULL) receives the new values.
+template <int SRC, int B>
+__device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, int k,
+                                                   const float *mv_old, const float *src_lds,
                                                    const float *old_lds, float *rs_lds) {
   const int nsls = A.nsls;
-  const int n = 24 * nsls;
-  float *memvar = A.memvar[b] + (size_t)e * n;
-  for (int idx = k; idx < n; idx += 25) {
-    const int q = idx & 3, c = (idx >> 2) % 6, jj = idx / 24;
-    if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
+  float *memvar = A.memvar + (size_t)e * 24 * nsls * B;
+  for (int jj = 0; jj < nsls; jj++) {
     const double Tt = A.tst[jj], Tm = A.tstm1[jj], E = A.expw[jj];
     const double amu = A.aj_mu[(size_t)e * nsls + jj];
-    const double m = (double)mv_lds[idx];
-    const float db = (float)(Tt * amu * (double)src_lds[4 + 4 * c + q] +
-                             Tm * amu * (double)old_lds[4 + 4 * c + q]);
-    float nv;
-    if (c < 3) {
-      const double ak = A.aj_k[(size_t)e * nsls + jj];
-      const float trb = (float)(Tt * ak * (double)src_lds[q] + Tm * ak * (double)old_lds[q]);
-      nv = (float)((E * m + (double)db) + (double)trb);
-    } else {
-      nv = (float)(E * m + (double)db);
+    const double ak = A.aj_k[(size_t)e * nsls + jj];
+    for (int r = k; r < 24 * B; r += 25) {
+      const int cq = r / B, b = r - cq * B;
+      const int q = cq & 3, c = cq >> 2;
+      if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
+      const int m = jj * 24 * B + r;
+      const double mv = (double)mv_old[m];
+      const float db = (float)(Tt * amu * (double)src_lds[(4 + cq) * B + b] +
+                               Tm * amu * (double)old_lds[(4 + cq) * B + b]);
+      float nv;
+      if (c < 3) {
+        const float trb = (float)(Tt * ak * (double)src_lds[q * B + b] +
+                                  Tm * ak * (double)old_lds[q * B + b]);
+        nv = (float)((E * mv + (double)db) + (double)trb);
+      } else {
+        nv = (float)(E * mv + (double)db);
+      }
+      memvar[m] = nv;
+      if (rs_lds) rs_lds[m] = nv;
     }
-    memvar[idx] = nv;
-    if (rs_lds) rs_lds[idx] = nv;
+  }
+}
+
+// gather the (predicted, masked) displacement of the tile's points into LDS
+// (monopole: components s and z)
+template <int NCOMP, int B>
+__device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, size_t slot, int t,
+                                            float *sU) {
+  float u[B];
+  if (valid) {
+    const int gp = A.gid[slot];
+#pragma unroll
+    for (int cc = 0; cc < NCOMP; cc++) {
+      const int c = (NCOMP == 2 && cc == 1) ? 2 : cc;
+      ldv<B>(A.disp + ((size_t)c * A.nglob + gp) * B, u);
+      stv<B>(sU + (size_t)cc * 256 * B + t * B, u);
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) u[b] = 0.0f;
+#pragma unroll
+    for (int cc = 0; cc < NCOMP; cc++) stv<B>(sU + (size_t)cc * 256 * B + t * B, u);
   }
 }
 
@@ -414,24 +436,24 @@ __device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int b, in
 // glob_anel_stiffness_*_cg4 and, for newmark2, time_step_memvars_cg4.
 // MODE 0: newmark2 (memory variables read, used, updated)
 // MODE 1: symplectic stage (sums of memory variables read)
-// One workgroup = one tile of 10 elements; the tile's stiffness terms are read
-// once and applied to the A.nb wavefields in turn.
+// One workgroup = one tile of 10 elements, one lane = one GLL point and the
+// B wavefields of the group; the lane's stiffness terms are read once.
 // ---------------------------------------------------------------------------
-template <int SRC, int MODE, bool ANEL>
+template <int SRC, int MODE, bool ANEL, int B>
 __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   constexpr int NC = (SRC == AXH_MONOPOLE) ? kNcMono
                      : (SRC == AXH_DIPOLE) ? kNcDi : kNcQuad;
   constexpr int NS = (SRC == AXH_MONOPOLE) ? 4 : 6;
-  __shared__ float sU[NCOMP][256];
-  __shared__ float sS[NS][256];
-  __shared__ float sV[kElemPerTile][8];
-  __shared__ float sMv[ANEL ? (MODE == 0 ? kElemPerTile * 24 * kMaxSls : 1) : 1];
-  __shared__ float sR[ANEL ? kElemPerTile * 24 : 1];
-  __shared__ float sSa[ANEL ? kElemPerTile * 24 : 1];
-  __shared__ float sSrc[ANEL ? kElemPerTile * 28 : 1];
-  __shared__ float sOld[ANEL ? kElemPerTile * 28 : 1];
-  __shared__ float sG[105];
+  constexpr bool MV = ANEL && MODE == 0;  // memory variables updated here
+  __shared__ __align__(16) float sU[NCOMP * 256 * B];
+  __shared__ __align__(16) float sS[NS][256 * B];
+  __shared__ __align__(16) float sV[kElemPerTile * 5 * B];
+  __shared__ __align__(16) float sR[ANEL ? kElemPerTile * 24 * B : 1];
+  __shared__ __align__(16) float sSa[ANEL ? kElemPerTile * 24 * B : 1];
+  __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
+  __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
+  __shared__ __align__(16) float sG[105];
 
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
@@ -441,278 +463,298 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
-  const int gp = valid ? A.gid[slot] : 0;
+  const int eb = le * 25 * B;  // the element's first entry in an LDS tile
 
   stage_g(A.G, sG);
-  // the lane's stiffness terms, read once for every wavefield (the packed
-  // rows cover all 256 lanes of a tile)
+  // the lane's stiffness terms (the packed rows cover all 256 lanes)
   float cf[NC];
   {
     const float *coef = A.coef + (size_t)tile * NC * 256 + t;
 #pragma unroll
     for (int a = 0; a < NC; a++) cf[a] = coef[a * 256];
   }
+  gather_disp<NCOMP, B>(A, valid, slot, t, sU);
+  __syncthreads();
 
-  for (int b = 0; b < A.nb; b++) {
-    __syncthreads();  // LDS of the previous wavefield consumed; sG staged
-    // gather the (predicted, masked) displacement of the element
-    const float *disp = A.disp[b];
-    if (valid) {
-      if (SRC == AXH_MONOPOLE) {
-        sU[0][t] = disp[gp];
-        sU[1][t] = disp[2 * (size_t)A.nglob + gp];
-      } else {
+  GRegs G;
+  load_g(sG, axial, i, j, G);
+  float acc[3][B];
+  const float *U0 = sU + eb, *U1 = sU + 256 * B + eb, *U2 = sU + 2 * 256 * B + eb;
+
+  if (SRC == AXH_MONOPOLE) {
+    const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], mw1 = cf[4],
+                m11s = cf[5], m21s = cf[6], m41s = cf[7], m12s = cf[8], m22s = cf[9],
+                m32s = cf[10], m42s = cf[11], m11z = cf[12], m21z = cf[13], m41z = cf[14];
+    float ls[B], lz[B], V1[B], V2[B], V3[B];
 #pragma unroll
-        for (int c = 0; c < 3; c++) sU[c][t] = disp[c * (size_t)A.nglob + gp];
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < NCOMP; c++) sU[c][t] = 0.0f;
-    }
-    if (ANEL && MODE == 0) {
-      // memory variables of the tile (contiguous per element)
-      const int n = 24 * A.nsls;
-      for (int idx = k; idx < n && valid; idx += 25)
-        sMv[le * 24 * kMaxSls + idx] = A.memvar[b][(size_t)e * n + idx];
+    for (int b = 0; b < B; b++) {
+      const float *Us = U0 + b, *Uz = U1 + b;
+      const float us = Us[k * B], uz = Uz[k * B];
+      const float X1 = amul<B>(G.gT, Us, i, j), X2 = amul<B>(G.gT, Uz, i, j);
+      const float X3 = mulb<B>(Us, G.g2, i, j), X4 = mulb<B>(Uz, G.g2, i, j);
+      ls[b] = m4 * X4 + m2 * X3 + m1 * X1 + m3 * X2 + us * mw1;
+      sS[0][t * B + b] = m11s * X3 + m21s * X1 + m12s * X4 + m22s * X2 + m1 * us;
+      sS[1][t * B + b] = m11s * X1 + m41s * X3 + m32s * X2 + m42s * X4 + m2 * us;
+      sS[2][t * B + b] = m11z * X4 + m21z * X2 + m32s * X3 + m22s * X1 + m3 * us;
+      sS[3][t * B + b] = m11z * X2 + m41z * X4 + m12s * X1 + m42s * X3 + m4 * us;
     }
     __syncthreads();
-
-    GRegs G;
-    load_g(sG, axial, i, j, G);
-    float acc[3];
-
-    if (SRC == AXH_MONOPOLE) {
-      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25];
-      const float us = U0[k], uz = U1[k];
-      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j);
-      const float X3 = mulb(U0, G.g2, i, j), X4 = mulb(U1, G.g2, i, j);
-      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], mw1 = cf[4],
-                  m11s = cf[5], m21s = cf[6], m41s = cf[7], m12s = cf[8], m22s = cf[9],
-                  m32s = cf[10], m42s = cf[11], m11z = cf[12], m21z = cf[13], m41z = cf[14];
-      float ls = m4 * X4 + m2 * X3 + m1 * X1 + m3 * X2 + us * mw1;
-      sS[0][t] = m11s * X3 + m21s * X1 + m12s * X4 + m22s * X2 + m1 * us;
-      sS[1][t] = m11s * X1 + m41s * X3 + m32s * X2 + m42s * X4 + m2 * us;
-      sS[2][t] = m11z * X4 + m21z * X2 + m32s * X3 + m22s * X1 + m3 * us;
-      sS[3][t] = m11z * X2 + m41z * X4 + m12s * X1 + m42s * X3 + m4 * us;
-      __syncthreads();
-      const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
-                  *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25];
-      const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2z, G.g2T, i, j);
-      const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1z, i, j);
-      ls = ls + Y1 + Y2;
-      float lz = Y3 + Y4;
-      // axial terms with M0_w1..3
-      float w1 = 0, w2 = 0, w3 = 0, V1 = 0, V2 = 0, V3 = 0;
+    // axial terms with M0_w1..3
+    float w1 = 0, w2 = 0, w3 = 0;
+    if (axial) {
+      const float *m0 = A.m0 + (size_t)ax * 50;
+      w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
+    }
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j),
+                  Y4 = mulb<B>(&sS[3][eb + b], G.g2T, i, j);
+      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[2][eb + b], i, j);
+      ls[b] = ls[b] + Y1 + Y2;
+      lz[b] = Y3 + Y4;
       if (axial) {
-        const float *m0 = A.m0 + (size_t)ax * 50;
-        w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
-        V1 = vmul(G.g0, U0, j);        // vxm(G0, us)
-        V2 = mulb(U1, G.g2, 0, j);     // vxm(uz(0,:), G2)
-        V3 = vmul(G.g0, U1, j);        // vxm(G0, uz)
-        if (i == 0) sV[le][j] = w3 * V1;
+        V1[b] = vgm<B>(G.g0, U0 + b, j);        // vxm(G0, us)
+        V2[b] = mulb<B>(U1 + b, G.g2, 0, j);    // vxm(uz(0,:), G2)
+        V3[b] = vgm<B>(G.g0, U1 + b, j);        // vxm(G0, uz)
+        if (i == 0) sV[(le * 5 + j) * B + b] = w3 * V1[b];
       }
-      __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; b++) {
       if (axial) {
-        float V4 = w1 * V1 + w3 * V2;
-        V4 = V4 + w2 * V3;
-        float X2a = G.g0i * (w2 * V1);
-        const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(w3*V1, G2T)
+        float V4 = w1 * V1[b] + w3 * V2[b];
+        V4 = V4 + w2 * V3[b];
+        float X2a = G.g0i * (w2 * V1[b]);
+        const float V1b = vmb<B>(&sV[le * 5 * B + b], G.g2T, j);  // vxm(w3*V1, G2T)
         if (i == 0) X2a = X2a + V1b;
-        ls = ls + G.g0i * V4;
-        lz = X2a + lz;
+        ls[b] = ls[b] + G.g0i * V4;
+        lz[b] = X2a + lz[b];
       }
-      acc[0] = ls;
-      acc[1] = 0.0f;
-      acc[2] = lz;
-    } else if (SRC == AXH_DIPOLE) {
-      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
-      const float u1 = U0[k], u2 = U1[k], u3 = U2[k];
-      const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
-                  X6 = mulb(U2, G.g2, i, j);
-      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
-                  X3 = amul(G.gT, U2, i, j);
+      acc[0][b] = ls[b];
+      acc[1][b] = 0.0f;
+      acc[2][b] = lz[b];
+    }
+  } else if (SRC == AXH_DIPOLE) {
+    const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
+                m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
+                m11s = cf[11], m21s = cf[12], m41s = cf[13], m12s = cf[14],
+                m22s = cf[15], m42s = cf[16], m13s = cf[17], m23s = cf[18],
+                m33s = cf[19], m43s = cf[20], m11z = cf[21], m21z = cf[22],
+                m41z = cf[23];
+    float ls2[B], ls3[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const float *Ua = U0 + b, *Ub = U1 + b, *Uc = U2 + b;
+      const float u2 = Ub[k * B], u3 = Uc[k * B];
+      const float X4 = mulb<B>(Ua, G.g2, i, j), X5 = mulb<B>(Ub, G.g2, i, j),
+                  X6 = mulb<B>(Uc, G.g2, i, j);
+      const float X1 = amul<B>(G.gT, Ua, i, j), X2 = amul<B>(G.gT, Ub, i, j),
+                  X3 = amul<B>(G.gT, Uc, i, j);
       const float X7 = X1 + X2, X8 = X4 + X5;
-      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
-                  m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
-                  m11s = cf[11], m21s = cf[12], m41s = cf[13], m12s = cf[14],
-                  m22s = cf[15], m42s = cf[16], m13s = cf[17], m23s = cf[18],
-                  m33s = cf[19], m43s = cf[20], m11z = cf[21], m21z = cf[22],
-                  m41z = cf[23];
-      const float ls2 = m8 * X6 + m7 * X3 + m1 * X1 + m5 * X2 + m2 * X4 + m6 * X5 +
-                        mw1 * u2 + mw2 * u3;
-      const float ls3 = m4 * X4 - m4 * X5 + m3 * X1 - m3 * X2 + mw2 * u2 + mw3 * u3;
+      ls2[b] = m8 * X6 + m7 * X3 + m1 * X1 + m5 * X2 + m2 * X4 + m6 * X5 + mw1 * u2 +
+               mw2 * u3;
+      ls3[b] = m4 * X4 - m4 * X5 + m3 * X1 - m3 * X2 + mw2 * u2 + mw3 * u3;
       float c1 = m13s * X6, c2 = m23s * X3, c3 = m3 * u3;
-      sS[0][t] = c1 + c2 + c3 + m11s * X4 + m21s * X1 + m12s * X5 + m22s * X2 + m1 * u2;
-      sS[1][t] = c1 + c2 - c3 + m11s * X5 + m21s * X2 + m12s * X4 + m22s * X1 + m5 * u2;
+      sS[0][t * B + b] = c1 + c2 + c3 + m11s * X4 + m21s * X1 + m12s * X5 + m22s * X2 + m1 * u2;
+      sS[1][t * B + b] = c1 + c2 - c3 + m11s * X5 + m21s * X2 + m12s * X4 + m22s * X1 + m5 * u2;
       c1 = m33s * X3;
       c2 = m43s * X6;
       c3 = m4 * u3;
-      sS[2][t] = c1 + c2 + c3 + m11s * X1 + m41s * X4 + m12s * X2 + m42s * X5 + m2 * u2;
-      sS[3][t] = c1 + c2 - c3 + m11s * X2 + m41s * X5 + m12s * X1 + m42s * X4 + m6 * u2;
-      sS[4][t] = m33s * X8 + m23s * X7 + m11z * X6 + m21z * X3 + m7 * u2;
-      sS[5][t] = m13s * X7 + m43s * X8 + m11z * X3 + m41z * X6 + m8 * u2;
-      __syncthreads();
-      const float *S1p = &sS[0][le * 25], *S1m = &sS[1][le * 25],
-                  *S2p = &sS[2][le * 25], *S2m = &sS[3][le * 25],
-                  *S1z = &sS[4][le * 25], *S2z = &sS[5][le * 25];
-      const float Y1 = amul(G.g, S1p, i, j), Y3 = amul(G.g, S1m, i, j),
-                  Y5 = amul(G.g, S1z, i, j);
-      const float Y2 = mulb(S2p, G.g2T, i, j), Y4 = mulb(S2m, G.g2T, i, j),
-                  Y6 = mulb(S2z, G.g2T, i, j);
-      float l1 = Y1 + Y2;
-      float l2 = Y3 + Y4 + ls2;
-      float l3 = Y5 + Y6 + ls3;
-      float a = 0, bb = 0, c = 0;
+      sS[2][t * B + b] = c1 + c2 + c3 + m11s * X1 + m41s * X4 + m12s * X2 + m42s * X5 + m2 * u2;
+      sS[3][t * B + b] = c1 + c2 - c3 + m11s * X2 + m41s * X5 + m12s * X1 + m42s * X4 + m6 * u2;
+      sS[4][t * B + b] = m33s * X8 + m23s * X7 + m11z * X6 + m21z * X3 + m7 * u2;
+      sS[5][t * B + b] = m13s * X7 + m43s * X8 + m11z * X3 + m41z * X6 + m8 * u2;
+    }
+    __syncthreads();
+    float w1 = 0, w2 = 0, w3 = 0, w4 = 0, w6 = 0, w7 = 0, w8 = 0, w9 = 0, w10 = 0;
+    if (axial) {
+      const float *m0 = A.m0 + (size_t)ax * 50;
+      w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j]; w4 = m0[3 * 5 + j];
+      w6 = m0[5 * 5 + j]; w7 = m0[6 * 5 + j]; w8 = m0[7 * 5 + j]; w9 = m0[8 * 5 + j];
+      w10 = m0[9 * 5 + j];
+    }
+    float va[B], vb[B], vc[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[1][eb + b], i, j),
+                  Y5 = amul<B>(G.g, &sS[4][eb + b], i, j);
+      const float Y2 = mulb<B>(&sS[2][eb + b], G.g2T, i, j),
+                  Y4 = mulb<B>(&sS[3][eb + b], G.g2T, i, j),
+                  Y6 = mulb<B>(&sS[5][eb + b], G.g2T, i, j);
+      acc[0][b] = Y1 + Y2;
+      acc[1][b] = Y3 + Y4 + ls2[b];
+      acc[2][b] = Y5 + Y6 + ls3[b];
       if (axial) {
-        const float *m0 = A.m0 + (size_t)ax * 50;
-        const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
-                    w4 = m0[3 * 5 + j], w6 = m0[5 * 5 + j], w7 = m0[6 * 5 + j],
-                    w8 = m0[7 * 5 + j], w9 = m0[8 * 5 + j], w10 = m0[9 * 5 + j];
-        const float V1 = vmul(G.g0, U0, j);
-        const float V2 = vmul(G.g0, U1, j);
-        const float V3 = vmul(G.g0, U2, j);
-        const float V4 = mulb(U0, G.g2, 0, j);   // vxm(u1(0,:), G2)
-        a = w1 * V2 + w3 * V3;
-        bb = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
-        c = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
-        if (i == 0) sV[le][j] = (w2 + w6) * V2 + (w4 + w8) * V3;
+        const float V1 = vgm<B>(G.g0, U0 + b, j);
+        const float V2 = vgm<B>(G.g0, U1 + b, j);
+        const float V3 = vgm<B>(G.g0, U2 + b, j);
+        const float V4 = mulb<B>(U0 + b, G.g2, 0, j);   // vxm(u1(0,:), G2)
+        va[b] = w1 * V2 + w3 * V3;
+        vb[b] = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
+        vc[b] = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
+        if (i == 0) sV[(le * 5 + j) * B + b] = (w2 + w6) * V2 + (w4 + w8) * V3;
       }
-      __syncthreads();
-      if (axial) {
-        float S1pa = G.g0i * a;
-        const float S1ma = G.g0i * bb, S1za = G.g0i * c;
-        const float V1b = vmul(&sV[le][0], G.g2T, j);  // vxm(V4, G2T)
+    }
+    __syncthreads();
+    if (axial) {
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        float S1pa = G.g0i * va[b];
+        const float S1ma = G.g0i * vb[b], S1za = G.g0i * vc[b];
+        const float V1b = vmb<B>(&sV[le * 5 * B + b], G.g2T, j);  // vxm(V4, G2T)
         if (i == 0) S1pa = S1pa + V1b;
-        l1 = l1 + S1pa;
-        l2 = l2 + S1ma;
-        l3 = l3 + S1za;
+        acc[0][b] = acc[0][b] + S1pa;
+        acc[1][b] = acc[1][b] + S1ma;
+        acc[2][b] = acc[2][b] + S1za;
       }
-      acc[0] = l1;
-      acc[1] = l2;
-      acc[2] = l3;
-    } else {
-      const float *U0 = &sU[0][le * 25], *U1 = &sU[1][le * 25], *U2 = &sU[2][le * 25];
-      const float us = U0[k], up = U1[k], uz = U2[k];
-      const float X1 = amul(G.gT, U0, i, j), X2 = amul(G.gT, U1, i, j),
-                  X3 = amul(G.gT, U2, i, j);
-      const float X4 = mulb(U0, G.g2, i, j), X5 = mulb(U1, G.g2, i, j),
-                  X6 = mulb(U2, G.g2, i, j);
-      const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
-                  m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
-                  mw4 = cf[11], mw5 = cf[12], m11s = cf[13], m21s = cf[14],
-                  m41s = cf[15], m12s = cf[16], m22s = cf[17], m32s = cf[18],
-                  m42s = cf[19], m11z = cf[20], m21z = cf[21], m41z = cf[22],
-                  m1phi = cf[23], m2phi = cf[24], m4phi = cf[25];
+    }
+  } else {
+    const float m1 = cf[0], m2 = cf[1], m3 = cf[2], m4 = cf[3], m5 = cf[4], m6 = cf[5],
+                m7 = cf[6], m8 = cf[7], mw1 = cf[8], mw2 = cf[9], mw3 = cf[10],
+                mw4 = cf[11], mw5 = cf[12], m11s = cf[13], m21s = cf[14],
+                m41s = cf[15], m12s = cf[16], m22s = cf[17], m32s = cf[18],
+                m42s = cf[19], m11z = cf[20], m21z = cf[21], m41z = cf[22],
+                m1phi = cf[23], m2phi = cf[24], m4phi = cf[25];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const float *Ua = U0 + b, *Ub = U1 + b, *Uc = U2 + b;
+      const float us = Ua[k * B], up = Ub[k * B], uz = Uc[k * B];
+      const float X1 = amul<B>(G.gT, Ua, i, j), X2 = amul<B>(G.gT, Ub, i, j),
+                  X3 = amul<B>(G.gT, Uc, i, j);
+      const float X4 = mulb<B>(Ua, G.g2, i, j), X5 = mulb<B>(Ub, G.g2, i, j),
+                  X6 = mulb<B>(Uc, G.g2, i, j);
       const float c1 = m2 * X4, c2 = m1 * X1, c3 = m6 * X5, c4 = m5 * X2,
                   c5 = m4 * X6, c6 = m3 * X3;
-      float ls = c1 + c2 + 2.0f * (c3 + c4) + c5 + c6 + mw1 * us + mw2 * up +
-                 2.0f * mw3 * uz;
-      float lp = -2.0f * (c1 + c2 + c5 + c6) - (c3 + c4) + mw2 * us + mw4 * up -
-                 mw3 * uz;
-      float lz = 2.0f * (m8 * X5 + m7 * X2) + mw3 * (2.0f * us - up) + mw5 * uz;
+      acc[0][b] = c1 + c2 + 2.0f * (c3 + c4) + c5 + c6 + mw1 * us + mw2 * up +
+                  2.0f * mw3 * uz;
+      acc[1][b] = -2.0f * (c1 + c2 + c5 + c6) - (c3 + c4) + mw2 * us + mw4 * up -
+                  mw3 * uz;
+      acc[2][b] = 2.0f * (m8 * X5 + m7 * X2) + mw3 * (2.0f * us - up) + mw5 * uz;
       const float usp = us - 2.0f * up, tus = 2.0f * us - up;
-      sS[0][t] = m11s * X4 + m21s * X1 + m12s * X6 + m22s * X3 + m1 * usp;   // S1s
-      sS[1][t] = m11s * X1 + m41s * X4 + m32s * X3 + m42s * X6 + m2 * usp;   // S2s
-      sS[2][t] = m11z * X6 + m21z * X3 + m32s * X4 + m22s * X1 + m3 * usp;   // S1z
-      sS[3][t] = m11z * X3 + m41z * X6 + m12s * X1 + m42s * X4 + m4 * usp;   // S2z
-      sS[4][t] = m1phi * X5 + m2phi * X2 + m5 * tus + 2.0f * m7 * uz;        // S1phi
-      sS[5][t] = m1phi * X2 + m4phi * X5 + m6 * tus + 2.0f * m8 * uz;        // S2phi
-      __syncthreads();
-      const float *S1s = &sS[0][le * 25], *S2s = &sS[1][le * 25],
-                  *S1z = &sS[2][le * 25], *S2z = &sS[3][le * 25],
-                  *S1p = &sS[4][le * 25], *S2p = &sS[5][le * 25];
-      const float Y2 = mulb(S2s, G.g2T, i, j), Y4 = mulb(S2p, G.g2T, i, j),
-                  Y6 = mulb(S2z, G.g2T, i, j);
-      const float Y1 = amul(G.g, S1s, i, j), Y3 = amul(G.g, S1p, i, j),
-                  Y5 = amul(G.g, S1z, i, j);
-      ls = ls + Y1 + Y2;
-      lp = lp + Y3 + Y4;
-      lz = lz + Y5 + Y6;
-      __syncthreads();  // uniform barrier count with the other source types
-      if (axial) {
-        const float *m0 = A.m0 + (size_t)ax * 50;
-        const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
-                    w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
-        const float V1 = vmul(G.g0, U0, j), V2 = vmul(G.g0, U1, j),
-                    V3 = vmul(G.g0, U2, j);
+      sS[0][t * B + b] = m11s * X4 + m21s * X1 + m12s * X6 + m22s * X3 + m1 * usp;   // S1s
+      sS[1][t * B + b] = m11s * X1 + m41s * X4 + m32s * X3 + m42s * X6 + m2 * usp;   // S2s
+      sS[2][t * B + b] = m11z * X6 + m21z * X3 + m32s * X4 + m22s * X1 + m3 * usp;   // S1z
+      sS[3][t * B + b] = m11z * X3 + m41z * X6 + m12s * X1 + m42s * X4 + m4 * usp;   // S2z
+      sS[4][t * B + b] = m1phi * X5 + m2phi * X2 + m5 * tus + 2.0f * m7 * uz;        // S1phi
+      sS[5][t * B + b] = m1phi * X2 + m4phi * X5 + m6 * tus + 2.0f * m8 * uz;        // S2phi
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j),
+                  Y4 = mulb<B>(&sS[5][eb + b], G.g2T, i, j),
+                  Y6 = mulb<B>(&sS[3][eb + b], G.g2T, i, j);
+      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[4][eb + b], i, j),
+                  Y5 = amul<B>(G.g, &sS[2][eb + b], i, j);
+      acc[0][b] = acc[0][b] + Y1 + Y2;
+      acc[1][b] = acc[1][b] + Y3 + Y4;
+      acc[2][b] = acc[2][b] + Y5 + Y6;
+    }
+    __syncthreads();  // uniform barrier count with the other source types
+    if (axial) {
+      const float *m0 = A.m0 + (size_t)ax * 50;
+      const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
+                  w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const float V1 = vgm<B>(G.g0, U0 + b, j), V2 = vgm<B>(G.g0, U1 + b, j),
+                    V3 = vgm<B>(G.g0, U2 + b, j);
         const float a = w1 * V1 + w2 * V2 + w3 * V3;
         const float bq = w2 * V1 + w4 * V2 + w5 * V3;
         const float c = w3 * V1 + w5 * V2 + w6 * V3;
-        ls = ls + G.g0i * a;
-        lp = lp + G.g0i * bq;
-        lz = lz + G.g0i * c;
+        acc[0][b] = acc[0][b] + G.g0i * a;
+        acc[1][b] = acc[1][b] + G.g0i * bq;
+        acc[2][b] = acc[2][b] + G.g0i * c;
       }
-      acc[0] = ls;
-      acc[1] = lp;
-      acc[2] = lz;
     }
+  }
 
-    if (ANEL) {
-      // sums of the memory variables over the SLS, (c, q) per lane k < 24
-      float *R = &sR[le * 24];
-      if (valid && k < 24) {
-        if (MODE == 0) {
+  if (ANEL) {
+    // sums of the memory variables over the SLS, (c, q) per lane k < 24
+    float *R = &sR[le * 24 * B];
+    if (valid && k < 24) {
+      if (MODE == 0) {
+        const float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
+#pragma unroll
+        for (int b = 0; b < B; b++) {
           float s = 0.0f;
-          for (int jj = 0; jj < A.nsls; jj++) s = s + sMv[le * 24 * kMaxSls + jj * 24 + k];
-          R[k] = s;
-        } else {
-          R[k] = A.rsum[b][(size_t)e * 24 + k];
+          for (int jj = 0; jj < A.nsls; jj++) s = s + mv[(jj * 24 + k) * B + b];
+          R[k * B + b] = s;
         }
+      } else {
+        float r[B];
+        ldv<B>(A.rsum + ((size_t)e * 24 + k) * B, r);
+        stv<B>(R + k * B, r);
       }
-      const float *cs = A.cg_stage + (size_t)(valid ? e : 0) * (kCgStage * 4);
-      __syncthreads();
-      float *Sa = &sSa[le * 24];
-      if (valid && k < 4) {
-        const int q = k;
-        const float vse = cs[1 * 4 + q], vsx = cs[2 * 4 + q], vze = cs[3 * 4 + q],
-                    vzx = cs[4 * 4 + q];
-        const float r1 = R[0 * 4 + q], r3 = R[2 * 4 + q], r4 = R[3 * 4 + q],
-                    r5 = R[4 * 4 + q], r6 = R[5 * 4 + q];
+    }
+    const float *cs = A.cg_stage + (size_t)(valid ? e : 0) * (kCgStage * 4);
+    __syncthreads();
+    float *Sa = &sSa[le * 24 * B];
+    if (valid && k < 4 * B) {
+      // lane k: cg point q = k % 4 of wavefield b = k / 4
+      const int q = k & 3, b = k >> 2;
+      const float vse = cs[1 * 4 + q], vsx = cs[2 * 4 + q], vze = cs[3 * 4 + q],
+                  vzx = cs[4 * 4 + q];
+      {
+        const float r1 = R[(0 * 4 + q) * B + b], r3 = R[(2 * 4 + q) * B + b],
+                    r4 = R[(3 * 4 + q) * B + b], r5 = R[(4 * 4 + q) * B + b],
+                    r6 = R[(5 * 4 + q) * B + b];
         if (SRC == AXH_MONOPOLE) {
-          Sa[0 * 4 + q] = vze * r1 + vse * r5;
-          Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
-          Sa[4 * 4 + q] = vze * r5 + vse * r3;
-          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+          Sa[(0 * 4 + q) * B + b] = vze * r1 + vse * r5;
+          Sa[(1 * 4 + q) * B + b] = vzx * r1 + vsx * r5;
+          Sa[(4 * 4 + q) * B + b] = vze * r5 + vse * r3;
+          Sa[(5 * 4 + q) * B + b] = vzx * r5 + vsx * r3;
         } else if (SRC == AXH_DIPOLE) {
-          Sa[0 * 4 + q] = vze * (r1 - r6) + vse * (r5 - r4);
-          Sa[1 * 4 + q] = vzx * (r1 - r6) + vsx * (r5 - r4);
-          Sa[2 * 4 + q] = vze * (r1 + r6) + vse * (r5 + r4);
-          Sa[3 * 4 + q] = vzx * (r1 + r6) + vsx * (r5 + r4);
-          Sa[4 * 4 + q] = vze * r5 + vse * r3;
-          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+          Sa[(0 * 4 + q) * B + b] = vze * (r1 - r6) + vse * (r5 - r4);
+          Sa[(1 * 4 + q) * B + b] = vzx * (r1 - r6) + vsx * (r5 - r4);
+          Sa[(2 * 4 + q) * B + b] = vze * (r1 + r6) + vse * (r5 + r4);
+          Sa[(3 * 4 + q) * B + b] = vzx * (r1 + r6) + vsx * (r5 + r4);
+          Sa[(4 * 4 + q) * B + b] = vze * r5 + vse * r3;
+          Sa[(5 * 4 + q) * B + b] = vzx * r5 + vsx * r3;
         } else {
-          Sa[0 * 4 + q] = vze * r1 + vse * r5;
-          Sa[1 * 4 + q] = vzx * r1 + vsx * r5;
-          Sa[2 * 4 + q] = vze * r6 + vse * r4;
-          Sa[3 * 4 + q] = vzx * r6 + vsx * r4;
-          Sa[4 * 4 + q] = vze * r5 + vse * r3;
-          Sa[5 * 4 + q] = vzx * r5 + vsx * r3;
+          Sa[(0 * 4 + q) * B + b] = vze * r1 + vse * r5;
+          Sa[(1 * 4 + q) * B + b] = vzx * r1 + vsx * r5;
+          Sa[(2 * 4 + q) * B + b] = vze * r6 + vse * r4;
+          Sa[(3 * 4 + q) * B + b] = vzx * r6 + vsx * r4;
+          Sa[(4 * 4 + q) * B + b] = vze * r5 + vse * r3;
+          Sa[(5 * 4 + q) * B + b] = vzx * r5 + vsx * r3;
         }
       }
-      __syncthreads();
-      if (valid) {
-        // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
-        // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
+    }
+    __syncthreads();
+    if (valid) {
+      // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
+      // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
+      const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
+      const int q = cgp ? ((i == 1 ? 0 : 2) + (j == 1 ? 0 : 1)) : 0;
+      const float yl = cs[0 * 4 + q];
+#pragma unroll
+      for (int b = 0; b < B; b++) {
         auto spb = [&](int s1) -> float {
-          if (j == 1) return G.g[i + 5] * Sa[s1 * 4 + 0] + G.g[i + 15] * Sa[s1 * 4 + 2];
-          if (j == 3) return G.g[i + 5] * Sa[s1 * 4 + 1] + G.g[i + 15] * Sa[s1 * 4 + 3];
+          if (j == 1)
+            return G.g[i + 5] * Sa[(s1 * 4 + 0) * B + b] + G.g[i + 15] * Sa[(s1 * 4 + 2) * B + b];
+          if (j == 3)
+            return G.g[i + 5] * Sa[(s1 * 4 + 1) * B + b] + G.g[i + 15] * Sa[(s1 * 4 + 3) * B + b];
           return 0.0f;
         };
         auto spa = [&](int s2) -> float {
-          if (i == 1) return Sa[s2 * 4 + 0] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 1] * G.g2T[3 + 5 * j];
-          if (i == 3) return Sa[s2 * 4 + 2] * G.g2T[1 + 5 * j] + Sa[s2 * 4 + 3] * G.g2T[3 + 5 * j];
+          if (i == 1)
+            return Sa[(s2 * 4 + 0) * B + b] * G.g2T[1 + 5 * j] +
+                   Sa[(s2 * 4 + 1) * B + b] * G.g2T[3 + 5 * j];
+          if (i == 3)
+            return Sa[(s2 * 4 + 2) * B + b] * G.g2T[1 + 5 * j] +
+                   Sa[(s2 * 4 + 3) * B + b] * G.g2T[3 + 5 * j];
           return 0.0f;
         };
-        const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
-        const int q = cgp ? ((i == 1 ? 0 : 2) + (j == 1 ? 0 : 1)) : 0;
-        const float yl = cs[0 * 4 + q];
-        const float r2 = R[1 * 4 + q], r4 = R[3 * 4 + q], r6 = R[5 * 4 + q];
+        const float r2 = R[(1 * 4 + q) * B + b], r4 = R[(3 * 4 + q) * B + b],
+                    r6 = R[(5 * 4 + q) * B + b];
         if (SRC == AXH_MONOPOLE) {
           float l1 = spb(0) + spa(1);
           if (cgp) l1 = l1 + yl * r2;
           const float l3 = spb(4) + spa(5);
-          acc[0] = acc[0] - l1;
-          acc[2] = acc[2] - l3;
+          acc[0][b] = acc[0][b] - l1;
+          acc[2][b] = acc[2][b] - l3;
         } else if (SRC == AXH_DIPOLE) {
           const float lp = spb(0) + spa(1);
           float lm = spb(2) + spa(3);
@@ -721,9 +763,9 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
             lm = lm + 2.0f * yl * (r2 - r6);
             lz = lz - yl * r4;
           }
-          acc[0] = acc[0] - lp;
-          acc[1] = acc[1] - lm;
-          acc[2] = acc[2] - lz;
+          acc[0][b] = acc[0][b] - lp;
+          acc[1][b] = acc[1][b] - lm;
+          acc[2][b] = acc[2][b] - lz;
         } else {
           float ls = spb(0) + spa(1);
           float lp = -spb(2) - spa(3);
@@ -733,77 +775,86 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
             lp = lp + yl * (r6 - 2.0f * r2);
             lz = lz - 2.0f * yl * r4;
           }
-          acc[0] = acc[0] - ls;
-          acc[1] = acc[1] - lp;
-          acc[2] = acc[2] - lz;
+          acc[0][b] = acc[0][b] - ls;
+          acc[1][b] = acc[1][b] - lp;
+          acc[2][b] = acc[2][b] - lz;
         }
       }
-      if (MODE == 0) {
-        // time_step_memvars_cg4 with the masked predicted displacement
-        float *src = &sSrc[le * 28], *old = &sOld[le * 28];
-        if (valid && k < 4)
-          memvar_prepare<SRC>(A, b, e, k, axial, &sU[0][le * 25],
-                              &sU[NCOMP == 2 ? 0 : 1][le * 25], &sU[NCOMP - 1][le * 25],
-                              src, old);
-        __syncthreads();
-        if (valid)
-          memvar_update_elem<SRC>(A, b, e, k, &sMv[le * 24 * kMaxSls], src, old, nullptr);
+    }
+    if (MODE == 0) {
+      // time_step_memvars_cg4 with the masked predicted displacement; lane
+      // k: cg point k % 4 of wavefield k / 4
+      float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
+      if (valid && k < 4 * B) {
+        const int b = k >> 2;
+        memvar_prepare<SRC, B>(A, b, e, k & 3, axial, U0 + b, (NCOMP == 2 ? U0 : U1) + b,
+                               (NCOMP == 2 ? U1 : U2) + b, src, old);
       }
+      __syncthreads();
+      if (valid)
+        memvar_update_elem<SRC, B>(A, e, k, A.memvar + (size_t)e * 24 * A.nsls * B, src, old,
+                                   nullptr);
     }
+  }
 
-    if (valid) {
-      float *force = A.force[b];
-      force[slot] = acc[0];
-      if (SRC != AXH_MONOPOLE) force[(size_t)A.nslot + slot] = acc[1];
-      force[2 * (size_t)A.nslot + slot] = acc[2];
-    }
+  if (valid) {
+    stv<B>(A.force + slot * B, acc[0]);
+    if (SRC != AXH_MONOPOLE) stv<B>(A.force + ((size_t)A.nslot + slot) * B, acc[1]);
+    stv<B>(A.force + (2 * (size_t)A.nslot + slot) * B, acc[2]);
   }
 }
 
 // memory-variable update of the symplectic scheme, once per step after the
 // final drift (SOLVER/time_evol_wave.F90:727-731); also refreshes the SLS sums
 // read by the stage kernels.
-template <int SRC>
+// Dynamic LDS: the tile's memory variables, 10 * 24 * nsls * B floats.
+template <int SRC, int B>
 __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
-  __shared__ float sU[NCOMP][256];
-  __shared__ float sMv[kElemPerTile * 24 * kMaxSls];
-  __shared__ float sSrc[kElemPerTile * 28];
-  __shared__ float sOld[kElemPerTile * 28];
+  extern __shared__ __align__(16) float sMv[];
+  __shared__ __align__(16) float sU[NCOMP * 256 * B];
+  __shared__ __align__(16) float sSrc[kElemPerTile * 28 * B];
+  __shared__ __align__(16) float sOld[kElemPerTile * 28 * B];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int e = tile * kElemPerTile + le;
   const bool valid = (t < 250) && (e < A.nel);
   const size_t slot = (size_t)tile * 256 + t;
-  const int n = 24 * A.nsls;
-  const int gp = valid ? A.gid[slot] : 0;
+  const int n = 24 * A.nsls * B;
+  const int eb = le * 25 * B;
+  gather_disp<NCOMP, B>(A, valid, slot, t, sU);
+  float *mv = &sMv[le * n];
+  {
+    // the tile's memory variables are one contiguous block (n is a multiple
+    // of 24): copied by the whole workgroup in 16-byte pieces
+    const int e0 = tile * kElemPerTile;
+    const int n4 = std::min(kElemPerTile, A.nel - e0) * (n / 4);
+    const float4 *src4 = reinterpret_cast<const float4 *>(A.memvar + (size_t)e0 * n);
+    float4 *dst4 = reinterpret_cast<float4 *>(sMv);
+    for (int x = t; x < n4; x += 256) dst4[x] = src4[x];
+  }
+  __syncthreads();
   const bool axial = valid && A.einfo[e] >= 0;
-  float *src = &sSrc[le * 28], *old = &sOld[le * 28];
-  float *mv = &sMv[le * 24 * kMaxSls];
-  for (int b = 0; b < A.nb; b++) {
-    __syncthreads();  // LDS of the previous wavefield consumed
-    if (valid) {
-      const float *disp = A.disp[b];
-      if (SRC == AXH_MONOPOLE) {
-        sU[0][t] = disp[gp];
-        sU[1][t] = disp[2 * (size_t)A.nglob + gp];
-      } else {
-        for (int c = 0; c < 3; c++) sU[c][t] = disp[c * (size_t)A.nglob + gp];
-      }
-      for (int idx = k; idx < n; idx += 25) mv[idx] = A.memvar[b][(size_t)e * n + idx];
-    }
-    __syncthreads();
-    if (valid && k < 4)
-      memvar_prepare<SRC>(A, b, e, k, axial, &sU[0][le * 25], &sU[NCOMP == 2 ? 0 : 1][le * 25],
-                          &sU[NCOMP - 1][le * 25], src, old);
-    __syncthreads();
-    if (valid) memvar_update_elem<SRC>(A, b, e, k, mv, src, old, mv);
-    __syncthreads();
-    if (valid && k < 24) {
+  float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
+  const float *U0 = sU + eb, *U1 = sU + 256 * B + eb, *U2 = sU + 2 * 256 * B + eb;
+  if (valid && k < 4 * B) {
+    // lane k: cg point k % 4 of wavefield k / 4
+    const int b = k >> 2;
+    memvar_prepare<SRC, B>(A, b, e, k & 3, axial, U0 + b, (NCOMP == 2 ? U0 : U1) + b,
+                           (NCOMP == 2 ? U1 : U2) + b, src, old);
+  }
+  __syncthreads();
+  if (valid) memvar_update_elem<SRC, B>(A, e, k, mv, src, old, mv);
+  __syncthreads();
+  if (valid && k < 24) {
+    float r[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
       float s = 0.0f;
-      for (int jj = 0; jj < A.nsls; jj++) s = s + mv[jj * 24 + k];
-      A.rsum[b][(size_t)e * 24 + k] = s;
+      for (int jj = 0; jj < A.nsls; jj++) s = s + mv[(jj * 24 + k) * B + b];
+      r[b] = s;
     }
+    stv<B>(A.rsum + ((size_t)e * 24 + k) * B, r);
   }
 }
 
@@ -811,10 +862,11 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
 // fluid stiffness glob_fluid_stiffness_4 (SOLVER/stiffness_fluid.f90:139-216)
 // + add_source_fl (SOLVER/time_evol_wave.F90:1062-1076)
 // ---------------------------------------------------------------------------
+template <int B>
 __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
-  __shared__ float sC[256];
-  __shared__ float sS[2][256];
-  __shared__ float sG[105];
+  __shared__ __align__(16) float sC[256 * B];
+  __shared__ __align__(16) float sS[2][256 * B];
+  __shared__ __align__(16) float sG[105];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int i = k % 5, j = k / 5;
@@ -823,32 +875,45 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
-  const int gp = valid ? A.gid[slot] : 0;
+  const int eb = le * 25 * B;
   stage_g(A.G, sG);
   const int nc = A.multipole ? kNcFluidMulti : kNcFluidMono;
   const float *coef = A.coef + (size_t)tile * nc * 256 + t;
   const float m1 = coef[0], m2 = coef[256], m4 = coef[512];
   const float mw = A.multipole ? coef[768] : 0.0f;
   const float w = (A.multipole && axial) ? A.m0[(size_t)ax * 5 + j] : 0.0f;
-  for (int b = 0; b < A.nb; b++) {
-    __syncthreads();  // LDS of the previous wavefield consumed; sG staged
-    sC[t] = valid ? A.chi[b][gp] : 0.0f;
-    __syncthreads();
-    GRegs G;
-    load_g(sG, axial, i, j, G);
-    const float *C = &sC[le * 25];
-    const float c0 = C[k];
-    const float X1 = amul(G.gT, C, i, j), X2 = mulb(C, G.g2, i, j);
-    sS[0][t] = m1 * X2 + m2 * X1;
-    sS[1][t] = m1 * X1 + m4 * X2;
-    __syncthreads();
-    const float Y1 = amul(G.g, &sS[0][le * 25], i, j);
-    const float Y2 = mulb(&sS[1][le * 25], G.g2T, i, j);
+  {
+    float c[B];
+    if (valid) {
+      ldv<B>(A.chi + (size_t)A.gid[slot] * B, c);
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; b++) c[b] = 0.0f;
+    }
+    stv<B>(sC + t * B, c);
+  }
+  __syncthreads();
+  GRegs G;
+  load_g(sG, axial, i, j, G);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const float *C = sC + eb + b;
+    const float X1 = amul<B>(G.gT, C, i, j), X2 = mulb<B>(C, G.g2, i, j);
+    sS[0][t * B + b] = m1 * X2 + m2 * X1;
+    sS[1][t * B + b] = m1 * X1 + m4 * X2;
+  }
+  __syncthreads();
+  float out[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const float *C = sC + eb + b;
+    const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j);
+    const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j);
     float loc = Y1 + Y2;
     if (A.multipole) {
-      loc = loc + mw * c0;
+      loc = loc + mw * C[k * B];
       if (axial) {
-        const float V1 = vmul(G.g0, C, j);
+        const float V1 = vgm<B>(G.g0, C, j);
         loc = loc + G.g0i * (w * V1);
       }
     }
@@ -858,16 +923,16 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
         const int ks = A.srcel[b][e];
         if (ks >= 0) loc = loc - A.src_fl[b][ks * 25 + k] * stf;
       }
-      A.force[b][slot] = loc;
     }
+    out[b] = loc;
   }
+  if (valid) stv<B>(A.force + slot * B, out);
 }
 
 // ---------------------------------------------------------------------------
 // predictor / drift per global point (SOLVER/time_evol_wave.F90:357-367,
 // 597-605, 720-725) with the axis masks (SOLVER/apply_masks.f90) and the
-// blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042).
-// blockIdx.y = batch member.
+// blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042)
 // ---------------------------------------------------------------------------
 struct PredArgs {
   int n, mask0, mask1;   // masked component range [mask0, mask1]
@@ -876,221 +941,292 @@ struct PredArgs {
   double dt, hdt2, cd;
   const uint8_t *flags;
   const int *axb_g;      // masked axis points on the solid-fluid boundary
-  float *u[kMaxBatch], *v[kMaxBatch];
-  const float *a0[kMaxBatch];
-  int *blow[kMaxBatch];
-  double thr[kMaxBatch];
-  float *axb_val[kMaxBatch];  // unmasked prediction of the axb points [naxb][3]
+  float *u, *v;          // [comp][stride][B]
+  const float *a0;
+  float *axb_val;        // unmasked prediction of the axb points [naxb][3][B]
+  int *blow[kGroup];
+  double thr[kGroup];
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
-  const int b = blockIdx.y;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
-  float *U = A.u[b];
-  const float *Vv = A.v[b];
   for (int c = 0; c < 3; c++) {
     if (!(A.comps & (1 << c))) continue;
-    const size_t p = (size_t)c * A.stride + g;
-    const float u = U[p], v = Vv[p];
-    if (A.check_blow && (fl & kFlagBlow) && (double)fabsf(u) > A.thr[b])
-      atomicMin(A.blow[b], A.iter);
-    float un;
-    if (A.newmark)
-      un = (float)((double)u + A.dt * (double)v + A.hdt2 * (double)A.a0[b][p]);
-    else
-      un = (float)((double)u + (double)v * A.cd);
+    const size_t p = ((size_t)c * A.stride + g) * B;
+    float u[B], v[B], un[B];
+    ldv<B>(A.u + p, u);
+    ldv<B>(A.v + p, v);
+    if (A.check_blow && (fl & kFlagBlow)) {
+#pragma unroll
+      for (int b = 0; b < B; b++)
+        if ((double)fabsf(u[b]) > A.thr[b]) atomicMin(A.blow[b], A.iter);
+    }
+    if (A.newmark) {
+      float a0[B];
+      ldv<B>(A.a0 + p, a0);
+#pragma unroll
+      for (int b = 0; b < B; b++)
+        un[b] = (float)((double)u[b] + A.dt * (double)v[b] + A.hdt2 * (double)a0[b]);
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; b++) un[b] = (float)((double)u[b] + (double)v[b] * A.cd);
+    }
     if (fl & kFlagAxisBdry) {
       for (int q = 0; q < A.naxb; q++)
-        if (A.axb_g[q] == g) A.axb_val[b][3 * q + c] = un;
+        if (A.axb_g[q] == g) stv<B>(A.axb_val + ((size_t)q * 3 + c) * B, un);
     }
-    if (A.apply_mask && (fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) un = 0.0f;
-    U[p] = un;
+    if (A.apply_mask && (fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
+#pragma unroll
+      for (int b = 0; b < B; b++) un[b] = 0.0f;
+    }
+    stv<B>(A.u + p, un);
   }
 }
 
 // ---------------------------------------------------------------------------
-// assembly + mass + update, per global point, for every batch member
+// assembly + mass + update, per global point
 // ---------------------------------------------------------------------------
 struct AsmSolidArgs {
-  int n, src, newmark, nslot, comps, mask0, mask1, nb;
+  int n, src, newmark, nslot, comps, mask0, mask1;
   const int *gstart, *gslot;
   const uint8_t *flags;
   const float *minv;
   double hdt, cv;
-  const float *force[kMaxBatch];
-  float *velo[kMaxBatch], *acc0[kMaxBatch];
-  int nsrc[kMaxBatch];
-  const int *src_g[kMaxBatch];
-  const float *src_val[kMaxBatch];
-  float stf[kMaxBatch];
+  const float *force;    // [3][nslot][B]
+  float *velo, *acc0;    // [3][n][B]
+  int nsrc[kGroup];
+  const int *src_g[kGroup];
+  const float *src_val[kGroup];
+  float stf[kGroup];
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
   const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
-  for (int b = 0; b < A.nb; b++) {
-    int si = -1;
-    const float stf = A.stf[b];
-    if ((fl & kFlagSource) && stf != 0.0f) {
+  int si[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    si[b] = -1;
+    if ((fl & kFlagSource) && A.stf[b] != 0.0f) {
       const int *sg = A.src_g[b];
       for (int s = 0; s < A.nsrc[b]; s++)
-        if (sg[s] == g) { si = s; break; }
+        if (sg[s] == g) { si[b] = s; break; }
     }
-    const int order[3] = {0, 2, 1};
-    for (int kk = 0; kk < 3; kk++) {
-      const int c = order[kk];
-      if (!(A.comps & (1 << c))) continue;
-      float s = 0.0f;
-      if ((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
-        s = 0.0f;  // masked copies are zero before the gather
-      } else {
-        const float *f = A.force[b] + (size_t)c * A.nslot;
-        for (int q = q0; q < q1; q++) s = s + f[A.gslot[q]];
-      }
-      if (si >= 0) s = s - A.src_val[b][3 * si + c] * stf;
-      const size_t p = (size_t)c * A.n + g;
-      float *velo = A.velo[b];
-      if (A.newmark) {
-        float a;
-        if (c == 2 && A.src == AXH_DIPOLE)
-          a = (float)(-2.0 * (double)mi * (double)s);
-        else
-          a = -mi * s;
-        float *acc0 = A.acc0[b];
-        const float a0 = acc0[p];
-        velo[p] = (float)((double)velo[p] + A.hdt * (double)(a0 + a));
-        acc0[p] = a;
-      } else {
-        const float a = -mi * s;
-        if (c == 2 && A.src == AXH_DIPOLE)
-          velo[p] = (float)((double)velo[p] + 2.0 * (double)a * A.cv);
-        else
-          velo[p] = (float)((double)velo[p] + (double)a * A.cv);
+  }
+  const int order[3] = {0, 2, 1};
+  for (int kk = 0; kk < 3; kk++) {
+    const int c = order[kk];
+    if (!(A.comps & (1 << c))) continue;
+    float s[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) s[b] = 0.0f;
+    if (!((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1)) {
+      // masked copies are zero before the gather
+      const float *f = A.force + (size_t)c * A.nslot * B;
+      for (int q = q0; q < q1; q++) {
+        float x[B];
+        ldv<B>(f + (size_t)A.gslot[q] * B, x);
+#pragma unroll
+        for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
       }
     }
+#pragma unroll
+    for (int b = 0; b < B; b++)
+      if (si[b] >= 0) s[b] = s[b] - A.src_val[b][3 * si[b] + c] * A.stf[b];
+    const size_t p = ((size_t)c * A.n + g) * B;
+    float v[B];
+    ldv<B>(A.velo + p, v);
+    if (A.newmark) {
+      float a0[B], a[B];
+      ldv<B>(A.acc0 + p, a0);
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        if (c == 2 && A.src == AXH_DIPOLE)
+          a[b] = (float)(-2.0 * (double)mi * (double)s[b]);
+        else
+          a[b] = -mi * s[b];
+        v[b] = (float)((double)v[b] + A.hdt * (double)(a0[b] + a[b]));
+      }
+      stv<B>(A.acc0 + p, a);
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const float a = -mi * s[b];
+        if (c == 2 && A.src == AXH_DIPOLE)
+          v[b] = (float)((double)v[b] + 2.0 * (double)a * A.cv);
+        else
+          v[b] = (float)((double)v[b] + (double)a * A.cv);
+      }
+    }
+    stv<B>(A.velo + p, v);
   }
 }
 
 struct AsmFluidArgs {
-  int n, newmark, mask_axis, mask_surface, nb;
+  int n, newmark, mask_axis, mask_surface;
   const int *gstart, *gslot;
   const uint8_t *flags;
   const float *minv;
   double hdt, cv;
-  const float *force[kMaxBatch];
-  float *dchi[kMaxBatch], *ddchi[kMaxBatch];
+  const float *force;        // [nslot_f][B]
+  float *dchi, *ddchi;       // [n][B]
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
   const bool masked = (A.mask_axis && (fl & kFlagAxis)) ||
                       (A.mask_surface && (fl & kFlagSurface));
-  const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
-  const float mi = A.minv[g];
-  for (int b = 0; b < A.nb; b++) {
-    float s = 0.0f;
-    if (!masked) {
-      const float *f = A.force[b];
-      for (int q = q0; q < q1; q++) s = s + f[A.gslot[q]];
+  float s[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) s[b] = 0.0f;
+  if (!masked)
+    for (int q = A.gstart[g]; q < A.gstart[g + 1]; q++) {
+      float x[B];
+      ldv<B>(A.force + (size_t)A.gslot[q] * B, x);
+#pragma unroll
+      for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
     }
-    float *dchi = A.dchi[b], *ddchi = A.ddchi[b];
-    if (A.newmark) {
-      const float d1 = -mi * s;
-      const float d0 = ddchi[g];
-      dchi[g] = (float)((double)dchi[g] + A.hdt * (double)(d0 + d1));
-      ddchi[g] = d1;
-    } else {
-      const float d = -s * mi;
-      ddchi[g] = d;
-      dchi[g] = (float)((double)dchi[g] + A.cv * (double)d);
+  const float mi = A.minv[g];
+  float d[B], dd[B];
+  ldv<B>(A.dchi + (size_t)g * B, d);
+  if (A.newmark) {
+    float d0[B];
+    ldv<B>(A.ddchi + (size_t)g * B, d0);
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      dd[b] = -mi * s[b];
+      d[b] = (float)((double)d[b] + A.hdt * (double)(d0[b] + dd[b]));
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      dd[b] = -s[b] * mi;
+      d[b] = (float)((double)d[b] + A.cv * (double)dd[b]);
     }
   }
+  stv<B>(A.ddchi + (size_t)g * B, dd);
+  stv<B>(A.dchi + (size_t)g * B, d);
 }
 
 // ---------------------------------------------------------------------------
 // solid-fluid coupling (SOLVER/time_evol_wave.F90:1532-1611), one lane per
-// boundary GLL point, blockIdx.y = batch member
+// boundary GLL point
 // ---------------------------------------------------------------------------
 struct CoupleArgs {
   int n, dipole, nslot_s, nglob_s;
   const int *ss, *gs, *sf, *gf;
-  const float *bm;     // [n][2]
-  const int *axb;      // side-buffer slot per boundary point or -1
-  const float *disp[kMaxBatch];     // [3][nglob_s]
-  const float *ddchi[kMaxBatch];    // [nglob_f]
-  float *force_f[kMaxBatch];        // [nslot_f]
-  float *force_s[kMaxBatch];        // [3][nslot_s]
-  const float *axb_val[kMaxBatch];
+  const float *bm;       // [n][2]
+  const int *axb;        // side-buffer slot per boundary point or -1
+  const float *disp;     // [3][nglob_s][B]
+  const float *ddchi;    // [nglob_f][B]
+  float *force_f;        // [nslot_f][B]
+  float *force_s;        // [3][nslot_s][B]
+  const float *axb_val;  // [naxb][3][B]
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_copy2fluid(CoupleArgs A) {
   const int p = blockIdx.x * 256 + threadIdx.x;
-  const int b = blockIdx.y;
   if (p >= A.n) return;
-  const int g = A.gs[p], k = A.axb[p];
-  float s1, s2, s3;
-  if (k >= 0) {  // prediction before the axis mask
-    const float *av = A.axb_val[b];
-    s1 = av[3 * k];
-    s2 = av[3 * k + 1];
-    s3 = av[3 * k + 2];
+  const int g = A.gs[p], kq = A.axb[p];
+  float s1[B], s2[B], s3[B];
+  if (kq >= 0) {  // prediction before the axis mask
+    ldv<B>(A.axb_val + ((size_t)kq * 3 + 0) * B, s1);
+    ldv<B>(A.axb_val + ((size_t)kq * 3 + 1) * B, s2);
+    ldv<B>(A.axb_val + ((size_t)kq * 3 + 2) * B, s3);
   } else {
-    const float *disp = A.disp[b];
-    s1 = disp[g];
-    s2 = disp[(size_t)A.nglob_s + g];
-    s3 = disp[2 * (size_t)A.nglob_s + g];
+    ldv<B>(A.disp + (size_t)g * B, s1);
+    ldv<B>(A.disp + ((size_t)A.nglob_s + g) * B, s2);
+    ldv<B>(A.disp + (2 * (size_t)A.nglob_s + g) * B, s3);
   }
   const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
-  float *ff = A.force_f[b];
-  float f = ff[A.sf[p]];
-  if (A.dipole)
-    f = f - bm1 * (s1 + s2) - bm2 * s3;
-  else
-    f = f - bm1 * s1 - bm2 * s3;
-  ff[A.sf[p]] = f;
+  float f[B];
+  float *ff = A.force_f + (size_t)A.sf[p] * B;
+  ldv<B>(ff, f);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    if (A.dipole)
+      f[b] = f[b] - bm1 * (s1[b] + s2[b]) - bm2 * s3[b];
+    else
+      f[b] = f[b] - bm1 * s1[b] - bm2 * s3[b];
+  }
+  stv<B>(ff, f);
 }
 
+template <int B>
 __global__ __launch_bounds__(256) void k_copy2solid(CoupleArgs A) {
   const int p = blockIdx.x * 256 + threadIdx.x;
-  const int b = blockIdx.y;
   if (p >= A.n) return;
-  const float fl = A.ddchi[b][A.gf[p]];
+  float fl[B];
+  ldv<B>(A.ddchi + (size_t)A.gf[p] * B, fl);
   const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
   const size_t s = A.ss[p];
-  float *fs = A.force_s[b];
-  fs[s] = fs[s] + bm1 * fl;
-  if (A.dipole) fs[(size_t)A.nslot_s + s] = fs[(size_t)A.nslot_s + s] + bm1 * fl;
-  fs[2 * (size_t)A.nslot_s + s] = fs[2 * (size_t)A.nslot_s + s] + bm2 * fl;
+  float x[B];
+  float *f0 = A.force_s + s * B;
+  ldv<B>(f0, x);
+#pragma unroll
+  for (int b = 0; b < B; b++) x[b] = x[b] + bm1 * fl[b];
+  stv<B>(f0, x);
+  if (A.dipole) {
+    float *f1 = A.force_s + ((size_t)A.nslot_s + s) * B;
+    ldv<B>(f1, x);
+#pragma unroll
+    for (int b = 0; b < B; b++) x[b] = x[b] + bm1 * fl[b];
+    stv<B>(f1, x);
+  }
+  float *f2 = A.force_s + (2 * (size_t)A.nslot_s + s) * B;
+  ldv<B>(f2, x);
+#pragma unroll
+  for (int b = 0; b < B; b++) x[b] = x[b] + bm2 * fl[b];
+  stv<B>(f2, x);
 }
 
 // receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
-// blockIdx.y = batch member; members without a sample this step have out NULL
+// blockIdx.y = wavefield of the group; members without a sample this step
+// have out NULL
 struct RecArgs {
   int src, nglob;
-  int nrec[kMaxBatch];
-  const int *rec_g[kMaxBatch];
-  const float *disp[kMaxBatch];
-  float *out[kMaxBatch];
+  const float *disp;     // [3][nglob][B]
+  int nrec[kGroup];
+  const int *rec_g[kGroup];
+  float *out[kGroup];
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y;
   if (!A.out[b] || r >= A.nrec[b]) return;
   const int g = A.rec_g[b][r];
-  const float *disp = A.disp[b];
-  const float d1 = disp[g], d2 = disp[(size_t)A.nglob + g], d3 = disp[2 * (size_t)A.nglob + g];
+  const float d1 = A.disp[(size_t)g * B + b], d2 = A.disp[((size_t)A.nglob + g) * B + b],
+              d3 = A.disp[(2 * (size_t)A.nglob + g) * B + b];
   float *o = A.out[b] + 3 * (size_t)r;
   if (A.src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
   else if (A.src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
   else { o[0] = d1; o[1] = d2; o[2] = d3; }
+}
+
+// moving one wavefield's state between a group's interleaved arrays and its
+// own (dst[x*B + b] = src[x], and back)
+__global__ __launch_bounds__(256) void k_interleave(float *dst, const float *src, size_t n,
+                                                    int B, int b) {
+  for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (size_t)gridDim.x * 256)
+    dst[x * B + b] = src[x];
+}
+__global__ __launch_bounds__(256) void k_deinterleave(float *dst, const float *src, size_t n,
+                                                      int B, int b) {
+  for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (size_t)gridDim.x * 256)
+    dst[x] = src[x * B + b];
 }
 
 // ---------------------------------------------------------------------------
@@ -1649,34 +1785,114 @@ static void mesh_add_sources(MeshDev &M, const std::vector<int> &src_g) {
                        hipMemcpyHostToDevice));
 }
 
-static void wave_build(const axh_ctx *ctx, MeshDev &M, WaveDev &W) {
+// ---------------------------------------------------------------------------
+// wavefield storage and sources
+// ---------------------------------------------------------------------------
+// Source, receivers and blow-up flag of one wavefield (not interleaved).
+struct Member {
+  DevAlloc mem;
+  int nsrc = 0;               // solid source points
+  std::vector<int> src_g_h;
+  int *src_g = nullptr;
+  float *src_val = nullptr;   // [nsrc][3]
+  int fluid_src = 0;
+  int *srcel_f = nullptr;     // per fluid element: source slot k or -1
+  float *src_fl = nullptr;    // source_term_fl (25 x 8)
+  int nrec = 0, seis_it = 1;
+  int *rec_g = nullptr;
+  float *seis_d = nullptr;
+  size_t seis_cap = 0;
+  int *blow = nullptr;
+  double blow_thr = 0.0;
+  std::vector<float> stf;     // [niter][nstages]
+};
+
+// State of B wavefields, interleaved (element x of wavefield b at x*B + b),
+// and their element-force scratch.
+struct Fields {
+  int B = 1;
+  DevAlloc mem;
+  float *disp = nullptr, *velo = nullptr, *acc0 = nullptr, *force_s = nullptr;
+  float *chi = nullptr, *dchi = nullptr, *ddchi = nullptr, *force_f = nullptr;
+  float *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr, *devm1 = nullptr;
+  float *axb_val = nullptr;   // unmasked prediction of the axis boundary points
+};
+
+// the state arrays of a wavefield (pointer into F, entries per wavefield)
+static std::vector<std::pair<float *, size_t>> state_arrays(const MeshDev &M, Fields &F) {
+  std::vector<std::pair<float *, size_t>> v = {
+      {F.disp, 3 * (size_t)M.nglob_s}, {F.velo, 3 * (size_t)M.nglob_s},
+      {F.acc0, 3 * (size_t)M.nglob_s}, {F.axb_val, 3 * (size_t)std::max(M.naxb, 1)}};
+  if (M.nel_f > 0) {
+    v.push_back({F.chi, (size_t)M.nglob_f});
+    v.push_back({F.dchi, (size_t)M.nglob_f});
+    v.push_back({F.ddchi, (size_t)M.nglob_f});
+  }
+  if (M.anel) {
+    v.push_back({F.memvar, (size_t)M.nel_s * 24 * M.nsls});
+    v.push_back({F.rsum, (size_t)M.nel_s * 24});
+    v.push_back({F.trm1, (size_t)M.nel_s * 4});
+    v.push_back({F.devm1, (size_t)M.nel_s * 24});
+  }
+  return v;
+}
+
+static void fields_alloc(const MeshDev &M, Fields &F, int B) {
+  F.B = B;
+  DevAlloc &m = F.mem;
+  const size_t b = B;
+  F.disp = m.zeros<float>(3 * (size_t)M.nglob_s * b);
+  F.velo = m.zeros<float>(3 * (size_t)M.nglob_s * b);
+  F.acc0 = m.zeros<float>(3 * (size_t)M.nglob_s * b);
+  F.force_s = m.zeros<float>(3 * (size_t)M.ntile_s * kTile * b);
+  F.axb_val = m.zeros<float>(3 * (size_t)std::max(M.naxb, 1) * b);
+  if (M.nel_f > 0) {
+    F.chi = m.zeros<float>((size_t)M.nglob_f * b);
+    F.dchi = m.zeros<float>((size_t)M.nglob_f * b);
+    F.ddchi = m.zeros<float>((size_t)M.nglob_f * b);
+    F.force_f = m.zeros<float>((size_t)M.ntile_f * kTile * b);
+  }
+  if (M.anel) {
+    F.memvar = m.zeros<float>((size_t)M.nel_s * 24 * M.nsls * b);
+    F.rsum = m.zeros<float>((size_t)M.nel_s * 24 * b);
+    F.trm1 = m.zeros<float>((size_t)M.nel_s * 4 * b);
+    F.devm1 = m.zeros<float>((size_t)M.nel_s * 24 * b);
+  }
+}
+
+// Copy wavefield src_b of S into wavefield dst_b of D (same mesh), through
+// the (de)interleaving kernels on stream s.
+static void copy_state(const MeshDev &M, Fields &D, int dst_b, Fields &S, int src_b,
+                       hipStream_t s) {
+  auto dv = state_arrays(M, D), sv = state_arrays(M, S);
+  for (size_t a = 0; a < dv.size(); a++) {
+    const size_t n = dv[a].second;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (S.B == 1) {
+      hipLaunchKernelGGL(k_interleave, dim3(nb), dim3(256), 0, s, dv[a].first, sv[a].first, n,
+                         D.B, dst_b);
+    } else if (D.B == 1) {
+      hipLaunchKernelGGL(k_deinterleave, dim3(nb), dim3(256), 0, s, dv[a].first, sv[a].first, n,
+                         S.B, src_b);
+    } else {
+      fail(AXH_ERR_STATE, "state copy between two groups");
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+static void member_build(const axh_ctx *ctx, MeshDev &M, Member &W) {
   const axh_dims &d = ctx->dims;
   WaveHost H;
   build_wave_layout(ctx, M.gid_s_h, M.nglob_s, H);
   DevAlloc &m = W.mem;
-  const size_t nslot_s = (size_t)M.ntile_s * kTile;
-  W.disp = m.zeros<float>(3 * (size_t)M.nglob_s);
-  W.velo = m.zeros<float>(3 * (size_t)M.nglob_s);
-  W.acc0 = m.zeros<float>(3 * (size_t)M.nglob_s);
-  W.force_s = m.zeros<float>(3 * nslot_s);
   W.nsrc = (int)H.src_g.size();
   W.src_g = m.upload(H.src_g);
   W.src_val = m.upload(H.src_val);
   W.fluid_src = d.fluid_src;
   if (M.nel_f > 0) {
-    W.chi = m.zeros<float>(M.nglob_f);
-    W.dchi = m.zeros<float>(M.nglob_f);
-    W.ddchi = m.zeros<float>(M.nglob_f);
-    W.force_f = m.zeros<float>((size_t)M.ntile_f * kTile);
     W.srcel_f = m.upload(H.srcel_f);
     W.src_fl = m.upload(H.src_fl);
-  }
-  W.axb_val = m.zeros<float>(3 * (size_t)std::max(M.naxb, 1));
-  if (M.anel) {
-    W.memvar = m.zeros<float>((size_t)M.nel_s * 24 * M.nsls);
-    W.rsum = m.zeros<float>((size_t)M.nel_s * 24);
-    W.trm1 = m.zeros<float>((size_t)M.nel_s * 4);
-    W.devm1 = m.zeros<float>((size_t)M.nel_s * 24);
   }
   W.nrec = d.num_rec;
   W.seis_it = d.seis_it;
@@ -1690,43 +1906,65 @@ static void wave_build(const axh_ctx *ctx, MeshDev &M, WaveDev &W) {
   mesh_add_sources(M, W.src_g_h);
 }
 
+// Stream, events and per-kernel timing of the runs of a context or a batch.
+struct Runner {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool profile = false;
+  std::vector<hipEvent_t> kev;  // pairs
+  std::vector<int> kev_class;
+  double kms[KC_N] = {0};
+  long kcount[KC_N] = {0};
+  void init() {
+    if (stream) return;
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+  }
+  ~Runner() {
+    // teardown: errors are not actionable here
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : kev) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// One context on the device: the (shared) mesh part, its source data, its own
+// fields (B = 1) and, while a batch steps it, where its state lives instead.
+struct Device {
+  std::shared_ptr<MeshDev> mesh;
+  Member member;
+  Fields own;
+  Fields *live = nullptr;  // a batch group holding the state, or NULL
+  int live_slot = 0;
+  Runner run;  // declared last: its stream is drained before buffers are freed
+};
+
+// Groups of a batch: interleaved fields of up to kGroup members each.
+struct BatchState {
+  std::vector<std::unique_ptr<Fields>> groups;
+  std::vector<std::vector<int>> index;  // members of each group
+  Runner run;
+};
+
 void device_build(axh_ctx *ctx) {
   std::unique_ptr<Device> D(new Device);
   D->mesh = mesh_build(ctx);
-  wave_build(ctx, *D->mesh, D->wave);
+  member_build(ctx, *D->mesh, D->member);
+  fields_alloc(*D->mesh, D->own, 1);
   HIPCHECK(hipDeviceSynchronize());
   ctx->dev.reset(D.release());
 }
 
-void batch_attach(const std::vector<axh_ctx *> &members) {
-  axh_ctx *first = members[0];
-  if (!first->dev) device_build(first);
-  std::shared_ptr<MeshDev> mesh = first->dev->mesh;
-  for (size_t b = 1; b < members.size(); b++) {
-    axh_ctx *c = members[b];
-    if (!c->dev) {
-      std::unique_ptr<Device> D(new Device);
-      D->mesh = mesh;
-      wave_build(c, *mesh, D->wave);
-      c->dev.reset(D.release());
-    } else if (c->dev->mesh != mesh) {
-      // same mesh uploads (checked by the caller): the global numbering is
-      // the same, so the wavefield part carries over unchanged
-      if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
-      mesh_add_sources(*mesh, c->dev->wave.src_g_h);
-      c->dev->mesh = mesh;
-    }
-  }
-  HIPCHECK(hipDeviceSynchronize());
-}
-
 void DeviceDeleter::operator()(Device *d) const { device_destroy(d); }
-void RunnerDeleter::operator()(Runner *r) const { delete r; }
+void BatchStateDeleter::operator()(BatchState *b) const { delete b; }
 
 void device_destroy(Device *d) { delete d; }
 
 size_t device_bytes(const Device *d) {
-  return d ? d->mesh->mem.bytes + d->wave.mem.bytes : 0;
+  return d ? d->mesh->mem.bytes + d->member.mem.bytes + d->own.mem.bytes : 0;
 }
 
 const char *kernel_class_name(int k) { return (k >= 0 && k < KC_N) ? kClassName[k] : nullptr; }
@@ -1743,11 +1981,90 @@ bool kernel_stats(const Device *d, int k, double *ms, int64_t *n) {
 }
 
 bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n) {
-  return runner_stats(b ? b->run.get() : nullptr, k, ms, n);
+  return runner_stats(b && b->state ? &b->state->run : nullptr, k, ms, n);
+}
+
+bool in_running_batch(const axh_ctx *ctx) { return ctx->dev && ctx->dev->live; }
+
+// Put every member on member 0's device mesh (building what is missing) and
+// move the members' state into the batch's interleaved groups.
+void batch_attach(axh_batch *bt) {
+  if (bt->state) return;
+  const std::vector<axh_ctx *> &members = bt->members;
+  axh_ctx *first = members[0];
+  if (!first->dev) device_build(first);
+  std::shared_ptr<MeshDev> mesh = first->dev->mesh;
+  for (size_t b = 1; b < members.size(); b++) {
+    axh_ctx *c = members[b];
+    if (!c->dev) {
+      std::unique_ptr<Device> D(new Device);
+      D->mesh = mesh;
+      member_build(c, *mesh, D->member);
+      fields_alloc(*mesh, D->own, 1);
+      c->dev.reset(D.release());
+    } else if (c->dev->mesh != mesh) {
+      // same mesh part (checked when the batch was created): the global
+      // numbering is the same, so the wavefield part carries over unchanged
+      if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
+      mesh_add_sources(*mesh, c->dev->member.src_g_h);
+      c->dev->mesh = mesh;
+    }
+  }
+  std::unique_ptr<BatchState, BatchStateDeleter> S(new BatchState);
+  S->run.init();
+  // groups of at most kGroup members, sizes as equal as possible
+  const int n = (int)members.size();
+  const int width = std::max(1, std::min(bt->group_width, kGroup));
+  const int ng = (n + width - 1) / width;
+  int next = 0;
+  for (int g = 0; g < ng; g++) {
+    const int size = n / ng + (g < n % ng ? 1 : 0);
+    std::unique_ptr<Fields> F(new Fields);
+    fields_alloc(*mesh, *F, size);
+    std::vector<int> idx;
+    for (int b = 0; b < size; b++, next++) {
+      Device &D = *members[next]->dev;
+      if (D.run.stream) HIPCHECK(hipStreamSynchronize(D.run.stream));
+      copy_state(*mesh, *F, b, D.own, 0, S->run.stream);
+      idx.push_back(next);
+    }
+    S->index.push_back(idx);
+    S->groups.push_back(std::move(F));
+  }
+  HIPCHECK(hipStreamSynchronize(S->run.stream));
+  for (size_t g = 0; g < S->groups.size(); g++)
+    for (size_t b = 0; b < S->index[g].size(); b++) {
+      Device &D = *members[S->index[g][b]]->dev;
+      D.live = S->groups[g].get();
+      D.live_slot = (int)b;
+    }
+  bt->state = std::move(S);
+}
+
+// Move the members' state back to their own fields; the groups are freed.
+void batch_detach(axh_batch *bt) {
+  if (!bt->state) return;
+  BatchState &S = *bt->state;
+  std::string err;
+  for (size_t g = 0; g < S.groups.size(); g++)
+    for (size_t b = 0; b < S.index[g].size(); b++) {
+      Device &D = *bt->members[S.index[g][b]]->dev;
+      try {
+        copy_state(*D.mesh, D.own, 0, *S.groups[g], (int)b, S.run.stream);
+      } catch (const Error &e) {
+        err = e.msg;
+      }
+      D.live = nullptr;
+      D.live_slot = 0;
+    }
+  const hipError_t rc = hipStreamSynchronize(S.run.stream);
+  bt->state.reset();
+  if (!err.empty()) fail(AXH_ERR_HIP, err);
+  if (rc != hipSuccess) fail(AXH_ERR_HIP, std::string("batch detach: ") + hipGetErrorString(rc));
 }
 
 // ---------------------------------------------------------------------------
-// the time loop, for one wavefield or a batch on one mesh
+// the time loop of one group (B wavefields on one mesh)
 // ---------------------------------------------------------------------------
 static inline int nblk(long n) { return (int)((n + 255) / 256); }
 
@@ -1770,18 +2087,19 @@ struct Launcher {
   }
 };
 
-template <int SRC, int MODE, bool ANEL>
+template <int SRC, int MODE, bool ANEL, int B>
 static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
-  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL>), dim3(M.ntile_s), dim3(256), 0, s, A);
+  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B>), dim3(M.ntile_s), dim3(256), 0, s, A);
 }
 
+template <int B>
 static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A, int mode) {
   const bool an = M.anel != 0;
-#define AXH_SOLID_CASE(S)                                                     \
-  if (mode == 0) {                                                            \
-    if (an) launch_solid<S, 0, true>(M, s, A); else launch_solid<S, 0, false>(M, s, A); \
-  } else {                                                                    \
-    if (an) launch_solid<S, 1, true>(M, s, A); else launch_solid<S, 1, false>(M, s, A); \
+#define AXH_SOLID_CASE(S)                                                              \
+  if (mode == 0) {                                                                     \
+    if (an) launch_solid<S, 0, true, B>(M, s, A); else launch_solid<S, 0, false, B>(M, s, A); \
+  } else {                                                                             \
+    if (an) launch_solid<S, 1, true, B>(M, s, A); else launch_solid<S, 1, false, B>(M, s, A); \
   }
   if (M.src == AXH_MONOPOLE) { AXH_SOLID_CASE(AXH_MONOPOLE) }
   else if (M.src == AXH_DIPOLE) { AXH_SOLID_CASE(AXH_DIPOLE) }
@@ -1789,40 +2107,37 @@ static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A, 
 #undef AXH_SOLID_CASE
 }
 
+template <int B>
 static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
+  const size_t lds = sizeof(float) * kElemPerTile * 24 * M.nsls * B;
   if (M.src == AXH_MONOPOLE)
-    hipLaunchKernelGGL(k_memvar<AXH_MONOPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_MONOPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
   else if (M.src == AXH_DIPOLE)
-    hipLaunchKernelGGL(k_memvar<AXH_DIPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_DIPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
   else
-    hipLaunchKernelGGL(k_memvar<AXH_QUADPOLE>, dim3(M.ntile_s), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
 }
 
-// Steps it0..it1 of the wavefields of members (all on one mesh, at most
-// kMaxBatch) on the stream of R; seis[b] (may be NULL) receives member b's
-// samples.  Returns the device time of the run [ms].
-static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int32_t it0,
-                          int32_t it1, float *const *seis) {
-  const int nb = (int)members.size();
-  if (nb < 1 || nb > kMaxBatch) fail(AXH_ERR_ARG, "batch size must be 1..8");
-  const MeshDev &M = *members[0]->dev->mesh;
-  std::vector<WaveDev *> W(nb);
-  for (int b = 0; b < nb; b++) {
-    if (members[b]->dev->mesh.get() != &M) fail(AXH_ERR_STATE, "batch members on different meshes");
-    W[b] = &members[b]->dev->wave;
-  }
-  const axh_dims &d = members[0]->dims;
+// Steps it0..it1 of the B wavefields of F (member data mb[b], dims of member
+// 0) on the stream of R; seis[b] (may be NULL) receives wavefield b's samples
+// and blow[b] its blow-up step (INT_MAX: none).  Returns the device time of
+// the run [ms].
+template <int B>
+static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
+                        const axh_ctx *const *ctx, int32_t it0, int32_t it1,
+                        float *const *seis, int *blow) {
+  if (F.B != B) fail(AXH_ERR_STATE, "group width mismatch");
+  const axh_dims &d = ctx[0]->dims;
   const bool multipole = M.src != AXH_MONOPOLE;
   const bool newmark = M.scheme == AXH_NEWMARK2;
   const double dt = d.deltat, hdt = 0.5 * dt, hdt2 = 0.5 * (dt * dt);
   const size_t nslot_s = (size_t)M.ntile_s * kTile;
-  R.init();
   Launcher L{R};
   hipStream_t st_ = R.stream;
 
-  for (int b = 0; b < nb; b++) {
-    WaveDev &w = *W[b];
-    const int nsamp = axh_seis_samples(members[b], it0, it1);
+  for (int b = 0; b < B; b++) {
+    Member &w = *mb[b];
+    const int nsamp = axh_seis_samples(ctx[b], it0, it1);
     if (nsamp > 0 && (size_t)nsamp * 3 * w.nrec > w.seis_cap) {
       w.seis_cap = (size_t)nsamp * 3 * w.nrec;
       w.seis_d = w.mem.alloc<float>(w.seis_cap);
@@ -1832,19 +2147,23 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
 
   SolidArgs SA{};
   SA.nel = M.nel_s; SA.nsls = M.nsls; SA.nglob = M.nglob_s; SA.nslot = (int)nslot_s;
-  SA.nb = nb; SA.G = M.G; SA.coef = M.coef_s; SA.m0 = M.m0_s; SA.einfo = M.einfo_s;
-  SA.gid = M.gid_s; SA.cg_stage = M.cg_stage; SA.cg_upd = M.cg_upd; SA.aj_mu = M.aj_mu;
-  SA.aj_k = M.aj_k;
+  SA.G = M.G; SA.coef = M.coef_s; SA.m0 = M.m0_s; SA.einfo = M.einfo_s; SA.gid = M.gid_s;
+  SA.cg_stage = M.cg_stage; SA.cg_upd = M.cg_upd; SA.aj_mu = M.aj_mu; SA.aj_k = M.aj_k;
   for (int s = 0; s < kMaxSls; s++) {
     SA.expw[s] = M.expw[s]; SA.tst[s] = M.tst[s]; SA.tstm1[s] = M.tstm1[s];
   }
+  SA.disp = F.disp; SA.force = F.force_s; SA.memvar = F.memvar; SA.rsum = F.rsum;
+  SA.trm1 = F.trm1; SA.devm1 = F.devm1;
   FluidArgs FA{};
-  FA.nel = M.nel_f; FA.multipole = multipole; FA.nb = nb; FA.G = M.G; FA.coef = M.coef_f;
-  FA.m0 = M.m0_f; FA.einfo = M.einfo_f; FA.gid = M.gid_f;
+  FA.nel = M.nel_f; FA.multipole = multipole; FA.G = M.G; FA.coef = M.coef_f;
+  FA.m0 = M.m0_f; FA.einfo = M.einfo_f; FA.gid = M.gid_f; FA.chi = F.chi;
+  FA.force = F.force_f;
   CoupleArgs CA{};
   CA.n = 5 * M.nbdry; CA.dipole = M.src == AXH_DIPOLE; CA.nslot_s = (int)nslot_s;
   CA.nglob_s = M.nglob_s; CA.ss = M.cpl_ss; CA.gs = M.cpl_gs; CA.sf = M.cpl_sf;
-  CA.gf = M.cpl_gf; CA.bm = M.cpl_bm; CA.axb = M.cpl_axb;
+  CA.gf = M.cpl_gf; CA.bm = M.cpl_bm; CA.axb = M.cpl_axb; CA.disp = F.disp;
+  CA.ddchi = F.ddchi; CA.force_f = F.force_f; CA.force_s = F.force_s;
+  CA.axb_val = F.axb_val;
   // solid axis masks: monopole s; dipole +,z (2,3); quadrupole all
   const int mask0 = (M.src == AXH_DIPOLE) ? 1 : 0;
   const int mask1 = (M.src == AXH_MONOPOLE) ? 0 : 2;
@@ -1852,43 +2171,37 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
   PredArgs PS{};
   PS.n = M.nglob_s; PS.mask0 = mask0; PS.mask1 = mask1; PS.comps = comps;
   PS.dt = dt; PS.hdt2 = hdt2; PS.flags = M.flags_s; PS.stride = M.nglob_s;
-  PS.naxb = M.naxb; PS.axb_g = M.axb_g;
+  PS.naxb = M.naxb; PS.axb_g = M.axb_g; PS.u = F.disp; PS.v = F.velo; PS.a0 = F.acc0;
+  PS.axb_val = F.axb_val;
   PredArgs PF{};
   PF.n = M.nglob_f; PF.mask0 = 0; PF.mask1 = 0; PF.comps = 1; PF.dt = dt; PF.hdt2 = hdt2;
   PF.flags = M.flags_f; PF.stride = M.nglob_f; PF.naxb = 0; PF.axb_g = nullptr;
+  PF.u = F.chi; PF.v = F.dchi; PF.a0 = F.ddchi; PF.axb_val = F.axb_val;
   AsmSolidArgs AS{};
   AS.n = M.nglob_s; AS.src = M.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
-  AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.nb = nb; AS.gstart = M.gstart_s;
+  AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = M.gstart_s;
   AS.gslot = M.gslot_s; AS.flags = M.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
+  AS.force = F.force_s; AS.velo = F.velo; AS.acc0 = F.acc0;
   AsmFluidArgs AF{};
   AF.n = M.nglob_f; AF.newmark = newmark; AF.mask_axis = multipole;
-  AF.mask_surface = newmark; AF.nb = nb; AF.gstart = M.gstart_f; AF.gslot = M.gslot_f;
-  AF.flags = M.flags_f; AF.minv = M.minv_f; AF.hdt = hdt;
+  AF.mask_surface = newmark; AF.gstart = M.gstart_f; AF.gslot = M.gslot_f;
+  AF.flags = M.flags_f; AF.minv = M.minv_f; AF.hdt = hdt; AF.force = F.force_f;
+  AF.dchi = F.dchi; AF.ddchi = F.ddchi;
   RecArgs RA{};
-  RA.src = M.src; RA.nglob = M.nglob_s;
+  RA.src = M.src; RA.nglob = M.nglob_s; RA.disp = F.disp;
   int maxrec = 0;
-  for (int b = 0; b < nb; b++) {
-    WaveDev &w = *W[b];
-    SA.disp[b] = w.disp; SA.force[b] = w.force_s; SA.memvar[b] = w.memvar;
-    SA.rsum[b] = w.rsum; SA.trm1[b] = w.trm1; SA.devm1[b] = w.devm1;
-    FA.chi[b] = w.chi; FA.force[b] = w.force_f; FA.srcel[b] = w.srcel_f;
-    FA.src_fl[b] = w.src_fl;
-    CA.disp[b] = w.disp; CA.ddchi[b] = w.ddchi; CA.force_f[b] = w.force_f;
-    CA.force_s[b] = w.force_s; CA.axb_val[b] = w.axb_val;
-    PS.u[b] = w.disp; PS.v[b] = w.velo; PS.a0[b] = w.acc0; PS.blow[b] = w.blow;
-    PS.thr[b] = w.blow_thr; PS.axb_val[b] = w.axb_val;
-    PF.u[b] = w.chi; PF.v[b] = w.dchi; PF.a0[b] = w.ddchi; PF.blow[b] = w.blow;
-    PF.axb_val[b] = w.axb_val;
-    AS.force[b] = w.force_s; AS.velo[b] = w.velo; AS.acc0[b] = w.acc0;
+  for (int b = 0; b < B; b++) {
+    Member &w = *mb[b];
+    FA.srcel[b] = w.srcel_f; FA.src_fl[b] = w.src_fl;
+    PS.blow[b] = w.blow; PS.thr[b] = w.blow_thr;
+    PF.blow[b] = w.blow; PF.thr[b] = w.blow_thr;
     AS.nsrc[b] = w.nsrc; AS.src_g[b] = w.src_g; AS.src_val[b] = w.src_val;
-    AF.force[b] = w.force_f; AF.dchi[b] = w.dchi; AF.ddchi[b] = w.ddchi;
-    RA.nrec[b] = w.nrec; RA.rec_g[b] = w.rec_g; RA.disp[b] = w.disp;
+    RA.nrec[b] = w.nrec; RA.rec_g[b] = w.rec_g;
     maxrec = std::max(maxrec, w.nrec);
   }
-  const dim3 gS(nblk(M.nglob_s), nb), gF(nblk(M.nglob_f), nb), gC(nblk(CA.n), nb);
 
   HIPCHECK(hipEventRecord(R.ev0, st_));
-  std::vector<int> isamp(nb, 0);
+  int isamp[kGroup] = {0, 0, 0, 0};
   for (int it = it0; it <= it1; it++) {
     for (int st = 0; st < M.nstages; st++) {
       // predictor / drift
@@ -1900,37 +2213,37 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
       PF.check_blow = 0;
       PS.iter = PF.iter = it;
       L(KC_PRED, [&] {
-        hipLaunchKernelGGL(k_predict, gS, dim3(256), 0, st_, PS);
-        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict, gF, dim3(256), 0, st_, PF);
+        hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PS.n)), dim3(256), 0, st_, PS);
+        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PF.n)), dim3(256), 0, st_, PF);
       });
-      for (int b = 0; b < nb; b++) {
-        const float stf = W[b]->stf[(size_t)(it - 1) * M.nstages + st];
-        FA.stf[b] = (newmark && W[b]->fluid_src) ? stf : 0.0f;
-        AS.stf[b] = (newmark && W[b]->fluid_src) ? 0.0f : stf;
+      for (int b = 0; b < B; b++) {
+        const float stf = mb[b]->stf[(size_t)(it - 1) * M.nstages + st];
+        FA.stf[b] = (newmark && mb[b]->fluid_src) ? stf : 0.0f;
+        AS.stf[b] = (newmark && mb[b]->fluid_src) ? 0.0f : stf;
       }
       if (M.nel_f > 0) {
         L(KC_FLUID, [&] {
-          hipLaunchKernelGGL(k_fluid, dim3(M.ntile_f), dim3(256), 0, st_, FA);
+          hipLaunchKernelGGL(k_fluid<B>, dim3(M.ntile_f), dim3(256), 0, st_, FA);
         });
         if (M.nbdry > 0)
           L(KC_COUPLE_F, [&] {
-            hipLaunchKernelGGL(k_copy2fluid, gC, dim3(256), 0, st_, CA);
+            hipLaunchKernelGGL(k_copy2fluid<B>, dim3(nblk(CA.n)), dim3(256), 0, st_, CA);
           });
       }
-      L(KC_SOLID, [&] { dispatch_solid(M, st_, SA, newmark ? 0 : 1); });
+      L(KC_SOLID, [&] { dispatch_solid<B>(M, st_, SA, newmark ? 0 : 1); });
       if (M.nel_f > 0) {
         AF.cv = newmark ? 0.0 : M.coefv[st];
         L(KC_ASM_F, [&] {
-          hipLaunchKernelGGL(k_asm_fluid, dim3(nblk(AF.n)), dim3(256), 0, st_, AF);
+          hipLaunchKernelGGL(k_asm_fluid<B>, dim3(nblk(AF.n)), dim3(256), 0, st_, AF);
         });
         if (M.nbdry > 0)
           L(KC_COUPLE_S, [&] {
-            hipLaunchKernelGGL(k_copy2solid, gC, dim3(256), 0, st_, CA);
+            hipLaunchKernelGGL(k_copy2solid<B>, dim3(nblk(CA.n)), dim3(256), 0, st_, CA);
           });
       }
       AS.cv = newmark ? 0.0 : M.coefv[st];
       L(KC_ASM_S, [&] {
-        hipLaunchKernelGGL(k_asm_solid, dim3(nblk(AS.n)), dim3(256), 0, st_, AS);
+        hipLaunchKernelGGL(k_asm_solid<B>, dim3(nblk(AS.n)), dim3(256), 0, st_, AS);
       });
     }
     if (!newmark) {
@@ -1941,14 +2254,15 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
       PD.apply_mask = PDF.apply_mask = 0;
       PD.check_blow = PDF.check_blow = 0;
       L(KC_DRIFT, [&] {
-        hipLaunchKernelGGL(k_predict, gS, dim3(256), 0, st_, PD);
-        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict, gF, dim3(256), 0, st_, PDF);
+        hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PD.n)), dim3(256), 0, st_, PD);
+        if (M.nel_f > 0)
+          hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PDF.n)), dim3(256), 0, st_, PDF);
       });
-      if (M.anel) L(KC_MEMVAR, [&] { dispatch_memvar(M, st_, SA); });
+      if (M.anel) L(KC_MEMVAR, [&] { dispatch_memvar<B>(M, st_, SA); });
     }
     bool any = false;
-    for (int b = 0; b < nb; b++) {
-      WaveDev &w = *W[b];
+    for (int b = 0; b < B; b++) {
+      Member &w = *mb[b];
       RA.out[b] = nullptr;
       if (w.nrec > 0 && it % w.seis_it == 0) {
         RA.out[b] = w.seis_d + (size_t)isamp[b] * 3 * w.nrec;
@@ -1958,17 +2272,16 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
     }
     if (any)
       L(KC_REC, [&] {
-        hipLaunchKernelGGL(k_receivers, dim3(nblk(maxrec), nb), dim3(256), 0, st_, RA);
+        hipLaunchKernelGGL(k_receivers<B>, dim3(nblk(maxrec), B), dim3(256), 0, st_, RA);
       });
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(R.ev1, st_));
-  std::vector<int> blow(nb, INT_MAX);
-  for (int b = 0; b < nb; b++) {
-    HIPCHECK(hipMemcpyAsync(&blow[b], W[b]->blow, sizeof(int), hipMemcpyDeviceToHost, st_));
+  for (int b = 0; b < B; b++) {
+    HIPCHECK(hipMemcpyAsync(&blow[b], mb[b]->blow, sizeof(int), hipMemcpyDeviceToHost, st_));
     if (seis && seis[b] && isamp[b] > 0)
-      HIPCHECK(hipMemcpyAsync(seis[b], W[b]->seis_d,
-                              (size_t)isamp[b] * 3 * W[b]->nrec * sizeof(float),
+      HIPCHECK(hipMemcpyAsync(seis[b], mb[b]->seis_d,
+                              (size_t)isamp[b] * 3 * mb[b]->nrec * sizeof(float),
                               hipMemcpyDeviceToHost, st_));
   }
   HIPCHECK(hipStreamSynchronize(st_));
@@ -1985,35 +2298,96 @@ static double run_members(Runner &R, const std::vector<axh_ctx *> &members, int3
     R.kev.clear();
     R.kev_class.clear();
   }
-  for (int b = 0; b < nb; b++) {
-    members[b]->iter_done = it1;
-    members[b]->last_run_ms = ms;
-  }
-  for (int b = 0; b < nb; b++)
-    if (blow[b] != INT_MAX)
-      fail(AXH_ERR_BLOWUP, (nb > 1 ? "batch member " + std::to_string(b) + ": " : std::string()) +
-                               "displacements blew up at time step " + std::to_string(blow[b]));
   return ms;
+}
+
+static double run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
+                      const axh_ctx *const *ctx, int32_t it0, int32_t it1, float *const *seis,
+                      int *blow) {
+  switch (F.B) {
+    case 1: return run_group<1>(R, M, F, mb, ctx, it0, it1, seis, blow);
+    case 2: return run_group<2>(R, M, F, mb, ctx, it0, it1, seis, blow);
+    case 3: return run_group<3>(R, M, F, mb, ctx, it0, it1, seis, blow);
+    case 4: return run_group<4>(R, M, F, mb, ctx, it0, it1, seis, blow);
+    default: fail(AXH_ERR_STATE, "group width out of range");
+  }
 }
 
 void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   Device &V = *ctx->dev;
+  if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
+  V.run.init();
   V.run.profile = ctx->profile;
+  Member *mb[1] = {&V.member};
+  const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
-  run_members(V.run, {ctx}, it0, it1, out);
+  int blow = INT_MAX;
+  ctx->last_run_ms = run_any(V.run, *V.mesh, V.own, mb, cx, it0, it1, out, &blow);
+  ctx->iter_done = it1;
+  if (blow != INT_MAX)
+    fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
 }
 
 void batch_run(axh_batch *bt, int32_t it0, int32_t it1, float *const *seis) {
-  if (!bt->run) {
-    std::unique_ptr<Runner, RunnerDeleter> r(new Runner);
-    bt->run = std::move(r);
+  BatchState &S = *bt->state;
+  S.run.profile = bt->profile;
+  const MeshDev &M = *bt->members[0]->dev->mesh;
+  double ms = 0.0;
+  std::vector<int> blow(bt->members.size(), INT_MAX);
+  for (size_t g = 0; g < S.groups.size(); g++) {
+    const std::vector<int> &idx = S.index[g];
+    Member *mb[kGroup];
+    const axh_ctx *cx[kGroup];
+    float *out[kGroup];
+    int bl[kGroup];
+    for (size_t b = 0; b < idx.size(); b++) {
+      axh_ctx *c = bt->members[idx[b]];
+      if (c->dev->mesh.get() != &M) fail(AXH_ERR_STATE, "batch members on different meshes");
+      mb[b] = &c->dev->member;
+      cx[b] = c;
+      out[b] = seis ? seis[idx[b]] : nullptr;
+    }
+    ms += run_any(S.run, M, *S.groups[g], mb, cx, it0, it1, out, bl);
+    for (size_t b = 0; b < idx.size(); b++) blow[idx[b]] = bl[b];
   }
-  bt->run->profile = bt->profile;
-  bt->last_run_ms = run_members(*bt->run, bt->members, it0, it1, seis);
+  bt->last_run_ms = ms;
+  for (axh_ctx *c : bt->members) {
+    c->iter_done = it1;
+    c->last_run_ms = ms;
+  }
+  for (size_t b = 0; b < blow.size(); b++)
+    if (blow[b] != INT_MAX)
+      fail(AXH_ERR_BLOWUP, "batch member " + std::to_string(b) +
+                               ": displacements blew up at time step " + std::to_string(blow[b]));
+}
+
+// The live copy of ctx's state: its own fields, or its slot in a batch group.
+static Fields &live_fields(Device &V, int &slot) {
+  if (V.live) {
+    slot = V.live_slot;
+    return *V.live;
+  }
+  slot = 0;
+  return V.own;
 }
 
 static void sync_member(Device &V) {
   if (V.run.stream) HIPCHECK(hipStreamSynchronize(V.run.stream));
+  if (V.live) HIPCHECK(hipDeviceSynchronize());
+}
+
+// one wavefield's values of an interleaved device array
+static std::vector<float> fetch(const float *dsrc, size_t n, int B, int slot) {
+  std::vector<float> h(n * B), out(n);
+  HIPCHECK(hipMemcpy(h.data(), dsrc, h.size() * 4, hipMemcpyDeviceToHost));
+  for (size_t x = 0; x < n; x++) out[x] = h[x * B + slot];
+  return out;
+}
+static void store(float *ddst, const std::vector<float> &v, int B, int slot) {
+  std::vector<float> h(v.size() * B);
+  HIPCHECK(hipMemcpy(h.data(), ddst, h.size() * 4, hipMemcpyDeviceToHost));
+  for (size_t x = 0; x < v.size(); x++) h[x * B + slot] = v[x];
+  HIPCHECK(hipMemcpy(ddst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
 }
 
 // element-local fields from the per-point device arrays
@@ -2022,27 +2396,27 @@ void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
   Device &V = *ctx->dev;
   const MeshDev &M = *V.mesh;
   sync_member(V);
+  int slot = 0;
+  Fields &F = live_fields(V, slot);
   auto scatter_s = [&](const float *dsrc, float *out) {
-    std::vector<float> h(3 * (size_t)M.nglob_s);
-    HIPCHECK(hipMemcpy(h.data(), dsrc, h.size() * 4, hipMemcpyDeviceToHost));
+    const std::vector<float> h = fetch(dsrc, 3 * (size_t)M.nglob_s, F.B, slot);
     for (int c = 0; c < 3; c++)
       for (int e = 0; e < M.nel_s; e++)
         for (int k = 0; k < kNpp; k++)
           out[((size_t)c * M.nel_s + e) * kNpp + k] =
               h[(size_t)c * M.nglob_s + M.gid_s_h[slot_of(e, k)]];
   };
-  if (disp) scatter_s(V.wave.disp, disp);
-  if (velo) scatter_s(V.wave.velo, velo);
+  if (disp) scatter_s(F.disp, disp);
+  if (velo) scatter_s(F.velo, velo);
   if (M.nel_f > 0 && (chi || dchi || ddchi)) {
     auto scatter_f = [&](const float *dsrc, float *out) {
-      std::vector<float> h(M.nglob_f);
-      HIPCHECK(hipMemcpy(h.data(), dsrc, h.size() * 4, hipMemcpyDeviceToHost));
+      const std::vector<float> h = fetch(dsrc, M.nglob_f, F.B, slot);
       for (int e = 0; e < M.nel_f; e++)
         for (int k = 0; k < kNpp; k++) out[(size_t)e * kNpp + k] = h[M.gid_f_h[slot_of(e, k)]];
     };
-    if (chi) scatter_f(V.wave.chi, chi);
-    if (dchi) scatter_f(V.wave.dchi, dchi);
-    if (ddchi) scatter_f(V.wave.ddchi, ddchi);
+    if (chi) scatter_f(F.chi, chi);
+    if (dchi) scatter_f(F.dchi, dchi);
+    if (ddchi) scatter_f(F.ddchi, ddchi);
   }
 }
 
@@ -2051,6 +2425,8 @@ void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
   Device &V = *ctx->dev;
   const MeshDev &M = *V.mesh;
   sync_member(V);
+  int slot = 0;
+  Fields &F = live_fields(V, slot);
   auto gather_s = [&](const float *in, float *ddst, const char *what) {
     std::vector<float> h(3 * (size_t)M.nglob_s, 0.0f);
     std::vector<char> seen(h.size(), 0);
@@ -2062,10 +2438,10 @@ void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
           if (!seen[g]) { h[g] = x; seen[g] = 1; }
           else if (h[g] != x) fail(AXH_ERR_ARG, std::string(what) + ": copies of a point differ");
         }
-    HIPCHECK(hipMemcpy(ddst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    store(ddst, h, F.B, slot);
   };
-  if (disp) gather_s(disp, V.wave.disp, "disp");
-  if (velo) gather_s(velo, V.wave.velo, "velo");
+  if (disp) gather_s(disp, F.disp, "disp");
+  if (velo) gather_s(velo, F.velo, "velo");
   if (M.nel_f > 0 && (chi || dchi || ddchi)) {
     auto gather_f = [&](const float *in, float *ddst, const char *what) {
       std::vector<float> h(M.nglob_f, 0.0f);
@@ -2077,11 +2453,11 @@ void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
           if (!seen[g]) { h[g] = x; seen[g] = 1; }
           else if (h[g] != x) fail(AXH_ERR_ARG, std::string(what) + ": copies of a point differ");
         }
-      HIPCHECK(hipMemcpy(ddst, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+      store(ddst, h, F.B, slot);
     };
-    if (chi) gather_f(chi, V.wave.chi, "chi");
-    if (dchi) gather_f(dchi, V.wave.dchi, "dchi");
-    if (ddchi) gather_f(ddchi, V.wave.ddchi, "ddchi");
+    if (chi) gather_f(chi, F.chi, "chi");
+    if (dchi) gather_f(dchi, F.dchi, "dchi");
+    if (ddchi) gather_f(ddchi, F.ddchi, "ddchi");
   }
 }
 

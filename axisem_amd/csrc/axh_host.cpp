@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "axh_internal.h"
@@ -445,6 +446,7 @@ int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
       fail(AXH_ERR_ARG, "batch size must be 1.." + std::to_string(AXH_MAX_BATCH));
     for (int i = 0; i < n; i++) {
       if (!members[i]) fail(AXH_ERR_ARG, "null batch member");
+      if (members[i]->batch_refs > 0) fail(AXH_ERR_STATE, "a context is already in a batch");
       for (int j = 0; j < i; j++)
         if (members[j] == members[i]) fail(AXH_ERR_ARG, "a context appears twice in a batch");
     }
@@ -465,8 +467,10 @@ int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
 int axh_batch_destroy(axh_batch *batch) {
   return guarded([&] {
     if (!batch) return;
+    // the members keep their state: it moves back to them
+    std::unique_ptr<axh_batch> owned(batch);
     for (auto *c : batch->members) c->batch_refs--;
-    delete batch;
+    batch_detach(batch);
   });
 }
 
@@ -480,7 +484,7 @@ int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end, float *con
       if (it_end < it_begin - 1 || it_end > c->dims.niter)
         fail(AXH_ERR_ARG, "step range outside 1..niter of a member");
     }
-    batch_attach(batch->members);
+    batch_attach(batch);
     batch_run(batch, it_begin, it_end, seis);
   });
 }
@@ -490,6 +494,16 @@ int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms) {
     check_batch(batch);
     if (!total_ms) fail(AXH_ERR_ARG, "null pointer");
     *total_ms = batch->last_run_ms;
+  });
+}
+
+int axh_batch_set_group_width(axh_batch *batch, int32_t width) {
+  return guarded([&] {
+    check_batch(batch);
+    if (width < 1 || width > AXH_MAX_GROUP)
+      fail(AXH_ERR_ARG, "group width must be 1.." + std::to_string(AXH_MAX_GROUP));
+    if (batch->state) fail(AXH_ERR_STATE, "the batch has already run");
+    batch->group_width = width;
   });
 }
 

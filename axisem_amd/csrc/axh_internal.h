@@ -39,9 +39,9 @@ struct Device;  // device-side layout and buffers (axh_device.hip)
 struct DeviceDeleter {
   void operator()(Device *d) const;
 };
-struct Runner;  // stream, events and kernel timing of runs (axh_device.hip)
-struct RunnerDeleter {
-  void operator()(Runner *r) const;
+struct BatchState;  // device groups of a batch (axh_device.hip)
+struct BatchStateDeleter {
+  void operator()(BatchState *b) const;
 };
 
 // Error raised inside the library; converted to a status code at the ABI.
@@ -73,12 +73,14 @@ struct axh_ctx {
 
 // Wavefields on one mesh stepped together (axh_batch_*).  The members are not
 // owned; while a batch holds a context, the context takes no uploads and
-// cannot be destroyed.
+// cannot be destroyed; from the first batch run until the batch is destroyed
+// the context's state lives in the batch's device groups.
 struct axh_batch {
   std::vector<axh_ctx *> members;
-  std::unique_ptr<axh::Runner, axh::RunnerDeleter> run;
+  std::unique_ptr<axh::BatchState, axh::BatchStateDeleter> state;  // NULL before the first run
   double last_run_ms = 0.0;
   bool profile = false;
+  int group_width = AXH_MAX_GROUP;  // wavefields per kernel instance
 };
 
 namespace axh {
@@ -101,8 +103,11 @@ void check_inputs(const axh_ctx *ctx);
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob);
 bool kernel_stats(const Device *d, int k, double *ms, int64_t *n);
 // batches: put every member on the first member's device mesh (building what
-// is missing), then step them together
-void batch_attach(const std::vector<axh_ctx *> &members);
+// is missing) and move their state into the batch's groups; step them
+// together; move the state back
+void batch_attach(axh_batch *b);
+void batch_detach(axh_batch *b);
+bool in_running_batch(const axh_ctx *ctx);
 void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]);
 void batch_run(axh_batch *b, int32_t it0, int32_t it1, float *const *seis);
 bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n);

@@ -99,6 +99,7 @@ def library():
         L.axh_batch_run.argtypes = [P, C.c_int32, C.c_int32, C.POINTER(P)]
         L.axh_batch_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
         L.axh_batch_set_profiling.argtypes = [P, C.c_int]
+        L.axh_batch_set_group_width.argtypes = [P, C.c_int32]
         L.axh_batch_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_int64)]
         _lib = L
@@ -286,12 +287,14 @@ class Batch:
     reads the mesh terms once.  Each member's seismograms and fields are the
     ones it gets when run alone."""
 
-    def __init__(self, solvers):
+    def __init__(self, solvers, group_width: int | None = None):
         self.solvers = list(solvers)
         n = len(self.solvers)
         arr = (C.c_void_p * n)(*[s._h for s in self.solvers])
         self._h = C.c_void_p()
         _check(library().axh_batch_create(arr, n, C.byref(self._h)))
+        if group_width is not None:
+            _check(library().axh_batch_set_group_width(self._h, group_width))
 
     def close(self):
         if self._h:

@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--batch", type=int, default=len(BATCH_DEPTHS_KM),
                     help="wavefields of the batched leg (1 = plain only)")
+    ap.add_argument("--group-width", type=int, default=None,
+                    help="wavefields per kernel instance in the batched leg (default: library)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=os.path.join(REPO, "data", "bench_10s"))
@@ -251,7 +253,7 @@ def main():
     batched = None
     if a.batch > 1:
         members = [Solver.from_state_file(p) for p in states]
-        with Batch(members) as b:
+        with Batch(members, group_width=a.group_width) as b:
             log("batched x%d: warmup %d steps" % (nbatch, a.warmup))
             b.run(a.warmup)
             log("batched x%d: timed %d steps" % (nbatch, a.steps))
@@ -271,6 +273,7 @@ def main():
         btotal = nbatch * gll * a.steps / bwall
         batched = {
             "wavefields": nbatch,
+            "group_width": a.group_width or 4,
             "sources": "dipole thetaforce at %s km" % "/".join("%g" % x for x in
                                                                BATCH_DEPTHS_KM[:nbatch]),
             # GLL-point updates of all member wavefields per second, against

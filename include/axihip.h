@@ -37,6 +37,7 @@ extern "C" {
 
 #define AXH_ABI_VERSION 1
 #define AXH_MAX_BATCH 8     /* wavefields per batch */
+#define AXH_MAX_GROUP 4     /* wavefields per kernel instance */
 
 enum axh_status {
   AXH_OK = 0,
@@ -255,6 +256,13 @@ int axh_batch_destroy(axh_batch *batch);
 int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end,
                   float *const *seis);
 int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms);
+
+/* Wavefields advanced by one kernel instance (1..AXH_MAX_GROUP, default
+ * AXH_MAX_GROUP): the members are split into groups of at most this many,
+ * each group with its wavefields interleaved in device memory.  Wider groups
+ * read the mesh terms fewer times, narrower ones need fewer registers per
+ * lane.  Only before the first axh_batch_run. */
+int axh_batch_set_group_width(axh_batch *batch, int32_t width);
 int axh_batch_set_profiling(axh_batch *batch, int on);
 int axh_batch_kernel_stats(const axh_batch *batch, int kclass,
                            double *total_ms, int64_t *launches);
