@@ -66,15 +66,18 @@ constexpr uint8_t kFlagSource = 2;    // source term of some wavefield on the me
 constexpr uint8_t kFlagBlow = 4;      // GLL point (1,1) of an element
 constexpr uint8_t kFlagSurface = 8;   // fluid free surface (masked)
 constexpr uint8_t kFlagAxisBdry = 16; // axis point on the solid-fluid boundary
+constexpr uint8_t kFlagCoupled = 32;  // owns a slot of the solid-fluid coupling
+constexpr uint8_t kFlagDirect = 64;   // one element copy and nothing else to add:
+                                      // kicked by the stiffness kernel itself
 constexpr int kMaxAxisBdry = 16;
 
 // kernel classes for timing
-enum KClass { KC_PRED = 0, KC_FLUID, KC_COUPLE_F, KC_SOLID, KC_ASM_F,
-              KC_COUPLE_S, KC_ASM_S, KC_DRIFT, KC_MEMVAR, KC_REC, KC_N };
+// (the solid-fluid coupling and the symplectic drifts after the first are
+// part of the assemble_update kernels)
+enum KClass { KC_PRED = 0, KC_FLUID, KC_SOLID, KC_ASM_F, KC_ASM_S, KC_MEMVAR, KC_REC, KC_N };
 static const char *const kClassName[KC_N] = {
-    "predict", "fluid_stiffness", "copy2fluid", "solid_stiffness",
-    "fluid_assemble_update", "copy2solid", "solid_assemble_update",
-    "final_drift", "memvar_update", "receivers"};
+    "predict", "fluid_stiffness", "solid_stiffness", "fluid_assemble_update",
+    "solid_assemble_update", "memvar_update", "receivers"};
 
 // Device allocations owned by one object and freed with it.
 struct DevAlloc {
@@ -118,12 +121,12 @@ struct MeshDev {
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
   // host copies used by wavefield builds and field transfers
   std::vector<int> gid_s_h, gid_f_h;
-  std::vector<uint8_t> flags_s_h;
+  std::vector<uint8_t> flags_s_h;  // solid point flags without source bits (a
+                                   // group's device copy adds its sources)
   float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
   // solid
   float *coef_s = nullptr, *m0_s = nullptr, *minv_s = nullptr;
   int *einfo_s = nullptr, *gid_s = nullptr, *gstart_s = nullptr, *gslot_s = nullptr;
-  uint8_t *flags_s = nullptr;
   // fluid
   float *coef_f = nullptr, *m0_f = nullptr, *minv_f = nullptr;
   int *einfo_f = nullptr, *gid_f = nullptr, *gstart_f = nullptr, *gslot_f = nullptr;
@@ -133,6 +136,8 @@ struct MeshDev {
   float *cpl_bm = nullptr;    // [nbdry*5][2]
   int *cpl_axb = nullptr;     // side-buffer slot of a masked axis point or -1
   int *axb_g = nullptr;       // global ids of the masked axis boundary points
+  int *cpl_slot_s = nullptr;  // per solid slot: coupling entry or -1
+  int *cpl_slot_f = nullptr;  // per fluid slot: coupling entry or -1
   // attenuation
   float *cg_stage = nullptr, *cg_upd = nullptr;
   double *aj_mu = nullptr, *aj_k = nullptr;
@@ -184,6 +189,14 @@ struct SolidArgs {
   float *rsum;            // [nel][6][4][B]
   float *trm1;            // [nel][4][B]
   float *devm1;           // [nel][6][4][B]
+  // points of kFlagDirect: the stiffness kernel does their assembly (one
+  // term), kick and, for symplectic stages, the drift that follows
+  int direct;
+  const uint8_t *flags;
+  const float *minv;
+  float *velo, *acc0;     // [3][nglob][B]
+  float *disp_w;          // disp, written by the drift
+  double hdt, cv, cd_next;
 };
 
 struct FluidArgs {
@@ -288,43 +301,6 @@ __device__ __forceinline__ void grad_cg4(const float *G, bool axial, const float
   gz = cu[0 * 4 + q] * m1 + cu[2 * 4 + q] * m2;
 }
 
-// Voigt strain E1..E6 at cg point q from a wavefield's displacement tile in
-// LDS (entry k at k*S).
-template <int SRC, int S>
-__device__ void strain_cg4(const float *G, bool axial, const float *cu, int q,
-                           const float *U1, const float *U2, const float *U3,
-                           float g[6]) {
-  const float *invs = cu + 4 * 4;
-  const int p = (cg_r(q) + 5 * cg_c(q)) * S;
-  float b1s, b1z, b2s, b2z;
-#pragma unroll
-  for (int c = 0; c < 6; c++) g[c] = 0.0f;
-  if (SRC == AXH_DIPOLE)
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] + U2[k * S]; }, b1s, b1z);
-  else
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S]; }, b1s, b1z);
-  grad_cg4(G, axial, cu, q, [&](int k) { return U3[k * S]; }, b2s, b2z);
-  g[0] = b1s;
-  g[2] = b2z;
-  g[4] = b1z + b2s;
-  if (SRC == AXH_MONOPOLE) {
-    g[1] = invs[q] * U1[p];
-  } else if (SRC == AXH_DIPOLE) {
-    g[1] = 2.0f * (invs[q] * U2[p]);
-    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] - U2[k * S]; }, b1s, b1z);
-    float f3 = invs[q] * U3[p];
-    g[3] = -f3 - b1z;
-    g[5] = -g[1] - b1s;
-  } else {
-    g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
-    grad_cg4(G, axial, cu, q, [&](int k) { return U2[k * S]; }, b1s, b1z);
-    float f3 = invs[q] * U3[p];
-    float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
-    g[3] = -(2.0f * f3) - b1z;
-    g[5] = f6 - b1s;
-  }
-}
-
 // source terms of the memory-variable update at cg point q
 template <int SRC>
 __device__ __forceinline__ void memvar_sources(const float *cu, int q, const float g[6],
@@ -345,17 +321,81 @@ __device__ __forceinline__ void memvar_sources(const float *cu, int q, const flo
   }
 }
 
+// Voigt strain E1..E6 at the cg points (compute_strain_att_el_cg4 with the
+// gradients of SOLVER/pointwise_derivatives.f90:216-246), in two parts.
+// First the gradients, spread over the element's 25 lanes: item
+// w = (gi * 4 + q) * B + b is gradient gi of cg point q of wavefield b, gi over
+// (U1+U2 | U1, U3, U1-U2 | U2) (dipole | monopole, quadrupole); U1..U3 are the
+// element's displacement tiles in LDS (entry x of wavefield b at x*B + b).
+// gr[2 w + {0,1}] = (d/ds, d/dz).
+template <int SRC, int B>
+__device__ __forceinline__ void strain_grads(const float *G, bool axial, const float *cu, int k,
+                                             const float *U1, const float *U2, const float *U3,
+                                             float *gr) {
+  constexpr int NG = (SRC == AXH_MONOPOLE) ? 2 : 3;
+  for (int w = k; w < NG * 4 * B; w += 25) {
+    const int gi = w / (4 * B), r = w - gi * 4 * B, q = r / B, b = r - q * B;
+    const float *u1 = U1 + b, *u2 = U2 + b, *u3 = U3 + b;
+    float gs, gz;
+    if (gi == 0) {
+      if (SRC == AXH_DIPOLE)
+        grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B] + u2[x * B]; }, gs, gz);
+      else
+        grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B]; }, gs, gz);
+    } else if (gi == 1) {
+      grad_cg4(G, axial, cu, q, [&](int x) { return u3[x * B]; }, gs, gz);
+    } else if (SRC == AXH_DIPOLE) {
+      grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B] - u2[x * B]; }, gs, gz);
+    } else {
+      grad_cg4(G, axial, cu, q, [&](int x) { return u2[x * B]; }, gs, gz);
+    }
+    gr[2 * w] = gs;
+    gr[2 * w + 1] = gz;
+  }
+}
+
+// Then the strain at cg point q of wavefield b from those gradients.
+template <int SRC, int B>
+__device__ __forceinline__ void strain_from_grads(const float *cu, int q, int b, const float *gr,
+                                                  const float *U1, const float *U2,
+                                                  const float *U3, float g[6]) {
+  const float *invs = cu + 4 * 4;
+  const int p = (cg_r(q) + 5 * cg_c(q)) * B + b;
+  const float *ga = gr + 2 * ((0 * 4 + q) * B + b), *gb = gr + 2 * ((1 * 4 + q) * B + b);
+#pragma unroll
+  for (int c = 0; c < 6; c++) g[c] = 0.0f;
+  g[0] = ga[0];
+  g[2] = gb[1];
+  g[4] = ga[1] + gb[0];
+  if (SRC == AXH_MONOPOLE) {
+    g[1] = invs[q] * U1[p];
+  } else if (SRC == AXH_DIPOLE) {
+    const float *gc = gr + 2 * ((2 * 4 + q) * B + b);
+    g[1] = 2.0f * (invs[q] * U2[p]);
+    float f3 = invs[q] * U3[p];
+    g[3] = -f3 - gc[1];
+    g[5] = -g[1] - gc[0];
+  } else {
+    const float *gc = gr + 2 * ((2 * 4 + q) * B + b);
+    g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
+    float f3 = invs[q] * U3[p];
+    float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
+    g[3] = -(2.0f * f3) - gc[1];
+    g[5] = f6 - gc[0];
+  }
+}
+
 // Strain sources of the memory-variable update at cg point q of element e
 // for wavefield b of the group (lanes k < 4 of the element, one cg point
 // each).  The sources of the previous update go to old, the new ones to src
 // and to the t-1 store; src/old are [28][B] per element.
 template <int SRC, int B>
-__device__ __forceinline__ void memvar_prepare(const SolidArgs &A, int b, int e, int q,
-                                               bool axial, const float *U1, const float *U2,
+__device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *gr, int b, int e,
+                                               int q, const float *U1, const float *U2,
                                                const float *U3, float *src, float *old) {
   const float *cu = A.cg_upd + (size_t)e * (kCgUpd * 4);
   float g[6], str, sdev[6];
-  strain_cg4<SRC, B>(A.G, axial, cu, q, U1, U2, U3, g);
+  strain_from_grads<SRC, B>(cu, q, b, gr, U1, U2, U3, g);
   memvar_sources<SRC>(cu, q, g, str, sdev);
   float *trm1 = A.trm1 + (size_t)e * 4 * B;
   float *devm1 = A.devm1 + (size_t)e * 24 * B;
@@ -411,11 +451,10 @@ __device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, in
 // gather the (predicted, masked) displacement of the tile's points into LDS
 // (monopole: components s and z)
 template <int NCOMP, int B>
-__device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, size_t slot, int t,
+__device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, int gp, int t,
                                             float *sU) {
   float u[B];
   if (valid) {
-    const int gp = A.gid[slot];
 #pragma unroll
     for (int cc = 0; cc < NCOMP; cc++) {
       const int c = (NCOMP == 2 && cc == 1) ? 2 : cc;
@@ -439,8 +478,11 @@ __device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, size
 // One workgroup = one tile of 10 elements, one lane = one GLL point and the
 // B wavefields of the group; the lane's stiffness terms are read once.
 // ---------------------------------------------------------------------------
+// (occupancy hint: 5 workgroups per CU for a single wavefield, which fits the
+// register budget without spilling; wider groups take what their registers
+// allow)
 template <int SRC, int MODE, bool ANEL, int B>
-__global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
+__global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(SolidArgs A) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   constexpr int NC = (SRC == AXH_MONOPOLE) ? kNcMono
                      : (SRC == AXH_DIPOLE) ? kNcDi : kNcQuad;
@@ -453,6 +495,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   __shared__ __align__(16) float sSa[ANEL ? kElemPerTile * 24 * B : 1];
   __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
+  __shared__ __align__(16) float sGr[MV ? kElemPerTile * 24 * B : 1];
   __shared__ __align__(16) float sG[105];
 
   const int t = threadIdx.x, tile = blockIdx.x;
@@ -464,6 +507,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   const int ax = valid ? A.einfo[e] : -1;
   const bool axial = ax >= 0;
   const int eb = le * 25 * B;  // the element's first entry in an LDS tile
+  const int gp = valid ? A.gid[slot] : 0;
 
   stage_g(A.G, sG);
   // the lane's stiffness terms (the packed rows cover all 256 lanes)
@@ -473,7 +517,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
 #pragma unroll
     for (int a = 0; a < NC; a++) cf[a] = coef[a * 256];
   }
-  gather_disp<NCOMP, B>(A, valid, slot, t, sU);
+  gather_disp<NCOMP, B>(A, valid, gp, t, sU);
   __syncthreads();
 
   GRegs G;
@@ -785,10 +829,14 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
       // time_step_memvars_cg4 with the masked predicted displacement; lane
       // k: cg point k % 4 of wavefield k / 4
       float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
+      float *gr = &sGr[le * 24 * B];
+      const float *V1 = U0, *V2 = (NCOMP == 2 ? U0 : U1), *V3 = (NCOMP == 2 ? U1 : U2);
+      if (valid)
+        strain_grads<SRC, B>(sG, axial, A.cg_upd + (size_t)e * (kCgUpd * 4), k, V1, V2, V3, gr);
+      __syncthreads();
       if (valid && k < 4 * B) {
-        const int b = k >> 2;
-        memvar_prepare<SRC, B>(A, b, e, k & 3, axial, U0 + b, (NCOMP == 2 ? U0 : U1) + b,
-                               (NCOMP == 2 ? U1 : U2) + b, src, old);
+        // lane k: cg point k % 4 of wavefield k / 4
+        memvar_prepare<SRC, B>(A, gr, k >> 2, e, k & 3, V1, V2, V3, src, old);
       }
       __syncthreads();
       if (valid)
@@ -798,9 +846,50 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs A) {
   }
 
   if (valid) {
-    stv<B>(A.force + slot * B, acc[0]);
-    if (SRC != AXH_MONOPOLE) stv<B>(A.force + ((size_t)A.nslot + slot) * B, acc[1]);
-    stv<B>(A.force + (2 * (size_t)A.nslot + slot) * B, acc[2]);
+    if (A.direct && (A.flags[gp] & kFlagDirect)) {
+      // the point's only copy: assembly (0 + f, as the gather sums), kick
+      // and drift as in k_asm_solid
+      const float mi = A.minv[gp];
+#pragma unroll
+      for (int cc = 0; cc < NCOMP; cc++) {
+        const int c = (NCOMP == 2 && cc == 1) ? 2 : cc;
+        const size_t p = ((size_t)c * A.nglob + gp) * B;
+        float v[B];
+        ldv<B>(A.velo + p, v);
+        if (MODE == 0) {
+          float a0[B], a[B];
+          ldv<B>(A.acc0 + p, a0);
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            const float sm = 0.0f + acc[c][b];
+            if (c == 2 && SRC == AXH_DIPOLE)
+              a[b] = (float)(-2.0 * (double)mi * (double)sm);
+            else
+              a[b] = -mi * sm;
+            v[b] = (float)((double)v[b] + A.hdt * (double)(a0[b] + a[b]));
+          }
+          stv<B>(A.acc0 + p, a);
+          stv<B>(A.velo + p, v);
+        } else {
+          float u[B];
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            const float a = -mi * (0.0f + acc[c][b]);
+            if (c == 2 && SRC == AXH_DIPOLE)
+              v[b] = (float)((double)v[b] + 2.0 * (double)a * A.cv);
+            else
+              v[b] = (float)((double)v[b] + (double)a * A.cv);
+            u[b] = (float)((double)sU[cc * 256 * B + t * B + b] + (double)v[b] * A.cd_next);
+          }
+          stv<B>(A.velo + p, v);
+          stv<B>(A.disp_w + p, u);
+        }
+      }
+    } else {
+      stv<B>(A.force + slot * B, acc[0]);
+      if (SRC != AXH_MONOPOLE) stv<B>(A.force + ((size_t)A.nslot + slot) * B, acc[1]);
+      stv<B>(A.force + (2 * (size_t)A.nslot + slot) * B, acc[2]);
+    }
   }
 }
 
@@ -813,8 +902,10 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   extern __shared__ __align__(16) float sMv[];
   __shared__ __align__(16) float sU[NCOMP * 256 * B];
+  __shared__ __align__(16) float sG[105];
   __shared__ __align__(16) float sSrc[kElemPerTile * 28 * B];
   __shared__ __align__(16) float sOld[kElemPerTile * 28 * B];
+  __shared__ __align__(16) float sGr[kElemPerTile * 24 * B];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int e = tile * kElemPerTile + le;
@@ -822,7 +913,8 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   const size_t slot = (size_t)tile * 256 + t;
   const int n = 24 * A.nsls * B;
   const int eb = le * 25 * B;
-  gather_disp<NCOMP, B>(A, valid, slot, t, sU);
+  stage_g(A.G, sG);
+  gather_disp<NCOMP, B>(A, valid, valid ? A.gid[slot] : 0, t, sU);
   float *mv = &sMv[le * n];
   {
     // the tile's memory variables are one contiguous block (n is a multiple
@@ -837,11 +929,14 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   const bool axial = valid && A.einfo[e] >= 0;
   float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
   const float *U0 = sU + eb, *U1 = sU + 256 * B + eb, *U2 = sU + 2 * 256 * B + eb;
+  float *gr = &sGr[le * 24 * B];
+  const float *V1 = U0, *V2 = (NCOMP == 2 ? U0 : U1), *V3 = (NCOMP == 2 ? U1 : U2);
+  if (valid)
+    strain_grads<SRC, B>(sG, axial, A.cg_upd + (size_t)e * (kCgUpd * 4), k, V1, V2, V3, gr);
+  __syncthreads();
   if (valid && k < 4 * B) {
     // lane k: cg point k % 4 of wavefield k / 4
-    const int b = k >> 2;
-    memvar_prepare<SRC, B>(A, b, e, k & 3, axial, U0 + b, (NCOMP == 2 ? U0 : U1) + b,
-                           (NCOMP == 2 ? U1 : U2) + b, src, old);
+    memvar_prepare<SRC, B>(A, gr, k >> 2, e, k & 3, V1, V2, V3, src, old);
   }
   __syncthreads();
   if (valid) memvar_update_elem<SRC, B>(A, e, k, mv, src, old, mv);
@@ -937,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
 struct PredArgs {
   int n, mask0, mask1;   // masked component range [mask0, mask1]
   int comps;             // bit c set: component c is stepped
-  int newmark, apply_mask, check_blow, iter, stride, naxb;
+  int newmark, apply_mask, check_blow, stride, naxb;
   double dt, hdt2, cd;
   const uint8_t *flags;
   const int *axb_g;      // masked axis points on the solid-fluid boundary
@@ -946,6 +1041,7 @@ struct PredArgs {
   float *axb_val;        // unmasked prediction of the axb points [naxb][3][B]
   int *blow[kGroup];
   double thr[kGroup];
+  int iter[kGroup];      // each wavefield's current step
 };
 
 template <int B>
@@ -962,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
     if (A.check_blow && (fl & kFlagBlow)) {
 #pragma unroll
       for (int b = 0; b < B; b++)
-        if ((double)fabsf(u[b]) > A.thr[b]) atomicMin(A.blow[b], A.iter);
+        if ((double)fabsf(u[b]) > A.thr[b]) atomicMin(A.blow[b], A.iter[b]);
     }
     if (A.newmark) {
       float a0[B];
@@ -991,6 +1087,8 @@ __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
 // ---------------------------------------------------------------------------
 struct AsmSolidArgs {
   int n, src, newmark, nslot, comps, mask0, mask1;
+  int npts;              // points assembled here (the others are kFlagDirect)
+  const int *pts;
   const int *gstart, *gslot;
   const uint8_t *flags;
   const float *minv;
@@ -1001,12 +1099,27 @@ struct AsmSolidArgs {
   const int *src_g[kGroup];
   const float *src_val[kGroup];
   float stf[kGroup];
+  // symplectic: the drift of the next stage (or the final drift) right after
+  // the kick, u = u + v*cd_next, masked when drift_mask is set
+  int drift, drift_mask, naxb;
+  double cd_next;
+  float *disp;           // [3][n][B]
+  const int *axb_g;
+  float *axb_val;        // [naxb][3][B]
+  // solid-fluid coupling (bdry_copy2solid): added to the force of a coupled
+  // slot before the slot enters the sum
+  int dipole;
+  const int *cpl_slot;   // per solid slot: coupling entry or -1
+  const int *cpl_gf;     // fluid point of a coupling entry
+  const float *cpl_bm;   // [entry][2]
+  const float *ddchi;    // [nglob_f][B]
 };
 
 template <int B>
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= A.n) return;
+  const int ip = blockIdx.x * 256 + threadIdx.x;
+  if (ip >= A.npts) return;
+  const int g = A.pts[ip];
   const uint8_t fl = A.flags[g];
   const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
@@ -1030,9 +1143,22 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
     if (!((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1)) {
       // masked copies are zero before the gather
       const float *f = A.force + (size_t)c * A.nslot * B;
+      const bool coupled = (fl & kFlagCoupled) && (c != 1 || A.dipole);
       for (int q = q0; q < q1; q++) {
+        const int sl = A.gslot[q];
         float x[B];
-        ldv<B>(f + (size_t)A.gslot[q] * B, x);
+        ldv<B>(f + (size_t)sl * B, x);
+        if (coupled) {
+          // bdry_copy2solid (SOLVER/time_evol_wave.F90:1582-1611)
+          const int p = A.cpl_slot[sl];
+          if (p >= 0) {
+            float fl2[B];
+            ldv<B>(A.ddchi + (size_t)A.cpl_gf[p] * B, fl2);
+            const float bmc = (c == 2) ? A.cpl_bm[2 * p + 1] : A.cpl_bm[2 * p];
+#pragma unroll
+            for (int b = 0; b < B; b++) x[b] = x[b] + bmc * fl2[b];
+          }
+        }
 #pragma unroll
         for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
       }
@@ -1066,6 +1192,22 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
       }
     }
     stv<B>(A.velo + p, v);
+    if (A.drift) {
+      // as k_predict's drift, with the velocity just computed
+      float u[B];
+      ldv<B>(A.disp + p, u);
+#pragma unroll
+      for (int b = 0; b < B; b++) u[b] = (float)((double)u[b] + (double)v[b] * A.cd_next);
+      if (fl & kFlagAxisBdry) {
+        for (int q = 0; q < A.naxb; q++)
+          if (A.axb_g[q] == g) stv<B>(A.axb_val + ((size_t)q * 3 + c) * B, u);
+      }
+      if (A.drift_mask && (fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
+#pragma unroll
+        for (int b = 0; b < B; b++) u[b] = 0.0f;
+      }
+      stv<B>(A.disp + p, u);
+    }
   }
 }
 
@@ -1077,6 +1219,20 @@ struct AsmFluidArgs {
   double hdt, cv;
   const float *force;        // [nslot_f][B]
   float *dchi, *ddchi;       // [n][B]
+  // symplectic: the drift of the next stage (or the final drift) right after
+  // the kick, chi = chi + dchi*cd_next, axis-masked when drift_mask is set
+  int drift, drift_mask;
+  double cd_next;
+  float *chi;                // [n][B]
+  // solid-fluid coupling (bdry_copy2fluid): added to the force of a coupled
+  // slot before the slot enters the sum
+  int dipole, nglob_s;
+  const int *cpl_slot;       // per fluid slot: coupling entry or -1
+  const int *cpl_gs;         // solid point of a coupling entry
+  const int *cpl_axb;        // its side-buffer slot or -1
+  const float *cpl_bm;       // [entry][2]
+  const float *disp;         // [3][nglob_s][B], the stage's prediction
+  const float *axb_val;      // [naxb][3][B]
 };
 
 template <int B>
@@ -1089,13 +1245,42 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
   float s[B];
 #pragma unroll
   for (int b = 0; b < B; b++) s[b] = 0.0f;
-  if (!masked)
+  if (!masked) {
+    const bool coupled = fl & kFlagCoupled;
     for (int q = A.gstart[g]; q < A.gstart[g + 1]; q++) {
+      const int sl = A.gslot[q];
       float x[B];
-      ldv<B>(A.force + (size_t)A.gslot[q] * B, x);
+      ldv<B>(A.force + (size_t)sl * B, x);
+      if (coupled) {
+        // bdry_copy2fluid (SOLVER/time_evol_wave.F90:1532-1580) with the
+        // prediction before the axis mask
+        const int p = A.cpl_slot[sl];
+        if (p >= 0) {
+          const int gs = A.cpl_gs[p], kq = A.cpl_axb[p];
+          float s1[B], s2[B], s3[B];
+          if (kq >= 0) {
+            ldv<B>(A.axb_val + ((size_t)kq * 3 + 0) * B, s1);
+            ldv<B>(A.axb_val + ((size_t)kq * 3 + 1) * B, s2);
+            ldv<B>(A.axb_val + ((size_t)kq * 3 + 2) * B, s3);
+          } else {
+            ldv<B>(A.disp + (size_t)gs * B, s1);
+            ldv<B>(A.disp + ((size_t)A.nglob_s + gs) * B, s2);
+            ldv<B>(A.disp + (2 * (size_t)A.nglob_s + gs) * B, s3);
+          }
+          const float bm1 = A.cpl_bm[2 * p], bm2 = A.cpl_bm[2 * p + 1];
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            if (A.dipole)
+              x[b] = x[b] - bm1 * (s1[b] + s2[b]) - bm2 * s3[b];
+            else
+              x[b] = x[b] - bm1 * s1[b] - bm2 * s3[b];
+          }
+        }
+      }
 #pragma unroll
       for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
     }
+  }
   const float mi = A.minv[g];
   float d[B], dd[B];
   ldv<B>(A.dchi + (size_t)g * B, d);
@@ -1116,79 +1301,17 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
   }
   stv<B>(A.ddchi + (size_t)g * B, dd);
   stv<B>(A.dchi + (size_t)g * B, d);
-}
-
-// ---------------------------------------------------------------------------
-// solid-fluid coupling (SOLVER/time_evol_wave.F90:1532-1611), one lane per
-// boundary GLL point
-// ---------------------------------------------------------------------------
-struct CoupleArgs {
-  int n, dipole, nslot_s, nglob_s;
-  const int *ss, *gs, *sf, *gf;
-  const float *bm;       // [n][2]
-  const int *axb;        // side-buffer slot per boundary point or -1
-  const float *disp;     // [3][nglob_s][B]
-  const float *ddchi;    // [nglob_f][B]
-  float *force_f;        // [nslot_f][B]
-  float *force_s;        // [3][nslot_s][B]
-  const float *axb_val;  // [naxb][3][B]
-};
-
-template <int B>
-__global__ __launch_bounds__(256) void k_copy2fluid(CoupleArgs A) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= A.n) return;
-  const int g = A.gs[p], kq = A.axb[p];
-  float s1[B], s2[B], s3[B];
-  if (kq >= 0) {  // prediction before the axis mask
-    ldv<B>(A.axb_val + ((size_t)kq * 3 + 0) * B, s1);
-    ldv<B>(A.axb_val + ((size_t)kq * 3 + 1) * B, s2);
-    ldv<B>(A.axb_val + ((size_t)kq * 3 + 2) * B, s3);
-  } else {
-    ldv<B>(A.disp + (size_t)g * B, s1);
-    ldv<B>(A.disp + ((size_t)A.nglob_s + g) * B, s2);
-    ldv<B>(A.disp + (2 * (size_t)A.nglob_s + g) * B, s3);
+  if (A.drift) {
+    float c[B];
+    ldv<B>(A.chi + (size_t)g * B, c);
+#pragma unroll
+    for (int b = 0; b < B; b++) c[b] = (float)((double)c[b] + (double)d[b] * A.cd_next);
+    if (A.drift_mask && (fl & kFlagAxis)) {
+#pragma unroll
+      for (int b = 0; b < B; b++) c[b] = 0.0f;
+    }
+    stv<B>(A.chi + (size_t)g * B, c);
   }
-  const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
-  float f[B];
-  float *ff = A.force_f + (size_t)A.sf[p] * B;
-  ldv<B>(ff, f);
-#pragma unroll
-  for (int b = 0; b < B; b++) {
-    if (A.dipole)
-      f[b] = f[b] - bm1 * (s1[b] + s2[b]) - bm2 * s3[b];
-    else
-      f[b] = f[b] - bm1 * s1[b] - bm2 * s3[b];
-  }
-  stv<B>(ff, f);
-}
-
-template <int B>
-__global__ __launch_bounds__(256) void k_copy2solid(CoupleArgs A) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= A.n) return;
-  float fl[B];
-  ldv<B>(A.ddchi + (size_t)A.gf[p] * B, fl);
-  const float bm1 = A.bm[2 * p], bm2 = A.bm[2 * p + 1];
-  const size_t s = A.ss[p];
-  float x[B];
-  float *f0 = A.force_s + s * B;
-  ldv<B>(f0, x);
-#pragma unroll
-  for (int b = 0; b < B; b++) x[b] = x[b] + bm1 * fl[b];
-  stv<B>(f0, x);
-  if (A.dipole) {
-    float *f1 = A.force_s + ((size_t)A.nslot_s + s) * B;
-    ldv<B>(f1, x);
-#pragma unroll
-    for (int b = 0; b < B; b++) x[b] = x[b] + bm1 * fl[b];
-    stv<B>(f1, x);
-  }
-  float *f2 = A.force_s + (2 * (size_t)A.nslot_s + s) * B;
-  ldv<B>(f2, x);
-#pragma unroll
-  for (int b = 0; b < B; b++) x[b] = x[b] + bm2 * fl[b];
-  stv<B>(f2, x);
 }
 
 // receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
@@ -1325,7 +1448,7 @@ struct MeshHost {
   double coefd[16] = {0}, coefv[16] = {0};
   std::vector<float> G, coef_s, m0_s, minv_s, coef_f, m0_f, minv_f, cpl_bm, cg_stage, cg_upd;
   std::vector<int> einfo_s, gid_s, gstart_s, gslot_s, einfo_f, gid_f, gstart_f, gslot_f,
-      cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g;
+      cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, cpl_slot_s, cpl_slot_f;
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
@@ -1474,6 +1597,8 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
     const int32_t *bjf = ctx->need("bdry_jpol_fluid").as<int32_t>();
     const float *bm = ctx->need("bdry_matr").as<float>();
     std::vector<char> used_s((size_t)H.ntile_s * kTile, 0), used_f((size_t)H.ntile_f * kTile, 0);
+    H.cpl_slot_s.assign((size_t)H.ntile_s * kTile, -1);
+    H.cpl_slot_f.assign((size_t)H.ntile_f * kTile, -1);
     for (int b = 0; b < d.nel_bdry; b++) {
       int es = bse[b] - 1, ef = bfe[b] - 1;
       if (es < 0 || es >= H.nel_s || ef < 0 || ef >= H.nel_f || bjs[b] < 0 || bjs[b] > 4 ||
@@ -1498,6 +1623,10 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
             H.flags_s[g] |= kFlagAxisBdry;
           }
         }
+        // the assembly adds the coupling term to these slots before summing
+        H.cpl_slot_s[s1] = H.cpl_slot_f[s2] = (int)H.cpl_ss.size();
+        H.flags_s[g] |= kFlagCoupled;
+        H.flags_f[H.gid_f[s2]] |= kFlagCoupled;
         H.cpl_ss.push_back((int)s1);
         H.cpl_gs.push_back(g);
         H.cpl_sf.push_back((int)s2);
@@ -1510,6 +1639,13 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
     H.naxb = (int)H.axb_g.size();
     if (H.naxb > kMaxAxisBdry) fail(AXH_ERR_UNSUPPORTED, "too many axis points on the boundary");
   }
+
+  // ---- points with a single element copy and nothing else to add (no axis
+  // mask, no coupling term; source points are excluded when a wavefield with
+  // a source there is attached) ----
+  for (int g = 0; g < H.nglob_s; g++)
+    if (H.gstart_s[g + 1] - H.gstart_s[g] == 1 && !(H.flags_s[g] & (kFlagAxis | kFlagCoupled)))
+      H.flags_s[g] |= kFlagDirect;
 
   // ---- attenuation ----
   if (d.anel) {
@@ -1662,7 +1798,12 @@ struct FpHash {
   template <class T> void vec(const std::vector<T> &v) { bytes(v.data(), v.size() * sizeof(T)); }
 };
 
-void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]) {
+void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]) {
+  if (ctx->fp_valid) {
+    fp[0] = ctx->fp[0];
+    fp[1] = ctx->fp[1];
+    return;
+  }
   MeshHost H;
   build_mesh_layout(ctx, H);
   FpHash h;
@@ -1681,14 +1822,15 @@ void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]) {
     h.vec(*v);
   for (const auto *v : {&H.einfo_s, &H.gid_s, &H.gstart_s, &H.gslot_s, &H.einfo_f, &H.gid_f,
                         &H.gstart_f, &H.gslot_f, &H.cpl_ss, &H.cpl_gs, &H.cpl_sf, &H.cpl_gf,
-                        &H.cpl_axb, &H.axb_g})
+                        &H.cpl_axb, &H.axb_g, &H.cpl_slot_s, &H.cpl_slot_f})
     h.vec(*v);
   h.vec(H.flags_s);
   h.vec(H.flags_f);
   h.vec(H.aj_mu);
   h.vec(H.aj_k);
-  fp[0] = h.a;
-  fp[1] = h.b;
+  fp[0] = ctx->fp[0] = h.a;
+  fp[1] = ctx->fp[1] = h.b;
+  ctx->fp_valid = true;
 }
 
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob) {
@@ -1738,7 +1880,6 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   V.gstart_s = m.upload(H.gstart_s);
   V.gslot_s = m.upload(H.gslot_s);
   V.minv_s = m.upload(H.minv_s);
-  V.flags_s = m.upload(H.flags_s);
   if (V.nel_f > 0) {
     V.coef_f = m.upload(H.coef_f);
     V.einfo_f = m.upload(H.einfo_f);
@@ -1756,6 +1897,8 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
     V.cpl_gf = m.upload(H.cpl_gf);
     V.cpl_axb = m.upload(H.cpl_axb);
     V.cpl_bm = m.upload(H.cpl_bm);
+    V.cpl_slot_s = m.upload(H.cpl_slot_s);
+    V.cpl_slot_f = m.upload(H.cpl_slot_f);
     V.axb_g = m.upload(H.axb_g);
   }
   if (V.anel) {
@@ -1768,21 +1911,6 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   V.gid_f_h = std::move(H.gid_f);
   V.flags_s_h = std::move(H.flags_s);
   return M;
-}
-
-// Mark the wavefield's source points in the mesh flags (the flag is the union
-// over the wavefields stepped on the mesh; the assembly looks a flagged point
-// up in its own wavefield's source list).
-static void mesh_add_sources(MeshDev &M, const std::vector<int> &src_g) {
-  bool changed = false;
-  for (int g : src_g)
-    if (!(M.flags_s_h[g] & kFlagSource)) {
-      M.flags_s_h[g] |= kFlagSource;
-      changed = true;
-    }
-  if (changed)
-    HIPCHECK(hipMemcpy(M.flags_s, M.flags_s_h.data(), M.flags_s_h.size(),
-                       hipMemcpyHostToDevice));
 }
 
 // ---------------------------------------------------------------------------
@@ -1816,6 +1944,12 @@ struct Fields {
   float *chi = nullptr, *dchi = nullptr, *ddchi = nullptr, *force_f = nullptr;
   float *memvar = nullptr, *rsum = nullptr, *trm1 = nullptr, *devm1 = nullptr;
   float *axb_val = nullptr;   // unmasked prediction of the axis boundary points
+  // solid point flags with the source points of the group's wavefields
+  // (kFlagSource set, kFlagDirect cleared) and the points the assembly
+  // kernel handles (all but kFlagDirect, ascending)
+  uint8_t *flags_s = nullptr;
+  int nasm = 0;
+  int *asm_pts = nullptr;
 };
 
 // the state arrays of a wavefield (pointer into F, entries per wavefield)
@@ -1837,9 +1971,31 @@ static std::vector<std::pair<float *, size_t>> state_arrays(const MeshDev &M, Fi
   return v;
 }
 
+// Point flags and assembly list of a group whose wavefields have their solid
+// source terms at the points srcs[...] (the mesh part stays unchanged).
+static void fields_set_sources(const MeshDev &M, Fields &F,
+                               const std::vector<const std::vector<int> *> &srcs) {
+  std::vector<uint8_t> fl = M.flags_s_h;
+  for (const auto *v : srcs)
+    for (int g : *v) {
+      fl[g] |= kFlagSource;
+      fl[g] &= (uint8_t)~kFlagDirect;
+    }
+  std::vector<int> pts;
+  pts.reserve(M.nglob_s);
+  for (int g = 0; g < M.nglob_s; g++)
+    if (!(fl[g] & kFlagDirect)) pts.push_back(g);
+  F.nasm = (int)pts.size();
+  HIPCHECK(hipMemcpy(F.flags_s, fl.data(), fl.size(), hipMemcpyHostToDevice));
+  if (!pts.empty())
+    HIPCHECK(hipMemcpy(F.asm_pts, pts.data(), pts.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
 static void fields_alloc(const MeshDev &M, Fields &F, int B) {
   F.B = B;
   DevAlloc &m = F.mem;
+  F.flags_s = m.alloc<uint8_t>(M.nglob_s);
+  F.asm_pts = m.alloc<int>(M.nglob_s);
   const size_t b = B;
   F.disp = m.zeros<float>(3 * (size_t)M.nglob_s * b);
   F.velo = m.zeros<float>(3 * (size_t)M.nglob_s * b);
@@ -1881,7 +2037,7 @@ static void copy_state(const MeshDev &M, Fields &D, int dst_b, Fields &S, int sr
   HIPCHECK(hipGetLastError());
 }
 
-static void member_build(const axh_ctx *ctx, MeshDev &M, Member &W) {
+static void member_build(const axh_ctx *ctx, const MeshDev &M, Member &W) {
   const axh_dims &d = ctx->dims;
   WaveHost H;
   build_wave_layout(ctx, M.gid_s_h, M.nglob_s, H);
@@ -1903,7 +2059,6 @@ static void member_build(const axh_ctx *ctx, MeshDev &M, Member &W) {
   W.blow_thr = 10.0 * std::fabs(d.magnitude);
   W.stf = std::move(H.stf);
   W.src_g_h = std::move(H.src_g);
-  mesh_add_sources(M, W.src_g_h);
 }
 
 // Stream, events and per-kernel timing of the runs of a context or a batch.
@@ -1951,9 +2106,10 @@ struct BatchState {
 
 void device_build(axh_ctx *ctx) {
   std::unique_ptr<Device> D(new Device);
-  D->mesh = mesh_build(ctx);
+  D->mesh = ctx->bound_mesh ? ctx->bound_mesh : mesh_build(ctx);
   member_build(ctx, *D->mesh, D->member);
   fields_alloc(*D->mesh, D->own, 1);
+  fields_set_sources(*D->mesh, D->own, {&D->member.src_g_h});
   HIPCHECK(hipDeviceSynchronize());
   ctx->dev.reset(D.release());
 }
@@ -2001,12 +2157,12 @@ void batch_attach(axh_batch *bt) {
       D->mesh = mesh;
       member_build(c, *mesh, D->member);
       fields_alloc(*mesh, D->own, 1);
+      fields_set_sources(*mesh, D->own, {&D->member.src_g_h});
       c->dev.reset(D.release());
     } else if (c->dev->mesh != mesh) {
       // same mesh part (checked when the batch was created): the global
       // numbering is the same, so the wavefield part carries over unchanged
       if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
-      mesh_add_sources(*mesh, c->dev->member.src_g_h);
       c->dev->mesh = mesh;
     }
   }
@@ -2021,6 +2177,9 @@ void batch_attach(axh_batch *bt) {
     const int size = n / ng + (g < n % ng ? 1 : 0);
     std::unique_ptr<Fields> F(new Fields);
     fields_alloc(*mesh, *F, size);
+    std::vector<const std::vector<int> *> srcs;
+    for (int b = 0; b < size; b++) srcs.push_back(&members[next + b]->dev->member.src_g_h);
+    fields_set_sources(*mesh, *F, srcs);
     std::vector<int> idx;
     for (int b = 0; b < size; b++, next++) {
       Device &D = *members[next]->dev;
@@ -2118,13 +2277,14 @@ static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A)
     hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
 }
 
-// Steps it0..it1 of the B wavefields of F (member data mb[b], dims of member
-// 0) on the stream of R; seis[b] (may be NULL) receives wavefield b's samples
+// Advances each of the B wavefields of F (member data mb[b], dims of member
+// 0) by nsteps, wavefield b through its steps start[b] .. start[b]+nsteps-1,
+// on the stream of R; seis[b] (may be NULL) receives wavefield b's samples
 // and blow[b] its blow-up step (INT_MAX: none).  Returns the device time of
 // the run [ms].
 template <int B>
 static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
-                        const axh_ctx *const *ctx, int32_t it0, int32_t it1,
+                        const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
                         float *const *seis, int *blow) {
   if (F.B != B) fail(AXH_ERR_STATE, "group width mismatch");
   const axh_dims &d = ctx[0]->dims;
@@ -2137,7 +2297,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
 
   for (int b = 0; b < B; b++) {
     Member &w = *mb[b];
-    const int nsamp = axh_seis_samples(ctx[b], it0, it1);
+    const int nsamp = axh_seis_samples(ctx[b], start[b], start[b] + nsteps - 1);
     if (nsamp > 0 && (size_t)nsamp * 3 * w.nrec > w.seis_cap) {
       w.seis_cap = (size_t)nsamp * 3 * w.nrec;
       w.seis_d = w.mem.alloc<float>(w.seis_cap);
@@ -2154,23 +2314,19 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   }
   SA.disp = F.disp; SA.force = F.force_s; SA.memvar = F.memvar; SA.rsum = F.rsum;
   SA.trm1 = F.trm1; SA.devm1 = F.devm1;
+  SA.flags = F.flags_s; SA.minv = M.minv_s; SA.velo = F.velo; SA.acc0 = F.acc0;
+  SA.disp_w = F.disp; SA.hdt = hdt;
   FluidArgs FA{};
   FA.nel = M.nel_f; FA.multipole = multipole; FA.G = M.G; FA.coef = M.coef_f;
   FA.m0 = M.m0_f; FA.einfo = M.einfo_f; FA.gid = M.gid_f; FA.chi = F.chi;
   FA.force = F.force_f;
-  CoupleArgs CA{};
-  CA.n = 5 * M.nbdry; CA.dipole = M.src == AXH_DIPOLE; CA.nslot_s = (int)nslot_s;
-  CA.nglob_s = M.nglob_s; CA.ss = M.cpl_ss; CA.gs = M.cpl_gs; CA.sf = M.cpl_sf;
-  CA.gf = M.cpl_gf; CA.bm = M.cpl_bm; CA.axb = M.cpl_axb; CA.disp = F.disp;
-  CA.ddchi = F.ddchi; CA.force_f = F.force_f; CA.force_s = F.force_s;
-  CA.axb_val = F.axb_val;
   // solid axis masks: monopole s; dipole +,z (2,3); quadrupole all
   const int mask0 = (M.src == AXH_DIPOLE) ? 1 : 0;
   const int mask1 = (M.src == AXH_MONOPOLE) ? 0 : 2;
   const int comps = multipole ? 7 : 5;
   PredArgs PS{};
   PS.n = M.nglob_s; PS.mask0 = mask0; PS.mask1 = mask1; PS.comps = comps;
-  PS.dt = dt; PS.hdt2 = hdt2; PS.flags = M.flags_s; PS.stride = M.nglob_s;
+  PS.dt = dt; PS.hdt2 = hdt2; PS.flags = F.flags_s; PS.stride = M.nglob_s;
   PS.naxb = M.naxb; PS.axb_g = M.axb_g; PS.u = F.disp; PS.v = F.velo; PS.a0 = F.acc0;
   PS.axb_val = F.axb_val;
   PredArgs PF{};
@@ -2180,13 +2336,20 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   AsmSolidArgs AS{};
   AS.n = M.nglob_s; AS.src = M.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
   AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = M.gstart_s;
-  AS.gslot = M.gslot_s; AS.flags = M.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
+  AS.npts = F.nasm; AS.pts = F.asm_pts;
+  AS.gslot = M.gslot_s; AS.flags = F.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
   AS.force = F.force_s; AS.velo = F.velo; AS.acc0 = F.acc0;
+  AS.disp = F.disp; AS.naxb = M.naxb; AS.axb_g = M.axb_g; AS.axb_val = F.axb_val;
+  AS.dipole = M.src == AXH_DIPOLE; AS.cpl_slot = M.cpl_slot_s; AS.cpl_gf = M.cpl_gf;
+  AS.cpl_bm = M.cpl_bm; AS.ddchi = F.ddchi;
   AsmFluidArgs AF{};
   AF.n = M.nglob_f; AF.newmark = newmark; AF.mask_axis = multipole;
   AF.mask_surface = newmark; AF.gstart = M.gstart_f; AF.gslot = M.gslot_f;
   AF.flags = M.flags_f; AF.minv = M.minv_f; AF.hdt = hdt; AF.force = F.force_f;
-  AF.dchi = F.dchi; AF.ddchi = F.ddchi;
+  AF.dchi = F.dchi; AF.ddchi = F.ddchi; AF.chi = F.chi;
+  AF.dipole = M.src == AXH_DIPOLE; AF.nglob_s = M.nglob_s; AF.cpl_slot = M.cpl_slot_f;
+  AF.cpl_gs = M.cpl_gs; AF.cpl_axb = M.cpl_axb; AF.cpl_bm = M.cpl_bm; AF.disp = F.disp;
+  AF.axb_val = F.axb_val;
   RecArgs RA{};
   RA.src = M.src; RA.nglob = M.nglob_s; RA.disp = F.disp;
   int maxrec = 0;
@@ -2202,69 +2365,62 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
 
   HIPCHECK(hipEventRecord(R.ev0, st_));
   int isamp[kGroup] = {0, 0, 0, 0};
-  for (int it = it0; it <= it1; it++) {
+  for (int step = 0; step < nsteps; step++) {
     for (int st = 0; st < M.nstages; st++) {
-      // predictor / drift
-      PS.newmark = PF.newmark = newmark;
-      PS.cd = PF.cd = newmark ? 0.0 : M.coefd[st];
-      PS.apply_mask = 1;
-      PF.apply_mask = multipole;
-      PS.check_blow = (st == 0);
-      PF.check_blow = 0;
-      PS.iter = PF.iter = it;
-      L(KC_PRED, [&] {
-        hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PS.n)), dim3(256), 0, st_, PS);
-        if (M.nel_f > 0) hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PF.n)), dim3(256), 0, st_, PF);
-      });
+      // predictor / first drift of the step; the symplectic drifts of the
+      // later stages and the final drift are fused into the kicks below
+      if (newmark || st == 0) {
+        PS.newmark = PF.newmark = newmark;
+        PS.cd = PF.cd = newmark ? 0.0 : M.coefd[st];
+        PS.apply_mask = 1;
+        PF.apply_mask = multipole;
+        PS.check_blow = 1;
+        PF.check_blow = 0;
+        for (int b = 0; b < B; b++) PS.iter[b] = PF.iter[b] = start[b] + step;
+        L(KC_PRED, [&] {
+          hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PS.n)), dim3(256), 0, st_, PS);
+          if (M.nel_f > 0)
+            hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PF.n)), dim3(256), 0, st_, PF);
+        });
+      }
+      // drift after the kick: coefd(st+1), masked except the final one
+      // (SOLVER/time_evol_wave.F90:597-605,720-725)
+      const bool last = st + 1 == M.nstages;
+      AS.drift = AF.drift = !newmark;
+      AS.cd_next = AF.cd_next = newmark ? 0.0 : M.coefd[st + 1];
+      AS.drift_mask = !last;
+      AF.drift_mask = !last && multipole;
       for (int b = 0; b < B; b++) {
-        const float stf = mb[b]->stf[(size_t)(it - 1) * M.nstages + st];
+        const float stf = mb[b]->stf[(size_t)(start[b] + step - 1) * M.nstages + st];
         FA.stf[b] = (newmark && mb[b]->fluid_src) ? stf : 0.0f;
         AS.stf[b] = (newmark && mb[b]->fluid_src) ? 0.0f : stf;
       }
-      if (M.nel_f > 0) {
+      if (M.nel_f > 0)
         L(KC_FLUID, [&] {
           hipLaunchKernelGGL(k_fluid<B>, dim3(M.ntile_f), dim3(256), 0, st_, FA);
         });
-        if (M.nbdry > 0)
-          L(KC_COUPLE_F, [&] {
-            hipLaunchKernelGGL(k_copy2fluid<B>, dim3(nblk(CA.n)), dim3(256), 0, st_, CA);
-          });
-      }
+      SA.direct = 1;
+      SA.cv = newmark ? 0.0 : M.coefv[st];
+      SA.cd_next = newmark ? 0.0 : M.coefd[st + 1];
       L(KC_SOLID, [&] { dispatch_solid<B>(M, st_, SA, newmark ? 0 : 1); });
+      SA.direct = 0;
       if (M.nel_f > 0) {
         AF.cv = newmark ? 0.0 : M.coefv[st];
         L(KC_ASM_F, [&] {
           hipLaunchKernelGGL(k_asm_fluid<B>, dim3(nblk(AF.n)), dim3(256), 0, st_, AF);
         });
-        if (M.nbdry > 0)
-          L(KC_COUPLE_S, [&] {
-            hipLaunchKernelGGL(k_copy2solid<B>, dim3(nblk(CA.n)), dim3(256), 0, st_, CA);
-          });
       }
       AS.cv = newmark ? 0.0 : M.coefv[st];
       L(KC_ASM_S, [&] {
-        hipLaunchKernelGGL(k_asm_solid<B>, dim3(nblk(AS.n)), dim3(256), 0, st_, AS);
+        hipLaunchKernelGGL(k_asm_solid<B>, dim3(nblk(AS.npts)), dim3(256), 0, st_, AS);
       });
     }
-    if (!newmark) {
-      // final drift with coefd(nstages+1), no mask (SOLVER/time_evol_wave.F90:720-725)
-      PredArgs PD = PS, PDF = PF;
-      PD.cd = PDF.cd = M.coefd[M.nstages];
-      PD.newmark = PDF.newmark = 0;
-      PD.apply_mask = PDF.apply_mask = 0;
-      PD.check_blow = PDF.check_blow = 0;
-      L(KC_DRIFT, [&] {
-        hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PD.n)), dim3(256), 0, st_, PD);
-        if (M.nel_f > 0)
-          hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PDF.n)), dim3(256), 0, st_, PDF);
-      });
-      if (M.anel) L(KC_MEMVAR, [&] { dispatch_memvar<B>(M, st_, SA); });
-    }
+    if (!newmark && M.anel) L(KC_MEMVAR, [&] { dispatch_memvar<B>(M, st_, SA); });
     bool any = false;
     for (int b = 0; b < B; b++) {
       Member &w = *mb[b];
       RA.out[b] = nullptr;
-      if (w.nrec > 0 && it % w.seis_it == 0) {
+      if (w.nrec > 0 && (start[b] + step) % w.seis_it == 0) {
         RA.out[b] = w.seis_d + (size_t)isamp[b] * 3 * w.nrec;
         isamp[b]++;
         any = true;
@@ -2302,13 +2458,13 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
 }
 
 static double run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
-                      const axh_ctx *const *ctx, int32_t it0, int32_t it1, float *const *seis,
-                      int *blow) {
+                      const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
+                      float *const *seis, int *blow) {
   switch (F.B) {
-    case 1: return run_group<1>(R, M, F, mb, ctx, it0, it1, seis, blow);
-    case 2: return run_group<2>(R, M, F, mb, ctx, it0, it1, seis, blow);
-    case 3: return run_group<3>(R, M, F, mb, ctx, it0, it1, seis, blow);
-    case 4: return run_group<4>(R, M, F, mb, ctx, it0, it1, seis, blow);
+    case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, blow);
+    case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, blow);
+    case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, blow);
+    case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, blow);
     default: fail(AXH_ERR_STATE, "group width out of range");
   }
 }
@@ -2322,13 +2478,14 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
   int blow = INT_MAX;
-  ctx->last_run_ms = run_any(V.run, *V.mesh, V.own, mb, cx, it0, it1, out, &blow);
+  const int32_t start[1] = {it0};
+  ctx->last_run_ms = run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, &blow);
   ctx->iter_done = it1;
   if (blow != INT_MAX)
     fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
 }
 
-void batch_run(axh_batch *bt, int32_t it0, int32_t it1, float *const *seis) {
+void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
   BatchState &S = *bt->state;
   S.run.profile = bt->profile;
   const MeshDev &M = *bt->members[0]->dev->mesh;
@@ -2339,6 +2496,7 @@ void batch_run(axh_batch *bt, int32_t it0, int32_t it1, float *const *seis) {
     Member *mb[kGroup];
     const axh_ctx *cx[kGroup];
     float *out[kGroup];
+    int32_t start[kGroup];
     int bl[kGroup];
     for (size_t b = 0; b < idx.size(); b++) {
       axh_ctx *c = bt->members[idx[b]];
@@ -2346,13 +2504,14 @@ void batch_run(axh_batch *bt, int32_t it0, int32_t it1, float *const *seis) {
       mb[b] = &c->dev->member;
       cx[b] = c;
       out[b] = seis ? seis[idx[b]] : nullptr;
+      start[b] = c->iter_done + 1;
     }
-    ms += run_any(S.run, M, *S.groups[g], mb, cx, it0, it1, out, bl);
+    ms += run_any(S.run, M, *S.groups[g], mb, cx, start, nsteps, out, bl);
     for (size_t b = 0; b < idx.size(); b++) blow[idx[b]] = bl[b];
   }
   bt->last_run_ms = ms;
   for (axh_ctx *c : bt->members) {
-    c->iter_done = it1;
+    c->iter_done += nsteps;
     c->last_run_ms = ms;
   }
   for (size_t b = 0; b < blow.size(); b++)
@@ -2460,5 +2619,67 @@ void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
     if (ddchi) gather_f(ddchi, F.ddchi, "ddchi");
   }
 }
+
+// ---------------------------------------------------------------------------
+// resident meshes and footprints
+// ---------------------------------------------------------------------------
+void footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes) {
+  const axh_dims &d = ctx->dims;
+  const size_t nel_s = d.nel_solid, nel_f = d.nel_fluid;
+  const size_t ns = (size_t)((nel_s + kElemPerTile - 1) / kElemPerTile) * kTile;
+  const size_t nf = (size_t)((nel_f + kElemPerTile - 1) / kElemPerTile) * kTile;
+  const size_t gs = d.nglob_solid, gf = d.nglob_fluid;
+  const size_t nc = d.src_type == AXH_MONOPOLE ? kNcMono : d.src_type == AXH_DIPOLE ? kNcDi
+                                                                                 : kNcQuad;
+  size_t nax = 0;
+  if (const Array *a = ctx->get("axis_solid"))
+    for (size_t e = 0; e < a->count(); e++) nax += a->as<int32_t>()[e] != 0;
+  size_t m = 105 * 4;
+  m += ns * nc * 4 + nel_s * 4 + nax * 50 * 4 + ns * 4 + (gs + 1) * 4 + nel_s * 25 * 4 + gs * 4;
+  if (nel_f) m += nf * 4 * 4 + nel_f * 4 + nel_f * 5 * 4 + nf * 4 + (gf + 1) * 4 + nel_f * 25 * 4 +
+                  gf * 4 + gf;
+  if (d.nel_bdry) m += (size_t)d.nel_bdry * 5 * (5 * 4 + 8) + kMaxAxisBdry * 4 + (ns + nf) * 4;
+  if (d.anel) m += nel_s * (kCgStage + kCgUpd) * 16 + 2 * nel_s * d.n_sls * 8;
+  size_t w = 3 * gs * 4 * 3 + 3 * ns * 4 + 3 * kMaxAxisBdry * 4 + gs + gs * 4;
+  if (nel_f) w += 3 * gf * 4 + nf * 4 + nel_f * 4 + 200 * 4;
+  if (d.anel) w += nel_s * (24 * d.n_sls + 24 + 4 + 24) * 4;
+  w += (size_t)std::max(d.nelsrc, 0) * 25 * 16 + (size_t)d.num_rec * 4 + 4;
+  *mesh_bytes = m;
+  *wave_bytes = w;
+}
+
+}  // namespace axh
+
+// a handle on a device mesh part
+struct axh_mesh {
+  std::shared_ptr<axh::MeshDev> mesh;
+  uint64_t fp[2];
+};
+
+namespace axh {
+
+axh_mesh *mesh_get(axh_ctx *ctx) {
+  uint64_t fp[2];
+  mesh_fingerprint(ctx, fp);
+  if (!ctx->dev) device_build(ctx);
+  std::unique_ptr<axh_mesh> m(new axh_mesh);
+  m->mesh = ctx->dev->mesh;
+  m->fp[0] = fp[0];
+  m->fp[1] = fp[1];
+  return m.release();
+}
+
+void mesh_bind(axh_ctx *ctx, const axh_mesh *m) {
+  if (ctx->dev) fail(AXH_ERR_STATE, "the context is already on the device");
+  uint64_t fp[2];
+  mesh_fingerprint(ctx, fp);
+  if (fp[0] != m->fp[0] || fp[1] != m->fp[1])
+    fail(AXH_ERR_ARG, "the context's mesh terms differ from the mesh's");
+  ctx->bound_mesh = m->mesh;
+}
+
+size_t mesh_bytes(const axh_mesh *m) { return m->mesh->mem.bytes; }
+
+void mesh_release(axh_mesh *m) { delete m; }
 
 }  // namespace axh

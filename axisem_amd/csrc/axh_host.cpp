@@ -111,6 +111,7 @@ void axh_ctx::put(const std::string &name, DType t, std::vector<uint64_t> dims_,
                   const void *src) {
   if (dev) fail(AXH_ERR_STATE, "uploads are closed after the first run: " + name);
   if (batch_refs > 0) fail(AXH_ERR_STATE, "uploads are closed while in a batch: " + name);
+  fp_valid = false;
   Array a;
   a.dtype = t;
   a.dims = std::move(dims_);
@@ -417,7 +418,7 @@ int axh_device_bytes(axh_ctx *ctx, size_t *bytes) {
 // leaves entries it never reads unset (e.g. M0_w_fl of non-axial elements,
 // SOLVER/def_precomp_terms.f90:2355-2455), so the comparison is on the built
 // layout, not on the raw uploads.
-static void check_same_mesh(const axh_ctx *a, const uint64_t fa[2], const axh_ctx *b, int idx) {
+static void check_same_mesh(const axh_ctx *a, const uint64_t fa[2], axh_ctx *b, int idx) {
   const std::string who = "batch member " + std::to_string(idx) + ": ";
   const axh_dims &x = a->dims, &y = b->dims;
   if (x.abi_version != y.abi_version || x.npol != y.npol || x.nel_solid != y.nel_solid ||
@@ -452,11 +453,7 @@ int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
     }
     uint64_t f0[2] = {0, 0};
     if (n > 1) mesh_fingerprint(members[0], f0);
-    for (int i = 1; i < n; i++) {
-      check_same_mesh(members[0], f0, members[i], i);
-      if (members[i]->iter_done != members[0]->iter_done)
-        fail(AXH_ERR_ARG, "batch members are at different time steps");
-    }
+    for (int i = 1; i < n; i++) check_same_mesh(members[0], f0, members[i], i);
     auto *b = new axh_batch;
     b->members.assign(members, members + n);
     for (auto *c : b->members) c->batch_refs++;
@@ -485,7 +482,19 @@ int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end, float *con
         fail(AXH_ERR_ARG, "step range outside 1..niter of a member");
     }
     batch_attach(batch);
-    batch_run(batch, it_begin, it_end, seis);
+    batch_run(batch, it_end - it_begin + 1, seis);
+  });
+}
+
+int axh_batch_step(axh_batch *batch, int32_t nsteps, float *const *seis) {
+  return guarded([&] {
+    check_batch(batch);
+    if (nsteps < 0) fail(AXH_ERR_ARG, "negative step count");
+    for (auto *c : batch->members)
+      if (c->iter_done + nsteps > c->dims.niter)
+        fail(AXH_ERR_ARG, "a member has fewer than nsteps steps left");
+    batch_attach(batch);
+    batch_run(batch, nsteps, seis);
   });
 }
 
@@ -522,6 +531,52 @@ int axh_batch_kernel_stats(const axh_batch *batch, int kclass, double *total_ms,
     if (!batch_kernel_stats(batch, kclass, total_ms, launches))
       fail(AXH_ERR_ARG, "unknown kernel class");
   });
+}
+
+// ---------------------------------------------------------------------------
+// resident meshes and footprints (request loops)
+// ---------------------------------------------------------------------------
+int axh_mesh_fingerprint(axh_ctx *ctx, uint64_t *fp) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!fp) fail(AXH_ERR_ARG, "null pointer");
+    mesh_fingerprint(ctx, fp);
+  });
+}
+
+int axh_footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!mesh_bytes || !wave_bytes) fail(AXH_ERR_ARG, "null pointer");
+    footprint(ctx, mesh_bytes, wave_bytes);
+  });
+}
+
+int axh_mesh_get(axh_ctx *ctx, axh_mesh **out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out) fail(AXH_ERR_ARG, "null pointer");
+    *out = mesh_get(ctx);
+  });
+}
+
+int axh_mesh_bind(axh_ctx *ctx, const axh_mesh *mesh) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!mesh) fail(AXH_ERR_ARG, "null mesh");
+    mesh_bind(ctx, mesh);
+  });
+}
+
+int axh_mesh_bytes(const axh_mesh *mesh, size_t *bytes) {
+  return guarded([&] {
+    if (!mesh || !bytes) fail(AXH_ERR_ARG, "null pointer");
+    *bytes = mesh_bytes(mesh);
+  });
+}
+
+int axh_mesh_release(axh_mesh *mesh) {
+  return guarded([&] { mesh_release(mesh); });
 }
 
 int axh_symplectic_coefficients(int32_t scheme, double deltat, double *coefd,

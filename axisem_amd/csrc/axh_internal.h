@@ -39,6 +39,7 @@ struct Device;  // device-side layout and buffers (axh_device.hip)
 struct DeviceDeleter {
   void operator()(Device *d) const;
 };
+struct MeshDev;     // device mesh part, shared (axh_device.hip)
 struct BatchState;  // device groups of a batch (axh_device.hip)
 struct BatchStateDeleter {
   void operator()(BatchState *b) const;
@@ -61,6 +62,11 @@ struct axh_ctx {
   double last_run_ms = 0.0;
   bool profile = false;
   int batch_refs = 0;     // batches holding this context
+  // device mesh part to use when the device part is built (axh_mesh_bind)
+  std::shared_ptr<axh::MeshDev> bound_mesh;
+  // cached fingerprint of the mesh part (cleared by every upload)
+  bool fp_valid = false;
+  uint64_t fp[2] = {0, 0};
 
   const axh::Array *get(const std::string &name) const {
     auto it = arrays.find(name);
@@ -108,8 +114,15 @@ bool kernel_stats(const Device *d, int k, double *ms, int64_t *n);
 void batch_attach(axh_batch *b);
 void batch_detach(axh_batch *b);
 bool in_running_batch(const axh_ctx *ctx);
-void mesh_fingerprint(const axh_ctx *ctx, uint64_t fp[2]);
-void batch_run(axh_batch *b, int32_t it0, int32_t it1, float *const *seis);
+void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]);
+// device bytes of the mesh part and of one wavefield (B = 1) without a GPU
+void footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes);
+axh_mesh *mesh_get(axh_ctx *ctx);
+void mesh_bind(axh_ctx *ctx, const axh_mesh *m);
+size_t mesh_bytes(const axh_mesh *m);
+void mesh_release(axh_mesh *m);
+// advance every member nsteps from its own last completed step
+void batch_run(axh_batch *b, int32_t nsteps, float *const *seis);
 bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n);
 int symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
                             double *cf);

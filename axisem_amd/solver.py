@@ -100,6 +100,13 @@ def library():
         L.axh_batch_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
         L.axh_batch_set_profiling.argtypes = [P, C.c_int]
         L.axh_batch_set_group_width.argtypes = [P, C.c_int32]
+        L.axh_batch_step.argtypes = [P, C.c_int32, C.POINTER(P)]
+        L.axh_mesh_fingerprint.argtypes = [P, P]
+        L.axh_footprint.argtypes = [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.axh_mesh_get.argtypes = [P, C.POINTER(P)]
+        L.axh_mesh_bind.argtypes = [P, P]
+        L.axh_mesh_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.axh_mesh_release.argtypes = [P]
         L.axh_batch_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_int64)]
         _lib = L
@@ -334,6 +341,24 @@ class Batch:
         if rc in (AXH_OK, AXH_ERR_BLOWUP):
             for s in self.solvers:
                 s.iter = it1
+        _check(rc)
+        return [a[:n] for a, n in outs]
+
+    def step(self, nsteps: int) -> list:
+        """Advance every member nsteps from its own current step (members may
+        be at different steps); returns one (nsamples, num_rec, 3) array per
+        member."""
+        outs, ptrs = [], []
+        for s in self.solvers:
+            n = library().axh_seis_samples(s._h, s.iter + 1, s.iter + nsteps)
+            a = np.zeros((max(n, 1), max(s.dims["num_rec"], 1), 3), np.float32)
+            outs.append((a, n))
+            ptrs.append(a.ctypes.data_as(C.c_void_p))
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        rc = library().axh_batch_step(self._h, nsteps, arr)
+        if rc in (AXH_OK, AXH_ERR_BLOWUP):
+            for s in self.solvers:
+                s.iter += nsteps
         _check(rc)
         return [a[:n] for a, n in outs]
 

@@ -255,6 +255,12 @@ int axh_batch_destroy(axh_batch *batch);
  * member b's samples in the layout of axh_run. */
 int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end,
                   float *const *seis);
+
+/* Advance every member nsteps from its own last completed step (members may
+ * be at different steps, e.g. requests admitted at different times); each
+ * must have nsteps steps left.  seis as for axh_batch_run, counted over each
+ * member's own steps (axh_seis_samples(member, done + 1, done + nsteps)). */
+int axh_batch_step(axh_batch *batch, int32_t nsteps, float *const *seis);
 int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms);
 
 /* Wavefields advanced by one kernel instance (1..AXH_MAX_GROUP, default
@@ -266,6 +272,34 @@ int axh_batch_set_group_width(axh_batch *batch, int32_t width);
 int axh_batch_set_profiling(axh_batch *batch, int on);
 int axh_batch_kernel_stats(const axh_batch *batch, int kclass,
                            double *total_ms, int64_t *launches);
+
+/* ---- resident meshes (request loops) --------------------------------------- */
+
+/* A mesh part on the device, shared by the contexts bound to it.  The
+ * reference prepares every run from scratch (SOLVER/main.f90:56-80); a request
+ * loop instead keeps meshes resident and binds new requests to them.  No
+ * counterpart in the reference. */
+typedef struct axh_mesh axh_mesh;
+
+/* 128-bit fingerprint of the mesh part ctx would build (every term the
+ * kernels read; not the per-source uploads).  Equal fingerprints: one device
+ * mesh serves both contexts.  No GPU. */
+int axh_mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]);
+
+/* Device bytes of ctx's mesh part and of one wavefield on it (fields,
+ * memory variables, element forces, source and receiver tables; not the
+ * seismogram buffer, which grows with the steps of a call).  No GPU. */
+int axh_footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes);
+
+/* A handle on ctx's device mesh part (building ctx's device part first).
+ * The mesh stays on the device while a handle or a context uses it. */
+int axh_mesh_get(axh_ctx *ctx, axh_mesh **out);
+
+/* Build ctx's device part on mesh instead of uploading its own mesh part;
+ * before ctx's first device use, with an equal fingerprint. */
+int axh_mesh_bind(axh_ctx *ctx, const axh_mesh *mesh);
+int axh_mesh_bytes(const axh_mesh *mesh, size_t *bytes);
+int axh_mesh_release(axh_mesh *mesh);
 
 /* ---- state files --------------------------------------------------------- */
 

@@ -115,3 +115,59 @@ def test_batch_member_blowup_is_reported(state_cache):
     finally:
         for s in solvers:
             s.close()
+
+
+def test_batch_members_at_different_steps(state_cache):
+    """Continuous batching: a member joins after others have advanced; each
+    member still reproduces its solo run (axh_batch_step)."""
+    names = cases.BATCHES["dipole_symplec4_anel"][:3]
+    paths = [state_cache(n)[1] for n in names]
+    print("staggered batch:", names, flush=True)
+    ref = [_solo(p, 150) for p in paths]
+    solvers = [Solver.from_state_file(p) for p in paths]
+    try:
+        got = [[solvers[0].run(30)], [solvers[1].run(70)], []]
+        with Batch(solvers) as b:          # members at steps 30, 70 and 0
+            for k, seis in enumerate(b.step(50)):
+                got[k].append(seis)
+        # finish alone: 150 steps each
+        for k, s in enumerate(solvers):
+            got[k].append(s.run(150 - s.iter))
+        for k in range(3):
+            np.testing.assert_array_equal(np.concatenate(got[k]), ref[k][0], err_msg=names[k])
+            _same_fields(solvers[k].fields(), ref[k][1], names[k])
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_mesh_handle_shares_the_device_mesh(state_cache):
+    """A request bound to a resident mesh (axh_mesh_bind) runs as if alone and
+    adds only its wavefield part to the device; the footprint estimate made
+    without a GPU matches the device allocation."""
+    import ctypes as C
+    from axisem_amd import solver as S
+    names = cases.BATCHES["mono_newmark_anel"][:2]
+    paths = [state_cache(n)[1] for n in names]
+    ref = _solo(paths[1], 80)
+    L = S.library()
+    a = Solver.from_state_file(paths[0])
+    b = Solver.from_state_file(paths[1])
+    try:
+        mb, wb = C.c_size_t(), C.c_size_t()
+        S._check(L.axh_footprint(a._h, C.byref(mb), C.byref(wb)))
+        mesh = C.c_void_p()
+        S._check(L.axh_mesh_get(a._h, C.byref(mesh)))
+        S._check(L.axh_mesh_bind(b._h, mesh))
+        dev_mesh = C.c_size_t()
+        S._check(L.axh_mesh_bytes(mesh, C.byref(dev_mesh)))
+        seis = b.run(80)
+        np.testing.assert_array_equal(seis, ref[0])
+        _same_fields(b.fields(), ref[1], "bound")
+        # estimate vs allocation (the estimate ignores a few small tables)
+        assert abs(dev_mesh.value - mb.value) <= 0.02 * dev_mesh.value
+        assert abs(a.device_bytes() - dev_mesh.value - wb.value) <= 0.05 * wb.value
+        S._check(L.axh_mesh_release(mesh))
+    finally:
+        a.close()
+        b.close()
