@@ -301,6 +301,43 @@ __device__ __forceinline__ void grad_cg4(const float *G, bool axial, const float
   gz = cu[0 * 4 + q] * m1 + cu[2 * 4 + q] * m2;
 }
 
+// Voigt strain E1..E6 at cg point q from a wavefield's displacement tile in
+// LDS (entry k at k*S).
+template <int SRC, int S>
+__device__ void strain_cg4(const float *G, bool axial, const float *cu, int q,
+                           const float *U1, const float *U2, const float *U3,
+                           float g[6]) {
+  const float *invs = cu + 4 * 4;
+  const int p = (cg_r(q) + 5 * cg_c(q)) * S;
+  float b1s, b1z, b2s, b2z;
+#pragma unroll
+  for (int c = 0; c < 6; c++) g[c] = 0.0f;
+  if (SRC == AXH_DIPOLE)
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] + U2[k * S]; }, b1s, b1z);
+  else
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S]; }, b1s, b1z);
+  grad_cg4(G, axial, cu, q, [&](int k) { return U3[k * S]; }, b2s, b2z);
+  g[0] = b1s;
+  g[2] = b2z;
+  g[4] = b1z + b2s;
+  if (SRC == AXH_MONOPOLE) {
+    g[1] = invs[q] * U1[p];
+  } else if (SRC == AXH_DIPOLE) {
+    g[1] = 2.0f * (invs[q] * U2[p]);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U1[k * S] - U2[k * S]; }, b1s, b1z);
+    float f3 = invs[q] * U3[p];
+    g[3] = -f3 - b1z;
+    g[5] = -g[1] - b1s;
+  } else {
+    g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
+    grad_cg4(G, axial, cu, q, [&](int k) { return U2[k * S]; }, b1s, b1z);
+    float f3 = invs[q] * U3[p];
+    float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
+    g[3] = -(2.0f * f3) - b1z;
+    g[5] = f6 - b1s;
+  }
+}
+
 // source terms of the memory-variable update at cg point q
 template <int SRC>
 __device__ __forceinline__ void memvar_sources(const float *cu, int q, const float g[6],
@@ -321,81 +358,18 @@ __device__ __forceinline__ void memvar_sources(const float *cu, int q, const flo
   }
 }
 
-// Voigt strain E1..E6 at the cg points (compute_strain_att_el_cg4 with the
-// gradients of SOLVER/pointwise_derivatives.f90:216-246), in two parts.
-// First the gradients, spread over the element's 25 lanes: item
-// w = (gi * 4 + q) * B + b is gradient gi of cg point q of wavefield b, gi over
-// (U1+U2 | U1, U3, U1-U2 | U2) (dipole | monopole, quadrupole); U1..U3 are the
-// element's displacement tiles in LDS (entry x of wavefield b at x*B + b).
-// gr[2 w + {0,1}] = (d/ds, d/dz).
-template <int SRC, int B>
-__device__ __forceinline__ void strain_grads(const float *G, bool axial, const float *cu, int k,
-                                             const float *U1, const float *U2, const float *U3,
-                                             float *gr) {
-  constexpr int NG = (SRC == AXH_MONOPOLE) ? 2 : 3;
-  for (int w = k; w < NG * 4 * B; w += 25) {
-    const int gi = w / (4 * B), r = w - gi * 4 * B, q = r / B, b = r - q * B;
-    const float *u1 = U1 + b, *u2 = U2 + b, *u3 = U3 + b;
-    float gs, gz;
-    if (gi == 0) {
-      if (SRC == AXH_DIPOLE)
-        grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B] + u2[x * B]; }, gs, gz);
-      else
-        grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B]; }, gs, gz);
-    } else if (gi == 1) {
-      grad_cg4(G, axial, cu, q, [&](int x) { return u3[x * B]; }, gs, gz);
-    } else if (SRC == AXH_DIPOLE) {
-      grad_cg4(G, axial, cu, q, [&](int x) { return u1[x * B] - u2[x * B]; }, gs, gz);
-    } else {
-      grad_cg4(G, axial, cu, q, [&](int x) { return u2[x * B]; }, gs, gz);
-    }
-    gr[2 * w] = gs;
-    gr[2 * w + 1] = gz;
-  }
-}
-
-// Then the strain at cg point q of wavefield b from those gradients.
-template <int SRC, int B>
-__device__ __forceinline__ void strain_from_grads(const float *cu, int q, int b, const float *gr,
-                                                  const float *U1, const float *U2,
-                                                  const float *U3, float g[6]) {
-  const float *invs = cu + 4 * 4;
-  const int p = (cg_r(q) + 5 * cg_c(q)) * B + b;
-  const float *ga = gr + 2 * ((0 * 4 + q) * B + b), *gb = gr + 2 * ((1 * 4 + q) * B + b);
-#pragma unroll
-  for (int c = 0; c < 6; c++) g[c] = 0.0f;
-  g[0] = ga[0];
-  g[2] = gb[1];
-  g[4] = ga[1] + gb[0];
-  if (SRC == AXH_MONOPOLE) {
-    g[1] = invs[q] * U1[p];
-  } else if (SRC == AXH_DIPOLE) {
-    const float *gc = gr + 2 * ((2 * 4 + q) * B + b);
-    g[1] = 2.0f * (invs[q] * U2[p]);
-    float f3 = invs[q] * U3[p];
-    g[3] = -f3 - gc[1];
-    g[5] = -g[1] - gc[0];
-  } else {
-    const float *gc = gr + 2 * ((2 * 4 + q) * B + b);
-    g[1] = invs[q] * (U1[p] - 2.0f * U2[p]);
-    float f3 = invs[q] * U3[p];
-    float f6 = invs[q] * (U2[p] - 2.0f * U1[p]);
-    g[3] = -(2.0f * f3) - gc[1];
-    g[5] = f6 - gc[0];
-  }
-}
-
 // Strain sources of the memory-variable update at cg point q of element e
 // for wavefield b of the group (lanes k < 4 of the element, one cg point
 // each).  The sources of the previous update go to old, the new ones to src
 // and to the t-1 store; src/old are [28][B] per element.
 template <int SRC, int B>
-__device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *gr, int b, int e,
-                                               int q, const float *U1, const float *U2,
-                                               const float *U3, float *src, float *old) {
+__device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *sG, bool axial,
+                                               int b, int e, int q, const float *U1,
+                                               const float *U2, const float *U3, float *src,
+                                               float *old) {
   const float *cu = A.cg_upd + (size_t)e * (kCgUpd * 4);
   float g[6], str, sdev[6];
-  strain_from_grads<SRC, B>(cu, q, b, gr, U1, U2, U3, g);
+  strain_cg4<SRC, B>(sG, axial, cu, q, U1 + b, U2 + b, U3 + b, g);
   memvar_sources<SRC>(cu, q, g, str, sdev);
   float *trm1 = A.trm1 + (size_t)e * 4 * B;
   float *devm1 = A.devm1 + (size_t)e * 24 * B;
@@ -495,7 +469,6 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
   __shared__ __align__(16) float sSa[ANEL ? kElemPerTile * 24 * B : 1];
   __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
-  __shared__ __align__(16) float sGr[MV ? kElemPerTile * 24 * B : 1];
   __shared__ __align__(16) float sG[105];
 
   const int t = threadIdx.x, tile = blockIdx.x;
@@ -829,14 +802,10 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
       // time_step_memvars_cg4 with the masked predicted displacement; lane
       // k: cg point k % 4 of wavefield k / 4
       float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
-      float *gr = &sGr[le * 24 * B];
       const float *V1 = U0, *V2 = (NCOMP == 2 ? U0 : U1), *V3 = (NCOMP == 2 ? U1 : U2);
-      if (valid)
-        strain_grads<SRC, B>(sG, axial, A.cg_upd + (size_t)e * (kCgUpd * 4), k, V1, V2, V3, gr);
-      __syncthreads();
       if (valid && k < 4 * B) {
         // lane k: cg point k % 4 of wavefield k / 4
-        memvar_prepare<SRC, B>(A, gr, k >> 2, e, k & 3, V1, V2, V3, src, old);
+        memvar_prepare<SRC, B>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
       }
       __syncthreads();
       if (valid)
@@ -846,7 +815,7 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
   }
 
   if (valid) {
-    if (A.direct && (A.flags[gp] & kFlagDirect)) {
+    if (B == 1 && A.direct && (A.flags[gp] & kFlagDirect)) {
       // the point's only copy: assembly (0 + f, as the gather sums), kick
       // and drift as in k_asm_solid
       const float mi = A.minv[gp];
@@ -905,7 +874,6 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   __shared__ __align__(16) float sG[105];
   __shared__ __align__(16) float sSrc[kElemPerTile * 28 * B];
   __shared__ __align__(16) float sOld[kElemPerTile * 28 * B];
-  __shared__ __align__(16) float sGr[kElemPerTile * 24 * B];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int e = tile * kElemPerTile + le;
@@ -929,14 +897,10 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   const bool axial = valid && A.einfo[e] >= 0;
   float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
   const float *U0 = sU + eb, *U1 = sU + 256 * B + eb, *U2 = sU + 2 * 256 * B + eb;
-  float *gr = &sGr[le * 24 * B];
   const float *V1 = U0, *V2 = (NCOMP == 2 ? U0 : U1), *V3 = (NCOMP == 2 ? U1 : U2);
-  if (valid)
-    strain_grads<SRC, B>(sG, axial, A.cg_upd + (size_t)e * (kCgUpd * 4), k, V1, V2, V3, gr);
-  __syncthreads();
   if (valid && k < 4 * B) {
     // lane k: cg point k % 4 of wavefield k / 4
-    memvar_prepare<SRC, B>(A, gr, k >> 2, e, k & 3, V1, V2, V3, src, old);
+    memvar_prepare<SRC, B>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
   }
   __syncthreads();
   if (valid) memvar_update_elem<SRC, B>(A, e, k, mv, src, old, mv);
@@ -1972,10 +1936,15 @@ static std::vector<std::pair<float *, size_t>> state_arrays(const MeshDev &M, Fi
 }
 
 // Point flags and assembly list of a group whose wavefields have their solid
-// source terms at the points srcs[...] (the mesh part stays unchanged).
+// source terms at the points srcs[...] (the mesh part stays unchanged).  Only
+// a single wavefield kicks its single-copy points in the stiffness kernel
+// (for wider groups that costs the stiffness kernel more than it saves the
+// assembly): groups of B > 1 assemble every point.
 static void fields_set_sources(const MeshDev &M, Fields &F,
                                const std::vector<const std::vector<int> *> &srcs) {
   std::vector<uint8_t> fl = M.flags_s_h;
+  if (F.B > 1)
+    for (auto &f : fl) f &= (uint8_t)~kFlagDirect;
   for (const auto *v : srcs)
     for (int g : *v) {
       fl[g] |= kFlagSource;

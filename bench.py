@@ -95,13 +95,40 @@ def prepare_states(workdir: str, nbatch: int) -> list:
         return list(ex.map(lambda j: prepare_state(*j), jobs))
 
 
-def solid_kernel_bytes(st, nwave: int = 1) -> int:
+def direct_points(st) -> int:
+    """Solid points the single-wavefield stiffness kernel kicks itself: one
+    element copy, not on the axis, not on the solid-fluid boundary, no
+    source term (the library's kFlagDirect)."""
+    import numpy as np
+    d = st.dims
+    ig = np.asarray(st["igloc_solid"], np.int64) - 1
+    nglob = int(ig.max()) + 1
+    single = np.bincount(ig, minlength=nglob) == 1
+    off = np.zeros(nglob, bool)
+    ax = np.asarray(st["axis_solid"]) != 0
+    k = np.arange(25)
+    off[ig.reshape(-1, 25)[ax][:, k % 5 == 0].ravel()] = True
+    for b in range(d["nel_bdry"]):
+        e, j = int(st["bdry_solid_el"][b]) - 1, int(st["bdry_jpol_solid"][b])
+        off[ig[e * 25 + 5 * j: e * 25 + 5 * j + 5]] = True
+    if not d["fluid_src"]:
+        term = np.asarray(st["source_term"], np.float32).reshape(3, 8, 25)
+        for kk, e in enumerate(np.asarray(st["ielsrc"])[: d["nelsrc"]]):
+            nz = np.any(term[:, kk, :] != 0, axis=0)
+            off[ig[(int(e) - 1) * 25 + np.nonzero(nz)[0]]] = True
+    return int(np.count_nonzero(single & ~off))
+
+
+def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
     """Algorithmic HBM bytes of one launch of the solid stiffness kernel in a
     symplectic stage advancing nwave wavefields: every precomputed term of
     every element-local point once, the element->global index, the axial
     index, the axial terms and the stage cg4 terms once (shared by the
     wavefields); per wavefield the displacement of every global point once,
-    the element-local force written once and the SLS sums."""
+    the element-local force written once and the SLS sums.  A single
+    wavefield (nwave 1) kicks its ndirect single-copy points itself: no force
+    for them, but their velocity read and written, their displacement
+    written, their inverse mass and the point flags read."""
     d = st.dims
     nel, nglob = d["nel_solid"], d["nglob_solid"]
     ncomp = 2 if d["src_type"] == 0 else 3
@@ -116,6 +143,10 @@ def solid_kernel_bytes(st, nwave: int = 1) -> int:
     if d["anel"]:
         mesh += nel * 5 * 4 * 4        # Y, V_{s,z}_{eta,xi} at the cg points
         wave += nel * 24 * 4           # SLS sums
+    if nwave == 1 and ndirect:
+        wave += nglob                                  # point flags
+        wave -= 4 * ncomp * ndirect                    # no force for them
+        wave += ndirect * (4 + 3 * 4 * ncomp)          # minv, velo r/w, disp w
     return mesh + nwave * wave
 
 
@@ -243,7 +274,8 @@ def main():
     solver.set_profiling(False)
     sk_ms, sk_n = stats["solid_stiffness"]
     sk_avg_s = sk_ms / sk_n * 1e-3
-    sk_bytes = solid_kernel_bytes(st)
+    ndirect = direct_points(st)
+    sk_bytes = solid_kernel_bytes(st, 1, ndirect)
     achieved = sk_bytes / sk_avg_s / 1e9
     kernels = kernel_table(stats)
     log("plain kernels: " + json.dumps(kernels))
@@ -296,13 +328,17 @@ def main():
         log("cpu baseline (reference program, ~%.0f s loop)" % a.cpu_seconds)
         cpu = cpu_baseline(a.cpu_seconds, d["deltat"])
 
-    traffic = None
+    traffic = traffic4 = None
     tfile = os.path.join(REPO, "profiles", "solid_stiffness_prem10s_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
+            tj = json.load(open(tfile))
+            traffic = tj.get("bytes_per_launch")
+            traffic4 = tj.get("batched_4", {}).get("bytes_per_launch")
         except ValueError:
             traffic = None
+    if batched is not None and batched["wavefields"] == 4:
+        batched["roofline"]["traffic"] = traffic4
 
     out = {
         "metric": "GLL-point updates/s per GPU, batched vs plain at \u22641e-4 misfit; p50/p99 latency",
