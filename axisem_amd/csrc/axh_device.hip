@@ -275,6 +275,36 @@ __device__ __forceinline__ float vmb(const float *v, const float *Bop, int j) {
   return s;
 }
 
+// The same products for the B wavefields of a field tile at once: each step
+// of the k sum reads the B values of one entry together (one LDS access) and
+// the operator entry once; every wavefield's sum runs over k in order.
+template <int B>
+__device__ __forceinline__ void amulv(const float *A, const float *M, int i, int j, float (&o)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; b++) o[b] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const float a = A[i + 5 * k];
+    float m[B];
+    ldv<B>(M + (k + 5 * j) * B, m);
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = o[b] + a * m[b];
+  }
+}
+template <int B>
+__device__ __forceinline__ void mulbv(const float *M, const float *Bop, int i, int j, float (&o)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; b++) o[b] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const float c = Bop[k + 5 * j];
+    float m[B];
+    ldv<B>(M + (i + 5 * k) * B, m);
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = o[b] + m[b] * c;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // coarse-grained strain and memory variables (SOLVER/attenuation.f90:81-202,
 // 472-537; SOLVER/pointwise_derivatives.f90:55-75,216-246)
@@ -503,12 +533,16 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
                 m11s = cf[5], m21s = cf[6], m41s = cf[7], m12s = cf[8], m22s = cf[9],
                 m32s = cf[10], m42s = cf[11], m11z = cf[12], m21z = cf[13], m41z = cf[14];
     float ls[B], lz[B], V1[B], V2[B], V3[B];
+    float vX1[B], vX2[B], vX3[B], vX4[B], vus[B];
+    amulv<B>(G.gT, U0, i, j, vX1);
+    amulv<B>(G.gT, U1, i, j, vX2);
+    mulbv<B>(U0, G.g2, i, j, vX3);
+    mulbv<B>(U1, G.g2, i, j, vX4);
+    ldv<B>(U0 + k * B, vus);
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const float *Us = U0 + b, *Uz = U1 + b;
-      const float us = Us[k * B], uz = Uz[k * B];
-      const float X1 = amul<B>(G.gT, Us, i, j), X2 = amul<B>(G.gT, Uz, i, j);
-      const float X3 = mulb<B>(Us, G.g2, i, j), X4 = mulb<B>(Uz, G.g2, i, j);
+      const float us = vus[b];
+      const float X1 = vX1[b], X2 = vX2[b], X3 = vX3[b], X4 = vX4[b];
       ls[b] = m4 * X4 + m2 * X3 + m1 * X1 + m3 * X2 + us * mw1;
       sS[0][t * B + b] = m11s * X3 + m21s * X1 + m12s * X4 + m22s * X2 + m1 * us;
       sS[1][t * B + b] = m11s * X1 + m41s * X3 + m32s * X2 + m42s * X4 + m2 * us;
@@ -522,13 +556,15 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
       const float *m0 = A.m0 + (size_t)ax * 50;
       w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
     }
+    float vY1[B], vY2[B], vY3[B], vY4[B];
+    mulbv<B>(&sS[1][eb], G.g2T, i, j, vY2);
+    mulbv<B>(&sS[3][eb], G.g2T, i, j, vY4);
+    amulv<B>(G.g, &sS[0][eb], i, j, vY1);
+    amulv<B>(G.g, &sS[2][eb], i, j, vY3);
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j),
-                  Y4 = mulb<B>(&sS[3][eb + b], G.g2T, i, j);
-      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[2][eb + b], i, j);
-      ls[b] = ls[b] + Y1 + Y2;
-      lz[b] = Y3 + Y4;
+      ls[b] = ls[b] + vY1[b] + vY2[b];
+      lz[b] = vY3[b] + vY4[b];
       if (axial) {
         V1[b] = vgm<B>(G.g0, U0 + b, j);        // vxm(G0, us)
         V2[b] = mulb<B>(U1 + b, G.g2, 0, j);    // vxm(uz(0,:), G2)
@@ -560,14 +596,19 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
                 m33s = cf[19], m43s = cf[20], m11z = cf[21], m21z = cf[22],
                 m41z = cf[23];
     float ls2[B], ls3[B];
+    float vX1[B], vX2[B], vX3[B], vX4[B], vX5[B], vX6[B], vu2[B], vu3[B];
+    mulbv<B>(U0, G.g2, i, j, vX4);
+    mulbv<B>(U1, G.g2, i, j, vX5);
+    mulbv<B>(U2, G.g2, i, j, vX6);
+    amulv<B>(G.gT, U0, i, j, vX1);
+    amulv<B>(G.gT, U1, i, j, vX2);
+    amulv<B>(G.gT, U2, i, j, vX3);
+    ldv<B>(U1 + k * B, vu2);
+    ldv<B>(U2 + k * B, vu3);
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const float *Ua = U0 + b, *Ub = U1 + b, *Uc = U2 + b;
-      const float u2 = Ub[k * B], u3 = Uc[k * B];
-      const float X4 = mulb<B>(Ua, G.g2, i, j), X5 = mulb<B>(Ub, G.g2, i, j),
-                  X6 = mulb<B>(Uc, G.g2, i, j);
-      const float X1 = amul<B>(G.gT, Ua, i, j), X2 = amul<B>(G.gT, Ub, i, j),
-                  X3 = amul<B>(G.gT, Uc, i, j);
+      const float u2 = vu2[b], u3 = vu3[b];
+      const float X1 = vX1[b], X2 = vX2[b], X3 = vX3[b], X4 = vX4[b], X5 = vX5[b], X6 = vX6[b];
       const float X7 = X1 + X2, X8 = X4 + X5;
       ls2[b] = m8 * X6 + m7 * X3 + m1 * X1 + m5 * X2 + m2 * X4 + m6 * X5 + mw1 * u2 +
                mw2 * u3;
@@ -592,16 +633,23 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
       w10 = m0[9 * 5 + j];
     }
     float va[B], vb[B], vc[B];
+    {
+      float Y[6][B];
+      amulv<B>(G.g, &sS[0][eb], i, j, Y[0]);
+      mulbv<B>(&sS[2][eb], G.g2T, i, j, Y[1]);
+      amulv<B>(G.g, &sS[1][eb], i, j, Y[2]);
+      mulbv<B>(&sS[3][eb], G.g2T, i, j, Y[3]);
+      amulv<B>(G.g, &sS[4][eb], i, j, Y[4]);
+      mulbv<B>(&sS[5][eb], G.g2T, i, j, Y[5]);
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        acc[0][b] = Y[0][b] + Y[1][b];
+        acc[1][b] = Y[2][b] + Y[3][b] + ls2[b];
+        acc[2][b] = Y[4][b] + Y[5][b] + ls3[b];
+      }
+    }
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[1][eb + b], i, j),
-                  Y5 = amul<B>(G.g, &sS[4][eb + b], i, j);
-      const float Y2 = mulb<B>(&sS[2][eb + b], G.g2T, i, j),
-                  Y4 = mulb<B>(&sS[3][eb + b], G.g2T, i, j),
-                  Y6 = mulb<B>(&sS[5][eb + b], G.g2T, i, j);
-      acc[0][b] = Y1 + Y2;
-      acc[1][b] = Y3 + Y4 + ls2[b];
-      acc[2][b] = Y5 + Y6 + ls3[b];
       if (axial) {
         const float V1 = vgm<B>(G.g0, U0 + b, j);
         const float V2 = vgm<B>(G.g0, U1 + b, j);
@@ -633,14 +681,20 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
                 m41s = cf[15], m12s = cf[16], m22s = cf[17], m32s = cf[18],
                 m42s = cf[19], m11z = cf[20], m21z = cf[21], m41z = cf[22],
                 m1phi = cf[23], m2phi = cf[24], m4phi = cf[25];
+    float vX1[B], vX2[B], vX3[B], vX4[B], vX5[B], vX6[B], vus[B], vup[B], vuz[B];
+    amulv<B>(G.gT, U0, i, j, vX1);
+    amulv<B>(G.gT, U1, i, j, vX2);
+    amulv<B>(G.gT, U2, i, j, vX3);
+    mulbv<B>(U0, G.g2, i, j, vX4);
+    mulbv<B>(U1, G.g2, i, j, vX5);
+    mulbv<B>(U2, G.g2, i, j, vX6);
+    ldv<B>(U0 + k * B, vus);
+    ldv<B>(U1 + k * B, vup);
+    ldv<B>(U2 + k * B, vuz);
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const float *Ua = U0 + b, *Ub = U1 + b, *Uc = U2 + b;
-      const float us = Ua[k * B], up = Ub[k * B], uz = Uc[k * B];
-      const float X1 = amul<B>(G.gT, Ua, i, j), X2 = amul<B>(G.gT, Ub, i, j),
-                  X3 = amul<B>(G.gT, Uc, i, j);
-      const float X4 = mulb<B>(Ua, G.g2, i, j), X5 = mulb<B>(Ub, G.g2, i, j),
-                  X6 = mulb<B>(Uc, G.g2, i, j);
+      const float us = vus[b], up = vup[b], uz = vuz[b];
+      const float X1 = vX1[b], X2 = vX2[b], X3 = vX3[b], X4 = vX4[b], X5 = vX5[b], X6 = vX6[b];
       const float c1 = m2 * X4, c2 = m1 * X1, c3 = m6 * X5, c4 = m5 * X2,
                   c5 = m4 * X6, c6 = m3 * X3;
       acc[0][b] = c1 + c2 + 2.0f * (c3 + c4) + c5 + c6 + mw1 * us + mw2 * up +
@@ -657,16 +711,20 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
       sS[5][t * B + b] = m1phi * X2 + m4phi * X5 + m6 * tus + 2.0f * m8 * uz;        // S2phi
     }
     __syncthreads();
+    {
+      float Y[6][B];
+      mulbv<B>(&sS[1][eb], G.g2T, i, j, Y[1]);
+      mulbv<B>(&sS[5][eb], G.g2T, i, j, Y[3]);
+      mulbv<B>(&sS[3][eb], G.g2T, i, j, Y[5]);
+      amulv<B>(G.g, &sS[0][eb], i, j, Y[0]);
+      amulv<B>(G.g, &sS[4][eb], i, j, Y[2]);
+      amulv<B>(G.g, &sS[2][eb], i, j, Y[4]);
 #pragma unroll
-    for (int b = 0; b < B; b++) {
-      const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j),
-                  Y4 = mulb<B>(&sS[5][eb + b], G.g2T, i, j),
-                  Y6 = mulb<B>(&sS[3][eb + b], G.g2T, i, j);
-      const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j), Y3 = amul<B>(G.g, &sS[4][eb + b], i, j),
-                  Y5 = amul<B>(G.g, &sS[2][eb + b], i, j);
-      acc[0][b] = acc[0][b] + Y1 + Y2;
-      acc[1][b] = acc[1][b] + Y3 + Y4;
-      acc[2][b] = acc[2][b] + Y5 + Y6;
+      for (int b = 0; b < B; b++) {
+        acc[0][b] = acc[0][b] + Y[0][b] + Y[1][b];
+        acc[1][b] = acc[1][b] + Y[2][b] + Y[3][b];
+        acc[2][b] = acc[2][b] + Y[4][b] + Y[5][b];
+      }
     }
     __syncthreads();  // uniform barrier count with the other source types
     if (axial) {
@@ -742,33 +800,30 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
     __syncthreads();
     if (valid) {
       // mxm_cg4_sparse_b(G, S1) has columns 1,3; mxm_cg4_sparse_a(S2, G2T)
-      // rows 1,3 (SOLVER/unrolled_loops.f90:76-121)
+      // rows 1,3 (SOLVER/unrolled_loops.f90:76-121).  The lane's entries are
+      // selected once: lanes outside the sparse pattern take an exact 0.
       const bool cgp = (i == 1 || i == 3) && (j == 1 || j == 3);
       const int q = cgp ? ((i == 1 ? 0 : 2) + (j == 1 ? 0 : 1)) : 0;
       const float yl = cs[0 * 4 + q];
+      const bool inb = (j == 1 || j == 3), ina = (i == 1 || i == 3);
+      const int qb0 = (j == 3) ? 1 : 0, qa0 = (i == 3) ? 2 : 0;
+      const float gb0 = G.g[i + 5], gb1 = G.g[i + 15];
+      const float ga0 = G.g2T[1 + 5 * j], ga1 = G.g2T[3 + 5 * j];
 #pragma unroll
       for (int b = 0; b < B; b++) {
         auto spb = [&](int s1) -> float {
-          if (j == 1)
-            return G.g[i + 5] * Sa[(s1 * 4 + 0) * B + b] + G.g[i + 15] * Sa[(s1 * 4 + 2) * B + b];
-          if (j == 3)
-            return G.g[i + 5] * Sa[(s1 * 4 + 1) * B + b] + G.g[i + 15] * Sa[(s1 * 4 + 3) * B + b];
-          return 0.0f;
+          const float v = gb0 * Sa[(s1 * 4 + qb0) * B + b] + gb1 * Sa[(s1 * 4 + qb0 + 2) * B + b];
+          return inb ? v : 0.0f;
         };
         auto spa = [&](int s2) -> float {
-          if (i == 1)
-            return Sa[(s2 * 4 + 0) * B + b] * G.g2T[1 + 5 * j] +
-                   Sa[(s2 * 4 + 1) * B + b] * G.g2T[3 + 5 * j];
-          if (i == 3)
-            return Sa[(s2 * 4 + 2) * B + b] * G.g2T[1 + 5 * j] +
-                   Sa[(s2 * 4 + 3) * B + b] * G.g2T[3 + 5 * j];
-          return 0.0f;
+          const float v = Sa[(s2 * 4 + qa0) * B + b] * ga0 + Sa[(s2 * 4 + qa0 + 1) * B + b] * ga1;
+          return ina ? v : 0.0f;
         };
         const float r2 = R[(1 * 4 + q) * B + b], r4 = R[(3 * 4 + q) * B + b],
                     r6 = R[(5 * 4 + q) * B + b];
         if (SRC == AXH_MONOPOLE) {
           float l1 = spb(0) + spa(1);
-          if (cgp) l1 = l1 + yl * r2;
+          l1 = cgp ? l1 + yl * r2 : l1;
           const float l3 = spb(4) + spa(5);
           acc[0][b] = acc[0][b] - l1;
           acc[2][b] = acc[2][b] - l3;
@@ -776,10 +831,8 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
           const float lp = spb(0) + spa(1);
           float lm = spb(2) + spa(3);
           float lz = spb(4) + spa(5);
-          if (cgp) {
-            lm = lm + 2.0f * yl * (r2 - r6);
-            lz = lz - yl * r4;
-          }
+          lm = cgp ? lm + 2.0f * yl * (r2 - r6) : lm;
+          lz = cgp ? lz - yl * r4 : lz;
           acc[0][b] = acc[0][b] - lp;
           acc[1][b] = acc[1][b] - lm;
           acc[2][b] = acc[2][b] - lz;
@@ -787,11 +840,9 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
           float ls = spb(0) + spa(1);
           float lp = -spb(2) - spa(3);
           float lz = spb(4) + spa(5);
-          if (cgp) {
-            ls = ls + yl * (r2 - 2.0f * r6);
-            lp = lp + yl * (r6 - 2.0f * r2);
-            lz = lz - 2.0f * yl * r4;
-          }
+          ls = cgp ? ls + yl * (r2 - 2.0f * r6) : ls;
+          lp = cgp ? lp + yl * (r6 - 2.0f * r2) : lp;
+          lz = cgp ? lz - 2.0f * yl * r4 : lz;
           acc[0][b] = acc[0][b] - ls;
           acc[1][b] = acc[1][b] - lp;
           acc[2][b] = acc[2][b] - lz;
@@ -954,21 +1005,26 @@ __global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
   __syncthreads();
   GRegs G;
   load_g(sG, axial, i, j, G);
+  {
+    float X1[B], X2[B], s0[B], s1[B];
+    amulv<B>(G.gT, sC + eb, i, j, X1);
+    mulbv<B>(sC + eb, G.g2, i, j, X2);
 #pragma unroll
-  for (int b = 0; b < B; b++) {
-    const float *C = sC + eb + b;
-    const float X1 = amul<B>(G.gT, C, i, j), X2 = mulb<B>(C, G.g2, i, j);
-    sS[0][t * B + b] = m1 * X2 + m2 * X1;
-    sS[1][t * B + b] = m1 * X1 + m4 * X2;
+    for (int b = 0; b < B; b++) {
+      s0[b] = m1 * X2[b] + m2 * X1[b];
+      s1[b] = m1 * X1[b] + m4 * X2[b];
+    }
+    stv<B>(&sS[0][t * B], s0);
+    stv<B>(&sS[1][t * B], s1);
   }
   __syncthreads();
-  float out[B];
+  float out[B], Y1[B], Y2[B];
+  amulv<B>(G.g, &sS[0][eb], i, j, Y1);
+  mulbv<B>(&sS[1][eb], G.g2T, i, j, Y2);
 #pragma unroll
   for (int b = 0; b < B; b++) {
     const float *C = sC + eb + b;
-    const float Y1 = amul<B>(G.g, &sS[0][eb + b], i, j);
-    const float Y2 = mulb<B>(&sS[1][eb + b], G.g2T, i, j);
-    float loc = Y1 + Y2;
+    float loc = Y1[b] + Y2[b];
     if (A.multipole) {
       loc = loc + mw * C[k * B];
       if (axial) {

@@ -59,23 +59,24 @@ def member_config(depth_km: float) -> inputs.RunConfig:
     return inputs.RunConfig(**{**WORKLOAD.__dict__, "source_depth_km": depth_km})
 
 
-def prepare_state(workdir: str, cfg: inputs.RunConfig = WORKLOAD) -> str:
-    """Record (once) the uploads of the drop-in host for one source of the
-    workload."""
-    if not os.path.exists(os.path.join(MESH, "meshdb.dat0000")):
+def prepare_state(workdir: str, cfg: inputs.RunConfig = WORKLOAD, mesh: str = MESH) -> str:
+    """Record (once) the uploads of the drop-in host for one source on a
+    mesh."""
+    if not os.path.exists(os.path.join(mesh, "meshdb.dat0000")):
         raise SystemExit("missing %s: run __graft_entry__.build() where the reference "
-                         "tree exists" % MESH)
+                         "tree exists" % mesh)
     if not os.path.exists(HOST):
         raise SystemExit("missing drop-in host %s" % HOST)
     os.makedirs(workdir, exist_ok=True)
     rundir = os.path.join(workdir, "run_gpu")
     state = os.path.join(workdir, "state.bin")
     stamp = os.path.join(workdir, "stamp.json")
-    key = json.dumps({"cfg": repr(cfg), "mesh": os.path.getsize(os.path.join(MESH, "meshdb.dat0000")),
+    key = json.dumps({"cfg": repr(cfg), "mesh": mesh,
+                      "meshsize": os.path.getsize(os.path.join(mesh, "meshdb.dat0000")),
                       "host": os.path.getmtime(HOST)})
     if os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key:
         return state
-    inputs.write_run_dir(rundir, MESH, cfg)
+    inputs.write_run_dir(rundir, mesh, cfg)
     t = time.time()
     inputs.record_state(HOST, rundir, state)
     log("recorded the state of the %.0f km source in %.1fs" % (cfg.source_depth_km,
@@ -117,6 +118,58 @@ def direct_points(st) -> int:
             nz = np.any(term[:, kk, :] != 0, axis=0)
             off[ig[(int(e) - 1) * 25 + np.nonzero(nz)[0]]] = True
     return int(np.count_nonzero(single & ~off))
+
+
+# serving stream: meshes, source kinds, depths and durations of the requests
+SERVE_MESHES = {"prem_iso_50s": 50.0, "prem_iso_20s": 20.0, "prem_iso_10s": 10.0}
+SERVE_KINDS = ("vertforce", "thetaforce", "explosion", "mt")
+MT_RUNS = ("mrr", "mtt_p_mpp", "mtr", "mtp")
+
+
+def serving_stream(nsolves: int, rate_per_s: float, seed: int = 0) -> list:
+    """Solves of the serving workload with Poisson arrivals (seed 0): mesh
+    uniform over SERVE_MESHES, source uniform over SERVE_KINDS, depth U(0,700)
+    km, duration U(600,3600) s, elastic FP32 newmark2, 181 receivers.  A
+    moment tensor with entries |N(0,1)| x 1e20 N m becomes its 4 runs (one
+    request each, same arrival).  Returns (arrival_s, mesh name, RunConfig)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for _ in range(nsolves):
+        t += rng.exponential(1.0 / rate_per_s)
+        mesh = list(SERVE_MESHES)[rng.integers(len(SERVE_MESHES))]
+        kind = SERVE_KINDS[rng.integers(len(SERVE_KINDS))]
+        depth = float(rng.uniform(0.0, 700.0))
+        dur = float(rng.uniform(600.0, 3600.0))
+        runs = ([(r, float(abs(rng.standard_normal())) * 1e20) for r in MT_RUNS]
+                if kind == "mt" else [(kind, 1e20)])
+        for src, amp in runs:
+            out.append((t, mesh, inputs.RunConfig(
+                source_type=src, source_depth_km=round(depth, 1), source_amplitude=amp,
+                time_scheme="newmark2", attenuation=False, seismogram_length_s=round(dur, 1),
+                stations_deg=[float(k) for k in range(181)])))
+    return out
+
+
+def serving_leg(workdir: str, nsolves: int, rate: float, chunk: int) -> dict:
+    """The request loop on the serving stream: states recorded first (in
+    parallel), then the requests submitted with their arrival times."""
+    from axisem_amd.serve import RequestLoop
+    stream = serving_stream(nsolves, rate)
+    jobs = [(os.path.join(workdir, "req%02d" % i), cfg, os.path.join(REPO, "data", "meshes", m))
+            for i, (_, m, cfg) in enumerate(stream)]
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        paths = list(ex.map(lambda j: prepare_state(*j), jobs))
+    loop = RequestLoop(hbm_budget_bytes=200e9, chunk_steps=chunk)
+    for (arr, _, _), p in zip(stream, paths):
+        loop.submit(p, arrival_s=arr)
+    st = loop.run()
+    st["solves"] = nsolves
+    st["arrival_rate_per_s"] = rate
+    st["chunk_steps"] = chunk
+    st["mix"] = ["%s %s %.0fkm %.0fs" % (m, c.source_type, c.source_depth_km, c.seismogram_length_s)
+                 for _, m, c in stream]
+    return st
 
 
 def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
@@ -226,6 +279,10 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--batch", type=int, default=len(BATCH_DEPTHS_KM),
                     help="wavefields of the batched leg (1 = plain only)")
+    ap.add_argument("--serve-solves", type=int, default=6,
+                    help="solves of the serving leg (0: no serving leg)")
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="arrivals per second")
+    ap.add_argument("--serve-chunk", type=int, default=200, help="steps per batch chunk")
     ap.add_argument("--group-width", type=int, default=None,
                     help="wavefields per kernel instance in the batched leg (default: library)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -323,6 +380,13 @@ def main():
         }
         log("batched kernels: " + json.dumps(batched["kernels_us"]))
 
+    serving = None
+    if a.serve_solves > 0:
+        log("serving: %d solves, %.2f arrivals/s" % (a.serve_solves, a.serve_rate))
+        serving = serving_leg(os.path.join(REPO, "data", "bench_serve"), a.serve_solves,
+                              a.serve_rate, a.serve_chunk)
+        log("serving: " + json.dumps({k: v for k, v in serving.items() if k != "mix"}))
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (reference program, ~%.0f s loop)" % a.cpu_seconds)
@@ -367,6 +431,7 @@ def main():
         "latency": latency,
         "kernels_us": kernels,
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
