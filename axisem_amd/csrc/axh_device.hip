@@ -72,12 +72,13 @@ constexpr uint8_t kFlagDirect = 64;   // one element copy and nothing else to ad
 constexpr int kMaxAxisBdry = 16;
 
 // kernel classes for timing
-// (the solid-fluid coupling and the symplectic drifts after the first are
-// part of the assemble_update kernels)
-enum KClass { KC_PRED = 0, KC_FLUID, KC_SOLID, KC_ASM_F, KC_ASM_S, KC_MEMVAR, KC_REC, KC_N };
+// (the fluid stiffness is part of the stiffness launch; the solid-fluid
+// coupling and the symplectic drifts after the first are part of the
+// assemble_update kernels)
+enum KClass { KC_PRED = 0, KC_SOLID, KC_ASM_F, KC_ASM_S, KC_MEMVAR, KC_REC, KC_N };
 static const char *const kClassName[KC_N] = {
-    "predict", "fluid_stiffness", "solid_stiffness", "fluid_assemble_update",
-    "solid_assemble_update", "memvar_update", "receivers"};
+    "predict", "stiffness", "fluid_assemble_update", "solid_assemble_update",
+    "memvar_update", "receivers"};
 
 // Device allocations owned by one object and freed with it.
 struct DevAlloc {
@@ -176,7 +177,7 @@ __device__ __forceinline__ void stv(float *p, const float (&o)[B]) {
 // per-wavefield source / receiver data of its members
 // ---------------------------------------------------------------------------
 struct SolidArgs {
-  int nel, nsls, nglob, nslot;
+  int nel, nsls, nglob, nslot, ntile;
   const float *G, *coef, *m0;
   const int *einfo, *gid;
   const float *cg_stage;  // [nel][5][4]
@@ -474,6 +475,83 @@ __device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, int 
 }
 
 // ---------------------------------------------------------------------------
+// fluid stiffness glob_fluid_stiffness_4 (SOLVER/stiffness_fluid.f90:139-216)
+// + add_source_fl (SOLVER/time_evol_wave.F90:1062-1076) of one fluid tile; run
+// by the workgroups of the solid stiffness launch past the solid tiles
+// ---------------------------------------------------------------------------
+// LDS (the stiffness kernel's own arrays): sC 256*B, sS0/sS1 256*B each,
+// sG 105 floats, 16-byte aligned.
+template <int B>
+__device__ __forceinline__ void fluid_tile(const FluidArgs &A, int tile, float *sC, float *sS0,
+                                           float *sS1, float *sG) {
+  const int t = threadIdx.x;
+  const int le = t / 25, k = t - 25 * le;
+  const int i = k % 5, j = k / 5;
+  const int e = tile * kElemPerTile + le;
+  const bool valid = (t < 250) && (e < A.nel);
+  const size_t slot = (size_t)tile * 256 + t;
+  const int ax = valid ? A.einfo[e] : -1;
+  const bool axial = ax >= 0;
+  const int eb = le * 25 * B;
+  stage_g(A.G, sG);
+  const int nc = A.multipole ? kNcFluidMulti : kNcFluidMono;
+  const float *coef = A.coef + (size_t)tile * nc * 256 + t;
+  const float m1 = coef[0], m2 = coef[256], m4 = coef[512];
+  const float mw = A.multipole ? coef[768] : 0.0f;
+  const float w = (A.multipole && axial) ? A.m0[(size_t)ax * 5 + j] : 0.0f;
+  {
+    float c[B];
+    if (valid) {
+      ldv<B>(A.chi + (size_t)A.gid[slot] * B, c);
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; b++) c[b] = 0.0f;
+    }
+    stv<B>(sC + t * B, c);
+  }
+  __syncthreads();
+  GRegs G;
+  load_g(sG, axial, i, j, G);
+  {
+    float X1[B], X2[B], s0[B], s1[B];
+    amulv<B>(G.gT, sC + eb, i, j, X1);
+    mulbv<B>(sC + eb, G.g2, i, j, X2);
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      s0[b] = m1 * X2[b] + m2 * X1[b];
+      s1[b] = m1 * X1[b] + m4 * X2[b];
+    }
+    stv<B>(&sS0[t * B], s0);
+    stv<B>(&sS1[t * B], s1);
+  }
+  __syncthreads();
+  float out[B], Y1[B], Y2[B];
+  amulv<B>(G.g, &sS0[eb], i, j, Y1);
+  mulbv<B>(&sS1[eb], G.g2T, i, j, Y2);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const float *C = sC + eb + b;
+    float loc = Y1[b] + Y2[b];
+    if (A.multipole) {
+      loc = loc + mw * C[k * B];
+      if (axial) {
+        const float V1 = vgm<B>(G.g0, C, j);
+        loc = loc + G.g0i * (w * V1);
+      }
+    }
+    if (valid) {
+      const float stf = A.stf[b];
+      if (stf != 0.0f) {
+        const int ks = A.srcel[b][e];
+        if (ks >= 0) loc = loc - A.src_fl[b][ks * 25 + k] * stf;
+      }
+    }
+    out[b] = loc;
+  }
+  if (valid) stv<B>(A.force + slot * B, out);
+}
+
+// ---------------------------------------------------------------------------
 // solid stiffness: glob_stiffness_{mono,di,quad}_4 (SOLVER/stiffness_mono.f90:
 // 60-157, stiffness_di.f90:60-256, stiffness_quad.f90:238-412) followed by
 // glob_anel_stiffness_*_cg4 and, for newmark2, time_step_memvars_cg4.
@@ -486,7 +564,8 @@ __device__ __forceinline__ void gather_disp(const SolidArgs &A, bool valid, int 
 // register budget without spilling; wider groups take what their registers
 // allow)
 template <int SRC, int MODE, bool ANEL, int B>
-__global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(SolidArgs A) {
+__global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2)))
+void k_solid(SolidArgs A, FluidArgs FA) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   constexpr int NC = (SRC == AXH_MONOPOLE) ? kNcMono
                      : (SRC == AXH_DIPOLE) ? kNcDi : kNcQuad;
@@ -501,6 +580,10 @@ __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2))) void k_solid(
   __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sG[105];
 
+  if ((int)blockIdx.x >= A.ntile) {  // a fluid tile, in the same LDS
+    fluid_tile<B>(FA, blockIdx.x - A.ntile, sU, sS[0], sS[1], sG);
+    return;
+  }
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int i = k % 5, j = k / 5;
@@ -966,82 +1049,6 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
     }
     stv<B>(A.rsum + ((size_t)e * 24 + k) * B, r);
   }
-}
-
-// ---------------------------------------------------------------------------
-// fluid stiffness glob_fluid_stiffness_4 (SOLVER/stiffness_fluid.f90:139-216)
-// + add_source_fl (SOLVER/time_evol_wave.F90:1062-1076)
-// ---------------------------------------------------------------------------
-template <int B>
-__global__ __launch_bounds__(256) void k_fluid(FluidArgs A) {
-  __shared__ __align__(16) float sC[256 * B];
-  __shared__ __align__(16) float sS[2][256 * B];
-  __shared__ __align__(16) float sG[105];
-  const int t = threadIdx.x, tile = blockIdx.x;
-  const int le = t / 25, k = t - 25 * le;
-  const int i = k % 5, j = k / 5;
-  const int e = tile * kElemPerTile + le;
-  const bool valid = (t < 250) && (e < A.nel);
-  const size_t slot = (size_t)tile * 256 + t;
-  const int ax = valid ? A.einfo[e] : -1;
-  const bool axial = ax >= 0;
-  const int eb = le * 25 * B;
-  stage_g(A.G, sG);
-  const int nc = A.multipole ? kNcFluidMulti : kNcFluidMono;
-  const float *coef = A.coef + (size_t)tile * nc * 256 + t;
-  const float m1 = coef[0], m2 = coef[256], m4 = coef[512];
-  const float mw = A.multipole ? coef[768] : 0.0f;
-  const float w = (A.multipole && axial) ? A.m0[(size_t)ax * 5 + j] : 0.0f;
-  {
-    float c[B];
-    if (valid) {
-      ldv<B>(A.chi + (size_t)A.gid[slot] * B, c);
-    } else {
-#pragma unroll
-      for (int b = 0; b < B; b++) c[b] = 0.0f;
-    }
-    stv<B>(sC + t * B, c);
-  }
-  __syncthreads();
-  GRegs G;
-  load_g(sG, axial, i, j, G);
-  {
-    float X1[B], X2[B], s0[B], s1[B];
-    amulv<B>(G.gT, sC + eb, i, j, X1);
-    mulbv<B>(sC + eb, G.g2, i, j, X2);
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-      s0[b] = m1 * X2[b] + m2 * X1[b];
-      s1[b] = m1 * X1[b] + m4 * X2[b];
-    }
-    stv<B>(&sS[0][t * B], s0);
-    stv<B>(&sS[1][t * B], s1);
-  }
-  __syncthreads();
-  float out[B], Y1[B], Y2[B];
-  amulv<B>(G.g, &sS[0][eb], i, j, Y1);
-  mulbv<B>(&sS[1][eb], G.g2T, i, j, Y2);
-#pragma unroll
-  for (int b = 0; b < B; b++) {
-    const float *C = sC + eb + b;
-    float loc = Y1[b] + Y2[b];
-    if (A.multipole) {
-      loc = loc + mw * C[k * B];
-      if (axial) {
-        const float V1 = vgm<B>(G.g0, C, j);
-        loc = loc + G.g0i * (w * V1);
-      }
-    }
-    if (valid) {
-      const float stf = A.stf[b];
-      if (stf != 0.0f) {
-        const int ks = A.srcel[b][e];
-        if (ks >= 0) loc = loc - A.src_fl[b][ks * 25 + k] * stf;
-      }
-    }
-    out[b] = loc;
-  }
-  if (valid) stv<B>(A.force + slot * B, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -2271,19 +2278,23 @@ struct Launcher {
   }
 };
 
+// one launch: the solid tiles, then the fluid tiles
 template <int SRC, int MODE, bool ANEL, int B>
-static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
-  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B>), dim3(M.ntile_s), dim3(256), 0, s, A);
+static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
+                         const FluidArgs &F) {
+  const int nblocks = M.ntile_s + (M.nel_f > 0 ? M.ntile_f : 0);
+  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B>), dim3(nblocks), dim3(256), 0, s, A, F);
 }
 
 template <int B>
-static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A, int mode) {
+static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
+                           const FluidArgs &F, int mode) {
   const bool an = M.anel != 0;
 #define AXH_SOLID_CASE(S)                                                              \
   if (mode == 0) {                                                                     \
-    if (an) launch_solid<S, 0, true, B>(M, s, A); else launch_solid<S, 0, false, B>(M, s, A); \
+    if (an) launch_solid<S, 0, true, B>(M, s, A, F); else launch_solid<S, 0, false, B>(M, s, A, F); \
   } else {                                                                             \
-    if (an) launch_solid<S, 1, true, B>(M, s, A); else launch_solid<S, 1, false, B>(M, s, A); \
+    if (an) launch_solid<S, 1, true, B>(M, s, A, F); else launch_solid<S, 1, false, B>(M, s, A, F); \
   }
   if (M.src == AXH_MONOPOLE) { AXH_SOLID_CASE(AXH_MONOPOLE) }
   else if (M.src == AXH_DIPOLE) { AXH_SOLID_CASE(AXH_DIPOLE) }
@@ -2332,6 +2343,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
 
   SolidArgs SA{};
   SA.nel = M.nel_s; SA.nsls = M.nsls; SA.nglob = M.nglob_s; SA.nslot = (int)nslot_s;
+  SA.ntile = M.ntile_s;
   SA.G = M.G; SA.coef = M.coef_s; SA.m0 = M.m0_s; SA.einfo = M.einfo_s; SA.gid = M.gid_s;
   SA.cg_stage = M.cg_stage; SA.cg_upd = M.cg_upd; SA.aj_mu = M.aj_mu; SA.aj_k = M.aj_k;
   for (int s = 0; s < kMaxSls; s++) {
@@ -2420,14 +2432,10 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         FA.stf[b] = (newmark && mb[b]->fluid_src) ? stf : 0.0f;
         AS.stf[b] = (newmark && mb[b]->fluid_src) ? 0.0f : stf;
       }
-      if (M.nel_f > 0)
-        L(KC_FLUID, [&] {
-          hipLaunchKernelGGL(k_fluid<B>, dim3(M.ntile_f), dim3(256), 0, st_, FA);
-        });
       SA.direct = 1;
       SA.cv = newmark ? 0.0 : M.coefv[st];
       SA.cd_next = newmark ? 0.0 : M.coefd[st + 1];
-      L(KC_SOLID, [&] { dispatch_solid<B>(M, st_, SA, newmark ? 0 : 1); });
+      L(KC_SOLID, [&] { dispatch_solid<B>(M, st_, SA, FA, newmark ? 0 : 1); });
       SA.direct = 0;
       if (M.nel_f > 0) {
         AF.cv = newmark ? 0.0 : M.coefv[st];
