@@ -145,6 +145,21 @@ struct MeshDev {
 };
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b to XCD b % 8), each with its own L2.  Neighbouring points
+// gather from the same element slots, so the assembly kernels give each XCD
+// one contiguous range of the n blocks: block b -> 8-way range b % 8,
+// position b / 8 in it.  A bijection on [0, n).  (The element kernels keep
+// the plain order: measured, the contiguous ranges did not help them.)
+// ---------------------------------------------------------------------------
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b % kXcd, r = b / kXcd;
+  const int q = n / kXcd, rem = n % kXcd;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
+// ---------------------------------------------------------------------------
 // B consecutive floats (one point, the wavefields of a group)
 // ---------------------------------------------------------------------------
 template <int B>
@@ -1144,7 +1159,7 @@ struct AsmSolidArgs {
 
 template <int B>
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
-  const int ip = blockIdx.x * 256 + threadIdx.x;
+  const int ip = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (ip >= A.npts) return;
   const int g = A.pts[ip];
   const uint8_t fl = A.flags[g];
@@ -1264,7 +1279,7 @@ struct AsmFluidArgs {
 
 template <int B>
 __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (g >= A.n) return;
   const uint8_t fl = A.flags[g];
   const bool masked = (A.mask_axis && (fl & kFlagAxis)) ||
