@@ -310,12 +310,13 @@ subroutine upload_all()
   character(len=1024)           :: path
   integer(c_int)                :: lpath
 
-  ! time scheme
+  ! time scheme (read_inparam_basic lower-cases TIME_SCHEME,
+  ! SOLVER/parameters.F90:471, so the McLachlan schemes arrive as ml_so*)
   select case (trim(time_scheme))
   case ('newmark2');  dims%scheme = AXH_NEWMARK2
   case ('symplec4');  dims%scheme = AXH_SYMPLEC4
-  case ('ML_SO4m5');  dims%scheme = AXH_ML_SO4M5
-  case ('ML_SO6m7');  dims%scheme = AXH_ML_SO6M7
+  case ('ML_SO4m5', 'ml_so4m5');  dims%scheme = AXH_ML_SO4M5
+  case ('ML_SO6m7', 'ml_so6m7');  dims%scheme = AXH_ML_SO6M7
   case default
      write(6,*) 'GPU time loop: unsupported time scheme ', trim(time_scheme)
      stop 2

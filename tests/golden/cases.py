@@ -65,7 +65,17 @@ BATCHES = {
                              "b_dipole_mtr_700", "b_dipole_mpr_2000"],
     "mono_newmark_anel": ["b_mono_mtt_p_mpp_300", "b_mono_explosion_100", "b_mono_vertforce_0"],
 }
-ALL_CONFIGS = {**CASES, **BATCH_MEMBERS}
+# Cases the reference program cannot run (it lower-cases TIME_SCHEME and its
+# symplectic_coefficients then matches neither McLachlan scheme): the GPU is
+# checked against the oracle only, whose schemes are pinned by their order.
+ORACLE_CASES = {
+    "mono_vertforce_mlso4m5": RunConfig("vertforce", 0.0, 1e20, "ML_SO4m5", False, 600.0,
+                                        STATIONS),
+    "dipole_thetaforce_mlso6m7_anel": RunConfig("thetaforce", 50.0, 1e20, "ML_SO6m7", True,
+                                                500.0, STATIONS),
+    "quad_mtp_mlso4m5_anel": RunConfig("mtp", 300.0, 1e20, "ML_SO4m5", True, 500.0, STATIONS),
+}
+ALL_CONFIGS = {**CASES, **BATCH_MEMBERS, **ORACLE_CASES}
 
 NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
          "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}

@@ -51,7 +51,11 @@ def _compare_fields(gpu, orc):
         np.testing.assert_array_equal(a, b, err_msg=nm)
 
 
-@pytest.mark.parametrize("name", sorted(cases.CASES))
+# the reference-run cases plus the McLachlan-scheme cases only the oracle runs
+ORACLE_RUNS = sorted({**cases.CASES, **cases.ORACLE_CASES})
+
+
+@pytest.mark.parametrize("name", ORACLE_RUNS)
 def test_uploads_match_oracle(name, state_cache):
     print("C-ABI uploads vs oracle:", name, flush=True)
     _, st = state_cache(name)
@@ -87,7 +91,7 @@ def _random_fields(st, seed):
     return disp, velo, f[0], f[1], f[2]
 
 
-@pytest.mark.parametrize("name", sorted(cases.CASES))
+@pytest.mark.parametrize("name", ORACLE_RUNS)
 def test_random_fields_match_oracle(name, state_cache):
     print("random initial fields vs oracle:", name, flush=True)
     _, st_path = state_cache(name)
