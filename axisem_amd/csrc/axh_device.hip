@@ -122,6 +122,7 @@ struct MeshDev {
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
   // host copies used by wavefield builds and field transfers
   std::vector<int> gid_s_h, gid_f_h;
+  std::vector<int> first_order_s_h;  // solid points in first-appearance order
   std::vector<uint8_t> flags_s_h;  // solid point flags without source bits (a
                                    // group's device copy adds its sources)
   float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
@@ -1157,6 +1158,13 @@ struct AsmSolidArgs {
   const float *ddchi;    // [nglob_f][B]
 };
 
+// The copies of a point are gathered kAsmChunk at a time with the three
+// components together, so that up to twelve loads are in flight per lane; each
+// component's sum still runs over the copies in ascending slot order from
+// zero.  The point's velocity (and acceleration, displacement) is loaded
+// before the gather for the same reason.
+template <int B>
+constexpr int kAsmChunk = (B >= 3 ? 2 : 4);
 template <int B>
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
   const int ip = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -1165,90 +1173,118 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
   const uint8_t fl = A.flags[g];
   const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
+  // a component is gathered unless it is off or axis-masked (masked copies
+  // are zero before the gather)
+  bool on[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    on[c] = (A.comps & (1 << c)) && !((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1);
+  float v[3][B], a0[3][B], u[3][B];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (!(A.comps & (1 << c))) continue;
+    const size_t p = ((size_t)c * A.n + g) * B;
+    ldv<B>(A.velo + p, v[c]);
+    if (A.newmark) ldv<B>(A.acc0 + p, a0[c]);
+    if (A.drift) ldv<B>(A.disp + p, u[c]);
+  }
+  float s[3][B];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int b = 0; b < B; b++) s[c][b] = 0.0f;
+  const size_t cstride = (size_t)A.nslot * B;
+  constexpr int W = kAsmChunk<B>;
+  for (int q = q0; q < q1; q += W) {
+    int sl[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) sl[w] = (q + w < q1) ? A.gslot[q + w] : -1;
+    float x[W][3][B];
+#pragma unroll
+    for (int w = 0; w < W; w++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        if (sl[w] >= 0 && on[c]) ldv<B>(A.force + c * cstride + (size_t)sl[w] * B, x[w][c]);
+    if (fl & kFlagCoupled) {
+      // bdry_copy2solid (SOLVER/time_evol_wave.F90:1582-1611), added to the
+      // coupled slot's force before it enters the sum
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const int p = sl[w] >= 0 ? A.cpl_slot[sl[w]] : -1;
+        if (p < 0) continue;
+        float f2[B];
+        ldv<B>(A.ddchi + (size_t)A.cpl_gf[p] * B, f2);
+        const float bm0 = A.cpl_bm[2 * p], bm1 = A.cpl_bm[2 * p + 1];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          if (!on[c] || (c == 1 && !A.dipole)) continue;
+          const float bmc = (c == 2) ? bm1 : bm0;
+#pragma unroll
+          for (int b = 0; b < B; b++) x[w][c][b] = x[w][c][b] + bmc * f2[b];
+        }
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        if (sl[w] >= 0 && on[c])
+#pragma unroll
+          for (int b = 0; b < B; b++) s[c][b] = s[c][b] + x[w][c][b];
+  }
   int si[B];
 #pragma unroll
   for (int b = 0; b < B; b++) {
     si[b] = -1;
     if ((fl & kFlagSource) && A.stf[b] != 0.0f) {
       const int *sg = A.src_g[b];
-      for (int s = 0; s < A.nsrc[b]; s++)
-        if (sg[s] == g) { si[b] = s; break; }
+      for (int k = 0; k < A.nsrc[b]; k++)
+        if (sg[k] == g) { si[b] = k; break; }
     }
   }
-  const int order[3] = {0, 2, 1};
-  for (int kk = 0; kk < 3; kk++) {
-    const int c = order[kk];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
     if (!(A.comps & (1 << c))) continue;
-    float s[B];
-#pragma unroll
-    for (int b = 0; b < B; b++) s[b] = 0.0f;
-    if (!((fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1)) {
-      // masked copies are zero before the gather
-      const float *f = A.force + (size_t)c * A.nslot * B;
-      const bool coupled = (fl & kFlagCoupled) && (c != 1 || A.dipole);
-      for (int q = q0; q < q1; q++) {
-        const int sl = A.gslot[q];
-        float x[B];
-        ldv<B>(f + (size_t)sl * B, x);
-        if (coupled) {
-          // bdry_copy2solid (SOLVER/time_evol_wave.F90:1582-1611)
-          const int p = A.cpl_slot[sl];
-          if (p >= 0) {
-            float fl2[B];
-            ldv<B>(A.ddchi + (size_t)A.cpl_gf[p] * B, fl2);
-            const float bmc = (c == 2) ? A.cpl_bm[2 * p + 1] : A.cpl_bm[2 * p];
-#pragma unroll
-            for (int b = 0; b < B; b++) x[b] = x[b] + bmc * fl2[b];
-          }
-        }
-#pragma unroll
-        for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
-      }
-    }
 #pragma unroll
     for (int b = 0; b < B; b++)
-      if (si[b] >= 0) s[b] = s[b] - A.src_val[b][3 * si[b] + c] * A.stf[b];
+      if (si[b] >= 0) s[c][b] = s[c][b] - A.src_val[b][3 * si[b] + c] * A.stf[b];
     const size_t p = ((size_t)c * A.n + g) * B;
-    float v[B];
-    ldv<B>(A.velo + p, v);
     if (A.newmark) {
-      float a0[B], a[B];
-      ldv<B>(A.acc0 + p, a0);
+      float a[B];
 #pragma unroll
       for (int b = 0; b < B; b++) {
         if (c == 2 && A.src == AXH_DIPOLE)
-          a[b] = (float)(-2.0 * (double)mi * (double)s[b]);
+          a[b] = (float)(-2.0 * (double)mi * (double)s[c][b]);
         else
-          a[b] = -mi * s[b];
-        v[b] = (float)((double)v[b] + A.hdt * (double)(a0[b] + a[b]));
+          a[b] = -mi * s[c][b];
+        v[c][b] = (float)((double)v[c][b] + A.hdt * (double)(a0[c][b] + a[b]));
       }
       stv<B>(A.acc0 + p, a);
     } else {
 #pragma unroll
       for (int b = 0; b < B; b++) {
-        const float a = -mi * s[b];
+        const float a = -mi * s[c][b];
         if (c == 2 && A.src == AXH_DIPOLE)
-          v[b] = (float)((double)v[b] + 2.0 * (double)a * A.cv);
+          v[c][b] = (float)((double)v[c][b] + 2.0 * (double)a * A.cv);
         else
-          v[b] = (float)((double)v[b] + (double)a * A.cv);
+          v[c][b] = (float)((double)v[c][b] + (double)a * A.cv);
       }
     }
-    stv<B>(A.velo + p, v);
+    stv<B>(A.velo + p, v[c]);
     if (A.drift) {
       // as k_predict's drift, with the velocity just computed
-      float u[B];
-      ldv<B>(A.disp + p, u);
 #pragma unroll
-      for (int b = 0; b < B; b++) u[b] = (float)((double)u[b] + (double)v[b] * A.cd_next);
+      for (int b = 0; b < B; b++)
+        u[c][b] = (float)((double)u[c][b] + (double)v[c][b] * A.cd_next);
       if (fl & kFlagAxisBdry) {
-        for (int q = 0; q < A.naxb; q++)
-          if (A.axb_g[q] == g) stv<B>(A.axb_val + ((size_t)q * 3 + c) * B, u);
+        for (int k = 0; k < A.naxb; k++)
+          if (A.axb_g[k] == g) stv<B>(A.axb_val + ((size_t)k * 3 + c) * B, u[c]);
       }
       if (A.drift_mask && (fl & kFlagAxis) && c >= A.mask0 && c <= A.mask1) {
 #pragma unroll
-        for (int b = 0; b < B; b++) u[b] = 0.0f;
+        for (int b = 0; b < B; b++) u[c][b] = 0.0f;
       }
-      stv<B>(A.disp + p, u);
+      stv<B>(A.disp + p, u[c]);
     }
   }
 }
@@ -1288,16 +1324,25 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
 #pragma unroll
   for (int b = 0; b < B; b++) s[b] = 0.0f;
   if (!masked) {
+    // copies gathered four at a time (loads in flight together), summed in
+    // ascending slot order from zero
     const bool coupled = fl & kFlagCoupled;
-    for (int q = A.gstart[g]; q < A.gstart[g + 1]; q++) {
-      const int sl = A.gslot[q];
-      float x[B];
-      ldv<B>(A.force + (size_t)sl * B, x);
+    const int q1 = A.gstart[g + 1];
+    for (int q = A.gstart[g]; q < q1; q += 4) {
+      int sl[4];
+#pragma unroll
+      for (int w = 0; w < 4; w++) sl[w] = (q + w < q1) ? A.gslot[q + w] : -1;
+      float x[4][B];
+#pragma unroll
+      for (int w = 0; w < 4; w++)
+        if (sl[w] >= 0) ldv<B>(A.force + (size_t)sl[w] * B, x[w]);
       if (coupled) {
         // bdry_copy2fluid (SOLVER/time_evol_wave.F90:1532-1580) with the
         // prediction before the axis mask
-        const int p = A.cpl_slot[sl];
-        if (p >= 0) {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+          const int p = sl[w] >= 0 ? A.cpl_slot[sl[w]] : -1;
+          if (p < 0) continue;
           const int gs = A.cpl_gs[p], kq = A.cpl_axb[p];
           float s1[B], s2[B], s3[B];
           if (kq >= 0) {
@@ -1313,14 +1358,17 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
 #pragma unroll
           for (int b = 0; b < B; b++) {
             if (A.dipole)
-              x[b] = x[b] - bm1 * (s1[b] + s2[b]) - bm2 * s3[b];
+              x[w][b] = x[w][b] - bm1 * (s1[b] + s2[b]) - bm2 * s3[b];
             else
-              x[b] = x[b] - bm1 * s1[b] - bm2 * s3[b];
+              x[w][b] = x[w][b] - bm1 * s1[b] - bm2 * s3[b];
           }
         }
       }
 #pragma unroll
-      for (int b = 0; b < B; b++) s[b] = s[b] + x[b];
+      for (int w = 0; w < 4; w++)
+        if (sl[w] >= 0)
+#pragma unroll
+          for (int b = 0; b < B; b++) s[b] = s[b] + x[w][b];
     }
   }
   const float mi = A.minv[g];
@@ -1493,6 +1541,7 @@ struct MeshHost {
       cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, cpl_slot_s, cpl_slot_f;
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
+  std::vector<int> first_order_s;  // solid points in first-appearance order
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
 };
 
@@ -1502,6 +1551,42 @@ struct WaveHost {
   std::vector<int> src_g, srcel_f, rec_g;
   std::vector<float> src_val, src_fl, stf;
 };
+
+// Renumber the solid points so that the points the assembly kernel handles
+// (all but kFlagDirect) come first, each class in its previous order: that
+// kernel then reads and writes one contiguous range of every per-point array,
+// and the stiffness kernel's single-copy points another.
+static void number_assembled_first(MeshHost &H) {
+  const int n = H.nglob_s;
+  std::vector<int> perm(n), inv(n);  // old -> new, new -> old
+  int next = 0;
+  for (int pass = 0; pass < 2; pass++)
+    for (int g = 0; g < n; g++)
+      if (((H.flags_s[g] & kFlagDirect) != 0) == (pass == 1)) {
+        perm[g] = next;
+        inv[next++] = g;
+      }
+  std::vector<int> gstart(n + 1, 0), gslot(H.gslot_s.size());
+  std::vector<float> minv(n);
+  std::vector<uint8_t> flags(n);
+  for (int p = 0; p < n; p++) {
+    const int g = inv[p];
+    const int q0 = H.gstart_s[g], q1 = H.gstart_s[g + 1];
+    std::copy(H.gslot_s.begin() + q0, H.gslot_s.begin() + q1, gslot.begin() + gstart[p]);
+    gstart[p + 1] = gstart[p] + (q1 - q0);
+    minv[p] = H.minv_s[g];
+    flags[p] = H.flags_s[g];
+  }
+  H.gstart_s.swap(gstart);
+  H.gslot_s.swap(gslot);
+  H.minv_s.swap(minv);
+  H.flags_s.swap(flags);
+  for (auto &x : H.gid_s)
+    if (x >= 0) x = perm[x];
+  for (auto &x : H.cpl_gs) x = perm[x];
+  for (auto &x : H.axb_g) x = perm[x];
+  H.first_order_s.swap(perm);  // perm[old] lists the new ids in the old order
+}
 
 static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
   const axh_dims &d = ctx->dims;
@@ -1688,6 +1773,7 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
   for (int g = 0; g < H.nglob_s; g++)
     if (H.gstart_s[g + 1] - H.gstart_s[g] == 1 && !(H.flags_s[g] & (kFlagAxis | kFlagCoupled)))
       H.flags_s[g] |= kFlagDirect;
+  number_assembled_first(H);
 
   // ---- attenuation ----
   if (d.anel) {
@@ -1877,13 +1963,13 @@ void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]) {
 
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob) {
   const bool solid = domain == 0;
-  const int nel = solid ? ctx->dims.nel_solid : ctx->dims.nel_fluid;
-  Numbering N = build_numbering(ctx->need(solid ? "igloc_solid" : "igloc_fluid").as<int32_t>(),
-                                nel, solid ? ctx->dims.nglob_solid : ctx->dims.nglob_fluid,
-                                solid ? "solid" : "fluid");
+  MeshHost H;
+  build_mesh_layout(ctx, H);
+  const int nel = solid ? H.nel_s : H.nel_f;
+  const std::vector<int> &g = solid ? H.gid_s : H.gid_f;
   for (int e = 0; e < nel; e++)
-    for (int k = 0; k < kNpp; k++) gid[(size_t)e * kNpp + k] = N.gid[slot_of(e, k)];
-  *nglob = N.nglob;
+    for (int k = 0; k < kNpp; k++) gid[(size_t)e * kNpp + k] = g[slot_of(e, k)];
+  *nglob = solid ? H.nglob_s : H.nglob_f;
 }
 
 static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
@@ -1950,6 +2036,7 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
     V.aj_k = m.upload(H.aj_k);
   }
   V.gid_s_h = std::move(H.gid_s);
+  V.first_order_s_h = std::move(H.first_order_s);
   V.gid_f_h = std::move(H.gid_f);
   V.flags_s_h = std::move(H.flags_s);
   return M;
@@ -2030,8 +2117,14 @@ static void fields_set_sources(const MeshDev &M, Fields &F,
     }
   std::vector<int> pts;
   pts.reserve(M.nglob_s);
-  for (int g = 0; g < M.nglob_s; g++)
-    if (!(fl[g] & kFlagDirect)) pts.push_back(g);
+  if (F.B > 1) {
+    // every point, in first-appearance (element) order: neighbouring lanes
+    // then gather neighbouring slots of the force array
+    pts = M.first_order_s_h;
+  } else {
+    for (int g = 0; g < M.nglob_s; g++)
+      if (!(fl[g] & kFlagDirect)) pts.push_back(g);
+  }
   F.nasm = (int)pts.size();
   HIPCHECK(hipMemcpy(F.flags_s, fl.data(), fl.size(), hipMemcpyHostToDevice));
   if (!pts.empty())

@@ -117,9 +117,21 @@ def test_inputs_pass_layout_checks_and_numbering_is_igloc(state_cache):
             # bijection between the two labellings
             pairs = np.unique(np.stack([ig, gid]), axis=1)
             assert pairs.shape[1] == n.value
-            # numbering by first appearance in element order
-            first = np.unique(gid, return_index=True)[1]
-            assert np.all(np.diff(first) > 0)
+            if dom == 1:
+                # fluid: numbering by first appearance in element order
+                first = np.unique(gid, return_index=True)[1]
+                assert np.all(np.diff(first) > 0)
+            else:
+                # solid: the points with several copies come first; only
+                # single-copy points on the axis or the fluid boundary may
+                # share their range
+                ncopy = np.bincount(gid)
+                multi = ncopy[gid] > 1
+                el = np.arange(ig.size) // 25
+                special = np.isin(el, np.nonzero(st["axis_solid"])[0]) | \
+                    np.isin(el, st["bdry_solid_el"] - 1)
+                nsingle_special = len(np.unique(gid[~multi & special]))
+                assert gid[multi].max() < len(np.unique(gid[multi])) + nsingle_special
         s.close()
 
 
