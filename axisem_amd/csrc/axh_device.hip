@@ -321,6 +321,34 @@ __device__ __forceinline__ void mulbv(const float *M, const float *Bop, int i, i
     for (int b = 0; b < B; b++) o[b] = o[b] + m[b] * c;
   }
 }
+// vgm and vmb for the B wavefields of a field tile (entry k of a wavefield at
+// k*B + b); every sum runs over k in order from zero
+template <int B>
+__device__ __forceinline__ void vgmv(const float *g, const float *M, int j, float (&o)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; b++) o[b] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const float a = g[k];
+    float m[B];
+    ldv<B>(M + (k + 5 * j) * B, m);
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = o[b] + a * m[b];
+  }
+}
+template <int B>
+__device__ __forceinline__ void vmbv(const float *v, const float *Bop, int j, float (&o)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; b++) o[b] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const float c = Bop[k + 5 * j];
+    float m[B];
+    ldv<B>(v + k * B, m);
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = o[b] + m[b] * c;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // coarse-grained strain and memory variables (SOLVER/attenuation.f90:81-202,
@@ -541,19 +569,17 @@ __device__ __forceinline__ void fluid_tile(const FluidArgs &A, int tile, float *
     stv<B>(&sS1[t * B], s1);
   }
   __syncthreads();
-  float out[B], Y1[B], Y2[B];
+  float out[B], Y1[B], Y2[B], V1[B];
   amulv<B>(G.g, &sS0[eb], i, j, Y1);
   mulbv<B>(&sS1[eb], G.g2T, i, j, Y2);
+  if (A.multipole && axial) vgmv<B>(G.g0, sC + eb, j, V1);
 #pragma unroll
   for (int b = 0; b < B; b++) {
     const float *C = sC + eb + b;
     float loc = Y1[b] + Y2[b];
     if (A.multipole) {
       loc = loc + mw * C[k * B];
-      if (axial) {
-        const float V1 = vgm<B>(G.g0, C, j);
-        loc = loc + G.g0i * (w * V1);
-      }
+      if (axial) loc = loc + G.g0i * (w * V1[b]);
     }
     if (valid) {
       const float stf = A.stf[b];
@@ -664,22 +690,28 @@ void k_solid(SolidArgs A, FluidArgs FA) {
     for (int b = 0; b < B; b++) {
       ls[b] = ls[b] + vY1[b] + vY2[b];
       lz[b] = vY3[b] + vY4[b];
-      if (axial) {
-        V1[b] = vgm<B>(G.g0, U0 + b, j);        // vxm(G0, us)
-        V2[b] = mulb<B>(U1 + b, G.g2, 0, j);    // vxm(uz(0,:), G2)
-        V3[b] = vgm<B>(G.g0, U1 + b, j);        // vxm(G0, uz)
-        if (i == 0) sV[(le * 5 + j) * B + b] = w3 * V1[b];
+    }
+    if (axial) {
+      vgmv<B>(G.g0, U0, j, V1);          // vxm(G0, us)
+      mulbv<B>(U1, G.g2, 0, j, V2);      // vxm(uz(0,:), G2)
+      vgmv<B>(G.g0, U1, j, V3);          // vxm(G0, uz)
+      if (i == 0) {
+        float x[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) x[b] = w3 * V1[b];
+        stv<B>(&sV[(le * 5 + j) * B], x);
       }
     }
     __syncthreads();
+    float V1b[B];
+    if (axial) vmbv<B>(&sV[le * 5 * B], G.g2T, j, V1b);  // vxm(w3*V1, G2T)
 #pragma unroll
     for (int b = 0; b < B; b++) {
       if (axial) {
         float V4 = w1 * V1[b] + w3 * V2[b];
         V4 = V4 + w2 * V3[b];
         float X2a = G.g0i * (w2 * V1[b]);
-        const float V1b = vmb<B>(&sV[le * 5 * B + b], G.g2T, j);  // vxm(w3*V1, G2T)
-        if (i == 0) X2a = X2a + V1b;
+        if (i == 0) X2a = X2a + V1b[b];
         ls[b] = ls[b] + G.g0i * V4;
         lz[b] = X2a + lz[b];
       }
@@ -747,27 +779,30 @@ void k_solid(SolidArgs A, FluidArgs FA) {
         acc[2][b] = Y[4][b] + Y[5][b] + ls3[b];
       }
     }
+    if (axial) {
+      float V1[B], V2[B], V3[B], V4[B], x[B];
+      vgmv<B>(G.g0, U0, j, V1);
+      vgmv<B>(G.g0, U1, j, V2);
+      vgmv<B>(G.g0, U2, j, V3);
+      mulbv<B>(U0, G.g2, 0, j, V4);   // vxm(u1(0,:), G2)
 #pragma unroll
-    for (int b = 0; b < B; b++) {
-      if (axial) {
-        const float V1 = vgm<B>(G.g0, U0 + b, j);
-        const float V2 = vgm<B>(G.g0, U1 + b, j);
-        const float V3 = vgm<B>(G.g0, U2 + b, j);
-        const float V4 = mulb<B>(U0 + b, G.g2, 0, j);   // vxm(u1(0,:), G2)
-        va[b] = w1 * V2 + w3 * V3;
-        vb[b] = w1 * V1 + (w2 + w6) * V4 + w9 * V2 + w10 * V3;
-        vc[b] = w3 * V1 + (w4 + w8) * V4 + w7 * V3 + w10 * V2;
-        if (i == 0) sV[(le * 5 + j) * B + b] = (w2 + w6) * V2 + (w4 + w8) * V3;
+      for (int b = 0; b < B; b++) {
+        va[b] = w1 * V2[b] + w3 * V3[b];
+        vb[b] = w1 * V1[b] + (w2 + w6) * V4[b] + w9 * V2[b] + w10 * V3[b];
+        vc[b] = w3 * V1[b] + (w4 + w8) * V4[b] + w7 * V3[b] + w10 * V2[b];
+        x[b] = (w2 + w6) * V2[b] + (w4 + w8) * V3[b];
       }
+      if (i == 0) stv<B>(&sV[(le * 5 + j) * B], x);
     }
     __syncthreads();
     if (axial) {
+      float V1b[B];
+      vmbv<B>(&sV[le * 5 * B], G.g2T, j, V1b);  // vxm(V4, G2T)
 #pragma unroll
       for (int b = 0; b < B; b++) {
         float S1pa = G.g0i * va[b];
         const float S1ma = G.g0i * vb[b], S1za = G.g0i * vc[b];
-        const float V1b = vmb<B>(&sV[le * 5 * B + b], G.g2T, j);  // vxm(V4, G2T)
-        if (i == 0) S1pa = S1pa + V1b;
+        if (i == 0) S1pa = S1pa + V1b[b];
         acc[0][b] = acc[0][b] + S1pa;
         acc[1][b] = acc[1][b] + S1ma;
         acc[2][b] = acc[2][b] + S1za;
@@ -830,10 +865,13 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       const float *m0 = A.m0 + (size_t)ax * 50;
       const float w1 = m0[0 * 5 + j], w2 = m0[1 * 5 + j], w3 = m0[2 * 5 + j],
                   w4 = m0[3 * 5 + j], w5 = m0[4 * 5 + j], w6 = m0[5 * 5 + j];
+      float vV1[B], vV2[B], vV3[B];
+      vgmv<B>(G.g0, U0, j, vV1);
+      vgmv<B>(G.g0, U1, j, vV2);
+      vgmv<B>(G.g0, U2, j, vV3);
 #pragma unroll
       for (int b = 0; b < B; b++) {
-        const float V1 = vgm<B>(G.g0, U0 + b, j), V2 = vgm<B>(G.g0, U1 + b, j),
-                    V3 = vgm<B>(G.g0, U2 + b, j);
+        const float V1 = vV1[b], V2 = vV2[b], V3 = vV3[b];
         const float a = w1 * V1 + w2 * V2 + w3 * V3;
         const float bq = w2 * V1 + w4 * V2 + w5 * V3;
         const float c = w3 * V1 + w5 * V2 + w6 * V3;
@@ -908,37 +946,61 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       const int qb0 = (j == 3) ? 1 : 0, qa0 = (i == 3) ? 2 : 0;
       const float gb0 = G.g[i + 5], gb1 = G.g[i + 15];
       const float ga0 = G.g2T[1 + 5 * j], ga1 = G.g2T[3 + 5 * j];
+      // the B wavefields' entries of Sa and R are adjacent: read together
+      auto spbv = [&](int s1, float (&o)[B]) {
+        float x0[B], x1[B];
+        ldv<B>(Sa + (s1 * 4 + qb0) * B, x0);
+        ldv<B>(Sa + (s1 * 4 + qb0 + 2) * B, x1);
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+          const float v = gb0 * x0[b] + gb1 * x1[b];
+          o[b] = inb ? v : 0.0f;
+        }
+      };
+      auto spav = [&](int s2, float (&o)[B]) {
+        float x0[B], x1[B];
+        ldv<B>(Sa + (s2 * 4 + qa0) * B, x0);
+        ldv<B>(Sa + (s2 * 4 + qa0 + 1) * B, x1);
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+          const float v = x0[b] * ga0 + x1[b] * ga1;
+          o[b] = ina ? v : 0.0f;
+        }
+      };
+      float vr2[B], vr4[B], vr6[B], pb0[B], pa1[B], pb2[B], pa3[B], pb4[B], pa5[B];
+      ldv<B>(R + (1 * 4 + q) * B, vr2);
+      ldv<B>(R + (3 * 4 + q) * B, vr4);
+      ldv<B>(R + (5 * 4 + q) * B, vr6);
+      spbv(0, pb0);
+      spav(1, pa1);
+      if (SRC != AXH_MONOPOLE) {
+        spbv(2, pb2);
+        spav(3, pa3);
+      }
+      spbv(4, pb4);
+      spav(5, pa5);
 #pragma unroll
       for (int b = 0; b < B; b++) {
-        auto spb = [&](int s1) -> float {
-          const float v = gb0 * Sa[(s1 * 4 + qb0) * B + b] + gb1 * Sa[(s1 * 4 + qb0 + 2) * B + b];
-          return inb ? v : 0.0f;
-        };
-        auto spa = [&](int s2) -> float {
-          const float v = Sa[(s2 * 4 + qa0) * B + b] * ga0 + Sa[(s2 * 4 + qa0 + 1) * B + b] * ga1;
-          return ina ? v : 0.0f;
-        };
-        const float r2 = R[(1 * 4 + q) * B + b], r4 = R[(3 * 4 + q) * B + b],
-                    r6 = R[(5 * 4 + q) * B + b];
+        const float r2 = vr2[b], r4 = vr4[b], r6 = vr6[b];
         if (SRC == AXH_MONOPOLE) {
-          float l1 = spb(0) + spa(1);
+          float l1 = pb0[b] + pa1[b];
           l1 = cgp ? l1 + yl * r2 : l1;
-          const float l3 = spb(4) + spa(5);
+          const float l3 = pb4[b] + pa5[b];
           acc[0][b] = acc[0][b] - l1;
           acc[2][b] = acc[2][b] - l3;
         } else if (SRC == AXH_DIPOLE) {
-          const float lp = spb(0) + spa(1);
-          float lm = spb(2) + spa(3);
-          float lz = spb(4) + spa(5);
+          const float lp = pb0[b] + pa1[b];
+          float lm = pb2[b] + pa3[b];
+          float lz = pb4[b] + pa5[b];
           lm = cgp ? lm + 2.0f * yl * (r2 - r6) : lm;
           lz = cgp ? lz - yl * r4 : lz;
           acc[0][b] = acc[0][b] - lp;
           acc[1][b] = acc[1][b] - lm;
           acc[2][b] = acc[2][b] - lz;
         } else {
-          float ls = spb(0) + spa(1);
-          float lp = -spb(2) - spa(3);
-          float lz = spb(4) + spa(5);
+          float ls = pb0[b] + pa1[b];
+          float lp = -pb2[b] - pa3[b];
+          float lz = pb4[b] + pa5[b];
           ls = cgp ? ls + yl * (r2 - 2.0f * r6) : ls;
           lp = cgp ? lp + yl * (r6 - 2.0f * r2) : lp;
           lz = cgp ? lz - 2.0f * yl * r4 : lz;
