@@ -681,16 +681,8 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       const float *m0 = A.m0 + (size_t)ax * 50;
       w1 = m0[0 * 5 + j]; w2 = m0[1 * 5 + j]; w3 = m0[2 * 5 + j];
     }
-    float vY1[B], vY2[B], vY3[B], vY4[B];
-    mulbv<B>(&sS[1][eb], G.g2T, i, j, vY2);
-    mulbv<B>(&sS[3][eb], G.g2T, i, j, vY4);
-    amulv<B>(G.g, &sS[0][eb], i, j, vY1);
-    amulv<B>(G.g, &sS[2][eb], i, j, vY3);
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-      ls[b] = ls[b] + vY1[b] + vY2[b];
-      lz[b] = vY3[b] + vY4[b];
-    }
+    // the axial terms first: their registers are free again before the
+    // products' LDS reads are issued
     if (axial) {
       vgmv<B>(G.g0, U0, j, V1);          // vxm(G0, us)
       mulbv<B>(U1, G.g2, 0, j, V2);      // vxm(uz(0,:), G2)
@@ -701,6 +693,16 @@ void k_solid(SolidArgs A, FluidArgs FA) {
         for (int b = 0; b < B; b++) x[b] = w3 * V1[b];
         stv<B>(&sV[(le * 5 + j) * B], x);
       }
+    }
+    float vY1[B], vY2[B], vY3[B], vY4[B];
+    mulbv<B>(&sS[1][eb], G.g2T, i, j, vY2);
+    mulbv<B>(&sS[3][eb], G.g2T, i, j, vY4);
+    amulv<B>(G.g, &sS[0][eb], i, j, vY1);
+    amulv<B>(G.g, &sS[2][eb], i, j, vY3);
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      ls[b] = ls[b] + vY1[b] + vY2[b];
+      lz[b] = vY3[b] + vY4[b];
     }
     __syncthreads();
     float V1b[B];
@@ -764,21 +766,8 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       w10 = m0[9 * 5 + j];
     }
     float va[B], vb[B], vc[B];
-    {
-      float Y[6][B];
-      amulv<B>(G.g, &sS[0][eb], i, j, Y[0]);
-      mulbv<B>(&sS[2][eb], G.g2T, i, j, Y[1]);
-      amulv<B>(G.g, &sS[1][eb], i, j, Y[2]);
-      mulbv<B>(&sS[3][eb], G.g2T, i, j, Y[3]);
-      amulv<B>(G.g, &sS[4][eb], i, j, Y[4]);
-      mulbv<B>(&sS[5][eb], G.g2T, i, j, Y[5]);
-#pragma unroll
-      for (int b = 0; b < B; b++) {
-        acc[0][b] = Y[0][b] + Y[1][b];
-        acc[1][b] = Y[2][b] + Y[3][b] + ls2[b];
-        acc[2][b] = Y[4][b] + Y[5][b] + ls3[b];
-      }
-    }
+    // the axial terms first: their registers are free again before the
+    // products' LDS reads are issued
     if (axial) {
       float V1[B], V2[B], V3[B], V4[B], x[B];
       vgmv<B>(G.g0, U0, j, V1);
@@ -793,6 +782,21 @@ void k_solid(SolidArgs A, FluidArgs FA) {
         x[b] = (w2 + w6) * V2[b] + (w4 + w8) * V3[b];
       }
       if (i == 0) stv<B>(&sV[(le * 5 + j) * B], x);
+    }
+    {
+      float Y[6][B];
+      amulv<B>(G.g, &sS[0][eb], i, j, Y[0]);
+      mulbv<B>(&sS[2][eb], G.g2T, i, j, Y[1]);
+      amulv<B>(G.g, &sS[1][eb], i, j, Y[2]);
+      mulbv<B>(&sS[3][eb], G.g2T, i, j, Y[3]);
+      amulv<B>(G.g, &sS[4][eb], i, j, Y[4]);
+      mulbv<B>(&sS[5][eb], G.g2T, i, j, Y[5]);
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        acc[0][b] = Y[0][b] + Y[1][b];
+        acc[1][b] = Y[2][b] + Y[3][b] + ls2[b];
+        acc[2][b] = Y[4][b] + Y[5][b] + ls3[b];
+      }
     }
     __syncthreads();
     if (axial) {
