@@ -616,8 +616,12 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   __shared__ __align__(16) float sU[NCOMP * 256 * B];
   __shared__ __align__(16) float sS[NS][256 * B];
   __shared__ __align__(16) float sV[kElemPerTile * 5 * B];
-  __shared__ __align__(16) float sR[ANEL ? kElemPerTile * 24 * B : 1];
-  __shared__ __align__(16) float sSa[ANEL ? kElemPerTile * 24 * B : 1];
+  // the SLS sums and their products with the cg4 terms live in the first two
+  // product buffers, which are dead once the second products are summed (a
+  // barrier separates the two uses)
+  static_assert(kElemPerTile * 24 <= 256, "SLS sums must fit a product buffer");
+  float *const sR = &sS[0][0];
+  float *const sSa = &sS[1][0];
   __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sG[105];
