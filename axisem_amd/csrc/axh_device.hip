@@ -650,6 +650,22 @@ void k_solid(SolidArgs A, FluidArgs FA) {
     for (int a = 0; a < NC; a++) cf[a] = coef[a * 256];
   }
   gather_disp<NCOMP, B>(A, valid, gp, t, sU);
+  // a single-copy point is kicked here at the end: its velocity (and, for
+  // newmark2, acceleration) and inverse mass are requested now
+  // (one wavefield only: for a group the extra state traffic at the end of
+  // this kernel costs more than the assembly kernel saves)
+  const bool direct = B == 1 && valid && A.direct && (A.flags[gp] & kFlagDirect);
+  float dv[NCOMP][B], da[MODE == 0 ? NCOMP : 1][B], dmi = 0.0f;
+  if (direct) {
+    dmi = A.minv[gp];
+#pragma unroll
+    for (int cc = 0; cc < NCOMP; cc++) {
+      const int c = (NCOMP == 2 && cc == 1) ? 2 : cc;
+      const size_t p = ((size_t)c * A.nglob + gp) * B;
+      ldv<B>(A.velo + p, dv[cc]);
+      if (MODE == 0) ldv<B>(A.acc0 + p, da[MODE == 0 ? cc : 0]);
+    }
+  }
   __syncthreads();
 
   GRegs G;
@@ -1035,19 +1051,18 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   }
 
   if (valid) {
-    if (B == 1 && A.direct && (A.flags[gp] & kFlagDirect)) {
+    if (direct) {
       // the point's only copy: assembly (0 + f, as the gather sums), kick
       // and drift as in k_asm_solid
-      const float mi = A.minv[gp];
+      const float mi = dmi;
 #pragma unroll
       for (int cc = 0; cc < NCOMP; cc++) {
         const int c = (NCOMP == 2 && cc == 1) ? 2 : cc;
         const size_t p = ((size_t)c * A.nglob + gp) * B;
-        float v[B];
-        ldv<B>(A.velo + p, v);
+        float *v = dv[cc];
         if (MODE == 0) {
-          float a0[B], a[B];
-          ldv<B>(A.acc0 + p, a0);
+          float a[B];
+          const float *a0 = da[MODE == 0 ? cc : 0];
 #pragma unroll
           for (int b = 0; b < B; b++) {
             const float sm = 0.0f + acc[c][b];
@@ -1058,7 +1073,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
             v[b] = (float)((double)v[b] + A.hdt * (double)(a0[b] + a[b]));
           }
           stv<B>(A.acc0 + p, a);
-          stv<B>(A.velo + p, v);
+          stv<B>(A.velo + p, dv[cc]);
         } else {
           float u[B];
 #pragma unroll
@@ -1070,7 +1085,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
               v[b] = (float)((double)v[b] + (double)a * A.cv);
             u[b] = (float)((double)sU[cc * 256 * B + t * B + b] + (double)v[b] * A.cd_next);
           }
-          stv<B>(A.velo + p, v);
+          stv<B>(A.velo + p, dv[cc]);
           stv<B>(A.disp_w + p, u);
         }
       }
