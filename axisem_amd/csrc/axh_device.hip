@@ -1100,21 +1100,29 @@ void k_solid(SolidArgs A, FluidArgs FA) {
 // memory-variable update of the symplectic scheme, once per step after the
 // final drift (SOLVER/time_evol_wave.F90:727-731); also refreshes the SLS sums
 // read by the stage kernels.
+// Each lane owns entries r = k + 25 w of its element's [24][B] block for every
+// SLS: it converts their sources once, runs the SLS in order for each entry
+// and sums the new values over the SLS (from zero, in SLS order) as it goes.
+// The products ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j of the element are
+// formed once per element, exactly as the per-entry expressions group them.
 // Dynamic LDS: the tile's memory variables, 10 * 24 * nsls * B floats.
 template <int SRC, int B>
 __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
+  constexpr int NR = (24 * B + 24) / 25;  // entries per lane
   extern __shared__ __align__(16) float sMv[];
   __shared__ __align__(16) float sU[NCOMP * 256 * B];
   __shared__ __align__(16) float sG[105];
   __shared__ __align__(16) float sSrc[kElemPerTile * 28 * B];
   __shared__ __align__(16) float sOld[kElemPerTile * 28 * B];
+  __shared__ double sCf[kElemPerTile][kMaxSls][4];
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int e = tile * kElemPerTile + le;
   const bool valid = (t < 250) && (e < A.nel);
   const size_t slot = (size_t)tile * 256 + t;
-  const int n = 24 * A.nsls * B;
+  const int nsls = A.nsls;
+  const int n = 24 * nsls * B;
   const int eb = le * 25 * B;
   stage_g(A.G, sG);
   gather_disp<NCOMP, B>(A, valid, valid ? A.gid[slot] : 0, t, sU);
@@ -1128,6 +1136,14 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
     float4 *dst4 = reinterpret_cast<float4 *>(sMv);
     for (int x = t; x < n4; x += 256) dst4[x] = src4[x];
   }
+  if (valid && k >= 4 * B && k < 4 * B + nsls) {
+    const int jj = k - 4 * B;
+    const double amu = A.aj_mu[(size_t)e * nsls + jj], ak = A.aj_k[(size_t)e * nsls + jj];
+    sCf[le][jj][0] = A.tst[jj] * amu;
+    sCf[le][jj][1] = A.tstm1[jj] * amu;
+    sCf[le][jj][2] = A.tst[jj] * ak;
+    sCf[le][jj][3] = A.tstm1[jj] * ak;
+  }
   __syncthreads();
   const bool axial = valid && A.einfo[e] >= 0;
   float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
@@ -1138,17 +1154,59 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
     memvar_prepare<SRC, B>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
   }
   __syncthreads();
-  if (valid) memvar_update_elem<SRC, B>(A, e, k, mv, src, old, mv);
-  __syncthreads();
-  if (valid && k < 24) {
-    float r[B];
+  if (!valid) return;
+  float *memvar = A.memvar + (size_t)e * n;
+  // per entry: its sources (converted once) and its running SLS sum; the SLS
+  // loop is outside so that the lane's entries proceed side by side
+  double sd[NR], od[NR], st[NR], ot[NR];
+  float rs[NR];
+  bool upd[NR];
 #pragma unroll
-    for (int b = 0; b < B; b++) {
-      float s = 0.0f;
-      for (int jj = 0; jj < A.nsls; jj++) s = s + mv[(jj * 24 + k) * B + b];
-      r[b] = s;
+  for (int w = 0; w < NR; w++) {
+    const int r = k + 25 * w;
+    const int cq = r / B, b = r - cq * B;
+    const int q = cq & 3, c = cq >> 2;
+    upd[w] = r < 24 * B && !(SRC == AXH_MONOPOLE && (c == 3 || c == 5));
+    rs[w] = 0.0f;
+    if (upd[w]) {
+      sd[w] = (double)src[(4 + cq) * B + b];
+      od[w] = (double)old[(4 + cq) * B + b];
+      st[w] = (double)src[q * B + b];
+      ot[w] = (double)old[q * B + b];
     }
-    stv<B>(A.rsum + ((size_t)e * 24 + k) * B, r);
+  }
+  for (int jj = 0; jj < nsls; jj++) {
+    const double E = A.expw[jj];
+    const double c0 = sCf[le][jj][0], c1 = sCf[le][jj][1], c2 = sCf[le][jj][2],
+                 c3 = sCf[le][jj][3];
+#pragma unroll
+    for (int w = 0; w < NR; w++) {
+      const int r = k + 25 * w;
+      if (r >= 24 * B) continue;
+      const int m = jj * 24 * B + r;
+      if (!upd[w]) {
+        // not updated for a monopole source: summed as they are
+        rs[w] = rs[w] + mv[m];
+        continue;
+      }
+      const int c = (r / B) >> 2;
+      const double mvo = (double)mv[m];
+      const float db = (float)(c0 * sd[w] + c1 * od[w]);
+      float nv;
+      if (c < 3) {
+        const float trb = (float)(c2 * st[w] + c3 * ot[w]);
+        nv = (float)((E * mvo + (double)db) + (double)trb);
+      } else {
+        nv = (float)(E * mvo + (double)db);
+      }
+      memvar[m] = nv;
+      rs[w] = rs[w] + nv;
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < NR; w++) {
+    const int r = k + 25 * w;
+    if (r < 24 * B) A.rsum[(size_t)e * 24 * B + r] = rs[w];
   }
 }
 
