@@ -460,15 +460,14 @@ __device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *
 }
 
 // Update of the memory variables of element e for the B wavefields of the
-// group.  For each SLS jj (uniform across the element's lanes) the 24 * B
-// entries r = (c*4 + q)*B + b of that SLS are spread over the element's 25
-// lanes.  mv_old holds the element's current memory variables (LDS or its own
-// place in HBM: every entry is read before the same lane overwrites it);
-// src/old in LDS; rs_lds (may be NULL) receives the new values.
+// group, in place (newmark2 stiffness kernel).  For each SLS jj (uniform
+// across the element's lanes) the 24 * B entries r = (c*4 + q)*B + b of that
+// SLS are spread over the element's 25 lanes; every entry is read before the
+// same lane overwrites it.  src/old in LDS.
 template <int SRC, int B>
 __device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, int k,
-                                                   const float *mv_old, const float *src_lds,
-                                                   const float *old_lds, float *rs_lds) {
+                                                   const float *src_lds,
+                                                   const float *old_lds) {
   const int nsls = A.nsls;
   float *memvar = A.memvar + (size_t)e * 24 * nsls * B;
   for (int jj = 0; jj < nsls; jj++) {
@@ -480,7 +479,7 @@ __device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, in
       const int q = cq & 3, c = cq >> 2;
       if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
       const int m = jj * 24 * B + r;
-      const double mv = (double)mv_old[m];
+      const double mv = (double)memvar[m];
       const float db = (float)(Tt * amu * (double)src_lds[(4 + cq) * B + b] +
                                Tm * amu * (double)old_lds[(4 + cq) * B + b]);
       float nv;
@@ -492,7 +491,6 @@ __device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, in
         nv = (float)(E * mv + (double)db);
       }
       memvar[m] = nv;
-      if (rs_lds) rs_lds[m] = nv;
     }
   }
 }
@@ -1045,8 +1043,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       }
       __syncthreads();
       if (valid)
-        memvar_update_elem<SRC, B>(A, e, k, A.memvar + (size_t)e * 24 * A.nsls * B, src, old,
-                                   nullptr);
+        memvar_update_elem<SRC, B>(A, e, k, src, old);
     }
   }
 
