@@ -292,6 +292,11 @@ __device__ __forceinline__ float vmb(const float *v, const float *Bop, int j) {
   return s;
 }
 
+// Keeps v computed at this point: the compiler would otherwise sink the
+// arithmetic producing it below the next barrier while the LDS reads feeding
+// it stay above, keeping all their registers live across the barrier.
+__device__ __forceinline__ void pin(float v) { asm volatile("" ::"v"(v)); }
+
 // The same products for the B wavefields of a field tile at once: each step
 // of the k sum reads the B values of one entry together (one LDS access) and
 // the operator entry once; every wavefield's sum runs over k in order.
@@ -721,6 +726,8 @@ void k_solid(SolidArgs A, FluidArgs FA) {
     for (int b = 0; b < B; b++) {
       ls[b] = ls[b] + vY1[b] + vY2[b];
       lz[b] = vY3[b] + vY4[b];
+      pin(ls[b]);
+      pin(lz[b]);
     }
     __syncthreads();
     float V1b[B];
