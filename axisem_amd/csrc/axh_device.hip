@@ -464,42 +464,6 @@ __device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *
   for (int c = 0; c < 6; c++) devm1[(4 * c + q) * B + b] = sdev[c];
 }
 
-// Update of the memory variables of element e for the B wavefields of the
-// group, in place (newmark2 stiffness kernel).  For each SLS jj (uniform
-// across the element's lanes) the 24 * B entries r = (c*4 + q)*B + b of that
-// SLS are spread over the element's 25 lanes; every entry is read before the
-// same lane overwrites it.  src/old in LDS.
-template <int SRC, int B>
-__device__ __forceinline__ void memvar_update_elem(const SolidArgs &A, int e, int k,
-                                                   const float *src_lds,
-                                                   const float *old_lds) {
-  const int nsls = A.nsls;
-  float *memvar = A.memvar + (size_t)e * 24 * nsls * B;
-  for (int jj = 0; jj < nsls; jj++) {
-    const double Tt = A.tst[jj], Tm = A.tstm1[jj], E = A.expw[jj];
-    const double amu = A.aj_mu[(size_t)e * nsls + jj];
-    const double ak = A.aj_k[(size_t)e * nsls + jj];
-    for (int r = k; r < 24 * B; r += 25) {
-      const int cq = r / B, b = r - cq * B;
-      const int q = cq & 3, c = cq >> 2;
-      if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
-      const int m = jj * 24 * B + r;
-      const double mv = (double)memvar[m];
-      const float db = (float)(Tt * amu * (double)src_lds[(4 + cq) * B + b] +
-                               Tm * amu * (double)old_lds[(4 + cq) * B + b]);
-      float nv;
-      if (c < 3) {
-        const float trb = (float)(Tt * ak * (double)src_lds[q * B + b] +
-                                  Tm * ak * (double)old_lds[q * B + b]);
-        nv = (float)((E * mv + (double)db) + (double)trb);
-      } else {
-        nv = (float)(E * mv + (double)db);
-      }
-      memvar[m] = nv;
-    }
-  }
-}
-
 // gather the (predicted, masked) displacement of the tile's points into LDS
 // (monopole: components s and z)
 template <int NCOMP, int B>
@@ -627,6 +591,8 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   float *const sSa = &sS[1][0];
   __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
   __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
+  // ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j per element (mu, then kappa)
+  __shared__ double sCf[MV ? kElemPerTile : 1][MV ? kMaxSls : 1][4];
   __shared__ __align__(16) float sG[105];
 
   if ((int)blockIdx.x >= A.ntile) {  // a fluid tile, in the same LDS
@@ -667,6 +633,24 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       const size_t p = ((size_t)c * A.nglob + gp) * B;
       ldv<B>(A.velo + p, dv[cc]);
       if (MODE == 0) ldv<B>(A.acc0 + p, da[MODE == 0 ? cc : 0]);
+    }
+  }
+  // newmark2 anelastic: lane k < 24 owns entries (k, b) of every SLS, read
+  // now; lane 24 forms the element's SLS coefficient products
+  float mvr[MV ? kMaxSls : 1][B];
+  if (MV && valid && k < 24) {
+    const float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
+#pragma unroll
+    for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++)
+      if (jj < A.nsls) ldv<B>(mv + (jj * 24 + k) * B, mvr[jj]);
+  }
+  if (MV && valid && k == 24) {
+    for (int jj = 0; jj < A.nsls; jj++) {
+      const double amu = A.aj_mu[(size_t)e * A.nsls + jj], ak = A.aj_k[(size_t)e * A.nsls + jj];
+      sCf[MV ? le : 0][MV ? jj : 0][0] = A.tst[jj] * amu;
+      sCf[MV ? le : 0][MV ? jj : 0][1] = A.tstm1[jj] * amu;
+      sCf[MV ? le : 0][MV ? jj : 0][2] = A.tst[jj] * ak;
+      sCf[MV ? le : 0][MV ? jj : 0][3] = A.tstm1[jj] * ak;
     }
   }
   __syncthreads();
@@ -916,11 +900,12 @@ void k_solid(SolidArgs A, FluidArgs FA) {
     float *R = &sR[le * 24 * B];
     if (valid && k < 24) {
       if (MODE == 0) {
-        const float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
 #pragma unroll
         for (int b = 0; b < B; b++) {
           float s = 0.0f;
-          for (int jj = 0; jj < A.nsls; jj++) s = s + mv[(jj * 24 + k) * B + b];
+#pragma unroll
+          for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++)
+            if (jj < A.nsls) s = s + mvr[jj][b];
           R[k * B + b] = s;
         }
       } else {
@@ -1049,8 +1034,39 @@ void k_solid(SolidArgs A, FluidArgs FA) {
         memvar_prepare<SRC, B>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
       }
       __syncthreads();
-      if (valid)
-        memvar_update_elem<SRC, B>(A, e, k, src, old);
+      // time_step_memvars_cg4's update of the entries (k, b) this lane read
+      // at the start (SOLVER/attenuation.f90:130-155); the monopole's
+      // components 3 and 5 are not updated
+      const int c = k >> 2, q = k & 3;
+      if (valid && k < 24 && !(SRC == AXH_MONOPOLE && (c == 3 || c == 5))) {
+        float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
+        double sd[B], od[B], st[B], ot[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+          sd[b] = (double)src[(4 + k) * B + b];
+          od[b] = (double)old[(4 + k) * B + b];
+          st[b] = (double)src[q * B + b];
+          ot[b] = (double)old[q * B + b];
+        }
+#pragma unroll
+        for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++) {
+          if (jj >= A.nsls) break;
+          const double E = A.expw[jj];
+          const double *cf = sCf[MV ? le : 0][MV ? jj : 0];
+          float nv[B];
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            const float db = (float)(cf[0] * sd[b] + cf[1] * od[b]);
+            if (c < 3) {
+              const float trb = (float)(cf[2] * st[b] + cf[3] * ot[b]);
+              nv[b] = (float)((E * (double)mvr[jj][b] + (double)db) + (double)trb);
+            } else {
+              nv[b] = (float)(E * (double)mvr[jj][b] + (double)db);
+            }
+          }
+          stv<B>(mv + (jj * 24 + k) * B, nv);
+        }
+      }
     }
   }
 
