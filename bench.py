@@ -277,6 +277,53 @@ def copy_calibration(nbytes: int, reps: int = 5) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
+# Other source types and time schemes on the 10 s mesh (not the headline
+# workload): plain and 4-wavefield step times, stiffness launch per kernel.
+SWEEP = (("mono_vertforce_newmark", "vertforce", "newmark2", False),
+         ("mono_vertforce_newmark_anel", "vertforce", "newmark2", True),
+         ("mono_explosion_symplec4_anel", "explosion", "symplec4", True),
+         ("dipole_thetaforce_newmark_anel", "thetaforce", "newmark2", True),
+         ("quad_mtp_symplec4_anel", "mtp", "symplec4", True))
+
+
+def sweep(workdir: str, steps: int = 300) -> dict:
+    from axisem_amd.solver import Batch, Solver
+    out = {}
+    for name, src, scheme, anel in SWEEP:
+        paths = []
+        for depth in BATCH_DEPTHS_KM:
+            cfg = inputs.RunConfig(source_type=src, source_depth_km=depth, source_amplitude=1e20,
+                                   time_scheme=scheme, attenuation=anel,
+                                   seismogram_length_s=1800.0,
+                                   stations_deg=[float(k) for k in range(181)])
+            paths.append(prepare_state(os.path.join(workdir, "%s_%g" % (name, depth)), cfg))
+        s = Solver.from_state_file(paths[0])
+        s.run(50)
+        t0 = time.perf_counter()
+        s.run(steps)
+        plain = (time.perf_counter() - t0) / steps * 1e3
+        s.set_profiling(True)
+        s.run(20)
+        ks = kernel_table(s.kernel_stats())
+        s.close()
+        members = [Solver.from_state_file(p) for p in paths]
+        with Batch(members) as b:
+            b.run(50)
+            t0 = time.perf_counter()
+            b.run(steps)
+            bat = (time.perf_counter() - t0) / steps * 1e3
+            b.set_profiling(True)
+            b.run(20)
+            kb = kernel_table(b.kernel_stats())
+        for m in members:
+            m.close()
+        out[name] = {"plain_ms_per_step": plain, "batch4_ms_per_step": bat,
+                     "batch4_vs_plain": 4 * plain / bat,
+                     "stiffness_us": [ks["stiffness"]["avg_us"], kb["stiffness"]["avg_us"]]}
+        log("sweep %s: %s" % (name, json.dumps(out[name])))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,9 +344,14 @@ def main():
     ap.add_argument("--workdir", default=os.path.join(REPO, "data", "bench_10s"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (counter calibration)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only time other source types and schemes (one JSON line)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.sweep:
+        print(json.dumps({"sweep": sweep(os.path.join(REPO, "data", "bench_sweep"))}))
         return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the time loop keeps the whole mesh resident on "
