@@ -59,11 +59,21 @@ BATCH_MEMBERS = {
     "b_mono_explosion_100": RunConfig("explosion", 100.0, 1e20, "newmark2", True, 900.0,
                                       STATIONS),
     "b_mono_vertforce_0": RunConfig("vertforce", 0.0, 2e20, "newmark2", True, 700.0, STATIONS),
+    "b_quad_mtp_100": RunConfig("mtp", 100.0, 1e20, "symplec4", True, 500.0, STATIONS),
+    "b_quad_mtt_m_mpp_1500": RunConfig("mtt_m_mpp", 1500.0, 2e20, "symplec4", True, 500.0,
+                                       STATIONS),
+    "b_quad_mtp_400": RunConfig("mtp", 400.0, 1e20, "symplec4", True, 500.0, STATIONS[1::2]),
+    "b_quad_mtt_m_mpp_20": RunConfig("mtt_m_mpp", 20.0, 1e20, "symplec4", True, 500.0,
+                                     STATIONS),
+    "b_quad_mtp_2600": RunConfig("mtp", 2600.0, 1e20, "symplec4", True, 500.0, STATIONS),
 }
 BATCHES = {
     "dipole_symplec4_anel": ["b_dipole_thetaforce_50", "b_dipole_phiforce_1000",
                              "b_dipole_mtr_700", "b_dipole_mpr_2000"],
     "mono_newmark_anel": ["b_mono_mtt_p_mpp_300", "b_mono_explosion_100", "b_mono_vertforce_0"],
+    # five members: one group of 4 and one of 1 at the default width
+    "quad_symplec4_anel": ["b_quad_mtp_100", "b_quad_mtt_m_mpp_1500", "b_quad_mtp_400",
+                           "b_quad_mtt_m_mpp_20", "b_quad_mtp_2600"],
 }
 # Cases the reference program cannot run (it lower-cases TIME_SCHEME and its
 # symplectic_coefficients then matches neither McLachlan scheme): the GPU is

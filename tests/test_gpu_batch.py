@@ -58,6 +58,27 @@ def test_batch_members_match_solo_and_oracle(batch, state_cache):
         _same_fields(solo[k][1], o.fields(), names[k] + " oracle")
 
 
+@pytest.mark.parametrize("width", [2, 3])
+def test_group_widths_equal_solo(width, state_cache):
+    """Narrower groups (the kernels for 2 and 3 wavefields, and a remainder
+    group) give every member its solo run, bit for bit."""
+    names = cases.BATCHES["dipole_symplec4_anel"]
+    print("group width", width, names, flush=True)
+    paths = [state_cache(n)[1] for n in names]
+    n = 80
+    solo = [_solo(p, n) for p in paths]
+    solvers = [Solver.from_state_file(p) for p in paths]
+    try:
+        with Batch(solvers, group_width=width) as b:
+            out = b.run(n)
+        for k, nm in enumerate(names):
+            np.testing.assert_array_equal(out[k], solo[k][0], err_msg=nm)
+            _same_fields(solvers[k].fields(), solo[k][1], nm)
+    finally:
+        for s in solvers:
+            s.close()
+
+
 def test_batch_continues_solo_runs(state_cache):
     """Members that ran alone first (each on its own device mesh) join one
     batch and continue exactly."""
