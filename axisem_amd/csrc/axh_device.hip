@@ -131,6 +131,7 @@ struct MeshDev {
   int *einfo_s = nullptr, *gid_s = nullptr, *gstart_s = nullptr, *gslot_s = nullptr;
   // fluid
   float *coef_f = nullptr, *m0_f = nullptr, *minv_f = nullptr;
+  float *gamma_s = nullptr, *gamma_f = nullptr;  // absorbing boundaries, or NULL
   int *einfo_f = nullptr, *gid_f = nullptr, *gstart_f = nullptr, *gslot_f = nullptr;
   uint8_t *flags_f = nullptr;
   // coupling
@@ -1298,6 +1299,7 @@ struct AsmSolidArgs {
   const int *gstart, *gslot;
   const uint8_t *flags;
   const float *minv;
+  const float *gamma;    // absorbing boundaries: damping factor per point, or NULL
   double hdt, cv;
   const float *force;    // [3][nslot][B]
   float *velo, *acc0;    // [3][n][B]
@@ -1349,8 +1351,11 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
     const size_t p = ((size_t)c * A.n + g) * B;
     ldv<B>(A.velo + p, v[c]);
     if (A.newmark) ldv<B>(A.acc0 + p, a0[c]);
-    if (A.drift) ldv<B>(A.disp + p, u[c]);
+    if (A.drift || A.gamma) ldv<B>(A.disp + p, u[c]);
   }
+  // absorbing boundaries: a -= 2 gamma v + gamma^2 u after the inverse mass
+  // (SOLVER/time_evol_wave.F90:468-491,690-711), with v and u of this stage
+  const float gm = A.gamma ? A.gamma[g] : 0.0f;
   float s[3][B];
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -1420,13 +1425,15 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
           a[b] = (float)(-2.0 * (double)mi * (double)s[c][b]);
         else
           a[b] = -mi * s[c][b];
+        if (A.gamma) a[b] = (a[b] - (2.0f * gm) * v[c][b]) - (gm * gm) * u[c][b];
         v[c][b] = (float)((double)v[c][b] + A.hdt * (double)(a0[c][b] + a[b]));
       }
       stv<B>(A.acc0 + p, a);
     } else {
 #pragma unroll
       for (int b = 0; b < B; b++) {
-        const float a = -mi * s[c][b];
+        float a = -mi * s[c][b];
+        if (A.gamma) a = (a - (2.0f * gm) * v[c][b]) - (gm * gm) * u[c][b];
         if (c == 2 && A.src == AXH_DIPOLE)
           v[c][b] = (float)((double)v[c][b] + 2.0 * (double)a * A.cv);
         else
@@ -1457,6 +1464,7 @@ struct AsmFluidArgs {
   const int *gstart, *gslot;
   const uint8_t *flags;
   const float *minv;
+  const float *gamma;        // absorbing boundaries: damping factor per point, or NULL
   double hdt, cv;
   const float *force;        // [nslot_f][B]
   float *dchi, *ddchi;       // [n][B]
@@ -1535,28 +1543,32 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
     }
   }
   const float mi = A.minv[g];
-  float d[B], dd[B];
+  float d[B], dd[B], c[B];
   ldv<B>(A.dchi + (size_t)g * B, d);
+  if (A.drift || A.gamma) ldv<B>(A.chi + (size_t)g * B, c);
+  // absorbing boundaries: ddchi -= 2 gamma dchi + gamma^2 chi after the
+  // inverse mass (SOLVER/time_evol_wave.F90:432-434,663-665)
+  const float gm = A.gamma ? A.gamma[g] : 0.0f;
   if (A.newmark) {
     float d0[B];
     ldv<B>(A.ddchi + (size_t)g * B, d0);
 #pragma unroll
     for (int b = 0; b < B; b++) {
       dd[b] = -mi * s[b];
+      if (A.gamma) dd[b] = (dd[b] - (2.0f * gm) * d[b]) - (gm * gm) * c[b];
       d[b] = (float)((double)d[b] + A.hdt * (double)(d0[b] + dd[b]));
     }
   } else {
 #pragma unroll
     for (int b = 0; b < B; b++) {
       dd[b] = -s[b] * mi;
+      if (A.gamma) dd[b] = (dd[b] - (2.0f * gm) * d[b]) - (gm * gm) * c[b];
       d[b] = (float)((double)d[b] + A.cv * (double)dd[b]);
     }
   }
   stv<B>(A.ddchi + (size_t)g * B, dd);
   stv<B>(A.dchi + (size_t)g * B, d);
   if (A.drift) {
-    float c[B];
-    ldv<B>(A.chi + (size_t)g * B, c);
 #pragma unroll
     for (int b = 0; b < B; b++) c[b] = (float)((double)c[b] + (double)d[b] * A.cd_next);
     if (A.drift_mask && (fl & kFlagAxis)) {
@@ -1705,6 +1717,8 @@ struct MeshHost {
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
   std::vector<int> first_order_s;  // solid points in first-appearance order
+  // absorbing boundaries: damping factor per point (empty when there are none)
+  std::vector<float> gamma_s, gamma_f;
   double expw[kMaxSls] = {0}, tst[kMaxSls] = {0}, tstm1[kMaxSls] = {0};
 };
 
@@ -1739,6 +1753,11 @@ static void number_assembled_first(MeshHost &H) {
     gstart[p + 1] = gstart[p] + (q1 - q0);
     minv[p] = H.minv_s[g];
     flags[p] = H.flags_s[g];
+  }
+  if (!H.gamma_s.empty()) {
+    std::vector<float> gm(n);
+    for (int p = 0; p < n; p++) gm[p] = H.gamma_s[inv[p]];
+    H.gamma_s.swap(gm);
   }
   H.gstart_s.swap(gstart);
   H.gslot_s.swap(gslot);
@@ -1778,6 +1797,10 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
                                  d.nglob_solid, "solid");
   H.nglob_s = NS.nglob;
   H.minv_s = per_point(NS, ctx->need("inv_mass_rho").as<float>(), H.nel_s, "inv_mass_rho");
+  const bool absorbing = ctx->get("solid_absorbing_gamma") != nullptr;
+  if (absorbing)
+    H.gamma_s = per_point(NS, ctx->need("solid_absorbing_gamma").as<float>(), H.nel_s,
+                          "solid_absorbing_gamma");
   H.flags_s.assign(NS.nglob, 0);
   std::vector<char> nonaxis_copy(NS.nglob, 0);
   for (int e = 0; e < H.nel_s; e++)
@@ -1845,6 +1868,9 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
     H.nglob_f = NF.nglob;
     H.minv_f = per_point(NF, ctx->need("inv_mass_fluid").as<float>(), H.nel_f,
                          "inv_mass_fluid");
+    if (absorbing)
+      H.gamma_f = per_point(NF, ctx->need("fluid_absorbing_gamma").as<float>(), H.nel_f,
+                            "fluid_absorbing_gamma");
     std::vector<float> surf = per_point(NF, ctx->need("fluid_free_surface_mask").as<float>(),
                                         H.nel_f, "fluid_free_surface_mask");
     H.flags_f.assign(NF.nglob, 0);
@@ -1934,7 +1960,8 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
   // mask, no coupling term; source points are excluded when a wavefield with
   // a source there is attached) ----
   for (int g = 0; g < H.nglob_s; g++)
-    if (H.gstart_s[g + 1] - H.gstart_s[g] == 1 && !(H.flags_s[g] & (kFlagAxis | kFlagCoupled)))
+    if (!absorbing && H.gstart_s[g + 1] - H.gstart_s[g] == 1 &&
+        !(H.flags_s[g] & (kFlagAxis | kFlagCoupled)))
       H.flags_s[g] |= kFlagDirect;
   number_assembled_first(H);
 
@@ -2109,7 +2136,7 @@ void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]) {
   h.bytes(H.tst, sizeof H.tst);
   h.bytes(H.tstm1, sizeof H.tstm1);
   for (const auto *v : {&H.G, &H.coef_s, &H.m0_s, &H.minv_s, &H.coef_f, &H.m0_f, &H.minv_f,
-                        &H.cpl_bm, &H.cg_stage, &H.cg_upd})
+                        &H.cpl_bm, &H.cg_stage, &H.cg_upd, &H.gamma_s, &H.gamma_f})
     h.vec(*v);
   for (const auto *v : {&H.einfo_s, &H.gid_s, &H.gstart_s, &H.gslot_s, &H.einfo_f, &H.gid_f,
                         &H.gstart_f, &H.gslot_f, &H.cpl_ss, &H.cpl_gs, &H.cpl_sf, &H.cpl_gf,
@@ -2171,6 +2198,7 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   V.gstart_s = m.upload(H.gstart_s);
   V.gslot_s = m.upload(H.gslot_s);
   V.minv_s = m.upload(H.minv_s);
+  if (!H.gamma_s.empty()) V.gamma_s = m.upload(H.gamma_s);
   if (V.nel_f > 0) {
     V.coef_f = m.upload(H.coef_f);
     V.einfo_f = m.upload(H.einfo_f);
@@ -2179,6 +2207,7 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
     V.gstart_f = m.upload(H.gstart_f);
     V.gslot_f = m.upload(H.gslot_f);
     V.minv_f = m.upload(H.minv_f);
+    if (!H.gamma_f.empty()) V.gamma_f = m.upload(H.gamma_f);
     V.flags_f = m.upload(H.flags_f);
   }
   if (V.nbdry > 0) {
@@ -2646,6 +2675,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = M.gstart_s;
   AS.npts = F.nasm; AS.pts = F.asm_pts;
   AS.gslot = M.gslot_s; AS.flags = F.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
+  AS.gamma = M.gamma_s;
   AS.force = F.force_s; AS.velo = F.velo; AS.acc0 = F.acc0;
   AS.disp = F.disp; AS.naxb = M.naxb; AS.axb_g = M.axb_g; AS.axb_val = F.axb_val;
   AS.dipole = M.src == AXH_DIPOLE; AS.cpl_slot = M.cpl_slot_s; AS.cpl_gf = M.cpl_gf;
@@ -2654,6 +2684,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   AF.n = M.nglob_f; AF.newmark = newmark; AF.mask_axis = multipole;
   AF.mask_surface = newmark; AF.gstart = M.gstart_f; AF.gslot = M.gslot_f;
   AF.flags = M.flags_f; AF.minv = M.minv_f; AF.hdt = hdt; AF.force = F.force_f;
+  AF.gamma = M.gamma_f;
   AF.dchi = F.dchi; AF.ddchi = F.ddchi; AF.chi = F.chi;
   AF.dipole = M.src == AXH_DIPOLE; AF.nglob_s = M.nglob_s; AF.cpl_slot = M.cpl_slot_f;
   AF.cpl_gs = M.cpl_gs; AF.cpl_axb = M.cpl_axb; AF.cpl_bm = M.cpl_bm; AF.disp = F.disp;

@@ -262,6 +262,19 @@ int axh_upload_fluid_surface_mask(axh_ctx *ctx, const float *mask) {
   });
 }
 
+int axh_upload_absorbing(axh_ctx *ctx, const float *gamma_solid, const float *gamma_fluid) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!gamma_solid || (ctx->dims.nel_fluid > 0 && !gamma_fluid))
+      fail(AXH_ERR_ARG, "absorbing boundaries need the solid and fluid damping factors");
+    ctx->put("solid_absorbing_gamma", kF32, {5, 5, (uint64_t)ctx->dims.nel_solid},
+             gamma_solid);
+    if (ctx->dims.nel_fluid > 0)
+      ctx->put("fluid_absorbing_gamma", kF32, {5, 5, (uint64_t)ctx->dims.nel_fluid},
+               gamma_fluid);
+  });
+}
+
 int axh_upload_source(axh_ctx *ctx, const int32_t *ielsrc,
                       const float *source_term, const float *stf, int32_t nstf) {
   return guarded([&] {

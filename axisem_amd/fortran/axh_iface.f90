@@ -104,6 +104,13 @@ module axh_iface
        real(c_float), intent(in) :: mask(*)
      end function
 
+     integer(c_int) function axh_upload_absorbing(ctx, gsol, gflu) &
+          bind(C, name='axh_upload_absorbing')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: gsol(*), gflu(*)
+     end function
+
      integer(c_int) function axh_upload_source(ctx, ielsrc, source_term, stf, nstf) &
           bind(C, name='axh_upload_source')
        import :: c_int, c_ptr, c_int32_t, c_float

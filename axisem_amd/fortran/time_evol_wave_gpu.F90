@@ -64,7 +64,7 @@ subroutine prepare_waves
   call compute_stf
   call compute_src
   call create_fluid_free_surface_mask()
-  call check_absorbing_boundaries()
+  call create_absorbing_gamma()
 
   call prepare_seismograms
   call open_hyp_epi_equ_anti
@@ -250,23 +250,77 @@ end subroutine create_fluid_free_surface_mask
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
-!> The absorbing boundary test of create_absorbing_gamma
-!! (SOLVER/time_evol_wave.F90:1633-1650): a mesh that does not reach the antipode has
-!! absorbing boundaries, which the GPU loop does not implement.
-subroutine check_absorbing_boundaries()
+!> create_absorbing_gamma (SOLVER/time_evol_wave.F90:1634-1713, private there): a mesh
+!! that does not reach the antipode gets absorbing boundaries; the damping factor rises
+!! as U0 (1 - sin^2(pi d / 2D)) within D = 15 h_max of the bottom or the lateral edge.
+subroutine create_absorbing_gamma()
 
-  use commun, only : pmax
-  real(kind=dp) :: thetamax
+  use utlity, only : rcoord, thetacoord, scoord, zcoord
+  use commun, only : pmax, pmin
+  integer            :: iel, ipol, jpol
+  integer, parameter :: num_elem = 15
+  real(kind=dp)      :: hmax, U0, distance_factor, rmin, thetamax, dist, r, t
 
   thetamax = maxval(atan2(crd_nodes(:, 1), crd_nodes(:, 2)))
   thetamax = pmax(thetamax)
   have_absorbing_bc = thetamax < pi - 1e-7
-  if (have_absorbing_bc) then
-     write(6,*) 'GPU time loop: unsupported option: absorbing boundaries (LOCAL_MAX_COLAT)'
-     stop 2
+  if (.not. have_absorbing_bc) return
+
+  hmax = 0.
+  do iel = 1, nelem
+     hmax = max(hmax, &
+        (scoord(0,0,iel) - scoord(0,npol,iel)) ** 2 + &
+           (zcoord(0,0,iel) - zcoord(0,npol,iel)) ** 2, &
+        (scoord(0,npol,iel) - scoord(npol,npol,iel)) ** 2 + &
+           (zcoord(0,npol,iel) - zcoord(npol,npol,iel)) ** 2, &
+        (scoord(npol,npol,iel) - scoord(npol,0,iel)) ** 2 + &
+           (zcoord(npol,npol,iel) - zcoord(npol,0,iel)) ** 2, &
+        (scoord(npol,0,iel) - scoord(0,0,iel)) ** 2 + &
+           (zcoord(npol,0,iel) - zcoord(0,0,iel)) ** 2)
+  enddo
+  hmax = hmax ** 0.5
+  rmin = dsqrt(minval(crd_nodes(:, 1) ** 2 + crd_nodes(:, 2) ** 2))
+  hmax = pmax(hmax)
+  rmin = pmin(rmin)
+
+  allocate(fluid_absorbing_gamma(0:npol, 0:npol, nel_fluid))
+  allocate(solid_absorbing_gamma(0:npol, 0:npol, nel_solid))
+  fluid_absorbing_gamma = 0
+  solid_absorbing_gamma = 0
+  distance_factor = num_elem * hmax
+  U0 = vpmax / (2 * hmax)
+  if (lpr) then
+     write(6,*) 'Absorbing Boundary Parameters:'
+     write(6,*) 'distance_factor: ', distance_factor
+     write(6,*) 'U0:              ', U0
   endif
 
-end subroutine check_absorbing_boundaries
+  do iel = 1, nel_fluid
+     do ipol = 0, npol
+        do jpol = 0, npol
+           r = rcoord(ipol, jpol, ielfluid(iel))
+           t = thetacoord(ipol, jpol, ielfluid(iel))
+           dist = min(r - rmin, (thetamax - t) * r)
+           if (dist < distance_factor) &
+              fluid_absorbing_gamma(ipol, jpol, iel) = &
+                 U0 * (1. - sin(pi * dist / (2 * distance_factor)) ** 2)
+        enddo
+     enddo
+  enddo
+  do iel = 1, nel_solid
+     do ipol = 0, npol
+        do jpol = 0, npol
+           r = rcoord(ipol, jpol, ielsolid(iel))
+           t = thetacoord(ipol, jpol, ielsolid(iel))
+           dist = min(r - rmin, (thetamax - t) * r)
+           if (dist < distance_factor) &
+              solid_absorbing_gamma(ipol, jpol, iel) = &
+                 U0 * (1. - sin(pi * dist / (2 * distance_factor)) ** 2)
+        enddo
+     enddo
+  enddo
+
+end subroutine create_absorbing_gamma
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
@@ -419,7 +473,12 @@ subroutine upload_all()
         call axh_check(axh_upload_fluid_term(ctx, AXH_M0_W_FL, M0_w_fl), 'M0_w_fl')
      call axh_check(axh_upload_mass(ctx, inv_mass_rho, inv_mass_fluid), 'mass')
      call axh_check(axh_upload_fluid_surface_mask(ctx, fluid_free_surface_mask), 'fsmask')
+     if (have_absorbing_bc) &
+        call axh_check(axh_upload_absorbing(ctx, solid_absorbing_gamma, &
+                                            fluid_absorbing_gamma), 'absorbing')
   else
+     if (have_absorbing_bc) &
+        call axh_check(axh_upload_absorbing(ctx, solid_absorbing_gamma, empty), 'absorbing')
      call axh_check(axh_upload_mass(ctx, inv_mass_rho, empty), 'mass')
   endif
 

@@ -29,7 +29,9 @@ class RunConfig:
 
 
 def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
-                       coarsening_layers: int = 3) -> None:
+                       coarsening_layers: int = 3, max_colat: float | None = None) -> None:
+    """inparam_mesh of the reference MESHER; max_colat (LOCAL_MAX_COLAT, degrees)
+    gives a regional mesh with absorbing boundaries."""
     with open(path, "w") as f:
         f.write("BACKGROUND_MODEL    %s\n" % model)
         f.write("DOMINANT_PERIOD     %.1f\n" % period_s)
@@ -37,15 +39,19 @@ def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
         f.write("ONLY_SUGGEST_NTHETA false\nWRITE_VTK           false\n")
         f.write("WRITE_1DMODEL       false\n")
         f.write("COARSENING_LAYERS   %d\n" % coarsening_layers)
+        if max_colat is not None:
+            f.write("LOCAL_MAX_COLAT     %.1f\n" % max_colat)
 
 
-def run_mesher(xmesh: str, outdir: str, period_s: float, model: str = "prem_iso") -> str:
+def run_mesher(xmesh: str, outdir: str, period_s: float, model: str = "prem_iso",
+               max_colat: float | None = None) -> str:
     """Run the reference MESHER binary into outdir; returns the meshdb path."""
     import subprocess
     import resource
 
     os.makedirs(os.path.join(outdir, "Diags"), exist_ok=True)
-    write_mesh_inparam(os.path.join(outdir, "inparam_mesh"), period_s, model)
+    write_mesh_inparam(os.path.join(outdir, "inparam_mesh"), period_s, model,
+                       max_colat=max_colat)
 
     def unlimited_stack():
         resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))

@@ -77,6 +77,7 @@ def library():
         L.axh_upload_mesh_maps.argtypes = [P, P, P, P, P]
         L.axh_upload_boundary.argtypes = [P, P, P, P, P, P]
         L.axh_upload_fluid_surface_mask.argtypes = [P, P]
+        L.axh_upload_absorbing.argtypes = [P, P, P]
         L.axh_upload_source.argtypes = [P, P, P, P, C.c_int32]
         L.axh_upload_attenuation_cg4.argtypes = [P, C.POINTER(Att)]
         L.axh_upload_receivers.argtypes = [P, P]
@@ -188,6 +189,10 @@ class Solver:
                 K(f32("bdry_matr"))))
         if "fluid_free_surface_mask" in st.arrays:
             _check(L.axh_upload_fluid_surface_mask(h, K(f32("fluid_free_surface_mask"))))
+        if "solid_absorbing_gamma" in st.arrays:
+            gf = (K(f32("fluid_absorbing_gamma")) if "fluid_absorbing_gamma" in st.arrays
+                  else None)
+            _check(L.axh_upload_absorbing(h, K(f32("solid_absorbing_gamma")), gf))
         stf = f32("stf")
         _check(L.axh_upload_source(h, K(i32("ielsrc")), K(f32("source_term")), K(stf),
                                    int(stf.size)))

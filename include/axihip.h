@@ -144,6 +144,17 @@ int axh_upload_boundary(axh_ctx *ctx, const int32_t *bdry_solid_el,
  * built by create_fluid_free_surface_mask (SOLVER/time_evol_wave.F90:1615). */
 int axh_upload_fluid_surface_mask(axh_ctx *ctx, const float *mask);
 
+/* Absorbing boundaries of a mesh that does not reach the antipode
+ * (LOCAL_MAX_COLAT): solid_absorbing_gamma(0:4,0:4,nel_solid) and
+ * fluid_absorbing_gamma(0:4,0:4,nel_fluid) (SOLVER/data_mesh.f90:184-187;
+ * may be NULL when nel_fluid is 0), as create_absorbing_gamma builds them
+ * (SOLVER/time_evol_wave.F90:1634-1713).  The loop then adds
+ * -2 gamma v - gamma^2 u to every acceleration after the inverse mass
+ * (SOLVER/time_evol_wave.F90:432-434,468-491,663-711).  Not uploading them
+ * means no absorbing boundaries. */
+int axh_upload_absorbing(axh_ctx *ctx, const float *gamma_solid,
+                         const float *gamma_fluid);
+
 /* Source (SOLVER/data_source.f90:47-49): ielsrc(nelsrc) 1-based, the element
  * source term source_term_el(0:4,0:4,8,3) as compute_src allocates it
  * (SOLVER/source.f90:376; source_term_fl(0:4,0:4,8) when fluid_src), and the

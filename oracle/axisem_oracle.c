@@ -56,6 +56,8 @@ typedef struct orc {
   const float *M0[10];   /* axial terms M0_w1..M0_w10               */
   const float *Mfl[4], *M0fl;
   const float *inv_mass_rho, *inv_mass_fluid;
+  /* absorbing boundaries (NULL when the mesh reaches the antipode) */
+  const float *gamma_s, *gamma_f;
   const int32_t *igloc_s, *igloc_f, *axis_s, *axis_f;
   const int32_t *bdry_solid_el, *bdry_fluid_el, *bdry_jpol_solid,
       *bdry_jpol_fluid;
@@ -878,6 +880,16 @@ static int blown_up(orc_t *o) {
 }
 
 /* One newmark2 step, sf_time_loop_newmark (SOLVER/time_evol_wave.F90:353-495) */
+/* Absorbing boundaries after the inverse mass (SOLVER/time_evol_wave.F90:432-434,
+ * 468-491, 663-665, 690-711): a = a - 2*gamma*v - gamma**2*u in FP32, left to right
+ * ((2*gamma)*v first, gamma**2 = gamma*gamma), for n element-local points. */
+static void absorb(float *a, const float *gamma, const float *v, const float *u, size_t n) {
+  for (size_t p = 0; p < n; p++) {
+    const float g = gamma[p];
+    a[p] = (a[p] - (2.0f * g) * v[p]) - (g * g) * u[p];
+  }
+}
+
 static void newmark_step(orc_t *o) {
   const size_t ns = (size_t)o->nel_s * NPP, nf = (size_t)o->nel_f * NPP;
   /* half_dt, half_dt_sq as parameters.F90:1124-1125 */
@@ -903,6 +915,7 @@ static void newmark_step(orc_t *o) {
   solid_stiffness(o, o->disp, o->acc1);
   if (o->anel) anel_stiff_cg4(o, o->acc1);
   for (size_t p = 0; p < nf; p++) o->ddchi1[p] = -o->inv_mass_fluid[p] * o->ddchi1[p];
+  if (o->gamma_f) absorb(o->ddchi1, o->gamma_f, o->dchi, o->chi, nf);
   copy2solid(o, o->acc1, o->ddchi1);
   solid_mask(o, o->acc1);
   pdistsum(o->igloc_s, o->gvec_s, o->nglob_s, o->acc1, o->nel_s, 3);
@@ -923,6 +936,7 @@ static void newmark_step(orc_t *o) {
         a1[p] = (float)(-2.0 * (double)o->inv_mass_rho[p] * (double)a1[p]);
       else
         a1[p] = -o->inv_mass_rho[p] * a1[p];
+      if (o->gamma_s) absorb(a1 + p, o->gamma_s + p, v + p, o->disp + c * ns + p, 1);
       v[p] = (float)((double)v[p] + hdt * (double)(a0[p] + a1[p]));
       a0[p] = a1[p];
     }
@@ -952,6 +966,7 @@ static void symplectic_step(orc_t *o) {
     solid_stiffness(o, o->disp, acc);
     if (o->anel) anel_stiff_cg4(o, acc);
     for (size_t p = 0; p < nf; p++) ddchi[p] = -ddchi[p] * o->inv_mass_fluid[p];
+    if (o->gamma_f) absorb(ddchi, o->gamma_f, o->dchi, o->chi, nf);
     copy2solid(o, acc, ddchi);
     solid_mask(o, acc);
     pdistsum(o->igloc_s, o->gvec_s, o->nglob_s, acc, o->nel_s, 3);
@@ -965,6 +980,7 @@ static void symplectic_step(orc_t *o) {
       float *a = acc + c * ns, *v = o->velo + c * ns;
       for (size_t p = 0; p < ns; p++) {
         a[p] = -o->inv_mass_rho[p] * a[p];
+        if (o->gamma_s) absorb(a + p, o->gamma_s + p, v + p, o->disp + c * ns + p, 1);
         if (c == 2 && o->src_type == DI)
           v[p] = (float)((double)v[p] + 2.0 * (double)a[p] * cv);
         else
@@ -1056,6 +1072,8 @@ orc_t *orc_load(const char *path) {
   o->M0fl = find(o, "M0_w_fl");
   o->inv_mass_rho = find(o, "inv_mass_rho");
   o->inv_mass_fluid = find(o, "inv_mass_fluid");
+  o->gamma_s = find(o, "solid_absorbing_gamma");
+  o->gamma_f = find(o, "fluid_absorbing_gamma");
   o->igloc_s = find(o, "igloc_solid"); o->igloc_f = find(o, "igloc_fluid");
   o->axis_s = find(o, "axis_solid"); o->axis_f = find(o, "axis_fluid");
   o->bdry_solid_el = find(o, "bdry_solid_el");

@@ -37,7 +37,7 @@ def state_cache(tmp_path_factory, host_exe):
     def get(name):
         if name not in done:
             rundir = str(base / name)
-            inputs.write_run_dir(rundir, cases.MESH_DIR, cases.ALL_CONFIGS[name])
+            inputs.write_run_dir(rundir, cases.mesh_of(name), cases.ALL_CONFIGS[name])
             st = os.path.join(rundir, "state.bin")
             inputs.record_state(host_exe, rundir, st)
             done[name] = (rundir, st)

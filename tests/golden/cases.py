@@ -15,6 +15,10 @@ from axisem_amd.inputs import RunConfig  # noqa: E402
 
 MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s")
 MESH_PERIOD = 200.0
+# a regional mesh (colatitude 0-60 degrees): absorbing boundaries
+REGIONAL_MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s_colat60")
+REGIONAL_MAX_COLAT = 60.0
+REGIONAL_STATIONS = [2.0 * k for k in range(30)]
 STATIONS = [10.0 * k for k in range(19)]
 SAMPLE_STRIDE = 5          # every 5th seismogram sample is stored
 
@@ -40,6 +44,12 @@ CASES = {
     # quadrupole Mtt-Mpp at depth, symplectic, anelastic
     "quad_mttmmpp_symplec4_anel": RunConfig("mtt_m_mpp", 600.0, 1e20, "symplec4", True, 600.0,
                                             STATIONS),
+    # regional mesh with absorbing boundaries: monopole, Newmark, elastic; and
+    # dipole, symplectic, anelastic
+    "reg_mono_vertforce_newmark": RunConfig("vertforce", 10.0, 1e20, "newmark2", False, 900.0,
+                                            REGIONAL_STATIONS),
+    "reg_dipole_thetaforce_symplec4_anel": RunConfig("thetaforce", 30.0, 1e20, "symplec4", True,
+                                                     600.0, REGIONAL_STATIONS),
 }
 
 # Source batches: several sources on the mesh above with one source type and
@@ -89,6 +99,11 @@ ALL_CONFIGS = {**CASES, **BATCH_MEMBERS, **ORACLE_CASES}
 
 NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
          "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}
+
+
+def mesh_of(name: str) -> str:
+    """The mesh directory a case runs on."""
+    return REGIONAL_MESH_DIR if name.startswith("reg_") else MESH_DIR
 
 
 def case_dir(name: str) -> str:

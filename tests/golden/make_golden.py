@@ -3,7 +3,8 @@
 Runs in the build container only (needs /root/reference for the reference
 programs built by oracle/build_ref.py):
 
-1. the reference MESHER -> tests/golden/mesh_prem_iso_200s/ (meshdb.dat0000);
+1. the reference MESHER -> tests/golden/mesh_prem_iso_200s/ (meshdb.dat0000),
+   and the regional mesh (LOCAL_MAX_COLAT 60) -> mesh_prem_iso_200s_colat60/;
 2. for every case, the unmodified reference SOLVER (oracle/_ref/axisem) on the
    case's inputs -> receiver seismograms, stored as float32 (the reference
    prints them with round-trip precision) every SAMPLE_STRIDE samples in
@@ -39,12 +40,12 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def make_mesh():
-    os.makedirs(cases.MESH_DIR, exist_ok=True)
+def make_mesh(mesh_dir, max_colat=None):
+    os.makedirs(mesh_dir, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
-        inputs.run_mesher(XMESH, tmp, cases.MESH_PERIOD)
+        inputs.run_mesher(XMESH, tmp, cases.MESH_PERIOD, max_colat=max_colat)
         for fn in ("meshdb.dat0000", "mesh_params.h"):
-            shutil.copy(os.path.join(tmp, fn), cases.MESH_DIR)
+            shutil.copy(os.path.join(tmp, fn), mesh_dir)
 
 
 def run_reference(name: str):
@@ -52,7 +53,7 @@ def run_reference(name: str):
     out = cases.case_dir(name)
     os.makedirs(out, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
-        inputs.write_run_dir(tmp, cases.MESH_DIR, cfg)
+        inputs.write_run_dir(tmp, cases.mesh_of(name), cfg)
         with open(os.path.join(tmp, "OUTPUT"), "w") as log:
             subprocess.run([AXISEM], cwd=tmp, stdout=log, stderr=subprocess.STDOUT, check=True,
                            preexec_fn=_unlimited_stack, timeout=3600)
@@ -72,7 +73,9 @@ def run_reference(name: str):
 
 def main(argv):
     if not os.path.exists(os.path.join(cases.MESH_DIR, "meshdb.dat0000")):
-        make_mesh()
+        make_mesh(cases.MESH_DIR)
+    if not os.path.exists(os.path.join(cases.REGIONAL_MESH_DIR, "meshdb.dat0000")):
+        make_mesh(cases.REGIONAL_MESH_DIR, cases.REGIONAL_MAX_COLAT)
     names = argv or list(cases.CASES)
     for n in names:
         run_reference(n)

@@ -33,7 +33,7 @@ def test_dropin_host_matches_reference(name, host_exe, tmp_path):
     print("drop-in GPU run:", name, flush=True)
     cfg = cases.CASES[name]
     rundir = str(tmp_path / name)
-    inputs.write_run_dir(rundir, cases.MESH_DIR, cfg)
+    inputs.write_run_dir(rundir, cases.mesh_of(name), cfg)
     with open(os.path.join(rundir, "OUTPUT"), "w") as log:
         r = subprocess.run([host_exe], cwd=rundir, stdout=log, stderr=subprocess.STDOUT,
                            timeout=300, preexec_fn=_unlimited_stack)
