@@ -134,6 +134,13 @@ module axh_iface
        real(c_float), intent(out) :: seis(*)
      end function
 
+     integer(c_int) function axh_download_fields(ctx, disp, velo, chi, dchi, ddchi) &
+          bind(C, name='axh_download_fields')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: disp(*), velo(*), chi(*), dchi(*), ddchi(*)
+     end function
+
      integer(c_int) function axh_check_inputs(ctx) bind(C, name='axh_check_inputs')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -9,6 +9,9 @@
 !! (SOLVER/seismograms.f90:742-780) and the generic epicentre/equator/antipode traces
 !! (SOLVER/seismograms.f90:688-735).
 !!
+!! Wavefield snapshots (SAVE_SNAPSHOTS, xdmf and vtk) are written by the reference's own
+!! writers from the fields copied back at every snap_it-th step.
+!!
 !! Environment switches of the host:
 !!   AXH_SAVE_STATE=<path>  write the uploads to a state file (axh_save_state)
 !!   AXH_NO_RUN=1           stop after prepare/upload (records states without a GPU)
@@ -33,6 +36,7 @@ module time_evol_wave
   integer,     save :: nrec_all = 0          !< receivers + generic traces
   integer,     save :: iepi = 0, ianti = 0, iequ = 0   !< rows of the generic traces
   logical,     save :: no_run = .false.
+  integer,     save :: isnap = 0             !< snapshots written (dump_stuff's isnap)
 
 contains
 
@@ -65,6 +69,8 @@ subroutine prepare_waves
   call compute_src
   call create_fluid_free_surface_mask()
   call create_absorbing_gamma()
+
+  call prepare_snapshots()
 
   call prepare_seismograms
   call open_hyp_epi_equ_anti
@@ -109,7 +115,10 @@ subroutine time_loop
      deallocate(seis)
   endif
 
-  ! chunks of at most 512 seismogram samples
+  ! dump_stuff(0, ...) also writes the snapshot of the initial state
+  if (dump_vtk .or. dump_xdmf) call write_snapshot(0.0_dp)
+
+  ! chunks of at most 512 seismogram samples, ending at the snapshot steps
   chunk = max(1, min(niter, 512 * seis_it))
   allocate(seis(3 * max(nrec_all, 1) * (chunk / seis_it + 1)))
   tloc = zero
@@ -117,6 +126,7 @@ subroutine time_loop
   it0 = 1
   do while (it0 <= niter)
      it1 = min(niter, it0 + chunk - 1)
+     if (dump_vtk .or. dump_xdmf) it1 = min(it1, (it0 + snap_it - 1) / snap_it * snap_it)
      ierr = axh_run(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis)
      if (ierr == AXH_ERR_BLOWUP) then
         write(6,*) procstrg, '!!!!!!!!!!!!!!! DISPLACEMENTS BLEW UP !!!!!!!!!!!!!!!'
@@ -133,6 +143,10 @@ subroutine time_loop
            call write_samples(seis(1 + 3 * nrec_all * (nsamp - 1):), 1, tloc)
         endif
      enddo
+     ! (nested: Fortran need not short-circuit, and snap_it is 0 without snapshots)
+     if (dump_vtk .or. dump_xdmf) then
+        if (mod(it1, snap_it) == 0) call write_snapshot(tloc)
+     endif
      if (lpr) then
         write(6,13) it1, tloc, real(it1) / real(niter) * 100.
         call flush(6)
@@ -148,6 +162,82 @@ subroutine time_loop
   ctx = c_null_ptr
 
 end subroutine time_loop
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Snapshot grids and files of prepare_waves (SOLVER/time_evol_wave.F90:117-175): the
+!! vtk midpoint grid, and the xdmf grid with its stream files (big endian, appended).
+subroutine prepare_snapshots()
+
+  use data_io
+  use meshes_io, only : dump_glob_grid_midpoint, dump_xdmf_grid
+  character(len=512) :: fname
+
+  if (dump_vtk) then
+     if (lpr) write(6,*)'  dumping global grids for snapshots...'
+     call dump_glob_grid_midpoint(ibeg, iend, jbeg, jend)
+  endif
+  if (dump_xdmf) then
+     if (lpr) write(6,*)'  dumping mesh for xdmf snapshots...'
+     fname = datapath(1:lfdata)//'/xdmf_snap_s_' //appmynum//'.dat'
+     open(13100, file=trim(fname), access='stream', status='unknown', &
+          form='unformatted', convert='big_endian', position='append')
+     if (.not. src_type(1) == 'monopole') then
+        fname = datapath(1:lfdata)//'/xdmf_snap_p_' //appmynum//'.dat'
+        open(13101, file=trim(fname), access='stream', status='unknown', &
+             form='unformatted', convert='big_endian', position='append')
+     endif
+     fname = datapath(1:lfdata)//'/xdmf_snap_z_' //appmynum//'.dat'
+     open(13102, file=trim(fname), access='stream', status='unknown', &
+          form='unformatted', convert='big_endian', position='append')
+     fname = datapath(1:lfdata)//'/xdmf_snap_trace_' //appmynum//'.dat'
+     open(13103, file=trim(fname), access='stream', status='unknown', &
+          form='unformatted', convert='big_endian', position='append')
+     fname = datapath(1:lfdata)//'/xdmf_snap_curlip_' //appmynum//'.dat'
+     open(13104, file=trim(fname), access='stream', status='unknown', &
+          form='unformatted', convert='big_endian', position='append')
+     call dump_xdmf_grid()
+  endif
+
+end subroutine prepare_snapshots
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> The snapshot part of dump_stuff (SOLVER/time_evol_wave.F90:1152-1177) for the
+!! current fields, copied back from the GPU (all zero before the first step).
+subroutine write_snapshot(time)
+
+  use data_io
+  use wavefields_io, only : glob_snapshot_midpoint, glob_snapshot_xdmf
+  real(kind=dp), intent(in)        :: time
+  real(kind=realkind), allocatable :: disp(:,:,:,:), velo(:,:,:,:), chi(:,:,:)
+  real(kind=realkind), allocatable :: dchi(:,:,:), ddchi(:,:,:)
+
+  allocate(disp(0:npol, 0:npol, nel_solid, 3), velo(0:npol, 0:npol, nel_solid, 3))
+  allocate(chi(0:npol, 0:npol, max(nel_fluid, 1)), dchi(0:npol, 0:npol, max(nel_fluid, 1)))
+  allocate(ddchi(0:npol, 0:npol, max(nel_fluid, 1)))
+  call axh_check(axh_download_fields(ctx, disp, velo, chi, dchi, ddchi), 'download fields')
+  if (dump_vtk) then
+     isnap = isnap + 1
+     if (lpr) then
+        write(6,*)
+        write(6,*) 'Writing global snap to file: ', isnap
+        write(6,*)
+     endif
+     call glob_snapshot_midpoint(disp, chi(:,:,1:nel_fluid), ibeg, iend, jbeg, jend, isnap)
+  endif
+  if (dump_xdmf) then
+     if (.not. dump_vtk) isnap = isnap + 1
+     if (lpr) then
+        write(6,*)
+        write(6,*)'Writing global xdmf snap to file:',isnap
+        write(6,*)
+     endif
+     call glob_snapshot_xdmf(disp, chi(:,:,1:nel_fluid), time, isnap)
+  endif
+  deallocate(disp, velo, chi, dchi, ddchi)
+
+end subroutine write_snapshot
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
@@ -217,7 +307,6 @@ subroutine check_supported_pre()
   if (npol /= 4)        why = 'npol /= 4'
   if (dump_energy)      why = 'SAVE_ENERGY'
   if (dump_wavefields)  why = 'KERNEL_WAVEFIELDS'
-  if (dump_vtk .or. dump_xdmf) why = 'SAVE_SNAPSHOTS'
   if (use_netcdf)       why = 'USE_NETCDF (ASCII seismograms only)'
   if (why /= ' ') then
      write(6,*) 'GPU time loop: unsupported option: ', trim(why)

@@ -26,6 +26,8 @@ class RunConfig:
     source_function: str = "gauss_0"
     n_sls: int = 5
     coarse_grained: bool = True
+    snapshots: str = ""                 # "", "xdmf", "vtk" or "both" (SAVE_SNAPSHOTS)
+    snapshot_dt: float = 20.0           # SNAPSHOT_DT, seconds
 
 
 def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
@@ -88,7 +90,7 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
         f.write("RECFILE_TYPE      stations\nMESHNAME           Mesh\n")
         f.write("LAT_HETEROGENEITY  false\n")
         f.write("ATTENUATION         %s\n" % ("true" if cfg.attenuation else "false"))
-        f.write("SAVE_SNAPSHOTS         false\n")
+        f.write("SAVE_SNAPSHOTS         %s\n" % ("true" if cfg.snapshots else "false"))
     with open(os.path.join(rundir, "inparam_advanced"), "w") as f:
         f.write("SAMPLING_PERIOD     0.0\nTIME_STEP           0.0\nSOURCE_PERIOD       0.0\n")
         f.write("SOURCE_FUNCTION     %s\n" % cfg.source_function)
@@ -100,6 +102,10 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
         f.write("SAVE_ENERGY         .false.\n")
         f.write("NR_LIN_SOLIDS       %d\n" % cfg.n_sls)
         f.write("COARSE_GRAINED      %s\n" % ("true" if cfg.coarse_grained else "false"))
+        if cfg.snapshots:
+            f.write("SNAPSHOT_DT         %.3f\n" % cfg.snapshot_dt)
+            f.write("SNAPSHOTS_FORMAT    %s\n" % cfg.snapshots)
+            f.write("XDMF_GLL_I       '3 0 2 4'\nXDMF_GLL_J       '3 0 2 4'\n")
     with open(os.path.join(rundir, "inparam_source"), "w") as f:
         f.write("SOURCE_TYPE       %s\n" % cfg.source_type)
         f.write("SOURCE_DEPTH      %.3f\n" % cfg.source_depth_km)

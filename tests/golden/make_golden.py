@@ -71,14 +71,44 @@ def run_reference(name: str):
     print(name, "samples", nsamp, "receivers", seis.shape[1], "components", ncomp)
 
 
+def snapshot_hashes(data_dir: str) -> dict:
+    import hashlib
+    out = {}
+    for fn in sorted(os.listdir(data_dir)):
+        if fn.startswith(cases.SNAPSHOT_FILES):
+            with open(os.path.join(data_dir, fn), "rb") as f:
+                out[fn] = hashlib.sha256(f.read()).hexdigest()
+    return out
+
+
+def run_snapshots(name: str):
+    """The reference run with snapshots: SHA-256 of every snapshot file."""
+    import json
+    cfg = cases.SNAPSHOT_CASES[name]
+    out = cases.case_dir(name)
+    os.makedirs(out, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        inputs.write_run_dir(tmp, cases.mesh_of(name), cfg)
+        with open(os.path.join(tmp, "OUTPUT"), "w") as log:
+            subprocess.run([AXISEM], cwd=tmp, stdout=log, stderr=subprocess.STDOUT, check=True,
+                           preexec_fn=_unlimited_stack, timeout=3600)
+        hashes = snapshot_hashes(os.path.join(tmp, "Data"))
+    with open(os.path.join(out, "snapshots.json"), "w") as f:
+        json.dump(hashes, f, indent=1, sort_keys=True)
+    print(name, len(hashes), "snapshot files")
+
+
 def main(argv):
     if not os.path.exists(os.path.join(cases.MESH_DIR, "meshdb.dat0000")):
         make_mesh(cases.MESH_DIR)
     if not os.path.exists(os.path.join(cases.REGIONAL_MESH_DIR, "meshdb.dat0000")):
         make_mesh(cases.REGIONAL_MESH_DIR, cases.REGIONAL_MAX_COLAT)
-    names = argv or list(cases.CASES)
+    names = argv or list(cases.CASES) + list(cases.SNAPSHOT_CASES)
     for n in names:
-        run_reference(n)
+        if n in cases.SNAPSHOT_CASES:
+            run_snapshots(n)
+        else:
+            run_reference(n)
 
 
 if __name__ == "__main__":
