@@ -10,7 +10,8 @@
 !! (SOLVER/seismograms.f90:688-735).
 !!
 !! Wavefield snapshots (SAVE_SNAPSHOTS, xdmf and vtk) are written by the reference's own
-!! writers from the fields copied back at every snap_it-th step.
+!! writers from the fields copied back at every snap_it-th step; SAVE_ENERGY copies the
+!! fields back after every step (the reference computes the energies every step).
 !!
 !! Environment switches of the host:
 !!   AXH_SAVE_STATE=<path>  write the uploads to a state file (axh_save_state)
@@ -70,6 +71,16 @@ subroutine prepare_waves
   call create_fluid_free_surface_mask()
   call create_absorbing_gamma()
 
+  ! energy files of prepare_waves (SOLVER/time_evol_wave.F90:102-110)
+  if (dump_energy .and. lpr) then
+     write(6,*)'  opening files for kinetic/potential energy...'
+     if (have_fluid) then
+        open(unit=4444, file=datapath(1:lfdata)//'/energy_sol.dat')
+        open(unit=4445, file=datapath(1:lfdata)//'/energy_flu.dat')
+     endif
+     open(unit=4446, file=datapath(1:lfdata)//'/energy_glob.dat')
+  endif
+
   call prepare_snapshots()
 
   call prepare_seismograms
@@ -115,11 +126,14 @@ subroutine time_loop
      deallocate(seis)
   endif
 
-  ! dump_stuff(0, ...) also writes the snapshot of the initial state
+  ! dump_stuff(0, ...) also writes the snapshot and the energies of the initial state
   if (dump_vtk .or. dump_xdmf) call write_snapshot(0.0_dp)
+  if (dump_energy) call energy_gpu(0.0_dp)
 
-  ! chunks of at most 512 seismogram samples, ending at the snapshot steps
+  ! chunks of at most 512 seismogram samples, ending at the snapshot steps (every
+  ! step with SAVE_ENERGY)
   chunk = max(1, min(niter, 512 * seis_it))
+  if (dump_energy) chunk = 1
   allocate(seis(3 * max(nrec_all, 1) * (chunk / seis_it + 1)))
   tloc = zero
   isamp_iter = 0
@@ -147,6 +161,7 @@ subroutine time_loop
      if (dump_vtk .or. dump_xdmf) then
         if (mod(it1, snap_it) == 0) call write_snapshot(tloc)
      endif
+     if (dump_energy) call energy_gpu(tloc)
      if (lpr) then
         write(6,13) it1, tloc, real(it1) / real(niter) * 100.
         call flush(6)
@@ -241,6 +256,94 @@ end subroutine write_snapshot
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
+!> energy (SOLVER/time_evol_wave.F90:1424-1527, private there) for the fields copied back
+!! from the GPU after the step ending at time: kinetic and potential energy of the solid
+!! (one more stiffness evaluation) and of the fluid, written every step.
+subroutine energy_gpu(time)
+
+  use data_matr, only : unassem_mass_rho_solid, unassem_mass_lam_fluid
+  use stiffness_mono
+  use stiffness_di
+  use stiffness_quad
+  use stiffness_fluid
+  use apply_masks
+  use commun
+  real(kind=dp), intent(in)        :: time
+  real(kind=realkind), allocatable :: disp(:,:,:,:), vel(:,:,:,:), chi(:,:,:)
+  real(kind=realkind), allocatable :: dchi(:,:,:), ddchi(:,:,:)
+  real(kind=realkind), allocatable :: stiff(:,:,:,:), stiff_flu(:,:,:)
+  real(kind=realkind) :: ekin_sol, epot_sol, ekin_flu, epot_flu
+
+  allocate(disp(0:npol, 0:npol, nel_solid, 3), vel(0:npol, 0:npol, nel_solid, 3))
+  allocate(chi(0:npol, 0:npol, max(nel_fluid, 1)), dchi(0:npol, 0:npol, max(nel_fluid, 1)))
+  allocate(ddchi(0:npol, 0:npol, max(nel_fluid, 1)))
+  allocate(stiff(0:npol, 0:npol, nel_solid, 3), stiff_flu(0:npol, 0:npol, max(nel_fluid, 1)))
+  call axh_check(axh_download_fields(ctx, disp, vel, chi, dchi, ddchi), 'download fields')
+  t = time
+
+  ekin_sol = zero
+  epot_sol = zero
+  ekin_flu = zero
+  epot_flu = zero
+  stiff = zero
+  stiff_flu = zero
+
+  select case (src_type(1))
+     case ('monopole')
+        call apply_axis_mask_onecomp(disp, nel_solid, ax_el_solid, naxel_solid)
+        call glob_stiffness_mono(stiff, disp)
+        call apply_axis_mask_onecomp(stiff, nel_solid, ax_el_solid, naxel_solid)
+     case ('dipole')
+        call apply_axis_mask_twocomp(disp, nel_solid, ax_el_solid, naxel_solid)
+        call glob_stiffness_di(stiff, disp)
+        call apply_axis_mask_twocomp(stiff, nel_solid, ax_el_solid, naxel_solid)
+     case ('quadpole')
+        call apply_axis_mask_threecomp(disp, nel_solid, ax_el_solid, naxel_solid)
+        call glob_stiffness_quad(stiff, disp)
+        call apply_axis_mask_threecomp(stiff, nel_solid, ax_el_solid, naxel_solid)
+  end select
+  stiff = stiff * disp
+  epot_sol = sum(stiff)
+  epot_sol = two * pi * psum(epot_sol)
+
+  stiff(:,:,:,1) = vel(:,:,:,1)**2 * unassem_mass_rho_solid
+  stiff(:,:,:,2) = vel(:,:,:,2)**2 * unassem_mass_rho_solid
+  stiff(:,:,:,3) = vel(:,:,:,3)**2 * unassem_mass_rho_solid
+  if (src_type(1) == 'dipole') stiff(:,:,:,3) = two * stiff(:,:,:,3)
+  ekin_sol = sum(stiff)
+  ekin_sol = two * pi * psum(ekin_sol)
+
+  if (have_fluid) then
+     stiff_flu = ddchi**2 * unassem_mass_lam_fluid
+     epot_flu = sum(stiff_flu)
+     epot_flu = two * pi * psum(epot_flu)
+     stiff_flu = zero
+     if (src_type(1) .ne. 'monopole') &
+        call apply_axis_mask_scal(dchi, nel_fluid, ax_el_fluid, naxel_fluid)
+     call glob_fluid_stiffness(stiff_flu, dchi)
+     if (src_type(1) .ne. 'monopole') &
+        call apply_axis_mask_scal(stiff_flu, nel_fluid, ax_el_fluid, naxel_fluid)
+     stiff_flu = stiff_flu * dchi
+     ekin_flu = sum(stiff_flu)
+     ekin_flu = two * pi * psum(ekin_flu)
+  endif
+
+  if (lpr) then
+     if (have_fluid) then
+        write(4444,8) t, ekin_sol, epot_sol
+        write(4445,8) t, epot_flu, ekin_flu
+     endif
+     write(4446,9) t, epot_sol + epot_flu, ekin_sol + ekin_flu, &
+                   half * (epot_sol + epot_flu + ekin_sol + ekin_flu)
+  endif
+8 format(3(1pe16.6))
+9 format(4(1pe16.6))
+  deallocate(disp, vel, chi, dchi, ddchi, stiff, stiff_flu)
+
+end subroutine energy_gpu
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
 !> One sample of compute_recfile_seis_bare (SOLVER/seismograms.f90:742-780) and of
 !! compute_hyp_epi_equ_anti (SOLVER/seismograms.f90:688-735), from the library's
 !! receiver rows s(3, nrec_all) in the same component order.
@@ -305,7 +408,6 @@ subroutine check_supported_pre()
   why = ' '
   if (nproc > 1)        why = 'more than one mesh slice (NTHETA/NRADIAL_SLICES > 1)'
   if (npol /= 4)        why = 'npol /= 4'
-  if (dump_energy)      why = 'SAVE_ENERGY'
   if (dump_wavefields)  why = 'KERNEL_WAVEFIELDS'
   if (use_netcdf)       why = 'USE_NETCDF (ASCII seismograms only)'
   if (why /= ' ') then

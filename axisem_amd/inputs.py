@@ -28,6 +28,7 @@ class RunConfig:
     coarse_grained: bool = True
     snapshots: str = ""                 # "", "xdmf", "vtk" or "both" (SAVE_SNAPSHOTS)
     snapshot_dt: float = 20.0           # SNAPSHOT_DT, seconds
+    energy: bool = False                # SAVE_ENERGY
 
 
 def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
@@ -99,7 +100,7 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
         f.write("USE_NETCDF          false\nCHECKPOINTING       false\n")
         f.write("VERBOSITY               1\nDIAGNOSTIC_FILE_OUTPUT false\n")
         f.write("MESH_TEST           false\nKERNEL_WAVEFIELDS   false\n")
-        f.write("SAVE_ENERGY         .false.\n")
+        f.write("SAVE_ENERGY         %s\n" % (".true." if cfg.energy else ".false."))
         f.write("NR_LIN_SOLIDS       %d\n" % cfg.n_sls)
         f.write("COARSE_GRAINED      %s\n" % ("true" if cfg.coarse_grained else "false"))
         if cfg.snapshots:

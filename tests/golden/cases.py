@@ -101,18 +101,23 @@ NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
          "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}
 
 
-# Runs with wavefield snapshots (SAVE_SNAPSHOTS): the reference's snapshot files
-# are fixed by their SHA-256 in tests/golden/<name>/snapshots.json.
-SNAPSHOT_CASES = {
+# Runs with wavefield snapshots (SAVE_SNAPSHOTS) or energies (SAVE_ENERGY): the
+# reference's output files are fixed by their SHA-256 in
+# tests/golden/<name>/outputs.json.
+OUTPUT_CASES = {
     "snap_mono_vertforce_newmark": RunConfig("vertforce", 0.0, 1e20, "newmark2", False, 300.0,
                                              STATIONS, snapshots="both", snapshot_dt=60.0),
     "snap_dipole_thetaforce_symplec4_anel": RunConfig("thetaforce", 50.0, 1e20, "symplec4", True,
                                                       300.0, STATIONS, snapshots="xdmf",
                                                       snapshot_dt=45.0),
+    "energy_dipole_thetaforce_newmark_anel": RunConfig("thetaforce", 50.0, 1e20, "newmark2", True,
+                                                       300.0, STATIONS, energy=True),
+    "energy_mono_explosion_symplec4": RunConfig("explosion", 600.0, 1e20, "symplec4", False,
+                                                300.0, STATIONS, energy=True),
 }
 # the output files the snapshot comparison covers
-SNAPSHOT_FILES = ("snap_", "xdmf_snap_", "glob_grid_", "xdmf_grid_", "xdmf_points_",
-                  "xdmf_xml_", "xdmf_meshonly_")
+OUTPUT_FILES = ("snap_", "xdmf_snap_", "glob_grid_", "xdmf_grid_", "xdmf_points_",
+                  "xdmf_xml_", "xdmf_meshonly_", "energy_")
 
 
 def mesh_of(name: str) -> str:

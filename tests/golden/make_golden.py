@@ -9,7 +9,9 @@ programs built by oracle/build_ref.py):
    case's inputs -> receiver seismograms, stored as float32 (the reference
    prints them with round-trip precision) every SAMPLE_STRIDE samples in
    tests/golden/<case>/ref_seis.npz, with niter, deltat and the seismogram
-   sampling of the run.
+   sampling of the run;
+3. for every case of OUTPUT_CASES (snapshots, energies), the SHA-256 of the
+   reference's output files in tests/golden/<case>/outputs.json.
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -71,20 +73,21 @@ def run_reference(name: str):
     print(name, "samples", nsamp, "receivers", seis.shape[1], "components", ncomp)
 
 
-def snapshot_hashes(data_dir: str) -> dict:
+def output_hashes(data_dir: str) -> dict:
     import hashlib
     out = {}
     for fn in sorted(os.listdir(data_dir)):
-        if fn.startswith(cases.SNAPSHOT_FILES):
+        if fn.startswith(cases.OUTPUT_FILES):
             with open(os.path.join(data_dir, fn), "rb") as f:
                 out[fn] = hashlib.sha256(f.read()).hexdigest()
     return out
 
 
-def run_snapshots(name: str):
-    """The reference run with snapshots: SHA-256 of every snapshot file."""
+def run_outputs(name: str):
+    """The reference run with snapshots or energies: SHA-256 of every such
+    output file."""
     import json
-    cfg = cases.SNAPSHOT_CASES[name]
+    cfg = cases.OUTPUT_CASES[name]
     out = cases.case_dir(name)
     os.makedirs(out, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
@@ -92,10 +95,10 @@ def run_snapshots(name: str):
         with open(os.path.join(tmp, "OUTPUT"), "w") as log:
             subprocess.run([AXISEM], cwd=tmp, stdout=log, stderr=subprocess.STDOUT, check=True,
                            preexec_fn=_unlimited_stack, timeout=3600)
-        hashes = snapshot_hashes(os.path.join(tmp, "Data"))
-    with open(os.path.join(out, "snapshots.json"), "w") as f:
+        hashes = output_hashes(os.path.join(tmp, "Data"))
+    with open(os.path.join(out, "outputs.json"), "w") as f:
         json.dump(hashes, f, indent=1, sort_keys=True)
-    print(name, len(hashes), "snapshot files")
+    print(name, len(hashes), "output files")
 
 
 def main(argv):
@@ -103,10 +106,10 @@ def main(argv):
         make_mesh(cases.MESH_DIR)
     if not os.path.exists(os.path.join(cases.REGIONAL_MESH_DIR, "meshdb.dat0000")):
         make_mesh(cases.REGIONAL_MESH_DIR, cases.REGIONAL_MAX_COLAT)
-    names = argv or list(cases.CASES) + list(cases.SNAPSHOT_CASES)
+    names = argv or list(cases.CASES) + list(cases.OUTPUT_CASES)
     for n in names:
-        if n in cases.SNAPSHOT_CASES:
-            run_snapshots(n)
+        if n in cases.OUTPUT_CASES:
+            run_outputs(n)
         else:
             run_reference(n)
 

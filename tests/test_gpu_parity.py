@@ -51,25 +51,25 @@ def _compare_fields(gpu, orc):
         np.testing.assert_array_equal(a, b, err_msg=nm)
 
 
-@pytest.mark.parametrize("name", sorted(cases.SNAPSHOT_CASES))
-def test_dropin_snapshots_match_reference(name, host_exe, tmp_path):
-    """SAVE_SNAPSHOTS (xdmf, vtk): the drop-in on the GPU writes the reference
-    program's snapshot files byte for byte (SHA-256 recorded from the
-    reference run by tests/golden/make_golden.py)."""
+@pytest.mark.parametrize("name", sorted(cases.OUTPUT_CASES))
+def test_dropin_output_files_match_reference(name, host_exe, tmp_path):
+    """SAVE_SNAPSHOTS (xdmf, vtk) and SAVE_ENERGY: the drop-in on the GPU
+    writes the reference program's snapshot and energy files byte for byte
+    (SHA-256 recorded from the reference run by tests/golden/make_golden.py)."""
     import hashlib
     import json
-    print("drop-in GPU run with snapshots:", name, flush=True)
+    print("drop-in GPU run with snapshots / energies:", name, flush=True)
     rundir = str(tmp_path / name)
-    inputs.write_run_dir(rundir, cases.mesh_of(name), cases.SNAPSHOT_CASES[name])
+    inputs.write_run_dir(rundir, cases.mesh_of(name), cases.OUTPUT_CASES[name])
     with open(os.path.join(rundir, "OUTPUT"), "w") as log:
         r = subprocess.run([host_exe], cwd=rundir, stdout=log, stderr=subprocess.STDOUT,
                            timeout=300, preexec_fn=_unlimited_stack)
     assert r.returncode == 0, open(os.path.join(rundir, "OUTPUT"), errors="replace").read()[-2000:]
-    want = json.load(open(os.path.join(cases.case_dir(name), "snapshots.json")))
+    want = json.load(open(os.path.join(cases.case_dir(name), "outputs.json")))
     data = os.path.join(rundir, "Data")
     got = {}
     for fn in sorted(os.listdir(data)):
-        if fn.startswith(cases.SNAPSHOT_FILES):
+        if fn.startswith(cases.OUTPUT_FILES):
             with open(os.path.join(data, fn), "rb") as f:
                 got[fn] = hashlib.sha256(f.read()).hexdigest()
     assert sorted(got) == sorted(want)
