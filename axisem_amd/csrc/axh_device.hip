@@ -52,6 +52,9 @@ namespace axh {
 
 // number of packed element terms per source type (order: see pack_terms)
 constexpr int kNcMono = 15, kNcDi = 24, kNcQuad = 26;
+// tiles of a group whose points include a source point of the group (up to
+// this many are passed to the stiffness kernel by value)
+constexpr int kMaxSrcTiles = 32;
 constexpr int kNcFluidMono = 3, kNcFluidMulti = 4;
 // coarse-grained terms per element, 4 cg points each.  Read by every stage:
 // Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi.  Read by the memory-variable update:
@@ -123,6 +126,7 @@ struct MeshDev {
   // host copies used by wavefield builds and field transfers
   std::vector<int> gid_s_h, gid_f_h;
   std::vector<int> first_order_s_h;  // solid points in first-appearance order
+  int first_direct = 0;  // single-copy points are [first_direct, nglob_s)
   std::vector<uint8_t> flags_s_h;  // solid point flags without source bits (a
                                    // group's device copy adds its sources)
   float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
@@ -210,6 +214,12 @@ struct SolidArgs {
   // points of kFlagDirect: the stiffness kernel does their assembly (one
   // term), kick and, for symplectic stages, the drift that follows
   int direct;
+  // the mesh's single-copy points are the range [first_direct, nglob); a
+  // group's source points among them are not direct, so the tiles holding
+  // source points (src_tile, or all tiles when nsrc_tile < 0) read the
+  // group's flags instead
+  int first_direct, nsrc_tile;
+  int src_tile[kMaxSrcTiles];
   const uint8_t *flags;
   const float *minv;
   float *velo, *acc0;     // [3][nglob][B]
@@ -624,7 +634,12 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   // newmark2, acceleration) and inverse mass are requested now
   // (one wavefield only: for a group the extra state traffic at the end of
   // this kernel costs more than the assembly kernel saves)
-  const bool direct = B == 1 && valid && A.direct && (A.flags[gp] & kFlagDirect);
+  bool direct = false;
+  if (B == 1 && A.direct) {
+    bool src_tile = A.nsrc_tile < 0;
+    for (int q = 0; q < A.nsrc_tile; q++) src_tile = src_tile || A.src_tile[q] == tile;
+    direct = valid && (src_tile ? (A.flags[gp] & kFlagDirect) != 0 : gp >= A.first_direct);
+  }
   float dv[NCOMP][B], da[MODE == 0 ? NCOMP : 1][B], dmi = 0.0f;
   if (direct) {
     dmi = A.minv[gp];
@@ -2231,6 +2246,9 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   V.first_order_s_h = std::move(H.first_order_s);
   V.gid_f_h = std::move(H.gid_f);
   V.flags_s_h = std::move(H.flags_s);
+  V.first_direct = V.nglob_s;
+  for (int g = 0; g < V.nglob_s; g++)
+    if (V.flags_s_h[g] & kFlagDirect) { V.first_direct = g; break; }
   return M;
 }
 
@@ -2271,6 +2289,9 @@ struct Fields {
   uint8_t *flags_s = nullptr;
   int nasm = 0;
   int *asm_pts = nullptr;
+  // solid tiles holding a source point of the group (-1 entries: more than
+  // kMaxSrcTiles, then every tile reads the flags)
+  std::vector<int> src_tiles;
 };
 
 // the state arrays of a wavefield (pointer into F, entries per wavefield)
@@ -2302,6 +2323,18 @@ static void fields_set_sources(const MeshDev &M, Fields &F,
   std::vector<uint8_t> fl = M.flags_s_h;
   if (F.B > 1)
     for (auto &f : fl) f &= (uint8_t)~kFlagDirect;
+  {
+    std::vector<char> is_src(M.nglob_s, 0);
+    for (const auto *v : srcs)
+      for (int g : *v) is_src[g] = 1;
+    F.src_tiles.clear();
+    for (size_t sl = 0; sl < M.gid_s_h.size(); sl++) {
+      const int g = M.gid_s_h[sl];
+      const int tl = (int)(sl / kTile);
+      if (g >= 0 && is_src[g] && (F.src_tiles.empty() || F.src_tiles.back() != tl))
+        F.src_tiles.push_back(tl);
+    }
+  }
   for (const auto *v : srcs)
     for (int g : *v) {
       fl[g] |= kFlagSource;
@@ -2652,6 +2685,13 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   SA.disp = F.disp; SA.force = F.force_s; SA.memvar = F.memvar; SA.rsum = F.rsum;
   SA.trm1 = F.trm1; SA.devm1 = F.devm1;
   SA.flags = F.flags_s; SA.minv = M.minv_s; SA.velo = F.velo; SA.acc0 = F.acc0;
+  SA.first_direct = M.first_direct;
+  if (F.src_tiles.size() > (size_t)kMaxSrcTiles) {
+    SA.nsrc_tile = -1;
+  } else {
+    SA.nsrc_tile = (int)F.src_tiles.size();
+    for (size_t q = 0; q < F.src_tiles.size(); q++) SA.src_tile[q] = F.src_tiles[q];
+  }
   SA.disp_w = F.disp; SA.hdt = hdt;
   FluidArgs FA{};
   FA.nel = M.nel_f; FA.multipole = multipole; FA.G = M.G; FA.coef = M.coef_f;

@@ -182,7 +182,8 @@ def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
     the element-local force written once and the SLS sums.  A single
     wavefield (nwave 1) kicks its ndirect single-copy points itself: no force
     for them, but their velocity read and written, their displacement
-    written, their inverse mass and the point flags read."""
+    written and their inverse mass read (they are one index range; only the
+    few tiles holding a source point read the point flags)."""
     d = st.dims
     nel, nglob = d["nel_solid"], d["nglob_solid"]
     ncomp = 2 if d["src_type"] == 0 else 3
@@ -198,7 +199,6 @@ def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
         mesh += nel * 5 * 4 * 4        # Y, V_{s,z}_{eta,xi} at the cg points
         wave += nel * 24 * 4           # SLS sums
     if nwave == 1 and ndirect:
-        wave += nglob                                  # point flags
         wave -= 4 * ncomp * ndirect                    # no force for them
         wave += ndirect * (4 + 3 * 4 * ncomp)          # minv, velo r/w, disp w
     # the fluid tiles of the same launch: terms (3, multipoles 4), index,
