@@ -1310,6 +1310,7 @@ __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
 struct AsmSolidArgs {
   int n, src, newmark, nslot, comps, mask0, mask1;
   int npts;              // points assembled here (the others are kFlagDirect)
+  int ncontig;           // pts[ip] == ip for ip < ncontig (not read there)
   const int *pts;
   const int *gstart, *gslot;
   const uint8_t *flags;
@@ -1349,7 +1350,7 @@ template <int B>
 __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
   const int ip = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (ip >= A.npts) return;
-  const int g = A.pts[ip];
+  const int g = ip < A.ncontig ? ip : A.pts[ip];
   const uint8_t fl = A.flags[g];
   const int q0 = A.gstart[g], q1 = A.gstart[g + 1];
   const float mi = A.minv[g];
@@ -2714,6 +2715,9 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   AS.n = M.nglob_s; AS.src = M.src; AS.newmark = newmark; AS.nslot = (int)nslot_s;
   AS.comps = comps; AS.mask0 = mask0; AS.mask1 = mask1; AS.gstart = M.gstart_s;
   AS.npts = F.nasm; AS.pts = F.asm_pts;
+  // one wavefield: the assembled points are first the mesh's range
+  // [0, first_direct) in order, then the group's source points beyond it
+  AS.ncontig = F.B == 1 ? std::min(M.first_direct, F.nasm) : 0;
   AS.gslot = M.gslot_s; AS.flags = F.flags_s; AS.minv = M.minv_s; AS.hdt = hdt;
   AS.gamma = M.gamma_s;
   AS.force = F.force_s; AS.velo = F.velo; AS.acc0 = F.acc0;
