@@ -1267,9 +1267,7 @@ struct PredArgs {
 };
 
 template <int B>
-__global__ __launch_bounds__(256) void k_predict(PredArgs A) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= A.n) return;
+__device__ __forceinline__ void predict_point(const PredArgs &A, int g) {
   const uint8_t fl = A.flags[g];
   for (int c = 0; c < 3; c++) {
     if (!(A.comps & (1 << c))) continue;
@@ -1301,6 +1299,19 @@ __global__ __launch_bounds__(256) void k_predict(PredArgs A) {
       for (int b = 0; b < B; b++) un[b] = 0.0f;
     }
     stv<B>(A.u + p, un);
+  }
+}
+
+// The solid and the fluid prediction in one launch: the first nblk_s
+// workgroups take the solid points, the others the fluid points.
+template <int B>
+__global__ __launch_bounds__(256) void k_predict(PredArgs S, PredArgs F, int nblk_s) {
+  if ((int)blockIdx.x < nblk_s) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g < S.n) predict_point<B>(S, g);
+  } else {
+    const int g = (blockIdx.x - nblk_s) * 256 + threadIdx.x;
+    if (g < F.n) predict_point<B>(F, g);
   }
 }
 
@@ -2761,9 +2772,8 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         PF.check_blow = 0;
         for (int b = 0; b < B; b++) PS.iter[b] = PF.iter[b] = start[b] + step;
         L(KC_PRED, [&] {
-          hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PS.n)), dim3(256), 0, st_, PS);
-          if (M.nel_f > 0)
-            hipLaunchKernelGGL(k_predict<B>, dim3(nblk(PF.n)), dim3(256), 0, st_, PF);
+          const int nbs = nblk(PS.n), nbf = M.nel_f > 0 ? nblk(PF.n) : 0;
+          hipLaunchKernelGGL(k_predict<B>, dim3(nbs + nbf), dim3(256), 0, st_, PS, PF, nbs);
         });
       }
       // drift after the kick: coefd(st+1), masked except the final one
