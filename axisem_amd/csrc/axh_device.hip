@@ -1133,6 +1133,29 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   }
 }
 
+// receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
+// blockIdx.y = wavefield of the group; members without a sample this step
+// have out NULL
+struct RecArgs {
+  int src, nglob;
+  const float *disp;     // [3][nglob][B]
+  int nrec[kGroup];
+  const int *rec_g[kGroup];
+  float *out[kGroup];
+};
+
+template <int B>
+__device__ __forceinline__ void rec_sample(const RecArgs &A, int r, int b) {
+  if (!A.out[b] || r >= A.nrec[b]) return;
+  const int g = A.rec_g[b][r];
+  const float d1 = A.disp[(size_t)g * B + b], d2 = A.disp[((size_t)A.nglob + g) * B + b],
+              d3 = A.disp[(2 * (size_t)A.nglob + g) * B + b];
+  float *o = A.out[b] + 3 * (size_t)r;
+  if (A.src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
+  else if (A.src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
+  else { o[0] = d1; o[1] = d2; o[2] = d3; }
+}
+
 // memory-variable update of the symplectic scheme, once per step after the
 // final drift (SOLVER/time_evol_wave.F90:727-731); also refreshes the SLS sums
 // read by the stage kernels.
@@ -1142,8 +1165,11 @@ void k_solid(SolidArgs A, FluidArgs FA) {
 // The products ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j of the element are
 // formed once per element, exactly as the per-entry expressions group them.
 // Dynamic LDS: the tile's memory variables, 10 * 24 * nsls * B floats.
+// Blocks past the solid tiles take the step's receiver samples (R; nothing
+// in the update writes the displacement they read), which saves the step a
+// launch.
 template <int SRC, int B>
-__global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
+__global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   constexpr int NR = (24 * B + 24) / 25;  // entries per lane
   extern __shared__ __align__(16) float sMv[];
@@ -1153,6 +1179,11 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A) {
   __shared__ __align__(16) float sOld[kElemPerTile * 28 * B];
   __shared__ double sCf[kElemPerTile][kMaxSls][4];
   const int t = threadIdx.x, tile = blockIdx.x;
+  if (tile >= A.ntile) {  // block-uniform: no barrier below is reached
+    const int q = tile - A.ntile;
+    rec_sample<B>(R, (q / B) * 256 + t, q % B);
+    return;
+  }
   const int le = t / 25, k = t - 25 * le;
   const int e = tile * kElemPerTile + le;
   const bool valid = (t < 250) && (e < A.nel);
@@ -1606,29 +1637,9 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
   }
 }
 
-// receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
-// blockIdx.y = wavefield of the group; members without a sample this step
-// have out NULL
-struct RecArgs {
-  int src, nglob;
-  const float *disp;     // [3][nglob][B]
-  int nrec[kGroup];
-  const int *rec_g[kGroup];
-  float *out[kGroup];
-};
-
 template <int B>
 __global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  const int b = blockIdx.y;
-  if (!A.out[b] || r >= A.nrec[b]) return;
-  const int g = A.rec_g[b][r];
-  const float d1 = A.disp[(size_t)g * B + b], d2 = A.disp[((size_t)A.nglob + g) * B + b],
-              d3 = A.disp[(2 * (size_t)A.nglob + g) * B + b];
-  float *o = A.out[b] + 3 * (size_t)r;
-  if (A.src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
-  else if (A.src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
-  else { o[0] = d1; o[1] = d2; o[2] = d3; }
+  rec_sample<B>(A, blockIdx.x * 256 + threadIdx.x, blockIdx.y);
 }
 
 // moving one wavefield's state between a group's interleaved arrays and its
@@ -2648,14 +2659,16 @@ static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
 }
 
 template <int B>
-static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A) {
+static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A,
+                            const RecArgs &R, int nrec_blk) {
   const size_t lds = sizeof(float) * kElemPerTile * 24 * M.nsls * B;
+  const dim3 grid(M.ntile_s + nrec_blk);
   if (M.src == AXH_MONOPOLE)
-    hipLaunchKernelGGL((k_memvar<AXH_MONOPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_MONOPOLE, B>), grid, dim3(256), lds, s, A, R);
   else if (M.src == AXH_DIPOLE)
-    hipLaunchKernelGGL((k_memvar<AXH_DIPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_DIPOLE, B>), grid, dim3(256), lds, s, A, R);
   else
-    hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), dim3(M.ntile_s), dim3(256), lds, s, A);
+    hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), grid, dim3(256), lds, s, A, R);
 }
 
 // Advances each of the B wavefields of F (member data mb[b], dims of member
@@ -2804,7 +2817,6 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         hipLaunchKernelGGL(k_asm_solid<B>, dim3(nblk(AS.npts)), dim3(256), 0, st_, AS);
       });
     }
-    if (!newmark && M.anel) L(KC_MEMVAR, [&] { dispatch_memvar<B>(M, st_, SA); });
     bool any = false;
     for (int b = 0; b < B; b++) {
       Member &w = *mb[b];
@@ -2814,6 +2826,12 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         isamp[b]++;
         any = true;
       }
+    }
+    if (!newmark && M.anel) {
+      // the receiver samples ride along as extra blocks of the update
+      const int nrb = any ? nblk(maxrec) * B : 0;
+      L(KC_MEMVAR, [&] { dispatch_memvar<B>(M, st_, SA, RA, nrb); });
+      any = false;
     }
     if (any)
       L(KC_REC, [&] {
