@@ -1997,14 +1997,17 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
   // ---- points with a single element copy and nothing else to add (no axis
   // mask, no coupling term; source points are excluded when a wavefield with
   // a source there is attached) ----
+  // (none with memory variables at all points: the stiffness kernel would
+  // kick them before the anelastic forces are subtracted)
+  const bool anel_full = d.anel && !d.coarse_grained;
   for (int g = 0; g < H.nglob_s; g++)
-    if (!absorbing && H.gstart_s[g + 1] - H.gstart_s[g] == 1 &&
+    if (!absorbing && !anel_full && H.gstart_s[g + 1] - H.gstart_s[g] == 1 &&
         !(H.flags_s[g] & (kFlagAxis | kFlagCoupled)))
       H.flags_s[g] |= kFlagDirect;
   number_assembled_first(H);
 
   // ---- attenuation ----
-  if (d.anel) {
+  if (d.anel && d.coarse_grained) {
     const char *stage_names[kCgStage] = {"Y_cg4", "V_s_eta_cg4", "V_s_xi_cg4", "V_z_eta_cg4",
                                          "V_z_xi_cg4"};
     const char *upd_names[kCgUpd - 1] = {"DsDeta_over_J_sol_cg4", "DzDeta_over_J_sol_cg4",
@@ -2202,6 +2205,9 @@ void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *ngl
 
 static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   const axh_dims &d = ctx->dims;
+  if (d.anel && !d.coarse_grained)
+    fail(AXH_ERR_UNSUPPORTED, "memory variables at all 25 points (COARSE_GRAINED false) "
+                              "are not stepped on the GPU");
   MeshHost H;
   build_mesh_layout(ctx, H);
   int ndev = 0;

@@ -141,8 +141,6 @@ static void validate_dims(const axh_dims &d) {
   if (d.scheme < AXH_NEWMARK2 || d.scheme > AXH_ML_SO6M7)
     fail(AXH_ERR_UNSUPPORTED, "unsupported time scheme");
   if (d.anel) {
-    if (!d.coarse_grained)
-      fail(AXH_ERR_UNSUPPORTED, "anelastic runs need coarse-grained memory variables");
     if (d.n_sls < 1 || d.n_sls > kMaxSls)
       fail(AXH_ERR_UNSUPPORTED, "n_sls must be in 1..8");
   }
@@ -316,6 +314,39 @@ int axh_upload_attenuation_cg4(axh_ctx *ctx, const axh_attenuation_cg4 *a) {
     ctx->put("inv_s_solid_cg4", kF32, {4, nel}, a->inv_s);
     ctx->put("delta_mu_cg4", kF32, {4, nel}, a->delta_mu);
     ctx->put("delta_kappa_cg4", kF32, {4, nel}, a->delta_kappa);
+    ctx->put("exp_w_j_deltat", kF64, {ns}, a->exp_w_j_deltat);
+    ctx->put("ts_fac_t", kF64, {ns}, a->ts_fac_t);
+    ctx->put("ts_fac_tm1", kF64, {ns}, a->ts_fac_tm1);
+    ctx->put("a_j_mu", kF64, {ns, nel}, a->a_j_mu);
+    ctx->put("a_j_kappa", kF64, {ns, nel}, a->a_j_kappa);
+  });
+}
+
+int axh_upload_attenuation_full(axh_ctx *ctx, const axh_attenuation_full *a) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!a) fail(AXH_ERR_ARG, "null attenuation terms");
+    const auto &d = ctx->dims;
+    if (!d.anel) fail(AXH_ERR_STATE, "attenuation upload for an elastic run");
+    if (d.coarse_grained) fail(AXH_ERR_STATE, "full attenuation terms for a coarse-grained run");
+    const uint64_t nel = (uint64_t)d.nel_solid, ns = (uint64_t)d.n_sls;
+    ctx->put("Y", kF32, {5, 5, nel}, a->Y);
+    ctx->put("V_s_eta", kF32, {5, 5, nel}, a->V_s_eta);
+    ctx->put("V_s_xi", kF32, {5, 5, nel}, a->V_s_xi);
+    ctx->put("V_z_eta", kF32, {5, 5, nel}, a->V_z_eta);
+    ctx->put("V_z_xi", kF32, {5, 5, nel}, a->V_z_xi);
+    ctx->put("Y0", kF32, {5, nel}, a->Y0);
+    ctx->put("V0_s_eta", kF32, {5, nel}, a->V0_s_eta);
+    ctx->put("V0_s_xi", kF32, {5, nel}, a->V0_s_xi);
+    ctx->put("V0_z_eta", kF32, {5, nel}, a->V0_z_eta);
+    ctx->put("V0_z_xi", kF32, {5, nel}, a->V0_z_xi);
+    ctx->put("DsDeta_over_J_sol", kF32, {5, 5, nel}, a->DsDeta_over_J);
+    ctx->put("DzDeta_over_J_sol", kF32, {5, 5, nel}, a->DzDeta_over_J);
+    ctx->put("DsDxi_over_J_sol", kF32, {5, 5, nel}, a->DsDxi_over_J);
+    ctx->put("DzDxi_over_J_sol", kF32, {5, 5, nel}, a->DzDxi_over_J);
+    ctx->put("inv_s_solid", kF32, {5, 5, nel}, a->inv_s);
+    ctx->put("delta_mu", kF32, {5, 5, nel}, a->delta_mu);
+    ctx->put("delta_kappa", kF32, {5, 5, nel}, a->delta_kappa);
     ctx->put("exp_w_j_deltat", kF64, {ns}, a->exp_w_j_deltat);
     ctx->put("ts_fac_t", kF64, {ns}, a->ts_fac_t);
     ctx->put("ts_fac_tm1", kF64, {ns}, a->ts_fac_tm1);

@@ -11,7 +11,7 @@
  *    ISO_Fortran_binding descriptor; reading the descriptors gives the exact
  *    values the reference loop would use.  This ties the drop-in host to a
  *    build of the reference with amdflang, which is how it is built.
- * 2. A flat-argument wrapper around axh_upload_attenuation_cg4 (Fortran
+ * 2. Flat-argument wrappers around axh_upload_attenuation_cg4/_full (Fortran
  *    cannot easily build a struct of pointers to allocatable arrays).
  * 3. getenv for the run-mode switches of the host.
  */
@@ -64,6 +64,25 @@ int axh_glue_upload_att(axh_ctx *ctx, const float *Y, const float *vse,
   a.exp_w_j_deltat = expw; a.ts_fac_t = tst; a.ts_fac_tm1 = tstm1;
   a.a_j_mu = ajmu; a.a_j_kappa = ajkap;
   return axh_upload_attenuation_cg4(ctx, &a);
+}
+
+/* The same for axh_upload_attenuation_full. */
+int axh_glue_upload_att_full(axh_ctx *ctx, const float *Y, const float *vse,
+                             const float *vsx, const float *vze, const float *vzx,
+                             const float *y0, const float *v0se, const float *v0sx,
+                             const float *v0ze, const float *v0zx, const float *dsde,
+                             const float *dzde, const float *dsdx, const float *dzdx,
+                             const float *invs, const float *dmu, const float *dkap,
+                             const double *expw, const double *tst, const double *tstm1,
+                             const double *ajmu, const double *ajkap) {
+  axh_attenuation_full a;
+  a.Y = Y; a.V_s_eta = vse; a.V_s_xi = vsx; a.V_z_eta = vze; a.V_z_xi = vzx;
+  a.Y0 = y0; a.V0_s_eta = v0se; a.V0_s_xi = v0sx; a.V0_z_eta = v0ze; a.V0_z_xi = v0zx;
+  a.DsDeta_over_J = dsde; a.DzDeta_over_J = dzde; a.DsDxi_over_J = dsdx;
+  a.DzDxi_over_J = dzdx; a.inv_s = invs; a.delta_mu = dmu; a.delta_kappa = dkap;
+  a.exp_w_j_deltat = expw; a.ts_fac_t = tst; a.ts_fac_tm1 = tstm1;
+  a.a_j_mu = ajmu; a.a_j_kappa = ajkap;
+  return axh_upload_attenuation_full(ctx, &a);
 }
 
 /* Copies the value of environment variable `name` (NUL-terminated) into val

@@ -181,6 +181,18 @@ module axh_iface
        real(c_double), intent(in) :: expw(*), tst(*), tstm1(*), ajmu(*), ajkap(*)
      end function
 
+     integer(c_int) function axh_glue_upload_att_full(ctx, Y, vse, vsx, vze, vzx, y0, v0se, &
+          v0sx, v0ze, v0zx, dsde, dzde, dsdx, dzdx, invs, dmu, dkap, expw, tst, tstm1, &
+          ajmu, ajkap) bind(C, name='axh_glue_upload_att_full')
+       import :: c_int, c_ptr, c_float, c_double
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: Y(*), vse(*), vsx(*), vze(*), vzx(*)
+       real(c_float), intent(in) :: y0(*), v0se(*), v0sx(*), v0ze(*), v0zx(*)
+       real(c_float), intent(in) :: dsde(*), dzde(*), dsdx(*), dzdx(*), invs(*)
+       real(c_float), intent(in) :: dmu(*), dkap(*)
+       real(c_double), intent(in) :: expw(*), tst(*), tstm1(*), ajmu(*), ajkap(*)
+     end function
+
      integer(c_int) function axh_glue_getenv(name, val, n) bind(C, name='axh_glue_getenv')
        import :: c_int, c_char
        character(kind=c_char), intent(in) :: name(*)

@@ -587,11 +587,7 @@ subroutine upload_all()
   dims%coarse_grained = 0
   dims%n_sls       = 0
   if (anel_true) then
-     if (.not. att_coarse_grained) then
-        write(6,*) 'GPU time loop: unsupported option: COARSE_GRAINED false'
-        stop 2
-     endif
-     dims%coarse_grained = 1
+     dims%coarse_grained = merge(1, 0, att_coarse_grained)
      dims%n_sls = n_sls_attenuation
   endif
   dims%niter     = niter
@@ -756,14 +752,15 @@ end subroutine upload_all
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
-!> Coarse-grained attenuation terms.  The per-element SLS weights a_j are those of
-!! time_step_memvars_cg4 (SOLVER/attenuation.f90:105-127), evaluated for every element
-!! from the reference's own fitted y_j and Q values.
+!> Attenuation terms, coarse-grained or at all 25 points (att_coarse_grained).  The
+!! per-element SLS weights a_j are those of time_step_memvars_cg4/_4
+!! (SOLVER/attenuation.f90:105-127, 244-262), evaluated for every element from the
+!! reference's own fitted y_j and Q values.
 subroutine upload_attenuation()
 
   use data_matr
   use data_pointwise
-  use attenuation,  only: n_sls_attenuation
+  use attenuation,  only: n_sls_attenuation, att_coarse_grained
 
   real(kind=dp)       :: expw(64), tst(64), tstm1(64), yj(64)
   integer(c_int)      :: docorr, n
@@ -791,6 +788,14 @@ subroutine upload_attenuation()
      endif
      ajk(:, iel) = yp / sum(yp)
   enddo
+
+  if (.not. att_coarse_grained) then
+     call axh_check(axh_glue_upload_att_full(ctx, Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi, &
+                    Y0, V0_s_eta, V0_s_xi, V0_z_eta, V0_z_xi, DsDeta_over_J_sol, &
+                    DzDeta_over_J_sol, DsDxi_over_J_sol, DzDxi_over_J_sol, inv_s_solid, &
+                    delta_mu, delta_kappa, expw, tst, tstm1, ajmu, ajk), 'attenuation')
+     return
+  endif
 
   ! f_over_s_solid_el_cg4 reads inv_s_solid at (1,1),(1,3),(3,1),(3,3)
   ! (SOLVER/pointwise_derivatives.f90:66-69)

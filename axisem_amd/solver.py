@@ -54,6 +54,14 @@ class Att(C.Structure):
         "exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa")]
 
 
+class AttFull(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "Y", "V_s_eta", "V_s_xi", "V_z_eta", "V_z_xi", "Y0", "V0_s_eta", "V0_s_xi",
+        "V0_z_eta", "V0_z_xi", "DsDeta_over_J", "DzDeta_over_J", "DsDxi_over_J",
+        "DzDxi_over_J", "inv_s", "delta_mu", "delta_kappa",
+        "exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa")]
+
+
 _lib = None
 
 
@@ -80,6 +88,7 @@ def library():
         L.axh_upload_absorbing.argtypes = [P, P, P]
         L.axh_upload_source.argtypes = [P, P, P, P, C.c_int32]
         L.axh_upload_attenuation_cg4.argtypes = [P, C.POINTER(Att)]
+        L.axh_upload_attenuation_full.argtypes = [P, C.POINTER(AttFull)]
         L.axh_upload_receivers.argtypes = [P, P]
         L.axh_run.argtypes = [P, C.c_int32, C.c_int32, P]
         L.axh_seis_samples.argtypes = [P, C.c_int32, C.c_int32]
@@ -196,7 +205,23 @@ class Solver:
         stf = f32("stf")
         _check(L.axh_upload_source(h, K(i32("ielsrc")), K(f32("source_term")), K(stf),
                                    int(stf.size)))
-        if d["anel"]:
+        if d["anel"] and not d["coarse_grained"]:
+            a = AttFull()
+            for fld, n in (("Y", "Y"), ("V_s_eta", "V_s_eta"), ("V_s_xi", "V_s_xi"),
+                           ("V_z_eta", "V_z_eta"), ("V_z_xi", "V_z_xi"), ("Y0", "Y0"),
+                           ("V0_s_eta", "V0_s_eta"), ("V0_s_xi", "V0_s_xi"),
+                           ("V0_z_eta", "V0_z_eta"), ("V0_z_xi", "V0_z_xi"),
+                           ("DsDeta_over_J", "DsDeta_over_J_sol"),
+                           ("DzDeta_over_J", "DzDeta_over_J_sol"),
+                           ("DsDxi_over_J", "DsDxi_over_J_sol"),
+                           ("DzDxi_over_J", "DzDxi_over_J_sol"),
+                           ("inv_s", "inv_s_solid"), ("delta_mu", "delta_mu"),
+                           ("delta_kappa", "delta_kappa")):
+                setattr(a, fld, K(f32(n)))
+            for fld in ("exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa"):
+                setattr(a, fld, K(np.ascontiguousarray(st[fld], np.float64)))
+            _check(L.axh_upload_attenuation_full(h, C.byref(a)))
+        elif d["anel"]:
             a = Att()
             for fld, n in (("Y", "Y_cg4"), ("V_s_eta", "V_s_eta_cg4"), ("V_s_xi", "V_s_xi_cg4"),
                            ("V_z_eta", "V_z_eta_cg4"), ("V_z_xi", "V_z_xi_cg4"),

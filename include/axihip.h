@@ -71,7 +71,7 @@ typedef struct axh_dims {
   int32_t src_type;         /* enum axh_src_type                          */
   int32_t scheme;           /* enum axh_scheme                            */
   int32_t anel;             /* anel_true                                  */
-  int32_t coarse_grained;   /* att_coarse_grained (only 1 is supported)   */
+  int32_t coarse_grained;   /* att_coarse_grained                         */
   int32_t n_sls;            /* n_sls_attenuation                          */
   int32_t niter;            /* number of time steps                       */
   int32_t seis_it;          /* seismogram sampling interval in steps      */
@@ -183,6 +183,28 @@ typedef struct axh_attenuation_cg4 {
   const double *a_j_mu, *a_j_kappa;
 } axh_attenuation_cg4;
 int axh_upload_attenuation_cg4(axh_ctx *ctx, const axh_attenuation_cg4 *att);
+
+/* Anelastic terms at all 25 points, for att_coarse_grained false
+ * (COARSE_GRAINED in inparam_advanced, SOLVER/attenuation.f90:812-813;
+ * used by glob_anel_stiffness_*_4 and time_step_memvars_4,
+ * SOLVER/stiffness_mono.f90:409-507, attenuation.f90:210-334).  Float arrays
+ * (0:4,0:4,nel_solid): Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi
+ * (SOLVER/data_matr.f90:101-110), DsDeta/DzDeta/DsDxi/DzDxi_over_J_sol and
+ * inv_s_solid (SOLVER/data_pointwise.f90), delta_mu, delta_kappa
+ * (SOLVER/attenuation.f90:1052-1054); (0:4,nel_solid): Y0, V0_s_eta,
+ * V0_s_xi, V0_z_eta, V0_z_xi (SOLVER/def_precomp_terms.f90:1294-1298).  The
+ * SLS coefficients are those of axh_upload_attenuation_cg4, with a_j_mu,
+ * a_j_kappa of time_step_memvars_4 (SOLVER/attenuation.f90:244-262). */
+typedef struct axh_attenuation_full {
+  const float *Y, *V_s_eta, *V_s_xi, *V_z_eta, *V_z_xi;
+  const float *Y0, *V0_s_eta, *V0_s_xi, *V0_z_eta, *V0_z_xi;
+  const float *DsDeta_over_J, *DzDeta_over_J, *DsDxi_over_J, *DzDxi_over_J;
+  const float *inv_s;
+  const float *delta_mu, *delta_kappa;
+  const double *exp_w_j_deltat, *ts_fac_t, *ts_fac_tm1;
+  const double *a_j_mu, *a_j_kappa;
+} axh_attenuation_full;
+int axh_upload_attenuation_full(axh_ctx *ctx, const axh_attenuation_full *att);
 
 /* Receivers: recfile_el(num_rec,3) = (element, ipol, jpol) in the solid,
  * element 1-based, ipol/jpol 0-based (SOLVER/data_mesh.f90:135). */

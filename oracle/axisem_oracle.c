@@ -67,6 +67,9 @@ typedef struct orc {
   int nstf;
   const float *Ycg, *Vseta, *Vsxi, *Vzeta, *Vzxi, *DsDeta, *DzDeta, *DsDxi,
       *DzDxi, *inv_s, *dmu, *dkappa;
+  /* COARSE_GRAINED false: the terms above at all 25 points, (0:4,0:4,nel),
+   * and the axial vectors (0:4,nel) */
+  const float *Y0, *V0seta, *V0sxi, *V0zeta, *V0zxi;
   const double *expw, *tsfac_t, *tsfac_tm1, *aj_mu, *aj_kappa;
   const int32_t *recfile_el;
   int32_t *ax_el_s, *ax_el_f;
@@ -736,6 +739,305 @@ static void memvars_cg4(orc_t *o, const float *disp) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* anelastic stiffness and memory variables at all 25 points                */
+/* (COARSE_GRAINED false)                                                   */
+/* ------------------------------------------------------------------------ */
+/* memory_var layout (0:4,0:4,6,n_sls,nel_solid) (SOLVER/time_evol_wave.F90:315) */
+#define MVF(p, c, j, e) \
+  o->memvar[(p) + NPP * ((c) + 6 * ((j) + (size_t)o->n_sls * (e)))]
+#define EF(a) ((a) + (size_t)e * NPP)
+
+/* glob_anel_stiffness_{mono,di,quad}_4 (SOLVER/stiffness_mono.f90:409-507,
+ * stiffness_di.f90:604-723, stiffness_quad.f90:555-668); subtracts from acc.
+ * The dipole z-component adds outerprod(G0, V2) of the p-component
+ * (stiffness_di.f90:706-707), as the reference does. */
+static void anel_stiff_full(orc_t *o, float *acc) {
+  const int nel = o->nel_s;
+  for (int e = 0; e < nel; e++) {
+    float r[6][NPP], Sa1[NPP], Sa2[NPP], Sb1[NPP], Sb2[NPP], Sc1[NPP], Sc2[NPP];
+    float X1[NPP], X2[NPP], X3[NPP], X4[NPP], X5[NPP], X6[NPP];
+    float l1[NPP], l2[NPP], l3[NPP], T[NPP];
+    float V1[5], V2[5], V3[5], V4[5], r10[5], r20[5], r30[5], r40[5], r50[5], r60[5];
+    for (int c = 0; c < 6; c++)
+      for (int p = 0; p < NPP; p++) {
+        float sm = 0.0f;
+        for (int j = 0; j < o->n_sls; j++) sm = sm + MVF(p, c, j, e);
+        r[c][p] = sm;
+      }
+    const float *r1 = r[0], *r2 = r[1], *r3 = r[2], *r4 = r[3], *r5 = r[4], *r6 = r[5];
+    const float *yl = EF(o->Ycg), *vse = EF(o->Vseta), *vsx = EF(o->Vsxi),
+                *vze = EF(o->Vzeta), *vzx = EF(o->Vzxi);
+    const int ax = o->axis_s[e];
+    const float *G = ax ? o->G1 : o->G2;
+    const float *y0 = o->Y0 + 5 * (size_t)e, *v0se = o->V0seta + 5 * (size_t)e,
+                *v0sx = o->V0sxi + 5 * (size_t)e, *v0ze = o->V0zeta + 5 * (size_t)e,
+                *v0zx = o->V0zxi + 5 * (size_t)e;
+    for (int jj = 0; jj < 5; jj++) {
+      r10[jj] = r1[5 * jj]; r20[jj] = r2[5 * jj]; r30[jj] = r3[5 * jj];
+      r40[jj] = r4[5 * jj]; r50[jj] = r5[5 * jj]; r60[jj] = r6[5 * jj];
+    }
+    float *a1 = acc + (size_t)e * NPP, *a2 = acc + (size_t)(e + nel) * NPP,
+          *a3 = acc + (size_t)(e + 2 * nel) * NPP;
+    if (o->src_type == MONO) {
+      for (int p = 0; p < NPP; p++) {
+        Sa1[p] = vze[p] * r1[p] + vse[p] * r5[p];
+        Sa2[p] = vzx[p] * r1[p] + vsx[p] * r5[p];
+        Sc1[p] = vze[p] * r5[p] + vse[p] * r3[p];
+        Sc2[p] = vzx[p] * r5[p] + vsx[p] * r3[p];
+      }
+      mxm(G, Sa1, X1);
+      mxm(G, Sc1, X3);
+      mxm(Sa2, o->G2T, X2);
+      mxm(Sc2, o->G2T, X4);
+      for (int p = 0; p < NPP; p++) {
+        l1[p] = X1[p] + X2[p] + yl[p] * r2[p];
+        l3[p] = X3[p] + X4[p];
+      }
+      if (ax) {
+        for (int jj = 0; jj < 5; jj++) {
+          V1[jj] = v0ze[jj] * r10[jj] + v0se[jj] * r50[jj] + y0[jj] * r20[jj];
+          V2[jj] = v0ze[jj] * r50[jj] + v0se[jj] * r30[jj];
+          V3[jj] = v0zx[jj] * r50[jj] + v0sx[jj] * r30[jj];
+        }
+        vxm(V3, o->G2T, V4);
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l1[p] = l1[p] + T[p];
+        outer(o->G0, V2, T);
+        for (int p = 0; p < NPP; p++) l3[p] = l3[p] + T[p];
+        for (int jj = 0; jj < 5; jj++) l3[5 * jj] = l3[5 * jj] + V4[jj];
+      }
+      for (int p = 0; p < NPP; p++) {
+        a1[p] = a1[p] - l1[p];
+        a3[p] = a3[p] - l3[p];
+      }
+    } else if (o->src_type == DI) {
+      for (int p = 0; p < NPP; p++) {
+        Sa1[p] = vze[p] * (r1[p] - r6[p]) + vse[p] * (r5[p] - r4[p]);
+        Sa2[p] = vzx[p] * (r1[p] - r6[p]) + vsx[p] * (r5[p] - r4[p]);
+        Sb1[p] = vze[p] * (r1[p] + r6[p]) + vse[p] * (r5[p] + r4[p]);
+        Sb2[p] = vzx[p] * (r1[p] + r6[p]) + vsx[p] * (r5[p] + r4[p]);
+        Sc1[p] = vze[p] * r5[p] + vse[p] * r3[p];
+        Sc2[p] = vzx[p] * r5[p] + vsx[p] * r3[p];
+      }
+      mxm(G, Sa1, X1);
+      mxm(G, Sb1, X3);
+      mxm(G, Sc1, X5);
+      mxm(Sa2, o->G2T, X2);
+      mxm(Sb2, o->G2T, X4);
+      mxm(Sc2, o->G2T, X6);
+      for (int p = 0; p < NPP; p++) {
+        l1[p] = X1[p] + X2[p];
+        l2[p] = X3[p] + X4[p] + 2.0f * yl[p] * (r2[p] - r6[p]);
+        l3[p] = X5[p] + X6[p] - yl[p] * r4[p];
+      }
+      if (ax) {
+        for (int jj = 0; jj < 5; jj++) {
+          V1[jj] = v0ze[jj] * (r10[jj] - r60[jj]) + v0se[jj] * (r50[jj] - r40[jj]);
+          V2[jj] = v0zx[jj] * (r10[jj] - r60[jj]) + v0sx[jj] * (r50[jj] - r40[jj]);
+        }
+        vxm(V2, o->G2T, V3);
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l1[p] = l1[p] + T[p];
+        for (int jj = 0; jj < 5; jj++) l1[5 * jj] = l1[5 * jj] + V3[jj];
+        for (int jj = 0; jj < 5; jj++)
+          V1[jj] = v0ze[jj] * (r10[jj] + r60[jj]) + v0se[jj] * (r50[jj] + r40[jj]) +
+                   y0[jj] * 2.0f * (r20[jj] - r60[jj]);
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l2[p] = l2[p] + T[p];
+        /* V1 of the z-component is computed and not used */
+        outer(o->G0, V2, T);
+        for (int p = 0; p < NPP; p++) l3[p] = l3[p] + T[p];
+      }
+      for (int p = 0; p < NPP; p++) {
+        a1[p] = a1[p] - l1[p];
+        a2[p] = a2[p] - l2[p];
+        a3[p] = a3[p] - l3[p];
+      }
+    } else {
+      for (int p = 0; p < NPP; p++) {
+        Sa1[p] = vze[p] * r1[p] + vse[p] * r5[p];
+        Sa2[p] = vzx[p] * r1[p] + vsx[p] * r5[p];
+        Sb1[p] = vze[p] * r6[p] + vse[p] * r4[p];
+        Sb2[p] = vzx[p] * r6[p] + vsx[p] * r4[p];
+        Sc1[p] = vze[p] * r5[p] + vse[p] * r3[p];
+        Sc2[p] = vzx[p] * r5[p] + vsx[p] * r3[p];
+      }
+      mxm(G, Sa1, X1);
+      mxm(G, Sb1, X3);
+      mxm(G, Sc1, X5);
+      mxm(Sa2, o->G2T, X2);
+      mxm(Sb2, o->G2T, X4);
+      mxm(Sc2, o->G2T, X6);
+      for (int p = 0; p < NPP; p++) {
+        l1[p] = X1[p] + X2[p] + yl[p] * (r2[p] - 2.0f * r6[p]);
+        l2[p] = -X3[p] - X4[p] + yl[p] * (r6[p] - 2.0f * r2[p]);
+        l3[p] = X5[p] + X6[p] - 2.0f * yl[p] * r4[p];
+      }
+      if (ax) {
+        for (int jj = 0; jj < 5; jj++)
+          V1[jj] = v0ze[jj] * r10[jj] + y0[jj] * (r20[jj] - 2.0f * r60[jj]);
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l1[p] = l1[p] + T[p];
+        for (int jj = 0; jj < 5; jj++)
+          V1[jj] = -(v0ze[jj] * r60[jj]) + y0[jj] * (r60[jj] - 2.0f * r20[jj]);
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l2[p] = l2[p] + T[p];
+        for (int jj = 0; jj < 5; jj++) V1[jj] = v0se[jj] * r30[jj];
+        outer(o->G0, V1, T);
+        for (int p = 0; p < NPP; p++) l3[p] = l3[p] + T[p];
+      }
+      for (int p = 0; p < NPP; p++) {
+        a1[p] = a1[p] - l1[p];
+        a2[p] = a2[p] - l2[p];
+        a3[p] = a3[p] - l3[p];
+      }
+    }
+  }
+}
+
+/* axisym_gradient_solid_el_4 (SOLVER/pointwise_derivatives.f90:252-286):
+ * g[0] = d/ds, g[1] = d/dz at the 25 points */
+static void grad_full(orc_t *o, const float *f, int e, float g[2][NPP]) {
+  float m1[NPP], m2[NPP];
+  mxm(o->axis_s[e] ? o->G1T : o->G2T, f, m1);
+  mxm(f, o->G2, m2);
+  const float *dze = EF(o->DzDeta), *dzx = EF(o->DzDxi), *dse = EF(o->DsDeta),
+              *dsx = EF(o->DsDxi);
+  for (int p = 0; p < NPP; p++) {
+    g[0][p] = dze[p] * m1[p] + dzx[p] * m2[p];
+    g[1][p] = dse[p] * m1[p] + dsx[p] * m2[p];
+  }
+}
+/* f_over_s_solid_el_4 (SOLVER/pointwise_derivatives.f90:104-124): f/s, and
+ * on the axis (ipol = 0 of axial elements) df/ds from dsdf_elem_solid
+ * (pointwise_derivatives.f90:419-445) */
+static void fos_full(orc_t *o, const float *f, int e, float r[NPP]) {
+  const float *is = EF(o->inv_s);
+  for (int p = 0; p < NPP; p++) r[p] = is[p] * f[p];
+  if (o->axis_s[e]) {
+    float g[2][NPP];
+    grad_full(o, f, e, g);
+    for (int jj = 0; jj < 5; jj++) r[5 * jj] = g[0][5 * jj];
+  }
+}
+
+/* compute_strain_att_el_4 (SOLVER/attenuation.f90:542-606): Voigt strain at
+ * the 25 points, g[c][p], c = 0..5 for E1..E6 */
+void orc_strain_full(orc_t *o, const float *u, int e, float g[6][NPP]) {
+  const int nel = o->nel_s;
+  const float *u1 = u + (size_t)e * NPP, *u2 = u + (size_t)(e + nel) * NPP,
+              *u3 = u + (size_t)(e + 2 * nel) * NPP;
+  float b1[2][NPP], b2[2][NPP], tmp[NPP], f[NPP], f2[NPP];
+  memset(g, 0, sizeof(float) * 6 * NPP);
+  if (o->src_type == DI) {
+    for (int p = 0; p < NPP; p++) tmp[p] = u1[p] + u2[p];
+    grad_full(o, tmp, e, b1);
+  } else {
+    grad_full(o, u1, e, b1);
+  }
+  grad_full(o, u3, e, b2);
+  for (int p = 0; p < NPP; p++) {
+    g[0][p] = b1[0][p];
+    g[2][p] = b2[1][p];
+    g[4][p] = b1[1][p] + b2[0][p];
+  }
+  if (o->src_type == MONO) {
+    fos_full(o, u1, e, f);
+    for (int p = 0; p < NPP; p++) g[1][p] = f[p];
+  } else if (o->src_type == DI) {
+    fos_full(o, u2, e, f);
+    for (int p = 0; p < NPP; p++) g[1][p] = 2.0f * f[p];
+    for (int p = 0; p < NPP; p++) tmp[p] = u1[p] - u2[p];
+    grad_full(o, tmp, e, b1);
+    fos_full(o, u3, e, f);
+    for (int p = 0; p < NPP; p++) {
+      g[3][p] = -f[p] - b1[1][p];
+      g[5][p] = -g[1][p] - b1[0][p];
+    }
+  } else {
+    for (int p = 0; p < NPP; p++) tmp[p] = u1[p] - 2.0f * u2[p];
+    fos_full(o, tmp, e, f);
+    for (int p = 0; p < NPP; p++) g[1][p] = f[p];
+    grad_full(o, u2, e, b1);
+    fos_full(o, u3, e, f);
+    for (int p = 0; p < NPP; p++) tmp[p] = u2[p] - 2.0f * u1[p];
+    fos_full(o, tmp, e, f2);
+    for (int p = 0; p < NPP; p++) {
+      g[3][p] = -(2.0f * f[p]) - b1[1][p];
+      g[5][p] = f2[p] - b1[0][p];
+    }
+  }
+}
+
+/* time_step_memvars_4 (SOLVER/attenuation.f90:210-334), with the per-element
+ * a_j of attenuation.f90:244-262 taken from the upload. */
+static void memvars_full(orc_t *o, const float *disp) {
+  const int nsls = o->n_sls;
+  const double third = 1.0 / 3.0;
+  for (int e = 0; e < o->nel_s; e++) {
+    float g[6][NPP], tr[NPP], str[NPP], sdev[6][NPP];
+    orc_strain_full(o, disp, e, g);
+    for (int p = 0; p < NPP; p++) tr[p] = g[0][p] + g[1][p] + g[2][p];
+    memset(sdev, 0, sizeof sdev);
+    const float *dmu = EF(o->dmu), *dkap = EF(o->dkappa);
+    for (int p = 0; p < NPP; p++) {
+      float dm = dmu[p];
+      str[p] = dkap[p] * tr[p];
+      for (int c = 0; c < 3; c++)
+        sdev[c][p] = (float)((double)(dm * 2.0f) *
+                             ((double)g[c][p] - (double)tr[p] * third));
+      sdev[4][p] = dm * g[4][p];
+      if (o->src_type != MONO) {
+        sdev[3][p] = dm * g[3][p];
+        sdev[5][p] = dm * g[5][p];
+      }
+    }
+    float *trm1 = o->src_tr_tm1 + NPP * (size_t)e;
+    float *devm1 = o->src_dev_tm1 + 6 * NPP * (size_t)e; /* (0:4,0:4,6,nel) */
+    const double *amu = o->aj_mu + (size_t)nsls * e,
+                 *akap = o->aj_kappa + (size_t)nsls * e;
+    for (int j = 0; j < nsls; j++) {
+      double Tt = o->tsfac_t[j], Tm = o->tsfac_tm1[j], E = o->expw[j];
+      for (int p = 0; p < NPP; p++) {
+        float trb = (float)(Tt * akap[j] * (double)str[p] +
+                            Tm * akap[j] * (double)trm1[p]);
+        float db[6];
+        for (int c = 0; c < 6; c++) {
+          if (o->src_type == MONO && (c == 3 || c == 5)) continue;
+          db[c] = (float)(Tt * amu[j] * (double)sdev[c][p] +
+                          Tm * amu[j] * (double)devm1[p + NPP * c]);
+        }
+        for (int c = 0; c < 3; c++)
+          MVF(p, c, j, e) = (float)((E * (double)MVF(p, c, j, e) + (double)db[c]) +
+                                    (double)trb);
+        MVF(p, 4, j, e) = (float)(E * (double)MVF(p, 4, j, e) + (double)db[4]);
+        if (o->src_type != MONO) {
+          MVF(p, 3, j, e) = (float)(E * (double)MVF(p, 3, j, e) + (double)db[3]);
+          MVF(p, 5, j, e) = (float)(E * (double)MVF(p, 5, j, e) + (double)db[5]);
+        }
+      }
+    }
+    for (int p = 0; p < NPP; p++) trm1[p] = str[p];
+    for (int c = 0; c < 6; c++)
+      for (int p = 0; p < NPP; p++) devm1[p + NPP * c] = sdev[c][p];
+  }
+}
+
+/* time_step_memvars / glob_anel_stiffness_* dispatch on att_coarse_grained
+ * (SOLVER/attenuation.f90:55-74, stiffness_mono.f90:285-303) */
+static void anel_stiff(orc_t *o, float *acc) {
+  if (o->cg) anel_stiff_cg4(o, acc);
+  else anel_stiff_full(o, acc);
+}
+static void memvars(orc_t *o, const float *disp) {
+  if (o->cg) memvars_cg4(o, disp);
+  else memvars_full(o, disp);
+}
+/* memory-variable storage per element: (1:4,6,n_sls) or (0:4,0:4,6,n_sls) */
+static size_t mv_per_el(const orc_t *o) { return (size_t)(o->cg ? 4 : NPP) * 6 * o->n_sls; }
+static size_t pts_per_el(const orc_t *o) { return o->cg ? 4 : NPP; }
+
+/* ------------------------------------------------------------------------ */
 /* assembly: pdistsum_solid / pdistsum_fluid (SOLVER/commun.F90:69-283)      */
 /* ------------------------------------------------------------------------ */
 /* The gather walks the 16 edge points of every element in the order of
@@ -913,13 +1215,13 @@ static void newmark_step(orc_t *o) {
   if (o->nel_f) pdistsum(o->igloc_f, o->gvec_f, o->nglob_f, o->ddchi1, o->nel_f, 1);
   solid_mask(o, o->disp);
   solid_stiffness(o, o->disp, o->acc1);
-  if (o->anel) anel_stiff_cg4(o, o->acc1);
+  if (o->anel) anel_stiff(o, o->acc1);
   for (size_t p = 0; p < nf; p++) o->ddchi1[p] = -o->inv_mass_fluid[p] * o->ddchi1[p];
   if (o->gamma_f) absorb(o->ddchi1, o->gamma_f, o->dchi, o->chi, nf);
   copy2solid(o, o->acc1, o->ddchi1);
   solid_mask(o, o->acc1);
   pdistsum(o->igloc_s, o->gvec_s, o->nglob_s, o->acc1, o->nel_s, 3);
-  if (o->anel) memvars_cg4(o, o->disp);
+  if (o->anel) memvars(o, o->disp);
   for (size_t p = 0; p < nf; p++) {
     o->dchi[p] = (float)((double)o->dchi[p] +
                          hdt * (double)(o->ddchi0[p] + o->ddchi1[p]));
@@ -964,7 +1266,7 @@ static void symplectic_step(orc_t *o) {
     if (o->nel_f) pdistsum(o->igloc_f, o->gvec_f, o->nglob_f, ddchi, o->nel_f, 1);
     solid_mask(o, o->disp);
     solid_stiffness(o, o->disp, acc);
-    if (o->anel) anel_stiff_cg4(o, acc);
+    if (o->anel) anel_stiff(o, acc);
     for (size_t p = 0; p < nf; p++) ddchi[p] = -ddchi[p] * o->inv_mass_fluid[p];
     if (o->gamma_f) absorb(ddchi, o->gamma_f, o->dchi, o->chi, nf);
     copy2solid(o, acc, ddchi);
@@ -996,7 +1298,7 @@ static void symplectic_step(orc_t *o) {
     float *d = o->disp + c * ns, *v = o->velo + c * ns;
     for (size_t p = 0; p < ns; p++) d[p] = (float)((double)d[p] + (double)v[p] * cd);
   }
-  if (o->anel) memvars_cg4(o, o->disp);
+  if (o->anel) memvars(o, o->disp);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1085,7 +1387,28 @@ orc_t *orc_load(const char *path) {
   o->ielsrc = find(o, "ielsrc"); o->source_term = find(o, "source_term");
   o->stf = find(o, "stf"); o->nstf = find_n(o, "stf");
   o->recfile_el = find(o, "recfile_el");
-  if (o->anel) {
+  if (o->anel && !o->cg) {
+    o->Ycg = find(o, "Y"); o->Vseta = find(o, "V_s_eta");
+    o->Vsxi = find(o, "V_s_xi"); o->Vzeta = find(o, "V_z_eta");
+    o->Vzxi = find(o, "V_z_xi");
+    o->Y0 = find(o, "Y0"); o->V0seta = find(o, "V0_s_eta");
+    o->V0sxi = find(o, "V0_s_xi"); o->V0zeta = find(o, "V0_z_eta");
+    o->V0zxi = find(o, "V0_z_xi");
+    o->DsDeta = find(o, "DsDeta_over_J_sol");
+    o->DzDeta = find(o, "DzDeta_over_J_sol");
+    o->DsDxi = find(o, "DsDxi_over_J_sol");
+    o->DzDxi = find(o, "DzDxi_over_J_sol");
+    o->inv_s = find(o, "inv_s_solid");
+    o->dmu = find(o, "delta_mu"); o->dkappa = find(o, "delta_kappa");
+    o->expw = find(o, "exp_w_j_deltat"); o->tsfac_t = find(o, "ts_fac_t");
+    o->tsfac_tm1 = find(o, "ts_fac_tm1"); o->aj_mu = find(o, "a_j_mu");
+    o->aj_kappa = find(o, "a_j_kappa");
+    if (!o->Ycg || !o->Y0 || !o->V0zxi || !o->DsDeta || !o->inv_s || !o->dkappa ||
+        !o->expw || !o->aj_mu) {
+      orc_free(o);
+      return NULL;
+    }
+  } else if (o->anel) {
     o->Ycg = find(o, "Y_cg4"); o->Vseta = find(o, "V_s_eta_cg4");
     o->Vsxi = find(o, "V_s_xi_cg4"); o->Vzeta = find(o, "V_z_eta_cg4");
     o->Vzxi = find(o, "V_z_xi_cg4");
@@ -1115,9 +1438,9 @@ orc_t *orc_load(const char *path) {
   o->chi = zalloc(nf); o->dchi = zalloc(nf); o->ddchi0 = zalloc(nf); o->ddchi1 = zalloc(nf);
   o->gvec_s = zalloc(3 * (size_t)o->nglob_s); o->gvec_f = zalloc((size_t)o->nglob_f);
   if (o->anel) {
-    o->memvar = zalloc(24 * (size_t)o->n_sls * o->nel_s);
-    o->src_tr_tm1 = zalloc(4 * (size_t)o->nel_s);
-    o->src_dev_tm1 = zalloc(24 * (size_t)o->nel_s);
+    o->memvar = zalloc(mv_per_el(o) * o->nel_s);
+    o->src_tr_tm1 = zalloc(pts_per_el(o) * o->nel_s);
+    o->src_dev_tm1 = zalloc(6 * pts_per_el(o) * o->nel_s);
   }
   o->nstages = 1;
   if (o->scheme != 0) {
@@ -1187,18 +1510,18 @@ void orc_anel_stiffness(orc_t *o, const float *memvar, float *acc) {
   enable_ftz();
   float *keep = o->memvar;
   o->memvar = (float *)memvar;
-  anel_stiff_cg4(o, acc);
+  anel_stiff(o, acc);
   o->memvar = keep;
 }
 void orc_memvar_update(orc_t *o, const float *disp) {
   enable_ftz();
-  memvars_cg4(o, disp);
+  memvars(o, disp);
 }
 void orc_get_memvars(orc_t *o, float *memvar, float *tr, float *dev) {
   if (!o->anel) return;
-  if (memvar) memcpy(memvar, o->memvar, 24 * (size_t)o->n_sls * o->nel_s * sizeof(float));
-  if (tr) memcpy(tr, o->src_tr_tm1, 4 * (size_t)o->nel_s * sizeof(float));
-  if (dev) memcpy(dev, o->src_dev_tm1, 24 * (size_t)o->nel_s * sizeof(float));
+  if (memvar) memcpy(memvar, o->memvar, mv_per_el(o) * o->nel_s * sizeof(float));
+  if (tr) memcpy(tr, o->src_tr_tm1, pts_per_el(o) * o->nel_s * sizeof(float));
+  if (dev) memcpy(dev, o->src_dev_tm1, 6 * pts_per_el(o) * o->nel_s * sizeof(float));
 }
 void orc_assemble_solid(orc_t *o, float *vec, int nc) {
   enable_ftz();

@@ -50,6 +50,14 @@ CASES = {
                                             REGIONAL_STATIONS),
     "reg_dipole_thetaforce_symplec4_anel": RunConfig("thetaforce", 30.0, 1e20, "symplec4", True,
                                                      600.0, REGIONAL_STATIONS),
+    # memory variables at all 25 points of an element (COARSE_GRAINED false):
+    # monopole Newmark, dipole symplectic, quadrupole Newmark
+    "mono_explosion_newmark_anelfull": RunConfig("explosion", 100.0, 1e20, "newmark2", True, 600.0,
+                                                 STATIONS, coarse_grained=False),
+    "dipole_thetaforce_symplec4_anelfull": RunConfig("thetaforce", 50.0, 1e20, "symplec4", True,
+                                                     500.0, STATIONS, coarse_grained=False),
+    "quad_mtp_newmark_anelfull": RunConfig("mtp", 300.0, 1e20, "newmark2", True, 600.0, STATIONS,
+                                           coarse_grained=False),
 }
 
 # Source batches: several sources on the mesh above with one source type and

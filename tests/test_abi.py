@@ -117,9 +117,11 @@ def test_inputs_pass_layout_checks_and_numbering_is_igloc(state_cache):
             # bijection between the two labellings
             pairs = np.unique(np.stack([ig, gid]), axis=1)
             assert pairs.shape[1] == n.value
-            if dom == 1 or "solid_absorbing_gamma" in st.arrays:
-                # fluid, and the solid of a mesh with absorbing boundaries
-                # (every point assembled): by first appearance in element order
+            full_memvars = st.dims["anel"] and not st.dims["coarse_grained"]
+            if dom == 1 or "solid_absorbing_gamma" in st.arrays or full_memvars:
+                # fluid, and the solid of a mesh with absorbing boundaries or
+                # memory variables at all points (every point assembled): by
+                # first appearance in element order
                 first = np.unique(gid, return_index=True)[1]
                 assert np.all(np.diff(first) > 0)
             else:
