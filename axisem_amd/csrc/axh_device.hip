@@ -119,6 +119,7 @@ struct DevAlloc {
 struct MeshDev {
   DevAlloc mem;
   int src = 0, scheme = 0, anel = 0, nsls = 0, nstages = 1;
+  int anel_full = 0;  // memory variables at all 25 points (anel is then 0)
   int nel_s = 0, nel_f = 0, ntile_s = 0, ntile_f = 0;
   int nglob_s = 0, nglob_f = 0, nbdry = 0, naxb = 0;
   double coefd[16] = {0}, coefv[16] = {0};
@@ -147,6 +148,7 @@ struct MeshDev {
   int *cpl_slot_f = nullptr;  // per fluid slot: coupling entry or -1
   // attenuation
   float *cg_stage = nullptr, *cg_upd = nullptr;
+  float *af_terms = nullptr, *af_axis = nullptr;  // anel_full
   double *aj_mu = nullptr, *aj_k = nullptr;
 };
 
@@ -1278,6 +1280,278 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
 }
 
 // ---------------------------------------------------------------------------
+// Memory variables at all 25 points of an element (att_coarse_grained
+// false), one wavefield: the anelastic forces of glob_anel_stiffness_*_4
+// (SOLVER/stiffness_mono.f90:409-507, stiffness_di.f90:604-723,
+// stiffness_quad.f90:555-668), subtracted from the element forces the
+// stiffness kernel wrote (before the assembly adds the coupling terms, the
+// reference's order), and time_step_memvars_4 (SOLVER/attenuation.f90:210-334)
+// with the strain of compute_strain_att_el_4 (attenuation.f90:542-606).  One
+// block per solid tile, a lane per element point (i = k % 5, j = k / 5).
+// Newmark runs both parts in one launch (the update reads the displacement the
+// forces were computed from); a symplectic step runs the forces after every
+// stage and the update once after the last drift.
+// ---------------------------------------------------------------------------
+constexpr int kAfTerms = 12;
+enum { AF_Y = 0, AF_VSE, AF_VSX, AF_VZE, AF_VZX, AF_DSDE, AF_DZDE, AF_DSDX, AF_DZDX, AF_INVS,
+       AF_DMU, AF_DK };
+
+struct AnelFullArgs {
+  int nel, nsls, nglob, nslot;
+  const float *G;         // G0 | G1 | G1T | G2 | G2T
+  const float *terms;     // [tile][kAfTerms][256]
+  const float *axis;      // [nel][5][5]: Y0, V0_s_eta, V0_s_xi, V0_z_eta, V0_z_xi
+  const int *einfo, *gid;
+  const double *aj_mu, *aj_k;  // [nel][nsls]
+  double expw[kMaxSls], tst[kMaxSls], tstm1[kMaxSls];
+  const float *disp;      // [3][nglob]
+  float *force;           // [3][nslot]
+  float *memvar;          // [tile][nsls][6][256]
+  float *trm1;            // [tile][256]
+  float *devm1;           // [tile][6][256]
+};
+
+template <int SRC, bool STIFF, bool UPD>
+__global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
+  __shared__ float sG[105];
+  __shared__ float sA[6][256];  // S terms (forces), then the strain's fields
+  __shared__ float sR[6][256];  // sums of the memory variables
+  const int t = threadIdx.x, tile = blockIdx.x;
+  const int le = t / 25, k = t - 25 * le;
+  const int i = k % 5, j = k / 5;
+  const int e = tile * kElemPerTile + le;
+  const bool valid = (t < 250) && (e < A.nel);
+  const int eb = le * 25;  // the element's first lane
+  const size_t slot = (size_t)tile * 256 + t;
+  stage_g(A.G, sG);
+  const float *G0 = sG, *G1 = sG + 5, *G1T = sG + 30, *G2 = sG + 55, *G2T = sG + 80;
+  const bool ax = valid && A.einfo[e] >= 0;
+  const float *Gs = ax ? G1 : G2, *GsT = ax ? G1T : G2T;
+  auto term = [&](int a) {
+    return valid ? A.terms[((size_t)tile * kAfTerms + a) * 256 + t] : 0.0f;
+  };
+  const size_t mvs = (size_t)256 * 6;  // one SLS of a tile
+  float *mv = A.memvar + (size_t)tile * A.nsls * mvs + t;
+  if (STIFF) {
+    float r[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      r[c] = 0.0f;
+      if (valid && (SRC != AXH_MONOPOLE || (c != 3 && c != 5)))
+        for (int q = 0; q < A.nsls; q++) r[c] = r[c] + mv[q * mvs + c * 256];
+      sR[c][t] = r[c];
+    }
+    const float r1 = r[0], r2 = r[1], r3 = r[2], r4 = r[3], r5 = r[4], r6 = r[5];
+    const float yl = term(AF_Y), vse = term(AF_VSE), vsx = term(AF_VSX), vze = term(AF_VZE),
+                vzx = term(AF_VZX);
+    if (SRC == AXH_MONOPOLE) {
+      sA[0][t] = vze * r1 + vse * r5;
+      sA[1][t] = vzx * r1 + vsx * r5;
+      sA[4][t] = vze * r5 + vse * r3;
+      sA[5][t] = vzx * r5 + vsx * r3;
+    } else if (SRC == AXH_DIPOLE) {
+      sA[0][t] = vze * (r1 - r6) + vse * (r5 - r4);
+      sA[1][t] = vzx * (r1 - r6) + vsx * (r5 - r4);
+      sA[2][t] = vze * (r1 + r6) + vse * (r5 + r4);
+      sA[3][t] = vzx * (r1 + r6) + vsx * (r5 + r4);
+      sA[4][t] = vze * r5 + vse * r3;
+      sA[5][t] = vzx * r5 + vsx * r3;
+    } else {
+      sA[0][t] = vze * r1 + vse * r5;
+      sA[1][t] = vzx * r1 + vsx * r5;
+      sA[2][t] = vze * r6 + vse * r4;
+      sA[3][t] = vzx * r6 + vsx * r4;
+      sA[4][t] = vze * r5 + vse * r3;
+      sA[5][t] = vzx * r5 + vsx * r3;
+    }
+    __syncthreads();
+    if (valid) {
+      const float X1 = amul<1>(Gs, &sA[0][eb], i, j), X2 = mulb<1>(&sA[1][eb], G2T, i, j);
+      const float X5 = amul<1>(Gs, &sA[4][eb], i, j), X6 = mulb<1>(&sA[5][eb], G2T, i, j);
+      float l1, l2 = 0.0f, l3;
+      if (SRC == AXH_MONOPOLE) {
+        l1 = X1 + X2 + yl * r2;
+        l3 = X5 + X6;
+      } else {
+        const float X3 = amul<1>(Gs, &sA[2][eb], i, j), X4 = mulb<1>(&sA[3][eb], G2T, i, j);
+        if (SRC == AXH_DIPOLE) {
+          l1 = X1 + X2;
+          l2 = X3 + X4 + 2.0f * yl * (r2 - r6);
+          l3 = X5 + X6 - yl * r4;
+        } else {
+          l1 = X1 + X2 + yl * (r2 - 2.0f * r6);
+          l2 = -X3 - X4 + yl * (r6 - 2.0f * r2);
+          l3 = X5 + X6 - 2.0f * yl * r4;
+        }
+      }
+      if (ax) {
+        // the axial vectors at this lane's j, from the sums at (0, j)
+        const float *av = A.axis + (size_t)e * 25;
+        const float y0 = av[j], v0se = av[5 + j], v0sx = av[10 + j], v0ze = av[15 + j],
+                    v0zx = av[20 + j];
+        const int p0 = eb + 5 * j;
+        const float r10 = sR[0][p0], r20 = sR[1][p0], r30 = sR[2][p0], r40 = sR[3][p0],
+                    r50 = sR[4][p0], r60 = sR[5][p0];
+        const float g0 = G0[i];
+        if (SRC == AXH_MONOPOLE) {
+          const float V1 = v0ze * r10 + v0se * r50 + y0 * r20;
+          const float V2 = v0ze * r50 + v0se * r30;
+          l1 = l1 + g0 * V1;
+          l3 = l3 + g0 * V2;
+          if (i == 0) {
+            // vxm_4(V3, G2T): V3 at every point (0, kk) of the element
+            float V4 = 0.0f;
+            for (int kk = 0; kk < 5; kk++) {
+              const int pk = eb + 5 * kk;
+              const float V3 = av[20 + kk] * sR[4][pk] + av[10 + kk] * sR[2][pk];
+              V4 = V4 + V3 * G2T[kk + 5 * j];
+            }
+            l3 = l3 + V4;
+          }
+        } else if (SRC == AXH_DIPOLE) {
+          const float V1 = v0ze * (r10 - r60) + v0se * (r50 - r40);
+          const float V2 = v0zx * (r10 - r60) + v0sx * (r50 - r40);
+          l1 = l1 + g0 * V1;
+          if (i == 0) {
+            float V3 = 0.0f;
+            for (int kk = 0; kk < 5; kk++) {
+              const int pk = eb + 5 * kk;
+              const float V2k = av[20 + kk] * (sR[0][pk] - sR[5][pk]) +
+                                av[10 + kk] * (sR[4][pk] - sR[3][pk]);
+              V3 = V3 + V2k * G2T[kk + 5 * j];
+            }
+            l1 = l1 + V3;
+          }
+          const float Vm = v0ze * (r10 + r60) + v0se * (r50 + r40) + y0 * 2.0f * (r20 - r60);
+          l2 = l2 + g0 * Vm;
+          // the z-component adds outerprod(G0, V2) (stiffness_di.f90:706-707)
+          l3 = l3 + g0 * V2;
+        } else {
+          const float Vs = v0ze * r10 + y0 * (r20 - 2.0f * r60);
+          const float Vp = -(v0ze * r60) + y0 * (r60 - 2.0f * r20);
+          const float Vz = v0se * r30;
+          l1 = l1 + g0 * Vs;
+          l2 = l2 + g0 * Vp;
+          l3 = l3 + g0 * Vz;
+        }
+      }
+      float *f0 = A.force + slot, *f2 = A.force + 2 * (size_t)A.nslot + slot;
+      *f0 = *f0 - l1;
+      if (SRC != AXH_MONOPOLE) {
+        float *f1 = A.force + (size_t)A.nslot + slot;
+        *f1 = *f1 - l2;
+      }
+      *f2 = *f2 - l3;
+    }
+    if (UPD) __syncthreads();  // sA is reused below
+  }
+  if (UPD) {
+    const int g = valid ? A.gid[slot] : 0;
+    const float u1 = valid ? A.disp[g] : 0.0f;
+    const float u2 = (valid && SRC != AXH_MONOPOLE) ? A.disp[(size_t)A.nglob + g] : 0.0f;
+    const float u3 = valid ? A.disp[2 * (size_t)A.nglob + g] : 0.0f;
+    // fields whose gradients the strain needs (f1 = u_z throughout)
+    if (SRC == AXH_MONOPOLE) {
+      sA[0][t] = u1;
+      sA[1][t] = u3;
+    } else if (SRC == AXH_DIPOLE) {
+      sA[0][t] = u1 + u2;
+      sA[1][t] = u3;
+      sA[2][t] = u2;
+      sA[3][t] = u1 - u2;
+    } else {
+      sA[0][t] = u1;
+      sA[1][t] = u3;
+      sA[2][t] = u2;
+      sA[3][t] = u1 - 2.0f * u2;
+      sA[4][t] = u2 - 2.0f * u1;
+    }
+    __syncthreads();
+    if (!valid) return;  // no barrier follows
+    const float dse = term(AF_DSDE), dze = term(AF_DZDE), dsx = term(AF_DSDX),
+                dzx = term(AF_DZDX), is = term(AF_INVS);
+    // axisym_gradient_solid_el_4: d/ds and d/dz of field f at this point
+    auto grad = [&](int f, float &ds, float &dz) {
+      const float m1 = amul<1>(GsT, &sA[f][eb], i, j);
+      const float m2 = mulb<1>(&sA[f][eb], G2, i, j);
+      ds = dze * m1 + dzx * m2;
+      dz = dse * m1 + dsx * m2;
+    };
+    // f_over_s_solid_el_4: f/s, and d/ds on the axis (dsdf_elem_solid)
+    auto fos = [&](int f, float val) {
+      if (ax && i == 0) {
+        float ds, dz;
+        grad(f, ds, dz);
+        return ds;
+      }
+      return is * val;
+    };
+    float E[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float b1s, b1z, b2s, b2z;
+    grad(0, b1s, b1z);
+    grad(1, b2s, b2z);
+    E[0] = b1s;
+    E[2] = b2z;
+    E[4] = b1z + b2s;
+    if (SRC == AXH_MONOPOLE) {
+      E[1] = fos(0, u1);
+    } else if (SRC == AXH_DIPOLE) {
+      E[1] = 2.0f * fos(2, u2);
+      float ps, pz;
+      grad(3, ps, pz);
+      const float f3 = fos(1, u3);
+      E[3] = -f3 - pz;
+      E[5] = -E[1] - ps;
+    } else {
+      E[1] = fos(3, u1 - 2.0f * u2);
+      float ps, pz;
+      grad(2, ps, pz);
+      const float f3 = fos(1, u3);
+      const float f4 = fos(4, u2 - 2.0f * u1);
+      E[3] = -(2.0f * f3) - pz;
+      E[5] = f4 - ps;
+    }
+    const double third = 1.0 / 3.0;
+    const float tr = E[0] + E[1] + E[2];
+    const float dm = term(AF_DMU), dk = term(AF_DK);
+    const float str = dk * tr;
+    float sdev[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      sdev[c] = (float)((double)(dm * 2.0f) * ((double)E[c] - (double)tr * third));
+    sdev[4] = dm * E[4];
+    if (SRC != AXH_MONOPOLE) {
+      sdev[3] = dm * E[3];
+      sdev[5] = dm * E[5];
+    }
+    float *ptr = A.trm1 + slot;
+    float *pdev = A.devm1 + (size_t)tile * mvs + t;
+    const float trm1 = *ptr;
+    float devm1[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) devm1[c] = pdev[c * 256];
+    const double *amu = A.aj_mu + (size_t)A.nsls * e, *akap = A.aj_k + (size_t)A.nsls * e;
+    for (int q = 0; q < A.nsls; q++) {
+      const double Tt = A.tst[q], Tm = A.tstm1[q], Ew = A.expw[q];
+      const float trb = (float)(Tt * akap[q] * (double)str + Tm * akap[q] * (double)trm1);
+      float *m = mv + q * mvs;
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
+        const float db = (float)(Tt * amu[q] * (double)sdev[c] + Tm * amu[q] * (double)devm1[c]);
+        const double old = (double)m[c * 256];
+        m[c * 256] = c < 3 ? (float)((Ew * old + (double)db) + (double)trb)
+                           : (float)(Ew * old + (double)db);
+      }
+    }
+    *ptr = str;
+#pragma unroll
+    for (int c = 0; c < 6; c++) pdev[c * 256] = sdev[c];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 // predictor / drift per global point (SOLVER/time_evol_wave.F90:357-367,
 // 597-605, 720-725) with the axis masks (SOLVER/apply_masks.f90) and the
 // blow-up test of runtime_info (SOLVER/time_evol_wave.F90:1042)
@@ -1754,6 +2028,9 @@ struct MeshHost {
       cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, cpl_slot_s, cpl_slot_f;
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
+  // memory variables at all points: packed [tile][kAfTerms][256] terms and
+  // the axial vectors [nel][5][5]
+  std::vector<float> af_terms, af_axis;
   std::vector<int> first_order_s;  // solid points in first-appearance order
   // absorbing boundaries: damping factor per point (empty when there are none)
   std::vector<float> gamma_s, gamma_f;
@@ -2027,6 +2304,21 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
         for (int q = 0; q < 4; q++)
           H.cg_upd[((size_t)e * kCgUpd + a) * 4 + q] = p[(size_t)e * 4 + q];
     }
+  }
+  if (anel_full) {
+    pack_terms(ctx, {"Y", "V_s_eta", "V_s_xi", "V_z_eta", "V_z_xi", "DsDeta_over_J_sol",
+                     "DzDeta_over_J_sol", "DsDxi_over_J_sol", "DzDxi_over_J_sol", "inv_s_solid",
+                     "delta_mu", "delta_kappa"},
+               H.nel_s, H.af_terms);
+    const char *axis_names[5] = {"Y0", "V0_s_eta", "V0_s_xi", "V0_z_eta", "V0_z_xi"};
+    H.af_axis.assign((size_t)H.nel_s * 25, 0.0f);
+    for (int a = 0; a < 5; a++) {
+      const float *p = ctx->need(axis_names[a]).as<float>();
+      for (int e = 0; e < H.nel_s; e++)
+        std::memcpy(&H.af_axis[(size_t)e * 25 + 5 * a], p + (size_t)e * 5, 5 * sizeof(float));
+    }
+  }
+  if (d.anel) {
     const double *ew = ctx->need("exp_w_j_deltat").as<double>();
     const double *tt = ctx->need("ts_fac_t").as<double>();
     const double *tm = ctx->need("ts_fac_tm1").as<double>();
@@ -2177,7 +2469,8 @@ void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]) {
   h.bytes(H.tst, sizeof H.tst);
   h.bytes(H.tstm1, sizeof H.tstm1);
   for (const auto *v : {&H.G, &H.coef_s, &H.m0_s, &H.minv_s, &H.coef_f, &H.m0_f, &H.minv_f,
-                        &H.cpl_bm, &H.cg_stage, &H.cg_upd, &H.gamma_s, &H.gamma_f})
+                        &H.cpl_bm, &H.cg_stage, &H.cg_upd, &H.gamma_s, &H.gamma_f,
+                        &H.af_terms, &H.af_axis})
     h.vec(*v);
   for (const auto *v : {&H.einfo_s, &H.gid_s, &H.gstart_s, &H.gslot_s, &H.einfo_f, &H.gid_f,
                         &H.gstart_f, &H.gslot_f, &H.cpl_ss, &H.cpl_gs, &H.cpl_sf, &H.cpl_gf,
@@ -2205,9 +2498,6 @@ void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *ngl
 
 static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   const axh_dims &d = ctx->dims;
-  if (d.anel && !d.coarse_grained)
-    fail(AXH_ERR_UNSUPPORTED, "memory variables at all 25 points (COARSE_GRAINED false) "
-                              "are not stepped on the GPU");
   MeshHost H;
   build_mesh_layout(ctx, H);
   int ndev = 0;
@@ -2217,7 +2507,8 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   MeshDev &V = *M;
   V.src = d.src_type;
   V.scheme = d.scheme;
-  V.anel = d.anel;
+  V.anel = d.anel && d.coarse_grained;
+  V.anel_full = d.anel && !d.coarse_grained;
   V.nsls = d.n_sls;
   V.nel_s = H.nel_s;
   V.nel_f = H.nel_f;
@@ -2268,6 +2559,12 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   if (V.anel) {
     V.cg_stage = m.upload(H.cg_stage);
     V.cg_upd = m.upload(H.cg_upd);
+  }
+  if (V.anel_full) {
+    V.af_terms = m.upload(H.af_terms);
+    V.af_axis = m.upload(H.af_axis);
+  }
+  if (V.anel || V.anel_full) {
     V.aj_mu = m.upload(H.aj_mu);
     V.aj_k = m.upload(H.aj_k);
   }
@@ -2339,6 +2636,12 @@ static std::vector<std::pair<float *, size_t>> state_arrays(const MeshDev &M, Fi
     v.push_back({F.trm1, (size_t)M.nel_s * 4});
     v.push_back({F.devm1, (size_t)M.nel_s * 24});
   }
+  if (M.anel_full) {
+    const size_t ns = (size_t)M.ntile_s * kTile;
+    v.push_back({F.memvar, ns * 6 * M.nsls});
+    v.push_back({F.trm1, ns});
+    v.push_back({F.devm1, ns * 6});
+  }
   return v;
 }
 
@@ -2407,6 +2710,14 @@ static void fields_alloc(const MeshDev &M, Fields &F, int B) {
     F.rsum = m.zeros<float>((size_t)M.nel_s * 24 * b);
     F.trm1 = m.zeros<float>((size_t)M.nel_s * 4 * b);
     F.devm1 = m.zeros<float>((size_t)M.nel_s * 24 * b);
+  }
+  if (M.anel_full) {
+    if (B != 1)
+      fail(AXH_ERR_UNSUPPORTED, "memory variables at all points: one wavefield per group");
+    const size_t ns = (size_t)M.ntile_s * kTile;
+    F.memvar = m.zeros<float>(ns * 6 * M.nsls);
+    F.trm1 = m.zeros<float>(ns);
+    F.devm1 = m.zeros<float>(ns * 6);
   }
 }
 
@@ -2564,7 +2875,8 @@ void batch_attach(axh_batch *bt) {
   S->run.init();
   // groups of at most kGroup members, sizes as equal as possible
   const int n = (int)members.size();
-  const int width = std::max(1, std::min(bt->group_width, kGroup));
+  // (memory variables at all points are stepped one wavefield per group)
+  const int width = mesh->anel_full ? 1 : std::max(1, std::min(bt->group_width, kGroup));
   const int ng = (n + width - 1) / width;
   int next = 0;
   for (int g = 0; g < ng; g++) {
@@ -2664,6 +2976,17 @@ static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
 #undef AXH_SOLID_CASE
 }
 
+template <bool STIFF, bool UPD>
+static void dispatch_anel_full(const MeshDev &M, hipStream_t s, const AnelFullArgs &A) {
+  const dim3 grid(M.ntile_s), blk(256);
+  if (M.src == AXH_MONOPOLE)
+    hipLaunchKernelGGL((k_anel_full<AXH_MONOPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+  else if (M.src == AXH_DIPOLE)
+    hipLaunchKernelGGL((k_anel_full<AXH_DIPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+  else
+    hipLaunchKernelGGL((k_anel_full<AXH_QUADPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+}
+
 template <int B>
 static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A,
                             const RecArgs &R, int nrec_blk) {
@@ -2724,6 +3047,18 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
     for (size_t q = 0; q < F.src_tiles.size(); q++) SA.src_tile[q] = F.src_tiles[q];
   }
   SA.disp_w = F.disp; SA.hdt = hdt;
+  AnelFullArgs AN{};
+  if (M.anel_full) {
+    if (B != 1) fail(AXH_ERR_STATE, "memory variables at all points: one wavefield per group");
+    AN.nel = M.nel_s; AN.nsls = M.nsls; AN.nglob = M.nglob_s; AN.nslot = (int)nslot_s;
+    AN.G = M.G; AN.terms = M.af_terms; AN.axis = M.af_axis; AN.einfo = M.einfo_s;
+    AN.gid = M.gid_s; AN.aj_mu = M.aj_mu; AN.aj_k = M.aj_k;
+    for (int s = 0; s < kMaxSls; s++) {
+      AN.expw[s] = M.expw[s]; AN.tst[s] = M.tst[s]; AN.tstm1[s] = M.tstm1[s];
+    }
+    AN.disp = F.disp; AN.force = F.force_s; AN.memvar = F.memvar; AN.trm1 = F.trm1;
+    AN.devm1 = F.devm1;
+  }
   FluidArgs FA{};
   FA.nel = M.nel_f; FA.multipole = multipole; FA.G = M.G; FA.coef = M.coef_f;
   FA.m0 = M.m0_f; FA.einfo = M.einfo_f; FA.gid = M.gid_f; FA.chi = F.chi;
@@ -2812,6 +3147,13 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       SA.cd_next = newmark ? 0.0 : M.coefd[st + 1];
       L(KC_SOLID, [&] { dispatch_solid<B>(M, st_, SA, FA, newmark ? 0 : 1); });
       SA.direct = 0;
+      if (M.anel_full) {
+        // anelastic forces; Newmark also updates the memory variables here
+        L(KC_MEMVAR, [&] {
+          if (newmark) dispatch_anel_full<true, true>(M, st_, AN);
+          else dispatch_anel_full<true, false>(M, st_, AN);
+        });
+      }
       if (M.nel_f > 0) {
         AF.cv = newmark ? 0.0 : M.coefv[st];
         L(KC_ASM_F, [&] {
@@ -2833,6 +3175,8 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         any = true;
       }
     }
+    if (!newmark && M.anel_full)
+      L(KC_MEMVAR, [&] { dispatch_anel_full<false, true>(M, st_, AN); });
     if (!newmark && M.anel) {
       // the receiver samples ride along as extra blocks of the update
       const int nrb = any ? nblk(maxrec) * B : 0;

@@ -24,12 +24,6 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 
-def _skip_unstepped(cfg):
-    if cfg.attenuation and not cfg.coarse_grained:
-        pytest.skip("memory variables at all 25 points (COARSE_GRAINED false) are not "
-                    "stepped on the GPU yet; the oracle is checked against the reference")
-
-
 def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
@@ -38,7 +32,6 @@ def _unlimited_stack():
 def test_dropin_host_matches_reference(name, host_exe, tmp_path):
     print("drop-in GPU run:", name, flush=True)
     cfg = cases.CASES[name]
-    _skip_unstepped(cfg)
     rundir = str(tmp_path / name)
     inputs.write_run_dir(rundir, cases.mesh_of(name), cfg)
     with open(os.path.join(rundir, "OUTPUT"), "w") as log:
@@ -91,7 +84,6 @@ ORACLE_RUNS = sorted({**cases.CASES, **cases.ORACLE_CASES})
 @pytest.mark.parametrize("name", ORACLE_RUNS)
 def test_uploads_match_oracle(name, state_cache):
     print("C-ABI uploads vs oracle:", name, flush=True)
-    _skip_unstepped(cases.ALL_CONFIGS[name])
     _, st = state_cache(name)
     d = read_dims(st)
     n = min(d["niter"], 2000)
@@ -128,7 +120,6 @@ def _random_fields(st, seed):
 @pytest.mark.parametrize("name", ORACLE_RUNS)
 def test_random_fields_match_oracle(name, state_cache):
     print("random initial fields vs oracle:", name, flush=True)
-    _skip_unstepped(cases.ALL_CONFIGS[name])
     _, st_path = state_cache(name)
     st = read_state(st_path)
     fields = _random_fields(st, seed=zlib.crc32(name.encode()))
