@@ -84,6 +84,11 @@ BATCH_MEMBERS = {
     "b_quad_mtt_m_mpp_20": RunConfig("mtt_m_mpp", 20.0, 1e20, "symplec4", True, 500.0,
                                      STATIONS),
     "b_quad_mtp_2600": RunConfig("mtp", 2600.0, 1e20, "symplec4", True, 500.0, STATIONS),
+    # memory variables at all 25 points (stepped one wavefield per group)
+    "bf_dipole_thetaforce_50": RunConfig("thetaforce", 50.0, 1e20, "symplec4", True, 400.0,
+                                         STATIONS, coarse_grained=False),
+    "bf_dipole_mtr_700": RunConfig("mtr", 700.0, 3e20, "symplec4", True, 400.0, STATIONS,
+                                   coarse_grained=False),
 }
 BATCHES = {
     "dipole_symplec4_anel": ["b_dipole_thetaforce_50", "b_dipole_phiforce_1000",
@@ -92,6 +97,7 @@ BATCHES = {
     # five members: one group of 4 and one of 1 at the default width
     "quad_symplec4_anel": ["b_quad_mtp_100", "b_quad_mtt_m_mpp_1500", "b_quad_mtp_400",
                            "b_quad_mtt_m_mpp_20", "b_quad_mtp_2600"],
+    "dipole_symplec4_anelfull": ["bf_dipole_thetaforce_50", "bf_dipole_mtr_700"],
 }
 # Cases the reference program cannot run (it lower-cases TIME_SCHEME and its
 # symplectic_coefficients then matches neither McLachlan scheme): the GPU is
