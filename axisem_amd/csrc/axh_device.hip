@@ -1290,7 +1290,10 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
 // block per solid tile, a lane per element point (i = k % 5, j = k / 5).
 // Newmark runs both parts in one launch (the update reads the displacement the
 // forces were computed from); a symplectic step runs the forces after every
-// stage and the update once after the last drift.
+// stage and the update once after the last drift.  The update also stores the
+// sums over the SLS of the new memory variables (from zero in SLS order, the
+// sums the forces form), so the stage launches read 6 floats per point instead
+// of 6 n_sls.
 // ---------------------------------------------------------------------------
 constexpr int kAfTerms = 12;
 enum { AF_Y = 0, AF_VSE, AF_VSX, AF_VZE, AF_VZX, AF_DSDE, AF_DZDE, AF_DSDX, AF_DZDX, AF_INVS,
@@ -1309,6 +1312,7 @@ struct AnelFullArgs {
   float *memvar;          // [tile][nsls][6][256]
   float *trm1;            // [tile][256]
   float *devm1;           // [tile][6][256]
+  float *rsum;            // [tile][6][256]: sums over the SLS
 };
 
 template <int SRC, bool STIFF, bool UPD>
@@ -1332,13 +1336,19 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
   };
   const size_t mvs = (size_t)256 * 6;  // one SLS of a tile
   float *mv = A.memvar + (size_t)tile * A.nsls * mvs + t;
+  float *const prs = A.rsum + (size_t)tile * mvs + t;
   if (STIFF) {
     float r[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       r[c] = 0.0f;
-      if (valid && (SRC != AXH_MONOPOLE || (c != 3 && c != 5)))
-        for (int q = 0; q < A.nsls; q++) r[c] = r[c] + mv[q * mvs + c * 256];
+      if (valid && (SRC != AXH_MONOPOLE || (c != 3 && c != 5))) {
+        if (UPD) {
+          for (int q = 0; q < A.nsls; q++) r[c] = r[c] + mv[q * mvs + c * 256];
+        } else {
+          r[c] = prs[c * 256];
+        }
+      }
       sR[c][t] = r[c];
     }
     const float r1 = r[0], r2 = r[1], r3 = r[2], r4 = r[3], r5 = r[4], r6 = r[5];
@@ -1531,6 +1541,7 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
 #pragma unroll
     for (int c = 0; c < 6; c++) devm1[c] = pdev[c * 256];
     const double *amu = A.aj_mu + (size_t)A.nsls * e, *akap = A.aj_k + (size_t)A.nsls * e;
+    float rs[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int q = 0; q < A.nsls; q++) {
       const double Tt = A.tst[q], Tm = A.tstm1[q], Ew = A.expw[q];
       const float trb = (float)(Tt * akap[q] * (double)str + Tm * akap[q] * (double)trm1);
@@ -1540,13 +1551,18 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
         if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
         const float db = (float)(Tt * amu[q] * (double)sdev[c] + Tm * amu[q] * (double)devm1[c]);
         const double old = (double)m[c * 256];
-        m[c * 256] = c < 3 ? (float)((Ew * old + (double)db) + (double)trb)
-                           : (float)(Ew * old + (double)db);
+        const float nv = c < 3 ? (float)((Ew * old + (double)db) + (double)trb)
+                               : (float)(Ew * old + (double)db);
+        m[c * 256] = nv;
+        rs[c] = rs[c] + nv;
       }
     }
     *ptr = str;
 #pragma unroll
-    for (int c = 0; c < 6; c++) pdev[c * 256] = sdev[c];
+    for (int c = 0; c < 6; c++) {
+      pdev[c * 256] = sdev[c];
+      if (!STIFF) prs[c * 256] = rs[c];  // read by the symplectic stages
+    }
   }
 }
 
@@ -2639,6 +2655,7 @@ static std::vector<std::pair<float *, size_t>> state_arrays(const MeshDev &M, Fi
   if (M.anel_full) {
     const size_t ns = (size_t)M.ntile_s * kTile;
     v.push_back({F.memvar, ns * 6 * M.nsls});
+    v.push_back({F.rsum, ns * 6});
     v.push_back({F.trm1, ns});
     v.push_back({F.devm1, ns * 6});
   }
@@ -2716,6 +2733,7 @@ static void fields_alloc(const MeshDev &M, Fields &F, int B) {
       fail(AXH_ERR_UNSUPPORTED, "memory variables at all points: one wavefield per group");
     const size_t ns = (size_t)M.ntile_s * kTile;
     F.memvar = m.zeros<float>(ns * 6 * M.nsls);
+    F.rsum = m.zeros<float>(ns * 6);
     F.trm1 = m.zeros<float>(ns);
     F.devm1 = m.zeros<float>(ns * 6);
   }
@@ -3057,7 +3075,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       AN.expw[s] = M.expw[s]; AN.tst[s] = M.tst[s]; AN.tstm1[s] = M.tstm1[s];
     }
     AN.disp = F.disp; AN.force = F.force_s; AN.memvar = F.memvar; AN.trm1 = F.trm1;
-    AN.devm1 = F.devm1;
+    AN.devm1 = F.devm1; AN.rsum = F.rsum;
   }
   FluidArgs FA{};
   FA.nel = M.nel_f; FA.multipole = multipole; FA.G = M.G; FA.coef = M.coef_f;

@@ -279,21 +279,27 @@ def copy_calibration(nbytes: int, reps: int = 5) -> None:
 
 # Other source types and time schemes on the 10 s mesh (not the headline
 # workload): plain and 4-wavefield step times, stiffness launch per kernel.
-SWEEP = (("mono_vertforce_newmark", "vertforce", "newmark2", False),
-         ("mono_vertforce_newmark_anel", "vertforce", "newmark2", True),
-         ("mono_explosion_symplec4_anel", "explosion", "symplec4", True),
-         ("dipole_thetaforce_newmark_anel", "thetaforce", "newmark2", True),
-         ("quad_mtp_symplec4_anel", "mtp", "symplec4", True))
+# name, source, scheme, attenuation, coarse-grained memory variables
+SWEEP = (("mono_vertforce_newmark", "vertforce", "newmark2", False, True),
+         ("mono_vertforce_newmark_anel", "vertforce", "newmark2", True, True),
+         ("mono_explosion_symplec4_anel", "explosion", "symplec4", True, True),
+         ("dipole_thetaforce_newmark_anel", "thetaforce", "newmark2", True, True),
+         ("quad_mtp_symplec4_anel", "mtp", "symplec4", True, True),
+         # memory variables at all 25 points (COARSE_GRAINED false; one
+         # wavefield per group, so the batched leg runs 4 groups of 1)
+         ("dipole_thetaforce_symplec4_anelfull", "thetaforce", "symplec4", True, False))
 
 
-def sweep(workdir: str, steps: int = 300) -> dict:
+def sweep(workdir: str, steps: int = 300, only=None) -> dict:
     from axisem_amd.solver import Batch, Solver
     out = {}
-    for name, src, scheme, anel in SWEEP:
+    for name, src, scheme, anel, cg in SWEEP:
+        if only and name not in only:
+            continue
         paths = []
         for depth in BATCH_DEPTHS_KM:
             cfg = inputs.RunConfig(source_type=src, source_depth_km=depth, source_amplitude=1e20,
-                                   time_scheme=scheme, attenuation=anel,
+                                   time_scheme=scheme, attenuation=anel, coarse_grained=cg,
                                    seismogram_length_s=1800.0,
                                    stations_deg=[float(k) for k in range(181)])
             paths.append(prepare_state(os.path.join(workdir, "%s_%g" % (name, depth)), cfg))
@@ -319,7 +325,8 @@ def sweep(workdir: str, steps: int = 300) -> dict:
             m.close()
         out[name] = {"plain_ms_per_step": plain, "batch4_ms_per_step": bat,
                      "batch4_vs_plain": 4 * plain / bat,
-                     "stiffness_us": [ks["stiffness"]["avg_us"], kb["stiffness"]["avg_us"]]}
+                     "stiffness_us": [ks["stiffness"]["avg_us"], kb["stiffness"]["avg_us"]],
+                     "kernels_plain": ks}
         log("sweep %s: %s" % (name, json.dumps(out[name])))
     return out
 
@@ -346,12 +353,16 @@ def main():
                     help="only run device copies of BYTES (counter calibration)")
     ap.add_argument("--sweep", action="store_true",
                     help="only time other source types and schemes (one JSON line)")
+    ap.add_argument("--sweep-only", default="",
+                    help="comma-separated sweep workloads (default: all)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
     if a.sweep:
-        print(json.dumps({"sweep": sweep(os.path.join(REPO, "data", "bench_sweep"))}))
+        only = [x for x in a.sweep_only.split(",") if x]
+        print(json.dumps({"sweep": sweep(os.path.join(REPO, "data", "bench_sweep"),
+                                         only=only)}))
         return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the time loop keeps the whole mesh resident on "
