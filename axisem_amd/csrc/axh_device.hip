@@ -1307,19 +1307,19 @@ struct AnelFullArgs {
   const int *einfo, *gid;
   const double *aj_mu, *aj_k;  // [nel][nsls]
   double expw[kMaxSls], tst[kMaxSls], tstm1[kMaxSls];
-  const float *disp;      // [3][nglob]
-  float *force;           // [3][nslot]
-  float *memvar;          // [tile][nsls][6][256]
-  float *trm1;            // [tile][256]
-  float *devm1;           // [tile][6][256]
-  float *rsum;            // [tile][6][256]: sums over the SLS
+  const float *disp;      // [3][nglob][B]
+  float *force;           // [3][nslot][B]
+  float *memvar;          // [tile][nsls][6][256][B]
+  float *trm1;            // [tile][256][B]
+  float *devm1;           // [tile][6][256][B]
+  float *rsum;            // [tile][6][256][B]: sums over the SLS
 };
 
-template <int SRC, bool STIFF, bool UPD>
+template <int SRC, bool STIFF, bool UPD, int B>
 __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
   __shared__ float sG[105];
-  __shared__ float sA[6][256];  // S terms (forces), then the strain's fields
-  __shared__ float sR[6][256];  // sums of the memory variables
+  __shared__ float sA[6][256 * B];  // S terms (forces), then the strain's fields
+  __shared__ float sR[6][256 * B];  // sums of the memory variables
   const int t = threadIdx.x, tile = blockIdx.x;
   const int le = t / 25, k = t - 25 * le;
   const int i = k % 5, j = k / 5;
@@ -1334,238 +1334,297 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
   auto term = [&](int a) {
     return valid ? A.terms[((size_t)tile * kAfTerms + a) * 256 + t] : 0.0f;
   };
-  const size_t mvs = (size_t)256 * 6;  // one SLS of a tile
-  float *mv = A.memvar + (size_t)tile * A.nsls * mvs + t;
-  float *const prs = A.rsum + (size_t)tile * mvs + t;
+  const size_t cs = (size_t)256 * B;  // one component of a tile
+  const size_t mvs = 6 * cs;          // one SLS of a tile
+  float *const mv = A.memvar + (size_t)tile * A.nsls * mvs + (size_t)t * B;
+  float *const prs = A.rsum + (size_t)tile * mvs + (size_t)t * B;
+  const int tb = t * B;
   if (STIFF) {
-    float r[6];
+    float r[6][B];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      r[c] = 0.0f;
+#pragma unroll
+      for (int w = 0; w < B; w++) r[c][w] = 0.0f;
       if (valid && (SRC != AXH_MONOPOLE || (c != 3 && c != 5))) {
         if (UPD) {
-          for (int q = 0; q < A.nsls; q++) r[c] = r[c] + mv[q * mvs + c * 256];
+          for (int q = 0; q < A.nsls; q++) {
+            float x[B];
+            ldv<B>(mv + q * mvs + c * cs, x);
+#pragma unroll
+            for (int w = 0; w < B; w++) r[c][w] = r[c][w] + x[w];
+          }
         } else {
-          r[c] = prs[c * 256];
+          ldv<B>(prs + c * cs, r[c]);
         }
       }
-      sR[c][t] = r[c];
+      stv<B>(&sR[c][tb], r[c]);
     }
-    const float r1 = r[0], r2 = r[1], r3 = r[2], r4 = r[3], r5 = r[4], r6 = r[5];
     const float yl = term(AF_Y), vse = term(AF_VSE), vsx = term(AF_VSX), vze = term(AF_VZE),
                 vzx = term(AF_VZX);
-    if (SRC == AXH_MONOPOLE) {
-      sA[0][t] = vze * r1 + vse * r5;
-      sA[1][t] = vzx * r1 + vsx * r5;
-      sA[4][t] = vze * r5 + vse * r3;
-      sA[5][t] = vzx * r5 + vsx * r3;
-    } else if (SRC == AXH_DIPOLE) {
-      sA[0][t] = vze * (r1 - r6) + vse * (r5 - r4);
-      sA[1][t] = vzx * (r1 - r6) + vsx * (r5 - r4);
-      sA[2][t] = vze * (r1 + r6) + vse * (r5 + r4);
-      sA[3][t] = vzx * (r1 + r6) + vsx * (r5 + r4);
-      sA[4][t] = vze * r5 + vse * r3;
-      sA[5][t] = vzx * r5 + vsx * r3;
-    } else {
-      sA[0][t] = vze * r1 + vse * r5;
-      sA[1][t] = vzx * r1 + vsx * r5;
-      sA[2][t] = vze * r6 + vse * r4;
-      sA[3][t] = vzx * r6 + vsx * r4;
-      sA[4][t] = vze * r5 + vse * r3;
-      sA[5][t] = vzx * r5 + vsx * r3;
+    {
+      float S[6][B];
+#pragma unroll
+      for (int w = 0; w < B; w++) {
+        const float r1 = r[0][w], r3 = r[2][w], r4 = r[3][w], r5 = r[4][w], r6 = r[5][w];
+        if (SRC == AXH_MONOPOLE) {
+          S[0][w] = vze * r1 + vse * r5;
+          S[1][w] = vzx * r1 + vsx * r5;
+          S[2][w] = 0.0f;
+          S[3][w] = 0.0f;
+        } else if (SRC == AXH_DIPOLE) {
+          S[0][w] = vze * (r1 - r6) + vse * (r5 - r4);
+          S[1][w] = vzx * (r1 - r6) + vsx * (r5 - r4);
+          S[2][w] = vze * (r1 + r6) + vse * (r5 + r4);
+          S[3][w] = vzx * (r1 + r6) + vsx * (r5 + r4);
+        } else {
+          S[0][w] = vze * r1 + vse * r5;
+          S[1][w] = vzx * r1 + vsx * r5;
+          S[2][w] = vze * r6 + vse * r4;
+          S[3][w] = vzx * r6 + vsx * r4;
+        }
+        S[4][w] = vze * r5 + vse * r3;
+        S[5][w] = vzx * r5 + vsx * r3;
+      }
+#pragma unroll
+      for (int f = 0; f < 6; f++)
+        if (SRC != AXH_MONOPOLE || (f != 2 && f != 3)) stv<B>(&sA[f][tb], S[f]);
     }
     __syncthreads();
     if (valid) {
-      const float X1 = amul<1>(Gs, &sA[0][eb], i, j), X2 = mulb<1>(&sA[1][eb], G2T, i, j);
-      const float X5 = amul<1>(Gs, &sA[4][eb], i, j), X6 = mulb<1>(&sA[5][eb], G2T, i, j);
-      float l1, l2 = 0.0f, l3;
-      if (SRC == AXH_MONOPOLE) {
-        l1 = X1 + X2 + yl * r2;
-        l3 = X5 + X6;
-      } else {
-        const float X3 = amul<1>(Gs, &sA[2][eb], i, j), X4 = mulb<1>(&sA[3][eb], G2T, i, j);
-        if (SRC == AXH_DIPOLE) {
-          l1 = X1 + X2;
-          l2 = X3 + X4 + 2.0f * yl * (r2 - r6);
-          l3 = X5 + X6 - yl * r4;
-        } else {
-          l1 = X1 + X2 + yl * (r2 - 2.0f * r6);
-          l2 = -X3 - X4 + yl * (r6 - 2.0f * r2);
-          l3 = X5 + X6 - 2.0f * yl * r4;
-        }
-      }
-      if (ax) {
-        // the axial vectors at this lane's j, from the sums at (0, j)
-        const float *av = A.axis + (size_t)e * 25;
-        const float y0 = av[j], v0se = av[5 + j], v0sx = av[10 + j], v0ze = av[15 + j],
-                    v0zx = av[20 + j];
-        const int p0 = eb + 5 * j;
-        const float r10 = sR[0][p0], r20 = sR[1][p0], r30 = sR[2][p0], r40 = sR[3][p0],
-                    r50 = sR[4][p0], r60 = sR[5][p0];
-        const float g0 = G0[i];
+      const float *av = A.axis + (size_t)e * 25;
+      const int p0 = eb + 5 * j;
+      float l1[B], l2[B], l3[B];
+#pragma unroll
+      for (int w = 0; w < B; w++) {
+        const int o = eb * B + w;
+        const float r2 = r[1][w], r4 = r[3][w], r6 = r[5][w];
+        const float X1 = amul<B>(Gs, &sA[0][o], i, j), X2 = mulb<B>(&sA[1][o], G2T, i, j);
+        const float X5 = amul<B>(Gs, &sA[4][o], i, j), X6 = mulb<B>(&sA[5][o], G2T, i, j);
         if (SRC == AXH_MONOPOLE) {
-          const float V1 = v0ze * r10 + v0se * r50 + y0 * r20;
-          const float V2 = v0ze * r50 + v0se * r30;
-          l1 = l1 + g0 * V1;
-          l3 = l3 + g0 * V2;
-          if (i == 0) {
-            // vxm_4(V3, G2T): V3 at every point (0, kk) of the element
-            float V4 = 0.0f;
-            for (int kk = 0; kk < 5; kk++) {
-              const int pk = eb + 5 * kk;
-              const float V3 = av[20 + kk] * sR[4][pk] + av[10 + kk] * sR[2][pk];
-              V4 = V4 + V3 * G2T[kk + 5 * j];
-            }
-            l3 = l3 + V4;
-          }
-        } else if (SRC == AXH_DIPOLE) {
-          const float V1 = v0ze * (r10 - r60) + v0se * (r50 - r40);
-          const float V2 = v0zx * (r10 - r60) + v0sx * (r50 - r40);
-          l1 = l1 + g0 * V1;
-          if (i == 0) {
-            float V3 = 0.0f;
-            for (int kk = 0; kk < 5; kk++) {
-              const int pk = eb + 5 * kk;
-              const float V2k = av[20 + kk] * (sR[0][pk] - sR[5][pk]) +
-                                av[10 + kk] * (sR[4][pk] - sR[3][pk]);
-              V3 = V3 + V2k * G2T[kk + 5 * j];
-            }
-            l1 = l1 + V3;
-          }
-          const float Vm = v0ze * (r10 + r60) + v0se * (r50 + r40) + y0 * 2.0f * (r20 - r60);
-          l2 = l2 + g0 * Vm;
-          // the z-component adds outerprod(G0, V2) (stiffness_di.f90:706-707)
-          l3 = l3 + g0 * V2;
+          l1[w] = X1 + X2 + yl * r2;
+          l2[w] = 0.0f;
+          l3[w] = X5 + X6;
         } else {
-          const float Vs = v0ze * r10 + y0 * (r20 - 2.0f * r60);
-          const float Vp = -(v0ze * r60) + y0 * (r60 - 2.0f * r20);
-          const float Vz = v0se * r30;
-          l1 = l1 + g0 * Vs;
-          l2 = l2 + g0 * Vp;
-          l3 = l3 + g0 * Vz;
+          const float X3 = amul<B>(Gs, &sA[2][o], i, j), X4 = mulb<B>(&sA[3][o], G2T, i, j);
+          if (SRC == AXH_DIPOLE) {
+            l1[w] = X1 + X2;
+            l2[w] = X3 + X4 + 2.0f * yl * (r2 - r6);
+            l3[w] = X5 + X6 - yl * r4;
+          } else {
+            l1[w] = X1 + X2 + yl * (r2 - 2.0f * r6);
+            l2[w] = -X3 - X4 + yl * (r6 - 2.0f * r2);
+            l3[w] = X5 + X6 - 2.0f * yl * r4;
+          }
+        }
+        if (ax) {
+          // the axial vectors at this lane's j, from the sums at (0, j)
+          const float y0 = av[j], v0se = av[5 + j], v0sx = av[10 + j], v0ze = av[15 + j],
+                      v0zx = av[20 + j];
+          const int q0 = p0 * B + w;
+          const float r10 = sR[0][q0], r20 = sR[1][q0], r30 = sR[2][q0], r40 = sR[3][q0],
+                      r50 = sR[4][q0], r60 = sR[5][q0];
+          const float g0 = G0[i];
+          if (SRC == AXH_MONOPOLE) {
+            const float V1 = v0ze * r10 + v0se * r50 + y0 * r20;
+            const float V2 = v0ze * r50 + v0se * r30;
+            l1[w] = l1[w] + g0 * V1;
+            l3[w] = l3[w] + g0 * V2;
+            if (i == 0) {
+              // vxm_4(V3, G2T): V3 at every point (0, kk) of the element
+              float V4 = 0.0f;
+              for (int kk = 0; kk < 5; kk++) {
+                const int pk = (eb + 5 * kk) * B + w;
+                const float V3 = av[20 + kk] * sR[4][pk] + av[10 + kk] * sR[2][pk];
+                V4 = V4 + V3 * G2T[kk + 5 * j];
+              }
+              l3[w] = l3[w] + V4;
+            }
+          } else if (SRC == AXH_DIPOLE) {
+            const float V1 = v0ze * (r10 - r60) + v0se * (r50 - r40);
+            const float V2 = v0zx * (r10 - r60) + v0sx * (r50 - r40);
+            l1[w] = l1[w] + g0 * V1;
+            if (i == 0) {
+              float V3 = 0.0f;
+              for (int kk = 0; kk < 5; kk++) {
+                const int pk = (eb + 5 * kk) * B + w;
+                const float V2k = av[20 + kk] * (sR[0][pk] - sR[5][pk]) +
+                                  av[10 + kk] * (sR[4][pk] - sR[3][pk]);
+                V3 = V3 + V2k * G2T[kk + 5 * j];
+              }
+              l1[w] = l1[w] + V3;
+            }
+            const float Vm = v0ze * (r10 + r60) + v0se * (r50 + r40) + y0 * 2.0f * (r20 - r60);
+            l2[w] = l2[w] + g0 * Vm;
+            // the z-component adds outerprod(G0, V2) (stiffness_di.f90:706-707)
+            l3[w] = l3[w] + g0 * V2;
+          } else {
+            const float Vs = v0ze * r10 + y0 * (r20 - 2.0f * r60);
+            const float Vp = -(v0ze * r60) + y0 * (r60 - 2.0f * r20);
+            const float Vz = v0se * r30;
+            l1[w] = l1[w] + g0 * Vs;
+            l2[w] = l2[w] + g0 * Vp;
+            l3[w] = l3[w] + g0 * Vz;
+          }
         }
       }
-      float *f0 = A.force + slot, *f2 = A.force + 2 * (size_t)A.nslot + slot;
-      *f0 = *f0 - l1;
+      float x[B];
+      float *f0 = A.force + slot * B, *f2 = A.force + (2 * (size_t)A.nslot + slot) * B;
+      ldv<B>(f0, x);
+#pragma unroll
+      for (int w = 0; w < B; w++) x[w] = x[w] - l1[w];
+      stv<B>(f0, x);
       if (SRC != AXH_MONOPOLE) {
-        float *f1 = A.force + (size_t)A.nslot + slot;
-        *f1 = *f1 - l2;
+        float *f1 = A.force + ((size_t)A.nslot + slot) * B;
+        ldv<B>(f1, x);
+#pragma unroll
+        for (int w = 0; w < B; w++) x[w] = x[w] - l2[w];
+        stv<B>(f1, x);
       }
-      *f2 = *f2 - l3;
+      ldv<B>(f2, x);
+#pragma unroll
+      for (int w = 0; w < B; w++) x[w] = x[w] - l3[w];
+      stv<B>(f2, x);
     }
     if (UPD) __syncthreads();  // sA is reused below
   }
   if (UPD) {
     const int g = valid ? A.gid[slot] : 0;
-    const float u1 = valid ? A.disp[g] : 0.0f;
-    const float u2 = (valid && SRC != AXH_MONOPOLE) ? A.disp[(size_t)A.nglob + g] : 0.0f;
-    const float u3 = valid ? A.disp[2 * (size_t)A.nglob + g] : 0.0f;
+    float u1[B], u2[B], u3[B];
+#pragma unroll
+    for (int w = 0; w < B; w++) u1[w] = u2[w] = u3[w] = 0.0f;
+    if (valid) {
+      ldv<B>(A.disp + (size_t)g * B, u1);
+      if (SRC != AXH_MONOPOLE) ldv<B>(A.disp + ((size_t)A.nglob + g) * B, u2);
+      ldv<B>(A.disp + (2 * (size_t)A.nglob + g) * B, u3);
+    }
     // fields whose gradients the strain needs (f1 = u_z throughout)
-    if (SRC == AXH_MONOPOLE) {
-      sA[0][t] = u1;
-      sA[1][t] = u3;
-    } else if (SRC == AXH_DIPOLE) {
-      sA[0][t] = u1 + u2;
-      sA[1][t] = u3;
-      sA[2][t] = u2;
-      sA[3][t] = u1 - u2;
-    } else {
-      sA[0][t] = u1;
-      sA[1][t] = u3;
-      sA[2][t] = u2;
-      sA[3][t] = u1 - 2.0f * u2;
-      sA[4][t] = u2 - 2.0f * u1;
+#pragma unroll
+    for (int w = 0; w < B; w++) {
+      if (SRC == AXH_MONOPOLE) {
+        sA[0][tb + w] = u1[w];
+        sA[1][tb + w] = u3[w];
+      } else if (SRC == AXH_DIPOLE) {
+        sA[0][tb + w] = u1[w] + u2[w];
+        sA[1][tb + w] = u3[w];
+        sA[2][tb + w] = u2[w];
+        sA[3][tb + w] = u1[w] - u2[w];
+      } else {
+        sA[0][tb + w] = u1[w];
+        sA[1][tb + w] = u3[w];
+        sA[2][tb + w] = u2[w];
+        sA[3][tb + w] = u1[w] - 2.0f * u2[w];
+        sA[4][tb + w] = u2[w] - 2.0f * u1[w];
+      }
     }
     __syncthreads();
     if (!valid) return;  // no barrier follows
     const float dse = term(AF_DSDE), dze = term(AF_DZDE), dsx = term(AF_DSDX),
                 dzx = term(AF_DZDX), is = term(AF_INVS);
-    // axisym_gradient_solid_el_4: d/ds and d/dz of field f at this point
-    auto grad = [&](int f, float &ds, float &dz) {
-      const float m1 = amul<1>(GsT, &sA[f][eb], i, j);
-      const float m2 = mulb<1>(&sA[f][eb], G2, i, j);
-      ds = dze * m1 + dzx * m2;
-      dz = dse * m1 + dsx * m2;
-    };
-    // f_over_s_solid_el_4: f/s, and d/ds on the axis (dsdf_elem_solid)
-    auto fos = [&](int f, float val) {
-      if (ax && i == 0) {
-        float ds, dz;
-        grad(f, ds, dz);
-        return ds;
-      }
-      return is * val;
-    };
-    float E[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float b1s, b1z, b2s, b2z;
-    grad(0, b1s, b1z);
-    grad(1, b2s, b2z);
-    E[0] = b1s;
-    E[2] = b2z;
-    E[4] = b1z + b2s;
-    if (SRC == AXH_MONOPOLE) {
-      E[1] = fos(0, u1);
-    } else if (SRC == AXH_DIPOLE) {
-      E[1] = 2.0f * fos(2, u2);
-      float ps, pz;
-      grad(3, ps, pz);
-      const float f3 = fos(1, u3);
-      E[3] = -f3 - pz;
-      E[5] = -E[1] - ps;
-    } else {
-      E[1] = fos(3, u1 - 2.0f * u2);
-      float ps, pz;
-      grad(2, ps, pz);
-      const float f3 = fos(1, u3);
-      const float f4 = fos(4, u2 - 2.0f * u1);
-      E[3] = -(2.0f * f3) - pz;
-      E[5] = f4 - ps;
-    }
-    const double third = 1.0 / 3.0;
-    const float tr = E[0] + E[1] + E[2];
     const float dm = term(AF_DMU), dk = term(AF_DK);
-    const float str = dk * tr;
-    float sdev[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const double third = 1.0 / 3.0;
+    float str[B], sdev[6][B];
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-      sdev[c] = (float)((double)(dm * 2.0f) * ((double)E[c] - (double)tr * third));
-    sdev[4] = dm * E[4];
-    if (SRC != AXH_MONOPOLE) {
-      sdev[3] = dm * E[3];
-      sdev[5] = dm * E[5];
+    for (int w = 0; w < B; w++) {
+      const int o = eb * B + w;
+      // axisym_gradient_solid_el_4: d/ds and d/dz of field f at this point
+      auto grad = [&](int f, float &ds, float &dz) {
+        const float m1 = amul<B>(GsT, &sA[f][o], i, j);
+        const float m2 = mulb<B>(&sA[f][o], G2, i, j);
+        ds = dze * m1 + dzx * m2;
+        dz = dse * m1 + dsx * m2;
+      };
+      // f_over_s_solid_el_4: f/s, and d/ds on the axis (dsdf_elem_solid)
+      auto fos = [&](int f, float val) {
+        if (ax && i == 0) {
+          float ds, dz;
+          grad(f, ds, dz);
+          return ds;
+        }
+        return is * val;
+      };
+      float E[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      float b1s, b1z, b2s, b2z;
+      grad(0, b1s, b1z);
+      grad(1, b2s, b2z);
+      E[0] = b1s;
+      E[2] = b2z;
+      E[4] = b1z + b2s;
+      if (SRC == AXH_MONOPOLE) {
+        E[1] = fos(0, u1[w]);
+      } else if (SRC == AXH_DIPOLE) {
+        E[1] = 2.0f * fos(2, u2[w]);
+        float ps, pz;
+        grad(3, ps, pz);
+        const float f3 = fos(1, u3[w]);
+        E[3] = -f3 - pz;
+        E[5] = -E[1] - ps;
+      } else {
+        E[1] = fos(3, u1[w] - 2.0f * u2[w]);
+        float ps, pz;
+        grad(2, ps, pz);
+        const float f3 = fos(1, u3[w]);
+        const float f4 = fos(4, u2[w] - 2.0f * u1[w]);
+        E[3] = -(2.0f * f3) - pz;
+        E[5] = f4 - ps;
+      }
+      const float tr = E[0] + E[1] + E[2];
+      str[w] = dk * tr;
+#pragma unroll
+      for (int c = 0; c < 6; c++) sdev[c][w] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        sdev[c][w] = (float)((double)(dm * 2.0f) * ((double)E[c] - (double)tr * third));
+      sdev[4][w] = dm * E[4];
+      if (SRC != AXH_MONOPOLE) {
+        sdev[3][w] = dm * E[3];
+        sdev[5][w] = dm * E[5];
+      }
     }
-    float *ptr = A.trm1 + slot;
-    float *pdev = A.devm1 + (size_t)tile * mvs + t;
-    const float trm1 = *ptr;
-    float devm1[6];
+    float *ptr = A.trm1 + slot * B;
+    float *pdev = A.devm1 + (size_t)tile * mvs + (size_t)t * B;
+    float trm1[B], devm1[6][B];
+    ldv<B>(ptr, trm1);
 #pragma unroll
-    for (int c = 0; c < 6; c++) devm1[c] = pdev[c * 256];
+    for (int c = 0; c < 6; c++) ldv<B>(pdev + c * cs, devm1[c]);
     const double *amu = A.aj_mu + (size_t)A.nsls * e, *akap = A.aj_k + (size_t)A.nsls * e;
-    float rs[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float rs[6][B];
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+#pragma unroll
+      for (int w = 0; w < B; w++) rs[c][w] = 0.0f;
     for (int q = 0; q < A.nsls; q++) {
       const double Tt = A.tst[q], Tm = A.tstm1[q], Ew = A.expw[q];
-      const float trb = (float)(Tt * akap[q] * (double)str + Tm * akap[q] * (double)trm1);
+      const double ak = akap[q], am = amu[q];
+      float trb[B];
+#pragma unroll
+      for (int w = 0; w < B; w++)
+        trb[w] = (float)(Tt * ak * (double)str[w] + Tm * ak * (double)trm1[w]);
       float *m = mv + q * mvs;
 #pragma unroll
       for (int c = 0; c < 6; c++) {
         if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
-        const float db = (float)(Tt * amu[q] * (double)sdev[c] + Tm * amu[q] * (double)devm1[c]);
-        const double old = (double)m[c * 256];
-        const float nv = c < 3 ? (float)((Ew * old + (double)db) + (double)trb)
-                               : (float)(Ew * old + (double)db);
-        m[c * 256] = nv;
-        rs[c] = rs[c] + nv;
+        float x[B];
+        ldv<B>(m + c * cs, x);
+#pragma unroll
+        for (int w = 0; w < B; w++) {
+          const float db =
+              (float)(Tt * am * (double)sdev[c][w] + Tm * am * (double)devm1[c][w]);
+          const double old = (double)x[w];
+          x[w] = c < 3 ? (float)((Ew * old + (double)db) + (double)trb[w])
+                       : (float)(Ew * old + (double)db);
+          rs[c][w] = rs[c][w] + x[w];
+        }
+        stv<B>(m + c * cs, x);
       }
     }
-    *ptr = str;
+    stv<B>(ptr, str);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      pdev[c * 256] = sdev[c];
-      if (!STIFF) prs[c * 256] = rs[c];  // read by the symplectic stages
+      stv<B>(pdev + c * cs, sdev[c]);
+      if (!STIFF) stv<B>(prs + c * cs, rs[c]);  // read by the symplectic stages
     }
   }
 }
-
 
 // ---------------------------------------------------------------------------
 // predictor / drift per global point (SOLVER/time_evol_wave.F90:357-367,
@@ -2729,13 +2788,11 @@ static void fields_alloc(const MeshDev &M, Fields &F, int B) {
     F.devm1 = m.zeros<float>((size_t)M.nel_s * 24 * b);
   }
   if (M.anel_full) {
-    if (B != 1)
-      fail(AXH_ERR_UNSUPPORTED, "memory variables at all points: one wavefield per group");
     const size_t ns = (size_t)M.ntile_s * kTile;
-    F.memvar = m.zeros<float>(ns * 6 * M.nsls);
-    F.rsum = m.zeros<float>(ns * 6);
-    F.trm1 = m.zeros<float>(ns);
-    F.devm1 = m.zeros<float>(ns * 6);
+    F.memvar = m.zeros<float>(ns * 6 * M.nsls * b);
+    F.rsum = m.zeros<float>(ns * 6 * b);
+    F.trm1 = m.zeros<float>(ns * b);
+    F.devm1 = m.zeros<float>(ns * 6 * b);
   }
 }
 
@@ -2893,8 +2950,7 @@ void batch_attach(axh_batch *bt) {
   S->run.init();
   // groups of at most kGroup members, sizes as equal as possible
   const int n = (int)members.size();
-  // (memory variables at all points are stepped one wavefield per group)
-  const int width = mesh->anel_full ? 1 : std::max(1, std::min(bt->group_width, kGroup));
+  const int width = std::max(1, std::min(bt->group_width, kGroup));
   const int ng = (n + width - 1) / width;
   int next = 0;
   for (int g = 0; g < ng; g++) {
@@ -2994,15 +3050,15 @@ static void dispatch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
 #undef AXH_SOLID_CASE
 }
 
-template <bool STIFF, bool UPD>
+template <bool STIFF, bool UPD, int B>
 static void dispatch_anel_full(const MeshDev &M, hipStream_t s, const AnelFullArgs &A) {
   const dim3 grid(M.ntile_s), blk(256);
   if (M.src == AXH_MONOPOLE)
-    hipLaunchKernelGGL((k_anel_full<AXH_MONOPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+    hipLaunchKernelGGL((k_anel_full<AXH_MONOPOLE, STIFF, UPD, B>), grid, blk, 0, s, A);
   else if (M.src == AXH_DIPOLE)
-    hipLaunchKernelGGL((k_anel_full<AXH_DIPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+    hipLaunchKernelGGL((k_anel_full<AXH_DIPOLE, STIFF, UPD, B>), grid, blk, 0, s, A);
   else
-    hipLaunchKernelGGL((k_anel_full<AXH_QUADPOLE, STIFF, UPD>), grid, blk, 0, s, A);
+    hipLaunchKernelGGL((k_anel_full<AXH_QUADPOLE, STIFF, UPD, B>), grid, blk, 0, s, A);
 }
 
 template <int B>
@@ -3067,7 +3123,6 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   SA.disp_w = F.disp; SA.hdt = hdt;
   AnelFullArgs AN{};
   if (M.anel_full) {
-    if (B != 1) fail(AXH_ERR_STATE, "memory variables at all points: one wavefield per group");
     AN.nel = M.nel_s; AN.nsls = M.nsls; AN.nglob = M.nglob_s; AN.nslot = (int)nslot_s;
     AN.G = M.G; AN.terms = M.af_terms; AN.axis = M.af_axis; AN.einfo = M.einfo_s;
     AN.gid = M.gid_s; AN.aj_mu = M.aj_mu; AN.aj_k = M.aj_k;
@@ -3168,8 +3223,8 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       if (M.anel_full) {
         // anelastic forces; Newmark also updates the memory variables here
         L(KC_MEMVAR, [&] {
-          if (newmark) dispatch_anel_full<true, true>(M, st_, AN);
-          else dispatch_anel_full<true, false>(M, st_, AN);
+          if (newmark) dispatch_anel_full<true, true, B>(M, st_, AN);
+          else dispatch_anel_full<true, false, B>(M, st_, AN);
         });
       }
       if (M.nel_f > 0) {
@@ -3194,7 +3249,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       }
     }
     if (!newmark && M.anel_full)
-      L(KC_MEMVAR, [&] { dispatch_anel_full<false, true>(M, st_, AN); });
+      L(KC_MEMVAR, [&] { dispatch_anel_full<false, true, B>(M, st_, AN); });
     if (!newmark && M.anel) {
       // the receiver samples ride along as extra blocks of the update
       const int nrb = any ? nblk(maxrec) * B : 0;

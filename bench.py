@@ -285,8 +285,7 @@ SWEEP = (("mono_vertforce_newmark", "vertforce", "newmark2", False, True),
          ("mono_explosion_symplec4_anel", "explosion", "symplec4", True, True),
          ("dipole_thetaforce_newmark_anel", "thetaforce", "newmark2", True, True),
          ("quad_mtp_symplec4_anel", "mtp", "symplec4", True, True),
-         # memory variables at all 25 points (COARSE_GRAINED false; one
-         # wavefield per group, so the batched leg runs 4 groups of 1)
+         # memory variables at all 25 points (COARSE_GRAINED false)
          ("dipole_thetaforce_symplec4_anelfull", "thetaforce", "symplec4", True, False))
 
 

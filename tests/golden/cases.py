@@ -84,11 +84,21 @@ BATCH_MEMBERS = {
     "b_quad_mtt_m_mpp_20": RunConfig("mtt_m_mpp", 20.0, 1e20, "symplec4", True, 500.0,
                                      STATIONS),
     "b_quad_mtp_2600": RunConfig("mtp", 2600.0, 1e20, "symplec4", True, 500.0, STATIONS),
-    # memory variables at all 25 points (stepped one wavefield per group)
+    # memory variables at all 25 points
     "bf_dipole_thetaforce_50": RunConfig("thetaforce", 50.0, 1e20, "symplec4", True, 400.0,
                                          STATIONS, coarse_grained=False),
     "bf_dipole_mtr_700": RunConfig("mtr", 700.0, 3e20, "symplec4", True, 400.0, STATIONS,
                                    coarse_grained=False),
+    "bf_dipole_phiforce_1000": RunConfig("phiforce", 1000.0, 1e20, "symplec4", True, 400.0,
+                                         STATIONS, coarse_grained=False),
+    "bf_dipole_mpr_2000": RunConfig("mpr", 2000.0, 1e20, "symplec4", True, 400.0,
+                                    STATIONS[::2], coarse_grained=False),
+    "bf_mono_explosion_100": RunConfig("explosion", 100.0, 1e20, "newmark2", True, 500.0,
+                                       STATIONS, coarse_grained=False),
+    "bf_mono_vertforce_0": RunConfig("vertforce", 0.0, 2e20, "newmark2", True, 500.0, STATIONS,
+                                     coarse_grained=False),
+    "bf_mono_mtt_p_mpp_300": RunConfig("mtt_p_mpp", 300.0, 1e20, "newmark2", True, 500.0,
+                                       STATIONS, coarse_grained=False),
 }
 BATCHES = {
     "dipole_symplec4_anel": ["b_dipole_thetaforce_50", "b_dipole_phiforce_1000",
@@ -97,7 +107,10 @@ BATCHES = {
     # five members: one group of 4 and one of 1 at the default width
     "quad_symplec4_anel": ["b_quad_mtp_100", "b_quad_mtt_m_mpp_1500", "b_quad_mtp_400",
                            "b_quad_mtt_m_mpp_20", "b_quad_mtp_2600"],
-    "dipole_symplec4_anelfull": ["bf_dipole_thetaforce_50", "bf_dipole_mtr_700"],
+    "dipole_symplec4_anelfull": ["bf_dipole_thetaforce_50", "bf_dipole_mtr_700",
+                                 "bf_dipole_phiforce_1000", "bf_dipole_mpr_2000"],
+    "mono_newmark_anelfull": ["bf_mono_explosion_100", "bf_mono_vertforce_0",
+                              "bf_mono_mtt_p_mpp_300"],
 }
 # Cases the reference program cannot run (it lower-cases TIME_SCHEME and its
 # symplectic_coefficients then matches neither McLachlan scheme): the GPU is

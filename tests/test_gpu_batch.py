@@ -58,11 +58,15 @@ def test_batch_members_match_solo_and_oracle(batch, state_cache):
         _same_fields(solo[k][1], o.fields(), names[k] + " oracle")
 
 
-@pytest.mark.parametrize("width", [2, 3])
-def test_group_widths_equal_solo(width, state_cache):
+@pytest.mark.parametrize("batch,width", [("dipole_symplec4_anel", 2),
+                                         ("dipole_symplec4_anel", 3),
+                                         ("dipole_symplec4_anelfull", 2),
+                                         ("dipole_symplec4_anelfull", 3)])
+def test_group_widths_equal_solo(batch, width, state_cache):
     """Narrower groups (the kernels for 2 and 3 wavefields, and a remainder
-    group) give every member its solo run, bit for bit."""
-    names = cases.BATCHES["dipole_symplec4_anel"]
+    group) give every member its solo run, bit for bit; also with memory
+    variables at all 25 points."""
+    names = cases.BATCHES[batch]
     print("group width", width, names, flush=True)
     paths = [state_cache(n)[1] for n in names]
     n = 80
