@@ -301,6 +301,8 @@ int axh_upload_attenuation_cg4(axh_ctx *ctx, const axh_attenuation_cg4 *a) {
     if (!a) fail(AXH_ERR_ARG, "null attenuation terms");
     const auto &d = ctx->dims;
     if (!d.anel) fail(AXH_ERR_STATE, "attenuation upload for an elastic run");
+    if (!d.coarse_grained)
+      fail(AXH_ERR_STATE, "coarse-grained terms for a run with memory variables at all points");
     const uint64_t nel = (uint64_t)d.nel_solid, ns = (uint64_t)d.n_sls;
     ctx->put("Y_cg4", kF32, {4, nel}, a->Y);
     ctx->put("V_s_eta_cg4", kF32, {4, nel}, a->V_s_eta);

@@ -92,6 +92,41 @@ def test_upload_size_checks(state_cache):
     assert lib.axh_destroy(h) == 0
 
 
+def test_attenuation_uploads_follow_the_memory_variable_mode(state_cache):
+    """axh_upload_attenuation_full serves COARSE_GRAINED false only, and the
+    coarse-grained upload the default mode: each refuses the other's context,
+    an elastic one and a NULL argument, naming the reason."""
+    lib = S.library()
+    st = read_state(state_cache("mono_explosion_newmark_anelfull")[1])
+    assert st.dims["anel"] == 1 and st.dims["coarse_grained"] == 0
+    assert "Y" in st.arrays and "Y_cg4" not in st.arrays
+
+    def ctx(anel, cg):
+        d = S.Dims()
+        for k in S.Dims._fields_:
+            if k[0] in st.dims:
+                setattr(d, k[0], st.dims[k[0]])
+        d.anel, d.coarse_grained = anel, cg
+        h = C.c_void_p()
+        assert lib.axh_create(C.byref(d), C.byref(h)) == 0, lib.axh_last_error()
+        return h
+
+    full, cg4 = S.AttFull(), S.Att()
+    h = ctx(1, 0)
+    assert lib.axh_upload_attenuation_full(h, None) == -1
+    assert lib.axh_upload_attenuation_cg4(h, C.byref(cg4)) == -2
+    assert b"all points" in lib.axh_last_error()
+    assert lib.axh_destroy(h) == 0
+    h = ctx(1, 1)
+    assert lib.axh_upload_attenuation_full(h, C.byref(full)) == -2
+    assert b"coarse-grained" in lib.axh_last_error()
+    assert lib.axh_destroy(h) == 0
+    h = ctx(0, 1)
+    assert lib.axh_upload_attenuation_full(h, C.byref(full)) == -2
+    assert b"elastic" in lib.axh_last_error()
+    assert lib.axh_destroy(h) == 0
+
+
 def test_symbols_have_reference_citations():
     """Every declaration in include/axihip.h documents what it replaces."""
     text = open(os.path.join(REPO, "include", "axihip.h")).read()
