@@ -18,6 +18,15 @@
  *
  * Inputs come from an axh state file (the uploads of the C-ABI written by
  * axh_save_state; the format is parsed here independently of the library).
+ *
+ * Pinning: the seismograms of this restatement equal the reference program's
+ * (built from the reference sources) bit for bit on the golden runs of
+ * tests/golden (monopole, dipole, quadrupole; Newmark and PEFRL; elastic,
+ * coarse-grained and all-point attenuation; a regional mesh with absorbing
+ * boundaries).  The McLachlan schemes, which the reference program cannot
+ * select, are pinned by their order of accuracy and against the Newmark run.
+ * add_source_fl (a source in the fluid): parity unpinned -- the reference
+ * cannot place a fluid source on the test meshes (SOLVER/source.f90:531).
  */
 #include <math.h>
 #include <stdint.h>
