@@ -629,7 +629,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   {
     const float *coef = A.coef + (size_t)tile * NC * 256 + t;
 #pragma unroll
-    for (int a = 0; a < NC; a++) cf[a] = coef[a * 256];
+    for (int a = 0; a < NC; a++) cf[a] = __builtin_nontemporal_load(coef + a * 256);
   }
   gather_disp<NCOMP, B>(A, valid, gp, t, sU);
   // a single-copy point is kicked here at the end: its velocity (and, for
