@@ -624,7 +624,10 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   const int gp = valid ? A.gid[slot] : 0;
 
   stage_g(A.G, sG);
-  // the lane's stiffness terms (the packed rows cover all 256 lanes)
+  // the lane's stiffness terms (the packed rows cover all 256 lanes), read
+  // non-temporally: they are streamed once per launch (221 MB at 10 s) and
+  // would otherwise evict the fields, forces and index lists that this and
+  // the following kernels re-read from the L2 and the Infinity Cache
   float cf[NC];
   {
     const float *coef = A.coef + (size_t)tile * NC * 256 + t;
