@@ -195,6 +195,40 @@ __device__ __forceinline__ void stv(float *p, const float (&o)[B]) {
   }
 }
 
+// the same for data read or written once per pass (non-temporal)
+template <int B>
+__device__ __forceinline__ void ldv_nt(const float *p, float (&o)[B]) {
+  if constexpr (B == 4) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else if constexpr (B == 2) {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(p));
+    o[0] = v.x; o[1] = v.y;
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) o[b] = __builtin_nontemporal_load(p + b);
+  }
+}
+template <int B>
+__device__ __forceinline__ void stv_nt(float *p, const float (&o)[B]) {
+  if constexpr (B == 4) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    v4f v;
+    v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(p));
+  } else if constexpr (B == 2) {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    v2f v;
+    v.x = o[0]; v.y = o[1];
+    __builtin_nontemporal_store(v, reinterpret_cast<v2f *>(p));
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; b++) __builtin_nontemporal_store(o[b], p + b);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // kernel argument blocks: mesh part, the group's interleaved fields, and the
 // per-wavefield source / receiver data of its members
@@ -1607,7 +1641,7 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
       for (int c = 0; c < 6; c++) {
         if (SRC == AXH_MONOPOLE && (c == 3 || c == 5)) continue;
         float x[B];
-        ldv<B>(m + c * cs, x);
+        ldv_nt<B>(m + c * cs, x);
 #pragma unroll
         for (int w = 0; w < B; w++) {
           const float db =
@@ -1617,7 +1651,7 @@ __global__ __launch_bounds__(256) void k_anel_full(AnelFullArgs A) {
                        : (float)(Ew * old + (double)db);
           rs[c][w] = rs[c][w] + x[w];
         }
-        stv<B>(m + c * cs, x);
+        stv_nt<B>(m + c * cs, x);
       }
     }
     stv<B>(ptr, str);
