@@ -3,14 +3,22 @@
 * ``build_library()``  -- hipcc for gfx950 -> ``axisem_amd/libaxihip.so``
   (the C-ABI of ``include/axihip.h``).
 * ``build_host(src)`` -- the drop-in Fortran host: the reference SOLVER's own
-  sources (read in place from ``src``, never copied) with its
-  ``time_evol_wave.F90`` replaced by ``fortran/time_evol_wave_gpu.F90``,
-  linked against ``libaxihip.so`` -> ``axisem_amd/fortran/_build/axisem_gpu``.
+  sources (read in place from ``src``) with its ``time_evol_wave.F90``
+  replaced by ``fortran/time_evol_wave_gpu.F90``, linked against
+  ``libaxihip.so`` -> ``axisem_amd/fortran/_build/axisem_gpu``.  The private
+  helpers the drop-in needs (``create_fluid_free_surface_mask``,
+  ``create_absorbing_gamma``, ``energy`` of ``time_evol_wave.F90``;
+  ``fast_correct`` of ``attenuation.f90``) are the reference's own: both
+  files are compiled a second time under the module names
+  ``axh_ref_time_evol_wave`` / ``axh_ref_attenuation`` with every entity
+  public (a rename of the module and of its default accessibility, written
+  into the build directory only).
 """
 from __future__ import annotations
 
 import glob
 import os
+import re
 import subprocess
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -44,6 +52,33 @@ def build_library(out: str = LIB) -> str:
     return out
 
 
+def public_module_copy(src: str, new_name: str, out_dir: str) -> str:
+    """The reference module in src as module new_name with its default
+    accessibility public, written to out_dir (the build directory); the
+    statements are otherwise the reference's, unchanged."""
+    txt = open(src).read()
+    old = re.search(r"^[ \t]*module[ \t]+(\w+)[ \t]*$", txt, re.I | re.M)
+    if not old:
+        raise RuntimeError("no module statement in " + src)
+    name = old.group(1)
+    txt, n = re.subn(r"^([ \t]*module[ \t]+)%s([ \t]*)$" % name, r"\g<1>%s\g<2>" % new_name, txt,
+                     count=1, flags=re.I | re.M)
+    txt = re.sub(r"^([ \t]*end[ \t]+module)[ \t]+%s[ \t]*$" % name, r"\g<1> %s" % new_name, txt,
+                 flags=re.I | re.M)
+    # the module's default-accessibility statement (a bare `private` before
+    # the first `contains`)
+    head, sep, rest = txt.partition("\ncontains")
+    head, m = re.subn(r"^[ \t]*private[ \t]*$", "  public", head, count=1, flags=re.I | re.M)
+    if n != 1 or m != 1 or not sep:
+        raise RuntimeError("unexpected layout of " + src)
+    os.makedirs(out_dir, exist_ok=True)
+    ext = os.path.splitext(src)[1]
+    out = os.path.join(out_dir, new_name + ext)
+    with open(out, "w") as f:
+        f.write(head + sep + rest)
+    return out
+
+
 def build_host(ref_root: str = "/root/reference", jobs: int = 8) -> str:
     """Drop-in host; needs the reference tree (present only where it is built)."""
     from .fortran import fortran_build as fb
@@ -57,6 +92,11 @@ def build_host(ref_root: str = "/root/reference", jobs: int = 8) -> str:
                   glob.glob(os.path.join(solver, "*.F90")))
     srcs = [s for s in srcs if os.path.basename(s) != "time_evol_wave.F90"]
     fdir = os.path.join(PKG, "fortran")
+    gen = os.path.join(HOST_DIR, "src")
+    srcs += [public_module_copy(os.path.join(solver, "time_evol_wave.F90"),
+                                "axh_ref_time_evol_wave", gen),
+             public_module_copy(os.path.join(solver, "attenuation.f90"),
+                                "axh_ref_attenuation", gen)]
     srcs += [os.path.join(fdir, "axh_iface.f90"), os.path.join(fdir, "time_evol_wave_gpu.F90")]
     obj = os.path.join(HOST_DIR, "obj")
     objs = fb.compile_all(srcs, obj, fflags=FFLAGS, jobs=jobs)

@@ -27,6 +27,11 @@ module time_evol_wave
   use rotations
   use data_io,          only: verbose
   use axh_iface
+  ! private helpers of the reference's own time_evol_wave and attenuation modules,
+  ! compiled at build time from the reference sources under these module names
+  use axh_ref_time_evol_wave, only: create_fluid_free_surface_mask, &
+                                    create_absorbing_gamma, ref_energy => energy
+  use axh_ref_attenuation,    only: ref_fast_correct => fast_correct
 
 
   implicit none
@@ -256,89 +261,22 @@ end subroutine write_snapshot
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
-!> energy (SOLVER/time_evol_wave.F90:1424-1527, private there) for the fields copied back
-!! from the GPU after the step ending at time: kinetic and potential energy of the solid
-!! (one more stiffness evaluation) and of the fluid, written every step.
+!> The energies after the step ending at time: the fields copied back from the GPU handed
+!! to the reference's own energy routine (SOLVER/time_evol_wave.F90:1424-1526, private
+!! there, reached through its build-time module axh_ref_time_evol_wave).
 subroutine energy_gpu(time)
 
-  use data_matr, only : unassem_mass_rho_solid, unassem_mass_lam_fluid
-  use stiffness_mono
-  use stiffness_di
-  use stiffness_quad
-  use stiffness_fluid
-  use apply_masks
-  use commun
   real(kind=dp), intent(in)        :: time
   real(kind=realkind), allocatable :: disp(:,:,:,:), vel(:,:,:,:), chi(:,:,:)
   real(kind=realkind), allocatable :: dchi(:,:,:), ddchi(:,:,:)
-  real(kind=realkind), allocatable :: stiff(:,:,:,:), stiff_flu(:,:,:)
-  real(kind=realkind) :: ekin_sol, epot_sol, ekin_flu, epot_flu
 
   allocate(disp(0:npol, 0:npol, nel_solid, 3), vel(0:npol, 0:npol, nel_solid, 3))
   allocate(chi(0:npol, 0:npol, max(nel_fluid, 1)), dchi(0:npol, 0:npol, max(nel_fluid, 1)))
   allocate(ddchi(0:npol, 0:npol, max(nel_fluid, 1)))
-  allocate(stiff(0:npol, 0:npol, nel_solid, 3), stiff_flu(0:npol, 0:npol, max(nel_fluid, 1)))
   call axh_check(axh_download_fields(ctx, disp, vel, chi, dchi, ddchi), 'download fields')
   t = time
-
-  ekin_sol = zero
-  epot_sol = zero
-  ekin_flu = zero
-  epot_flu = zero
-  stiff = zero
-  stiff_flu = zero
-
-  select case (src_type(1))
-     case ('monopole')
-        call apply_axis_mask_onecomp(disp, nel_solid, ax_el_solid, naxel_solid)
-        call glob_stiffness_mono(stiff, disp)
-        call apply_axis_mask_onecomp(stiff, nel_solid, ax_el_solid, naxel_solid)
-     case ('dipole')
-        call apply_axis_mask_twocomp(disp, nel_solid, ax_el_solid, naxel_solid)
-        call glob_stiffness_di(stiff, disp)
-        call apply_axis_mask_twocomp(stiff, nel_solid, ax_el_solid, naxel_solid)
-     case ('quadpole')
-        call apply_axis_mask_threecomp(disp, nel_solid, ax_el_solid, naxel_solid)
-        call glob_stiffness_quad(stiff, disp)
-        call apply_axis_mask_threecomp(stiff, nel_solid, ax_el_solid, naxel_solid)
-  end select
-  stiff = stiff * disp
-  epot_sol = sum(stiff)
-  epot_sol = two * pi * psum(epot_sol)
-
-  stiff(:,:,:,1) = vel(:,:,:,1)**2 * unassem_mass_rho_solid
-  stiff(:,:,:,2) = vel(:,:,:,2)**2 * unassem_mass_rho_solid
-  stiff(:,:,:,3) = vel(:,:,:,3)**2 * unassem_mass_rho_solid
-  if (src_type(1) == 'dipole') stiff(:,:,:,3) = two * stiff(:,:,:,3)
-  ekin_sol = sum(stiff)
-  ekin_sol = two * pi * psum(ekin_sol)
-
-  if (have_fluid) then
-     stiff_flu = ddchi**2 * unassem_mass_lam_fluid
-     epot_flu = sum(stiff_flu)
-     epot_flu = two * pi * psum(epot_flu)
-     stiff_flu = zero
-     if (src_type(1) .ne. 'monopole') &
-        call apply_axis_mask_scal(dchi, nel_fluid, ax_el_fluid, naxel_fluid)
-     call glob_fluid_stiffness(stiff_flu, dchi)
-     if (src_type(1) .ne. 'monopole') &
-        call apply_axis_mask_scal(stiff_flu, nel_fluid, ax_el_fluid, naxel_fluid)
-     stiff_flu = stiff_flu * dchi
-     ekin_flu = sum(stiff_flu)
-     ekin_flu = two * pi * psum(ekin_flu)
-  endif
-
-  if (lpr) then
-     if (have_fluid) then
-        write(4444,8) t, ekin_sol, epot_sol
-        write(4445,8) t, epot_flu, ekin_flu
-     endif
-     write(4446,9) t, epot_sol + epot_flu, ekin_sol + ekin_flu, &
-                   half * (epot_sol + epot_flu + ekin_sol + ekin_flu)
-  endif
-8 format(3(1pe16.6))
-9 format(4(1pe16.6))
-  deallocate(disp, vel, chi, dchi, ddchi, stiff, stiff_flu)
+  call ref_energy(disp, vel, dchi(:,:,1:nel_fluid), ddchi(:,:,1:nel_fluid))
+  deallocate(disp, vel, chi, dchi, ddchi)
 
 end subroutine energy_gpu
 !-----------------------------------------------------------------------------------------
@@ -418,120 +356,6 @@ subroutine check_supported_pre()
 end subroutine check_supported_pre
 !-----------------------------------------------------------------------------------------
 
-!-----------------------------------------------------------------------------------------
-!> create_fluid_free_surface_mask (SOLVER/time_evol_wave.F90:1615-1630): zero at fluid
-!! points on the outer surface (r > router - 1 m).
-subroutine create_fluid_free_surface_mask()
-
-  use utlity, only : rcoord
-  integer :: iel, ipol, jpol
-
-  allocate(fluid_free_surface_mask(0:npol, 0:npol, nel_fluid))
-  fluid_free_surface_mask = 1
-  do iel = 1, nel_fluid
-     do ipol = 0, npol
-        do jpol = 0, npol
-           if (rcoord(ipol, jpol, ielfluid(iel)) > router - 1) &
-              fluid_free_surface_mask(ipol, jpol, iel) = 0.
-        enddo
-     enddo
-  enddo
-
-end subroutine create_fluid_free_surface_mask
-!-----------------------------------------------------------------------------------------
-
-!-----------------------------------------------------------------------------------------
-!> create_absorbing_gamma (SOLVER/time_evol_wave.F90:1634-1713, private there): a mesh
-!! that does not reach the antipode gets absorbing boundaries; the damping factor rises
-!! as U0 (1 - sin^2(pi d / 2D)) within D = 15 h_max of the bottom or the lateral edge.
-subroutine create_absorbing_gamma()
-
-  use utlity, only : rcoord, thetacoord, scoord, zcoord
-  use commun, only : pmax, pmin
-  integer            :: iel, ipol, jpol
-  integer, parameter :: num_elem = 15
-  real(kind=dp)      :: hmax, U0, distance_factor, rmin, thetamax, dist, r, t
-
-  thetamax = maxval(atan2(crd_nodes(:, 1), crd_nodes(:, 2)))
-  thetamax = pmax(thetamax)
-  have_absorbing_bc = thetamax < pi - 1e-7
-  if (.not. have_absorbing_bc) return
-
-  hmax = 0.
-  do iel = 1, nelem
-     hmax = max(hmax, &
-        (scoord(0,0,iel) - scoord(0,npol,iel)) ** 2 + &
-           (zcoord(0,0,iel) - zcoord(0,npol,iel)) ** 2, &
-        (scoord(0,npol,iel) - scoord(npol,npol,iel)) ** 2 + &
-           (zcoord(0,npol,iel) - zcoord(npol,npol,iel)) ** 2, &
-        (scoord(npol,npol,iel) - scoord(npol,0,iel)) ** 2 + &
-           (zcoord(npol,npol,iel) - zcoord(npol,0,iel)) ** 2, &
-        (scoord(npol,0,iel) - scoord(0,0,iel)) ** 2 + &
-           (zcoord(npol,0,iel) - zcoord(0,0,iel)) ** 2)
-  enddo
-  hmax = hmax ** 0.5
-  rmin = dsqrt(minval(crd_nodes(:, 1) ** 2 + crd_nodes(:, 2) ** 2))
-  hmax = pmax(hmax)
-  rmin = pmin(rmin)
-
-  allocate(fluid_absorbing_gamma(0:npol, 0:npol, nel_fluid))
-  allocate(solid_absorbing_gamma(0:npol, 0:npol, nel_solid))
-  fluid_absorbing_gamma = 0
-  solid_absorbing_gamma = 0
-  distance_factor = num_elem * hmax
-  U0 = vpmax / (2 * hmax)
-  if (lpr) then
-     write(6,*) 'Absorbing Boundary Parameters:'
-     write(6,*) 'distance_factor: ', distance_factor
-     write(6,*) 'U0:              ', U0
-  endif
-
-  do iel = 1, nel_fluid
-     do ipol = 0, npol
-        do jpol = 0, npol
-           r = rcoord(ipol, jpol, ielfluid(iel))
-           t = thetacoord(ipol, jpol, ielfluid(iel))
-           dist = min(r - rmin, (thetamax - t) * r)
-           if (dist < distance_factor) &
-              fluid_absorbing_gamma(ipol, jpol, iel) = &
-                 U0 * (1. - sin(pi * dist / (2 * distance_factor)) ** 2)
-        enddo
-     enddo
-  enddo
-  do iel = 1, nel_solid
-     do ipol = 0, npol
-        do jpol = 0, npol
-           r = rcoord(ipol, jpol, ielsolid(iel))
-           t = thetacoord(ipol, jpol, ielsolid(iel))
-           dist = min(r - rmin, (thetamax - t) * r)
-           if (dist < distance_factor) &
-              solid_absorbing_gamma(ipol, jpol, iel) = &
-                 U0 * (1. - sin(pi * dist / (2 * distance_factor)) ** 2)
-        enddo
-     enddo
-  enddo
-
-end subroutine create_absorbing_gamma
-!-----------------------------------------------------------------------------------------
-
-!-----------------------------------------------------------------------------------------
-!> fast_correct (SOLVER/attenuation.f90:1139-1155): first-order correction of the
-!! linearised SLS weights for low Q, yp_j = y_j * delta_j.
-pure subroutine sls_fast_correct(y_j, yp_j)
-
-  real(kind=dp), intent(in)  :: y_j(:)
-  real(kind=dp), intent(out) :: yp_j(size(y_j))
-  real(kind=dp)              :: dy_j(size(y_j))
-  integer                    :: k
-
-  dy_j(1) = 1 + .5 * y_j(1)
-  do k = 2, size(y_j)
-     dy_j(k) = dy_j(k-1) + (dy_j(k-1) - .5) * y_j(k-1) + .5 * y_j(k)
-  enddo
-  yp_j = y_j * dy_j
-
-end subroutine sls_fast_correct
-!-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
 !> Hands every array of the loop to the library (and optionally to a state file).
@@ -776,13 +600,13 @@ subroutine upload_attenuation()
   allocate(ajmu(n, nel_solid), ajk(n, nel_solid), yp(n))
   do iel = 1, nel_solid
      if (docorr /= 0) then
-        call sls_fast_correct(yj(1:n) / Q_mu(iel), yp)
+        call ref_fast_correct(yj(1:n) / Q_mu(iel), yp)
      else
         yp = yj(1:n) / Q_mu(iel)
      endif
      ajmu(:, iel) = yp / sum(yp)
      if (docorr /= 0) then
-        call sls_fast_correct(yj(1:n) / Q_kappa(iel), yp)
+        call ref_fast_correct(yj(1:n) / Q_kappa(iel), yp)
      else
         yp = yj(1:n) / Q_kappa(iel)
      endif
