@@ -25,8 +25,10 @@
  * coarse-grained and all-point attenuation; a regional mesh with absorbing
  * boundaries).  The McLachlan schemes, which the reference program cannot
  * select, are pinned by their order of accuracy and against the Newmark run.
- * add_source_fl (a source in the fluid): parity unpinned -- the reference
- * cannot place a fluid source on the test meshes (SOLVER/source.f90:531).
+ * add_source_fl (a source in the fluid), which the reference program cannot
+ * place on its meshes (SOLVER/source.f90:531), is pinned by a closed form:
+ * with every operator zeroed, a Newmark step from rest is a point-mass update
+ * of the source elements' points (tests/test_oracle_math.py).
  */
 #include <math.h>
 #include <stdint.h>

@@ -9,11 +9,16 @@
   convergence on the harmonic oscillator; ML_SO4m5 and ML_SO6m7 cannot be run
   by the reference program itself (it lower-cases the scheme name before its
   select case), so these order checks are what pins them.
+* Source injection in the fluid (add_source_fl, which the reference program
+  cannot place on its meshes): with every operator zeroed, one Newmark step
+  from rest is a point-mass update, ddchi = inv_mass_fluid * source_term_fl *
+  stf(iter) at the source elements' points and zero everywhere else.
 """
 import numpy as np
 import pytest
 
-from axisem_amd.state import read_state
+from axisem_amd.state import read_state, write_state
+from fluid_source import fluid_source_state, pick_fluid_elements, zero_operators
 from oracle.oracle import Oracle, mxm, symplectic_coefficients
 
 
@@ -129,3 +134,56 @@ def test_library_and_oracle_coefficients_agree():
             b = symplectic_coefficients(scheme, dt)
             for x, y in zip(a, b):
                 np.testing.assert_array_equal(x, y)
+
+
+def test_add_source_fl_formula(state_cache, tmp_path):
+    """add_source_fl (SOLVER/time_evol_wave.F90:1062-1076) with the stiffness
+    and coupling operators zeroed: from rest, step n of newmark2
+    (SOLVER/time_evol_wave.F90:357-492) leaves chi = 0 and
+    ddchi = -inv_mass_fluid * (0 - source_term_fl(:,:,k) * stf(n)) on the
+    points of fluid element ielsrc(k); stf(n) = 0 injects nothing.  The source
+    term is set on the interior points only (one element copy each, so the
+    assembly adds nothing to it) and is asymmetric in (i, j)."""
+    _, path = state_cache("mono_vertforce_newmark")
+    st = read_state(path)
+    d = st.dims
+    elems = pick_fluid_elements(st, 3)
+    stf = np.zeros(d["niter"], np.float32)
+    stf[1], stf[2] = 0.75, -1.25          # nothing at step 1
+    fs = zero_operators(fluid_source_state(st, elems, seed=3, interior_only=True, stf=stf))
+    p = str(tmp_path / "fluid_src.bin")
+    write_state(fs, p)
+    o = Oracle(p)
+    nf = 25 * d["nel_fluid"]
+    minv = np.asarray(st["inv_mass_fluid"], np.float32)
+    term = fs["source_term"][: 25 * len(elems)].reshape(len(elems), 25)
+    hdt = 0.5 * d["deltat"]
+
+    def expected(s):
+        dd = np.zeros(nf, np.float32)
+        for k, e in enumerate(elems):
+            x = term[k] * np.float32(s)               # source_term_fl * stf, FP32
+            dd[25 * e: 25 * e + 25] = minv[25 * e: 25 * e + 25] * x
+        return dd
+
+    o.run(1, 1, d["num_rec"], d["seis_it"])
+    disp, velo, chi, dchi, ddchi = o.fields()
+    for a in (disp, velo, chi, dchi, ddchi):
+        assert not a.any()
+    o.run(2, 2, d["num_rec"], d["seis_it"])
+    disp, velo, chi, dchi, ddchi = o.fields()
+    dd2 = expected(stf[1])
+    assert np.count_nonzero(dd2) == 9 * len(elems)
+    np.testing.assert_array_equal(ddchi, dd2)
+    np.testing.assert_array_equal(dchi, (hdt * dd2.astype(np.float64)).astype(np.float32))
+    assert not chi.any() and not disp.any() and not velo.any()
+    # the next step: chi from the predictor, the new source on top of it
+    o.run(3, 3, d["num_rec"], d["seis_it"])
+    _, _, chi3, dchi3, ddchi3 = o.fields()
+    dd3 = expected(stf[2])
+    np.testing.assert_array_equal(ddchi3, dd3)
+    dt, hdt2 = d["deltat"], 0.5 * (d["deltat"] * d["deltat"])
+    chi_pred = (0.0 + dt * dchi.astype(np.float64) + hdt2 * dd2.astype(np.float64))
+    np.testing.assert_array_equal(chi3, chi_pred.astype(np.float32))
+    np.testing.assert_array_equal(
+        dchi3, (dchi.astype(np.float64) + hdt * (dd2 + dd3).astype(np.float64)).astype(np.float32))
