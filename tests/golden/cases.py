@@ -15,12 +15,22 @@ from axisem_amd.inputs import RunConfig  # noqa: E402
 
 MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s")
 MESH_PERIOD = 200.0
+# the 50 s prem_iso mesh of the plain CPU-runnable benchmark configuration
+MESH50_DIR = os.path.join(HERE, "mesh_prem_iso_50s")
+MESH50_PERIOD = 50.0
+STATIONS_3DEG = [3.0 * k for k in range(61)]
 # a regional mesh (colatitude 0-60 degrees): absorbing boundaries
 REGIONAL_MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s_colat60")
 REGIONAL_MAX_COLAT = 60.0
 REGIONAL_STATIONS = [2.0 * k for k in range(30)]
 STATIONS = [10.0 * k for k in range(19)]
 SAMPLE_STRIDE = 5          # every 5th seismogram sample is stored
+# (every 20th for the 50 s run of 12569 samples)
+STRIDES = {"p50_mono_vertforce_newmark": 20}
+
+
+def sample_stride(name: str) -> int:
+    return STRIDES.get(name, SAMPLE_STRIDE)
 
 CASES = {
     # monopole, surface vertical force, Newmark, elastic
@@ -58,6 +68,11 @@ CASES = {
                                                      500.0, STATIONS, coarse_grained=False),
     "quad_mtp_newmark_anelfull": RunConfig("mtp", 300.0, 1e20, "newmark2", True, 600.0, STATIONS,
                                            coarse_grained=False),
+    # the plain CPU-runnable benchmark configuration, full length: prem_iso
+    # 50 s, elastic, newmark2, monopole vertical surface force, Gaussian STF at
+    # the mesh period, 1800 s at 61 surface receivers 3 degrees apart
+    "p50_mono_vertforce_newmark": RunConfig("vertforce", 0.0, 1e20, "newmark2", False, 1800.0,
+                                            STATIONS_3DEG),
 }
 
 # Source batches: several sources on the mesh above with one source type and
@@ -149,7 +164,11 @@ OUTPUT_FILES = ("snap_", "xdmf_snap_", "glob_grid_", "xdmf_grid_", "xdmf_points_
 
 def mesh_of(name: str) -> str:
     """The mesh directory a case runs on."""
-    return REGIONAL_MESH_DIR if name.startswith("reg_") else MESH_DIR
+    if name.startswith("reg_"):
+        return REGIONAL_MESH_DIR
+    if name.startswith("p50_"):
+        return MESH50_DIR
+    return MESH_DIR
 
 
 def case_dir(name: str) -> str:

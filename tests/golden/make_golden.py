@@ -4,10 +4,11 @@ Runs in the build container only (needs /root/reference for the reference
 programs built by oracle/build_ref.py):
 
 1. the reference MESHER -> tests/golden/mesh_prem_iso_200s/ (meshdb.dat0000),
-   and the regional mesh (LOCAL_MAX_COLAT 60) -> mesh_prem_iso_200s_colat60/;
+   the regional mesh (LOCAL_MAX_COLAT 60) -> mesh_prem_iso_200s_colat60/ and
+   the 50 s mesh -> mesh_prem_iso_50s/;
 2. for every case, the unmodified reference SOLVER (oracle/_ref/axisem) on the
    case's inputs -> receiver seismograms, stored as float32 (the reference
-   prints them with round-trip precision) every SAMPLE_STRIDE samples in
+   prints them with round-trip precision) every sample_stride(case) samples in
    tests/golden/<case>/ref_seis.npz, with niter, deltat and the seismogram
    sampling of the run;
 3. for every case of OUTPUT_CASES (snapshots, energies), the SHA-256 of the
@@ -42,10 +43,10 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def make_mesh(mesh_dir, max_colat=None):
+def make_mesh(mesh_dir, max_colat=None, period=cases.MESH_PERIOD):
     os.makedirs(mesh_dir, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
-        inputs.run_mesher(XMESH, tmp, cases.MESH_PERIOD, max_colat=max_colat)
+        inputs.run_mesher(XMESH, tmp, period, max_colat=max_colat)
         for fn in ("meshdb.dat0000", "mesh_params.h"):
             shutil.copy(os.path.join(tmp, fn), mesh_dir)
 
@@ -64,9 +65,9 @@ def run_reference(name: str):
         text = open(os.path.join(tmp, "OUTPUT"), errors="replace").read()
     # the first row is the t = 0 sample written before the loop
     nsamp = seis.shape[0] - 1
-    np.savez_compressed(os.path.join(out, "ref_seis.npz"),
-                        seis=seis[1::cases.SAMPLE_STRIDE],
-                        nsamples=np.int64(nsamp), stride=np.int64(cases.SAMPLE_STRIDE))
+    stride = cases.sample_stride(name)
+    np.savez_compressed(os.path.join(out, "ref_seis.npz"), seis=seis[1::stride],
+                        nsamples=np.int64(nsamp), stride=np.int64(stride))
     for line in text.splitlines():
         if "Time loop routine" in line:
             print(name, line.strip())
@@ -106,6 +107,8 @@ def main(argv):
         make_mesh(cases.MESH_DIR)
     if not os.path.exists(os.path.join(cases.REGIONAL_MESH_DIR, "meshdb.dat0000")):
         make_mesh(cases.REGIONAL_MESH_DIR, cases.REGIONAL_MAX_COLAT)
+    if not os.path.exists(os.path.join(cases.MESH50_DIR, "meshdb.dat0000")):
+        make_mesh(cases.MESH50_DIR, period=cases.MESH50_PERIOD)
     names = argv or list(cases.CASES) + list(cases.OUTPUT_CASES)
     for n in names:
         if n in cases.OUTPUT_CASES:

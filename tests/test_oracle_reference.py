@@ -15,7 +15,8 @@ from oracle.oracle import Oracle
 
 # full-length comparisons for a subset, the first steps for the rest, so the
 # CPU suite stays within a few minutes
-FULL = {"mono_vertforce_newmark", "dipole_thetaforce_newmark_anel", "quad_mttmmpp_symplec4_anel"}
+FULL = {"mono_vertforce_newmark", "dipole_thetaforce_newmark_anel", "quad_mttmmpp_symplec4_anel",
+        "p50_mono_vertforce_newmark"}
 PREFIX_STEPS = 1200
 
 
@@ -29,7 +30,7 @@ def test_oracle_matches_reference(name, state_cache):
     cfg = cases.CASES[name]
     nc = cases.NCOMP[cfg.source_type]
     ref = np.load(cases.case_dir(name) + "/ref_seis.npz")["seis"]
-    mine = seis[::cases.SAMPLE_STRIDE, :len(cfg.stations_deg), :nc]
+    mine = seis[::cases.sample_stride(name), :len(cfg.stations_deg), :nc]
     n = min(len(mine), len(ref))
     assert n > 0
     assert np.abs(ref[:n]).max() > 0
