@@ -52,9 +52,12 @@ namespace axh {
 
 // number of packed element terms per source type (order: see pack_terms)
 constexpr int kNcMono = 15, kNcDi = 24, kNcQuad = 26;
-// tiles of a group whose points include a source point of the group (up to
-// this many are passed to the stiffness kernel by value)
-constexpr int kMaxSrcTiles = 32;
+// tiles of a group whose points include a source point of the group, passed
+// to the stiffness kernel by value.  Every element holding a copy of a source
+// point is a source element (the wavefield layout rejects a nonzero term
+// missing from a copy), so a wavefield's source points lie in at most its
+// nelsrc <= 8 elements' tiles and a group's in at most 8 per member.
+constexpr int kMaxSrcTiles = 8 * AXH_MAX_GROUP;
 constexpr int kNcFluidMono = 3, kNcFluidMulti = 4;
 // coarse-grained terms per element, 4 cg points each.  Read by every stage:
 // Y, V_s_eta, V_s_xi, V_z_eta, V_z_xi.  Read by the memory-variable update:
@@ -252,8 +255,7 @@ struct SolidArgs {
   int direct;
   // the mesh's single-copy points are the range [first_direct, nglob); a
   // group's source points among them are not direct, so the tiles holding
-  // source points (src_tile, or all tiles when nsrc_tile < 0) read the
-  // group's flags instead
+  // source points (src_tile) read the group's flags instead
   int first_direct, nsrc_tile;
   int src_tile[kMaxSrcTiles];
   const uint8_t *flags;
@@ -675,7 +677,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   // this kernel costs more than the assembly kernel saves)
   bool direct = false;
   if (B == 1 && A.direct) {
-    bool src_tile = A.nsrc_tile < 0;
+    bool src_tile = false;
     for (int q = 0; q < A.nsrc_tile; q++) src_tile = src_tile || A.src_tile[q] == tile;
     direct = valid && (src_tile ? (A.flags[gp] & kFlagDirect) != 0 : gp >= A.first_direct);
   }
@@ -2705,8 +2707,11 @@ struct Member {
   float *src_fl = nullptr;    // source_term_fl (25 x 8)
   int nrec = 0, seis_it = 1;
   int *rec_g = nullptr;
+  // receiver samples of a run, staged on the device: room for seis_cap
+  // samples, sized when the member is built (a longer run copies them out
+  // whenever the buffer is full), so a run allocates nothing
   float *seis_d = nullptr;
-  size_t seis_cap = 0;
+  int seis_cap = 0;
   int *blow = nullptr;
   double blow_thr = 0.0;
   std::vector<float> stf;     // [niter][nstages]
@@ -2727,8 +2732,7 @@ struct Fields {
   uint8_t *flags_s = nullptr;
   int nasm = 0;
   int *asm_pts = nullptr;
-  // solid tiles holding a source point of the group (-1 entries: more than
-  // kMaxSrcTiles, then every tile reads the flags)
+  // solid tiles holding a source point of the group (at most kMaxSrcTiles)
   std::vector<int> src_tiles;
 };
 
@@ -2854,6 +2858,14 @@ static void copy_state(const MeshDev &M, Fields &D, int dst_b, Fields &S, int sr
   HIPCHECK(hipGetLastError());
 }
 
+// Receiver samples a member stages on the device: a whole run's up to
+// kSeisStage (4.3 MB at 181 receivers).
+constexpr int kSeisStage = 2048;
+static int seis_staging_samples(const axh_dims &d) {
+  const int total = d.seis_it > 0 ? d.niter / d.seis_it : d.niter;
+  return std::max(1, std::min(total, kSeisStage));
+}
+
 static void member_build(const axh_ctx *ctx, const MeshDev &M, Member &W) {
   const axh_dims &d = ctx->dims;
   WaveHost H;
@@ -2869,7 +2881,11 @@ static void member_build(const axh_ctx *ctx, const MeshDev &M, Member &W) {
   }
   W.nrec = d.num_rec;
   W.seis_it = d.seis_it;
-  if (W.nrec > 0) W.rec_g = m.upload(H.rec_g);
+  if (W.nrec > 0) {
+    W.rec_g = m.upload(H.rec_g);
+    W.seis_cap = seis_staging_samples(d);
+    W.seis_d = m.alloc<float>((size_t)W.seis_cap * 3 * W.nrec);
+  }
   W.blow = m.alloc<int>(1);
   // blow-up threshold of runtime_info: maxval(abs(disp(1,1,:,:))) >
   // 10*abs(magnitude), the real(4) maximum promoted to double
@@ -3129,15 +3145,8 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   Launcher L{R};
   hipStream_t st_ = R.stream;
 
-  for (int b = 0; b < B; b++) {
-    Member &w = *mb[b];
-    const int nsamp = axh_seis_samples(ctx[b], start[b], start[b] + nsteps - 1);
-    if (nsamp > 0 && (size_t)nsamp * 3 * w.nrec > w.seis_cap) {
-      w.seis_cap = (size_t)nsamp * 3 * w.nrec;
-      w.seis_d = w.mem.alloc<float>(w.seis_cap);
-    }
-    HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)w.blow, INT_MAX, 1, st_));
-  }
+  for (int b = 0; b < B; b++)
+    HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)mb[b]->blow, INT_MAX, 1, st_));
 
   SolidArgs SA{};
   SA.nel = M.nel_s; SA.nsls = M.nsls; SA.nglob = M.nglob_s; SA.nslot = (int)nslot_s;
@@ -3151,12 +3160,10 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   SA.trm1 = F.trm1; SA.devm1 = F.devm1;
   SA.flags = F.flags_s; SA.minv = M.minv_s; SA.velo = F.velo; SA.acc0 = F.acc0;
   SA.first_direct = M.first_direct;
-  if (F.src_tiles.size() > (size_t)kMaxSrcTiles) {
-    SA.nsrc_tile = -1;
-  } else {
-    SA.nsrc_tile = (int)F.src_tiles.size();
-    for (size_t q = 0; q < F.src_tiles.size(); q++) SA.src_tile[q] = F.src_tiles[q];
-  }
+  if (F.src_tiles.size() > (size_t)kMaxSrcTiles)
+    fail(AXH_ERR_STATE, "source points in more tiles than the group's source elements");
+  SA.nsrc_tile = (int)F.src_tiles.size();
+  for (size_t q = 0; q < F.src_tiles.size(); q++) SA.src_tile[q] = F.src_tiles[q];
   SA.disp_w = F.disp; SA.hdt = hdt;
   AnelFullArgs AN{};
   if (M.anel_full) {
@@ -3222,7 +3229,17 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   }
 
   HIPCHECK(hipEventRecord(R.ev0, st_));
-  int isamp[kGroup] = {0, 0, 0, 0};
+  // samples staged on the device (isamp) and already copied out (nout)
+  int isamp[kGroup] = {0, 0, 0, 0}, nout[kGroup] = {0, 0, 0, 0};
+  auto copy_out = [&](int b) {
+    const Member &w = *mb[b];
+    if (seis && seis[b] && isamp[b] > 0)
+      HIPCHECK(hipMemcpyAsync(seis[b] + (size_t)nout[b] * 3 * w.nrec, w.seis_d,
+                              (size_t)isamp[b] * 3 * w.nrec * sizeof(float),
+                              hipMemcpyDeviceToHost, st_));
+    nout[b] += isamp[b];
+    isamp[b] = 0;
+  };
   for (int step = 0; step < nsteps; step++) {
     for (int st = 0; st < M.nstages; st++) {
       // predictor / first drift of the step; the symplectic drifts of the
@@ -3280,6 +3297,9 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       Member &w = *mb[b];
       RA.out[b] = nullptr;
       if (w.nrec > 0 && (start[b] + step) % w.seis_it == 0) {
+        // staging buffer full: its samples go out first (stream order keeps
+        // the copy ahead of the kernels that refill it)
+        if (isamp[b] == w.seis_cap) copy_out(b);
         RA.out[b] = w.seis_d + (size_t)isamp[b] * 3 * w.nrec;
         isamp[b]++;
         any = true;
@@ -3302,10 +3322,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   HIPCHECK(hipEventRecord(R.ev1, st_));
   for (int b = 0; b < B; b++) {
     HIPCHECK(hipMemcpyAsync(&blow[b], mb[b]->blow, sizeof(int), hipMemcpyDeviceToHost, st_));
-    if (seis && seis[b] && isamp[b] > 0)
-      HIPCHECK(hipMemcpyAsync(seis[b], mb[b]->seis_d,
-                              (size_t)isamp[b] * 3 * mb[b]->nrec * sizeof(float),
-                              hipMemcpyDeviceToHost, st_));
+    copy_out(b);
   }
   HIPCHECK(hipStreamSynchronize(st_));
   float ms = 0.0f;
@@ -3511,6 +3528,7 @@ void footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes) {
   if (nel_f) w += 3 * gf * 4 + nf * 4 + nel_f * 4 + 200 * 4;
   if (d.anel) w += nel_s * (24 * d.n_sls + 24 + 4 + 24) * 4;
   w += (size_t)std::max(d.nelsrc, 0) * 25 * 16 + (size_t)d.num_rec * 4 + 4;
+  if (d.num_rec > 0) w += (size_t)seis_staging_samples(d) * 3 * d.num_rec * 4;
   *mesh_bytes = m;
   *wave_bytes = w;
 }

@@ -320,8 +320,10 @@ typedef struct axh_mesh axh_mesh;
 int axh_mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]);
 
 /* Device bytes of ctx's mesh part and of one wavefield on it (fields,
- * memory variables, element forces, source and receiver tables; not the
- * seismogram buffer, which grows with the steps of a call).  No GPU. */
+ * memory variables, element forces, source and receiver tables and the
+ * fixed-size device staging of the receiver samples).  No GPU.  After the
+ * first run of a context, axh_run and the batch runs allocate no device
+ * memory. */
 int axh_footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes);
 
 /* A handle on ctx's device mesh part (building ctx's device part first).
