@@ -15,6 +15,8 @@ from axisem_amd.inputs import RunConfig  # noqa: E402
 
 MESH_DIR = os.path.join(HERE, "mesh_prem_iso_200s")
 MESH_PERIOD = 200.0
+# the transversely isotropic PREM (prem_ani) at 200 s
+ANI_MESH_DIR = os.path.join(HERE, "mesh_prem_ani_200s")
 # the 50 s prem_iso mesh of the plain CPU-runnable benchmark configuration
 MESH50_DIR = os.path.join(HERE, "mesh_prem_iso_50s")
 MESH50_PERIOD = 50.0
@@ -68,6 +70,16 @@ CASES = {
                                                      500.0, STATIONS, coarse_grained=False),
     "quad_mtp_newmark_anelfull": RunConfig("mtp", 300.0, 1e20, "newmark2", True, 600.0, STATIONS,
                                            coarse_grained=False),
+    # transversely isotropic PREM, anelastic (the large single-GPU
+    # configuration's model and physics at 200 s): the moment-tensor runs
+    "ani_mono_mrr_symplec4_anel": RunConfig("mrr", 100.0, 1e20, "symplec4", True, 600.0,
+                                            STATIONS),
+    "ani_dipole_mtr_symplec4_anel": RunConfig("mtr", 100.0, 1e20, "symplec4", True, 600.0,
+                                              STATIONS),
+    "ani_quad_mtp_symplec4_anel": RunConfig("mtp", 100.0, 1e20, "symplec4", True, 600.0,
+                                            STATIONS),
+    "ani_dipole_thetaforce_newmark_anel": RunConfig("thetaforce", 50.0, 1e20, "newmark2", True,
+                                                    900.0, STATIONS),
     # the plain CPU-runnable benchmark configuration, full length: prem_iso
     # 50 s, elastic, newmark2, monopole vertical surface force, Gaussian STF at
     # the mesh period, 1800 s at 61 surface receivers 3 degrees apart
@@ -115,7 +127,27 @@ BATCH_MEMBERS = {
     "bf_mono_mtt_p_mpp_300": RunConfig("mtt_p_mpp", 300.0, 1e20, "newmark2", True, 500.0,
                                        STATIONS, coarse_grained=False),
 }
+# the four runs of a moment-tensor source (monopole Mzz and Mxx+Myy, dipole
+# Mxz, quadrupole Mxy) at several depths: one batch of mixed source types
+MT_RUNS = ("mrr", "mtt_p_mpp", "mtr", "mtp")
+MT_DEPTHS = (10.0, 100.0, 300.0, 600.0)
+for _d in MT_DEPTHS:
+    for _r in MT_RUNS:
+        BATCH_MEMBERS["mt_%s_%g" % (_r, _d)] = RunConfig(_r, _d, 1e20, "newmark2", False, 600.0,
+                                                       STATIONS)
+for _r in MT_RUNS:
+    BATCH_MEMBERS["mta_%s_100" % _r] = RunConfig(_r, 100.0, 1e20, "symplec4", True, 500.0,
+                                                 STATIONS)
+    BATCH_MEMBERS["ani_mt_%s_100" % _r] = RunConfig(_r, 100.0, 1e20, "symplec4", True, 400.0,
+                                                    STATIONS)
 BATCHES = {
+    # mixed source types: one moment tensor at 100 km (elastic newmark2), the
+    # same at four depths (16 wavefields), and anelastic symplec4 at 100 km
+    "mt4_newmark": ["mt_%s_100" % r for r in MT_RUNS],
+    "mt16_newmark": ["mt_%s_%g" % (r, d) for d in MT_DEPTHS for r in MT_RUNS],
+    "mt4_symplec4_anel": ["mta_%s_100" % r for r in MT_RUNS],
+    # the same on transversely isotropic PREM
+    "ani_mt4_symplec4_anel": ["ani_mt_%s_100" % r for r in MT_RUNS],
     "dipole_symplec4_anel": ["b_dipole_thetaforce_50", "b_dipole_phiforce_1000",
                              "b_dipole_mtr_700", "b_dipole_mpr_2000"],
     "mono_newmark_anel": ["b_mono_mtt_p_mpp_300", "b_mono_explosion_100", "b_mono_vertforce_0"],
@@ -168,6 +200,8 @@ def mesh_of(name: str) -> str:
         return REGIONAL_MESH_DIR
     if name.startswith("p50_"):
         return MESH50_DIR
+    if name.startswith("ani"):
+        return ANI_MESH_DIR
     return MESH_DIR
 
 

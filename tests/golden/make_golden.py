@@ -5,7 +5,8 @@ programs built by oracle/build_ref.py):
 
 1. the reference MESHER -> tests/golden/mesh_prem_iso_200s/ (meshdb.dat0000),
    the regional mesh (LOCAL_MAX_COLAT 60) -> mesh_prem_iso_200s_colat60/ and
-   the 50 s mesh -> mesh_prem_iso_50s/;
+   the 50 s mesh -> mesh_prem_iso_50s/ and the transversely isotropic
+   prem_ani 200 s mesh -> mesh_prem_ani_200s/;
 2. for every case, the unmodified reference SOLVER (oracle/_ref/axisem) on the
    case's inputs -> receiver seismograms, stored as float32 (the reference
    prints them with round-trip precision) every sample_stride(case) samples in
@@ -43,10 +44,10 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def make_mesh(mesh_dir, max_colat=None, period=cases.MESH_PERIOD):
+def make_mesh(mesh_dir, max_colat=None, period=cases.MESH_PERIOD, model="prem_iso"):
     os.makedirs(mesh_dir, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
-        inputs.run_mesher(XMESH, tmp, period, max_colat=max_colat)
+        inputs.run_mesher(XMESH, tmp, period, model, max_colat=max_colat)
         for fn in ("meshdb.dat0000", "mesh_params.h"):
             shutil.copy(os.path.join(tmp, fn), mesh_dir)
 
@@ -109,6 +110,8 @@ def main(argv):
         make_mesh(cases.REGIONAL_MESH_DIR, cases.REGIONAL_MAX_COLAT)
     if not os.path.exists(os.path.join(cases.MESH50_DIR, "meshdb.dat0000")):
         make_mesh(cases.MESH50_DIR, period=cases.MESH50_PERIOD)
+    if not os.path.exists(os.path.join(cases.ANI_MESH_DIR, "meshdb.dat0000")):
+        make_mesh(cases.ANI_MESH_DIR, model="prem_ani")
     names = argv or list(cases.CASES) + list(cases.OUTPUT_CASES)
     for n in names:
         if n in cases.OUTPUT_CASES:
