@@ -81,25 +81,12 @@ static_assert(sizeof(axh_dims) == 104, "axh_dims layout");
 
 static thread_local std::string g_last_error;
 
-static int set_error(int code, const std::string &msg) {
+namespace axh {
+int set_error(int code, const std::string &msg) {
   g_last_error = msg;
   return code;
 }
-
-template <class F> static int guarded(F &&f) {
-  try {
-    f();
-    return AXH_OK;
-  } catch (const Error &e) {
-    return set_error(e.code, e.msg);
-  } catch (const std::bad_alloc &) {
-    return set_error(AXH_ERR_ARG, "host allocation failed");
-  } catch (const std::exception &e) {
-    return set_error(AXH_ERR_ARG, e.what());
-  } catch (...) {
-    return set_error(AXH_ERR_ARG, "unknown error");
-  }
-}
+}  // namespace axh
 
 const axh::Array &axh_ctx::need(const std::string &name) const {
   auto *a = get(name);
@@ -383,6 +370,28 @@ int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
   });
 }
 
+int axh_set_arithmetic(axh_ctx *ctx, int32_t mode) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (mode != AXH_ARITH_EXACT && mode != AXH_ARITH_FMA) fail(AXH_ERR_ARG, "unknown arithmetic");
+    ctx->arith = mode;
+  });
+}
+
+int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (it_begin != ctx->iter_done + 1)
+      fail(AXH_ERR_ARG, "axh_run_noalloc must continue at step " +
+                            std::to_string(ctx->iter_done + 1));
+    if (it_end < it_begin - 1 || it_end > ctx->dims.niter)
+      fail(AXH_ERR_ARG, "step range outside 1..niter");
+    if (!ctx->dev)
+      fail(AXH_ERR_STATE, "the context is not on the device: run it once with axh_run first");
+    device_run_noalloc(ctx, it_begin, it_end, seis);
+  });
+}
+
 int axh_download_fields(axh_ctx *ctx, float *disp, float *velo, float *chi,
                         float *dchi, float *ddchi) {
   return guarded([&] {
@@ -458,25 +467,41 @@ int axh_device_bytes(axh_ctx *ctx, size_t *bytes) {
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
-// Members of a batch step on member 0's device mesh: dims and the mesh part
-// the device would build (every term the kernels read; not the per-source
-// uploads ielsrc, source_term, stf, recfile_el) must agree.  The reference
-// leaves entries it never reads unset (e.g. M0_w_fl of non-axial elements,
-// SOLVER/def_precomp_terms.f90:2355-2455), so the comparison is on the built
-// layout, not on the raw uploads.
-static void check_same_mesh(const axh_ctx *a, const uint64_t fa[2], axh_ctx *b, int idx) {
+// Members of a batch share one mesh and one time scheme: equal dims (the
+// source type aside) and equal global numbering.  Members of one source type
+// step on one device mesh part, so for them the mesh part the device would
+// build (every term the kernels read; not the per-source uploads ielsrc,
+// source_term, stf, recfile_el) must agree.  The reference leaves entries it
+// never reads unset (e.g. M0_w_fl of non-axial elements,
+// SOLVER/def_precomp_terms.f90:2355-2455), so that comparison is on the built
+// layout, not on the raw uploads.  Members of different source types (the
+// monopole, dipole and quadrupole runs of a moment tensor,
+// SOLVER/source.f90:40-140) each use their type's mesh part.
+static void check_same_mesh(const axh_ctx *a, axh_ctx *b, int idx) {
   const std::string who = "batch member " + std::to_string(idx) + ": ";
   const axh_dims &x = a->dims, &y = b->dims;
   if (x.abi_version != y.abi_version || x.npol != y.npol || x.nel_solid != y.nel_solid ||
       x.nel_fluid != y.nel_fluid || x.nglob_solid != y.nglob_solid ||
-      x.nglob_fluid != y.nglob_fluid || x.nel_bdry != y.nel_bdry || x.src_type != y.src_type ||
+      x.nglob_fluid != y.nglob_fluid || x.nel_bdry != y.nel_bdry ||
       x.scheme != y.scheme || x.anel != y.anel || x.coarse_grained != y.coarse_grained ||
       x.n_sls != y.n_sls || std::memcmp(&x.deltat, &y.deltat, sizeof(double)) != 0)
-    fail(AXH_ERR_ARG, who + "mesh, source type or time scheme differs from member 0");
-  uint64_t fb[2];
+    fail(AXH_ERR_ARG, who + "mesh or time scheme differs from member 0");
+  for (const char *nm : {"igloc_solid", "igloc_fluid"}) {
+    const Array *p = a->get(nm), *q = b->get(nm);
+    if (!p || !q) fail(AXH_ERR_STATE, who + "missing upload " + nm);
+    if (p->bytes.size() != q->bytes.size() ||
+        std::memcmp(p->bytes.data(), q->bytes.data(), p->bytes.size()) != 0)
+      fail(AXH_ERR_ARG, who + "global numbering differs from member 0");
+  }
+}
+
+static void check_same_terms(axh_ctx *a, axh_ctx *b, int idx) {
+  uint64_t fa[2], fb[2];
+  mesh_fingerprint(a, fa);
   mesh_fingerprint(b, fb);
   if (fa[0] != fb[0] || fa[1] != fb[1])
-    fail(AXH_ERR_ARG, who + "mesh terms differ from member 0");
+    fail(AXH_ERR_ARG, "batch member " + std::to_string(idx) +
+                          ": mesh terms differ from an earlier member of its source type");
 }
 
 static void check_batch(const axh_batch *b) {
@@ -497,9 +522,14 @@ int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
       for (int j = 0; j < i; j++)
         if (members[j] == members[i]) fail(AXH_ERR_ARG, "a context appears twice in a batch");
     }
-    uint64_t f0[2] = {0, 0};
-    if (n > 1) mesh_fingerprint(members[0], f0);
-    for (int i = 1; i < n; i++) check_same_mesh(members[0], f0, members[i], i);
+    for (int i = 1; i < n; i++) {
+      check_same_mesh(members[0], members[i], i);
+      for (int j = 0; j < i; j++)
+        if (members[j]->dims.src_type == members[i]->dims.src_type) {
+          check_same_terms(members[j], members[i], i);
+          break;
+        }
+    }
     auto *b = new axh_batch;
     b->members.assign(members, members + n);
     for (auto *c : b->members) c->batch_refs++;
@@ -562,6 +592,14 @@ int axh_batch_set_group_width(axh_batch *batch, int32_t width) {
   });
 }
 
+int axh_batch_set_arithmetic(axh_batch *batch, int32_t mode) {
+  return guarded([&] {
+    check_batch(batch);
+    if (mode != AXH_ARITH_EXACT && mode != AXH_ARITH_FMA) fail(AXH_ERR_ARG, "unknown arithmetic");
+    batch->arith = mode;
+  });
+}
+
 int axh_batch_set_profiling(axh_batch *batch, int on) {
   return guarded([&] {
     check_batch(batch);
@@ -582,6 +620,14 @@ int axh_batch_kernel_stats(const axh_batch *batch, int kclass, double *total_ms,
 // ---------------------------------------------------------------------------
 // resident meshes and footprints (request loops)
 // ---------------------------------------------------------------------------
+int axh_batch_key(axh_ctx *ctx, uint64_t *key) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!key) fail(AXH_ERR_ARG, "null pointer");
+    batch_key(ctx, key);
+  });
+}
+
 int axh_mesh_fingerprint(axh_ctx *ctx, uint64_t *fp) {
   return guarded([&] {
     check_ctx(ctx);

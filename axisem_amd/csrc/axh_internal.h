@@ -2,7 +2,9 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
 #include <map>
+#include <new>
 #include <memory>
 #include <string>
 #include <vector>
@@ -52,6 +54,24 @@ struct Error {
 };
 [[noreturn]] void fail(int code, const std::string &msg);
 
+// Runs f, converting a failure into its status code (the message goes to
+// axh_last_error): no exception crosses the C-ABI.
+int set_error(int code, const std::string &msg);
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return AXH_OK;
+  } catch (const Error &e) {
+    return set_error(e.code, e.msg);
+  } catch (const std::bad_alloc &) {
+    return set_error(AXH_ERR_ARG, "host allocation failed");
+  } catch (const std::exception &e) {
+    return set_error(AXH_ERR_ARG, e.what());
+  } catch (...) {
+    return set_error(AXH_ERR_ARG, "unknown error");
+  }
+}
+
 }  // namespace axh
 
 struct axh_ctx {
@@ -61,6 +81,7 @@ struct axh_ctx {
   int32_t iter_done = 0;  // last completed step
   double last_run_ms = 0.0;
   bool profile = false;
+  int arith = AXH_ARITH_EXACT;  // enum axh_arith
   int batch_refs = 0;     // batches holding this context
   // device mesh part to use when the device part is built (axh_mesh_bind)
   std::shared_ptr<axh::MeshDev> bound_mesh;
@@ -86,6 +107,7 @@ struct axh_batch {
   std::unique_ptr<axh::BatchState, axh::BatchStateDeleter> state;  // NULL before the first run
   double last_run_ms = 0.0;
   bool profile = false;
+  int arith = AXH_ARITH_EXACT;      // enum axh_arith
   int group_width = AXH_MAX_GROUP;  // wavefields per kernel instance
 };
 
@@ -98,6 +120,8 @@ extern const char *const kFluid3dNames[4];
 void device_build(axh_ctx *ctx);
 void device_destroy(Device *d);
 void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_run_noalloc(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+size_t noalloc_max_samples(int nrec);
 void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
                      float *dchi, float *ddchi);
 void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
@@ -115,6 +139,7 @@ void batch_attach(axh_batch *b);
 void batch_detach(axh_batch *b);
 bool in_running_batch(const axh_ctx *ctx);
 void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]);
+void batch_key(const axh_ctx *ctx, uint64_t key[2]);
 // device bytes of the mesh part and of one wavefield (B = 1) without a GPU
 void footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes);
 axh_mesh *mesh_get(axh_ctx *ctx);
