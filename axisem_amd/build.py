@@ -42,6 +42,7 @@ FFLAGS = ["-O2", "-cpp", "-Dserial", "-Dsolver"]
 
 def build_library(out: str = LIB) -> str:
     srcs = [os.path.join(PKG, "csrc", "axh_host.cpp"),
+            os.path.join(PKG, "csrc", "axh_meshdb.cpp"),
             os.path.join(PKG, "csrc", "axh_device.hip")]
     tmp = out + ".tmp"
     cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO, "include"), *srcs, "-o", tmp]

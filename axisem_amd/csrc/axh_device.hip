@@ -2599,6 +2599,24 @@ void mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]) {
   ctx->fp_valid = true;
 }
 
+// Key of the batches a context may join: the dims every member shares and the
+// mesh maps (numbering and axis flags), not the source type or its terms.
+void batch_key(const axh_ctx *ctx, uint64_t key[2]) {
+  const axh_dims &d = ctx->dims;
+  FpHash h;
+  const int ints[] = {d.abi_version, d.npol, d.nel_solid, d.nel_fluid, d.nglob_solid,
+                      d.nglob_fluid, d.nel_bdry, d.scheme, d.anel, d.coarse_grained, d.n_sls};
+  h.bytes(ints, sizeof ints);
+  h.bytes(&d.deltat, sizeof(double));
+  for (const char *nm : {"igloc_solid", "igloc_fluid", "axis_solid", "axis_fluid"}) {
+    const Array *a = ctx->get(nm);
+    if (!a) fail(AXH_ERR_STATE, std::string("missing upload: ") + nm);
+    h.vec(a->bytes);
+  }
+  key[0] = h.a;
+  key[1] = h.b;
+}
+
 void global_numbering(const axh_ctx *ctx, int domain, int32_t *gid, int32_t *nglob) {
   const bool solid = domain == 0;
   MeshHost H;
@@ -2933,7 +2951,8 @@ struct Device {
 // Groups of a batch: interleaved fields of up to kGroup members each.
 struct BatchState {
   std::vector<std::unique_ptr<Fields>> groups;
-  std::vector<std::vector<int>> index;  // members of each group
+  std::vector<std::vector<int>> index;          // members of each group
+  std::vector<std::shared_ptr<MeshDev>> mesh;   // each group's mesh part
   Runner run;
 };
 
@@ -2975,53 +2994,62 @@ bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n) {
 
 bool in_running_batch(const axh_ctx *ctx) { return ctx->dev && ctx->dev->live; }
 
-// Put every member on member 0's device mesh (building what is missing) and
-// move the members' state into the batch's interleaved groups.
+// Put every member on the device mesh of the first member of its source type
+// (building what is missing) and move the members' state into the batch's
+// interleaved groups: the members of each source type, in batch order, split
+// into groups of at most the group width, sizes as equal as possible.
 void batch_attach(axh_batch *bt) {
   if (bt->state) return;
   const std::vector<axh_ctx *> &members = bt->members;
-  axh_ctx *first = members[0];
-  if (!first->dev) device_build(first);
-  std::shared_ptr<MeshDev> mesh = first->dev->mesh;
-  for (size_t b = 1; b < members.size(); b++) {
-    axh_ctx *c = members[b];
-    if (!c->dev) {
-      std::unique_ptr<Device> D(new Device);
-      D->mesh = mesh;
-      member_build(c, *mesh, D->member);
-      fields_alloc(*mesh, D->own, 1);
-      fields_set_sources(*mesh, D->own, {&D->member.src_g_h});
-      c->dev.reset(D.release());
-    } else if (c->dev->mesh != mesh) {
-      // same mesh part (checked when the batch was created): the global
-      // numbering is the same, so the wavefield part carries over unchanged
-      if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
-      c->dev->mesh = mesh;
-    }
-  }
   std::unique_ptr<BatchState, BatchStateDeleter> S(new BatchState);
   S->run.init();
-  // groups of at most kGroup members, sizes as equal as possible
-  const int n = (int)members.size();
   const int width = std::max(1, std::min(bt->group_width, kGroup));
-  const int ng = (n + width - 1) / width;
-  int next = 0;
-  for (int g = 0; g < ng; g++) {
-    const int size = n / ng + (g < n % ng ? 1 : 0);
-    std::unique_ptr<Fields> F(new Fields);
-    fields_alloc(*mesh, *F, size);
-    std::vector<const std::vector<int> *> srcs;
-    for (int b = 0; b < size; b++) srcs.push_back(&members[next + b]->dev->member.src_g_h);
-    fields_set_sources(*mesh, *F, srcs);
-    std::vector<int> idx;
-    for (int b = 0; b < size; b++, next++) {
-      Device &D = *members[next]->dev;
-      if (D.run.stream) HIPCHECK(hipStreamSynchronize(D.run.stream));
-      copy_state(*mesh, *F, b, D.own, 0, S->run.stream);
-      idx.push_back(next);
+  for (int type = AXH_MONOPOLE; type <= AXH_QUADPOLE; type++) {
+    std::vector<int> of_type;
+    for (size_t b = 0; b < members.size(); b++)
+      if (members[b]->dims.src_type == type) of_type.push_back((int)b);
+    if (of_type.empty()) continue;
+    axh_ctx *first = members[of_type[0]];
+    if (!first->dev) device_build(first);
+    std::shared_ptr<MeshDev> mesh = first->dev->mesh;
+    for (size_t q = 1; q < of_type.size(); q++) {
+      axh_ctx *c = members[of_type[q]];
+      if (!c->dev) {
+        std::unique_ptr<Device> D(new Device);
+        D->mesh = mesh;
+        member_build(c, *mesh, D->member);
+        fields_alloc(*mesh, D->own, 1);
+        fields_set_sources(*mesh, D->own, {&D->member.src_g_h});
+        c->dev.reset(D.release());
+      } else if (c->dev->mesh != mesh) {
+        // same mesh part (checked when the batch was created): the global
+        // numbering is the same, so the wavefield part carries over unchanged
+        if (c->dev->run.stream) HIPCHECK(hipStreamSynchronize(c->dev->run.stream));
+        c->dev->mesh = mesh;
+      }
     }
-    S->index.push_back(idx);
-    S->groups.push_back(std::move(F));
+    const int n = (int)of_type.size();
+    const int ng = (n + width - 1) / width;
+    int next = 0;
+    for (int g = 0; g < ng; g++) {
+      const int size = n / ng + (g < n % ng ? 1 : 0);
+      std::unique_ptr<Fields> F(new Fields);
+      fields_alloc(*mesh, *F, size);
+      std::vector<const std::vector<int> *> srcs;
+      for (int b = 0; b < size; b++)
+        srcs.push_back(&members[of_type[next + b]]->dev->member.src_g_h);
+      fields_set_sources(*mesh, *F, srcs);
+      std::vector<int> idx;
+      for (int b = 0; b < size; b++, next++) {
+        Device &D = *members[of_type[next]]->dev;
+        if (D.run.stream) HIPCHECK(hipStreamSynchronize(D.run.stream));
+        copy_state(*mesh, *F, b, D.own, 0, S->run.stream);
+        idx.push_back(of_type[next]);
+      }
+      S->index.push_back(idx);
+      S->groups.push_back(std::move(F));
+      S->mesh.push_back(mesh);
+    }
   }
   HIPCHECK(hipStreamSynchronize(S->run.stream));
   for (size_t g = 0; g < S->groups.size(); g++)
@@ -3372,11 +3400,11 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
 void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
   BatchState &S = *bt->state;
   S.run.profile = bt->profile;
-  const MeshDev &M = *bt->members[0]->dev->mesh;
   double ms = 0.0;
   std::vector<int> blow(bt->members.size(), INT_MAX);
   for (size_t g = 0; g < S.groups.size(); g++) {
     const std::vector<int> &idx = S.index[g];
+    const MeshDev &M = *S.mesh[g];
     Member *mb[kGroup];
     const axh_ctx *cx[kGroup];
     float *out[kGroup];

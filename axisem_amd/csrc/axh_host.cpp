@@ -370,28 +370,6 @@ int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
   });
 }
 
-int axh_set_arithmetic(axh_ctx *ctx, int32_t mode) {
-  return guarded([&] {
-    check_ctx(ctx);
-    if (mode != AXH_ARITH_EXACT && mode != AXH_ARITH_FMA) fail(AXH_ERR_ARG, "unknown arithmetic");
-    ctx->arith = mode;
-  });
-}
-
-int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
-  return guarded([&] {
-    check_ctx(ctx);
-    if (it_begin != ctx->iter_done + 1)
-      fail(AXH_ERR_ARG, "axh_run_noalloc must continue at step " +
-                            std::to_string(ctx->iter_done + 1));
-    if (it_end < it_begin - 1 || it_end > ctx->dims.niter)
-      fail(AXH_ERR_ARG, "step range outside 1..niter");
-    if (!ctx->dev)
-      fail(AXH_ERR_STATE, "the context is not on the device: run it once with axh_run first");
-    device_run_noalloc(ctx, it_begin, it_end, seis);
-  });
-}
-
 int axh_download_fields(axh_ctx *ctx, float *disp, float *velo, float *chi,
                         float *dchi, float *ddchi) {
   return guarded([&] {
@@ -589,14 +567,6 @@ int axh_batch_set_group_width(axh_batch *batch, int32_t width) {
       fail(AXH_ERR_ARG, "group width must be 1.." + std::to_string(AXH_MAX_GROUP));
     if (batch->state) fail(AXH_ERR_STATE, "the batch has already run");
     batch->group_width = width;
-  });
-}
-
-int axh_batch_set_arithmetic(axh_batch *batch, int32_t mode) {
-  return guarded([&] {
-    check_batch(batch);
-    if (mode != AXH_ARITH_EXACT && mode != AXH_ARITH_FMA) fail(AXH_ERR_ARG, "unknown arithmetic");
-    batch->arith = mode;
   });
 }
 

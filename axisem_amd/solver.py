@@ -23,7 +23,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libaxihip.so")
 
 AXH_OK, AXH_ERR_BLOWUP = 0, -6
-MAX_BATCH = 8
+MAX_BATCH = 16
 
 SOLID_TERMS = ["M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M11z", "M21z",
                "M41z", "M32s", "M13s", "M33s", "M43s", "M1phi", "M2phi", "M4phi",
@@ -60,6 +60,15 @@ class AttFull(C.Structure):
         "V0_z_eta", "V0_z_xi", "DsDeta_over_J", "DzDeta_over_J", "DsDxi_over_J",
         "DzDxi_over_J", "inv_s", "delta_mu", "delta_kappa",
         "exp_w_j_deltat", "ts_fac_t", "ts_fac_tm1", "a_j_mu", "a_j_kappa")]
+
+
+class MeshDBInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "nproc", "npol", "nelem", "npoint", "nel_solid", "nel_fluid", "nglob_solid",
+        "nglob_fluid", "nel_bdry", "ndisc", "npoin", "nglob", "naxel", "naxel_solid",
+        "naxel_fluid", "have_bdry", "have_fluid", "realkind_bytes")] + \
+        [(n, C.c_double) for n in ("pts_wavelngth", "period", "courant", "deltat", "router")] + \
+        [("bkgrdmodel", C.c_char * 64)]
 
 
 _lib = None
@@ -119,6 +128,12 @@ def library():
         L.axh_mesh_release.argtypes = [P]
         L.axh_batch_kernel_stats.argtypes = [P, C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_int64)]
+        L.axh_batch_key.argtypes = [P, P]
+        L.axh_meshdb_open.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.axh_meshdb_close.argtypes = [P]
+        L.axh_meshdb_get_info.argtypes = [P, C.POINTER(MeshDBInfo)]
+        L.axh_meshdb_array.argtypes = [P, C.c_char_p, P, C.c_size_t]
+        L.axh_meshdb_upload.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -403,3 +418,57 @@ class Batch:
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""
         return _kernel_stats(library().axh_batch_kernel_stats, self._h)
+
+
+# element counts of the arrays axh_meshdb_array returns: name -> (dtype, size)
+def _meshdb_shapes(I):
+    ns, nf, nb, ne = I["nel_solid"], I["nel_fluid"], I["nel_bdry"], I["nelem"]
+    i4, f4, f8 = np.int32, np.float32, np.float64
+    return {
+        "igloc_solid": (i4, 25 * ns), "igloc_fluid": (i4, 25 * nf),
+        "ielsolid": (i4, ns), "ielfluid": (i4, nf),
+        "axis_solid": (i4, ns), "axis_fluid": (i4, nf),
+        "ax_el_solid": (i4, I["naxel_solid"]), "ax_el_fluid": (i4, I["naxel_fluid"]),
+        "bdry_solid_el": (i4, nb if I["have_bdry"] else 0),
+        "bdry_fluid_el": (i4, nb if I["have_bdry"] else 0),
+        "bdry_jpol_solid": (i4, nb if I["have_bdry"] else 0),
+        "bdry_jpol_fluid": (i4, nb if I["have_bdry"] else 0),
+        "G0": (f4, 5), "G1": (f4, 25), "G1T": (f4, 25), "G2": (f4, 25), "G2T": (f4, 25),
+        "crd_s": (f8, I["npoin"]), "crd_z": (f8, I["npoin"]),
+        "lnods": (i4, 8 * ne), "coarsing": (i4, ne), "eltype": (np.uint8, 6 * ne),
+    }
+
+
+class MeshDB:
+    """The reference MESHER's mesh database (Mesh/meshdb.dat0000) read by the
+    library (axh_meshdb_*): the counts of its header and its arrays in the
+    layout the SOLVER reads them (SOLVER/get_mesh.f90:47-409)."""
+
+    def __init__(self, path: str):
+        self._h = C.c_void_p()
+        _check(library().axh_meshdb_open(path.encode(), C.byref(self._h)))
+        raw = MeshDBInfo()
+        _check(library().axh_meshdb_get_info(self._h, C.byref(raw)))
+        self.info = {k: getattr(raw, k) for k, _ in MeshDBInfo._fields_}
+        self.info["bkgrdmodel"] = raw.bkgrdmodel.decode(errors="replace").strip()
+
+    def array(self, name: str) -> np.ndarray:
+        dt, n = _meshdb_shapes(self.info)[name]
+        a = np.zeros(max(n, 1), dt)
+        _check(library().axh_meshdb_array(self._h, name.encode(), _p(a), n * a.itemsize))
+        return a[:n]
+
+    def upload(self, solver: "Solver"):
+        """Give a context its mesh-only uploads from the database."""
+        _check(library().axh_meshdb_upload(self._h, solver._h))
+
+    def close(self):
+        if self._h:
+            library().axh_meshdb_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define AXH_ABI_VERSION 1
-#define AXH_MAX_BATCH 8     /* wavefields per batch */
+#define AXH_MAX_BATCH 16    /* wavefields per batch */
 #define AXH_MAX_GROUP 4     /* wavefields per kernel instance */
 
 enum axh_status {
@@ -272,13 +272,18 @@ int axh_device_bytes(axh_ctx *ctx, size_t *bytes);
  * once per source; a batch is that loop over sources with the mesh terms read
  * from HBM once per step for all members.
  *
- * Members must have equal dims except nelsrc, fluid_src, num_rec, seis_it,
- * niter and magnitude, and byte-equal uploads except ielsrc, source_term, stf
- * and recfile_el (the per-source inputs).  Each member's fields, seismograms
- * and blow-up test are its own and bit-identical to stepping it alone with
- * axh_run.  1 <= n <= AXH_MAX_BATCH.  The batch does not own the members; a
- * member in a batch takes no uploads and axh_destroy refuses it until
- * axh_batch_destroy.  Like axh_create, creating a batch touches no GPU. */
+ * Members must have equal dims except src_type, nelsrc, fluid_src, num_rec,
+ * seis_it, niter and magnitude, and the same global numbering.  Members of
+ * one source type must have byte-equal mesh terms (every upload except
+ * ielsrc, source_term, stf and recfile_el, the per-source inputs); members of
+ * different source types -- the monopole, dipole and quadrupole runs of one
+ * moment tensor (SOLVER/source.f90:40-140, one SOLVER run each in the
+ * reference's submit.py:276-300) -- each step on their type's terms.  Each
+ * member's fields, seismograms and blow-up test are its own and bit-identical
+ * to stepping it alone with axh_run.  1 <= n <= AXH_MAX_BATCH.  The batch
+ * does not own the members; a member in a batch takes no uploads and
+ * axh_destroy refuses it until axh_batch_destroy.  Like axh_create, creating
+ * a batch touches no GPU. */
 typedef struct axh_batch axh_batch;
 int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out);
 int axh_batch_destroy(axh_batch *batch);
@@ -319,6 +324,12 @@ typedef struct axh_mesh axh_mesh;
  * mesh serves both contexts.  No GPU. */
 int axh_mesh_fingerprint(axh_ctx *ctx, uint64_t fp[2]);
 
+/* 128-bit key of the batches ctx may join: contexts with equal keys (same
+ * mesh numbering, time scheme, attenuation mode and time step; any source
+ * type) may share a batch, provided members of one source type also have
+ * equal fingerprints.  Needs only the dims and the mesh maps.  No GPU. */
+int axh_batch_key(axh_ctx *ctx, uint64_t key[2]);
+
 /* Device bytes of ctx's mesh part and of one wavefield on it (fields,
  * memory variables, element forces, source and receiver tables and the
  * fixed-size device staging of the receiver samples).  No GPU.  After the
@@ -335,6 +346,44 @@ int axh_mesh_get(axh_ctx *ctx, axh_mesh **out);
 int axh_mesh_bind(axh_ctx *ctx, const axh_mesh *mesh);
 int axh_mesh_bytes(const axh_mesh *mesh, size_t *bytes);
 int axh_mesh_release(axh_mesh *mesh);
+
+/* ---- the MESHER's mesh database ---------------------------------------------- */
+
+/* Reader of Mesh/meshdb.dat0000, the Fortran unformatted sequential file the
+ * reference MESHER writes (MESHER/pdb.f90:2213-2386) and the SOLVER reads
+ * (SOLVER/get_mesh.f90:47-409 with read_mesh_basics/advanced/axel of
+ * SOLVER/data_mesh.f90:195-328).  It replaces the SOLVER's own reading for
+ * hosts that drive the library without the Fortran program (request loops,
+ * tests).  Single-slice (nproc_mesh = 1), npol = 4, realkind 4 databases. */
+typedef struct axh_meshdb axh_meshdb;
+typedef struct axh_meshdb_info {
+  int32_t nproc, npol, nelem, npoint;
+  int32_t nel_solid, nel_fluid, nglob_solid, nglob_fluid;
+  int32_t nel_bdry, ndisc, npoin, nglob;
+  int32_t naxel, naxel_solid, naxel_fluid;
+  int32_t have_bdry, have_fluid, realkind_bytes;
+  double pts_wavelngth, period, courant, deltat, router;
+  char bkgrdmodel[64];
+} axh_meshdb_info;
+
+/* Read the whole database (AXH_ERR_IO: not a mesh database or truncated;
+ * AXH_ERR_UNSUPPORTED: several slices, npol /= 4 or realkind /= 4). */
+int axh_meshdb_open(const char *path, axh_meshdb **out);
+int axh_meshdb_close(axh_meshdb *db);
+int axh_meshdb_get_info(const axh_meshdb *db, axh_meshdb_info *info);
+
+/* Copy one array of the database, nbytes must be its exact size:
+ * int32: igloc_solid(25 nel_solid), igloc_fluid(25 nel_fluid), ielsolid,
+ * ielfluid, axis_solid/axis_fluid (per element 0/1), ax_el_solid,
+ * ax_el_fluid, bdry_solid_el, bdry_fluid_el, bdry_jpol_solid,
+ * bdry_jpol_fluid (nel_bdry), lnods(8, nelem), coarsing(nelem);
+ * float: G0(5), G1, G1T, G2, G2T (25); double: crd_s, crd_z(npoin);
+ * char: eltype(6, nelem). */
+int axh_meshdb_array(const axh_meshdb *db, const char *name, void *out, size_t nbytes);
+
+/* The mesh-only uploads of ctx from the database (axh_upload_spectral and
+ * axh_upload_mesh_maps); ctx's dims must match the database's counts. */
+int axh_meshdb_upload(const axh_meshdb *db, axh_ctx *ctx);
 
 /* ---- state files --------------------------------------------------------- */
 
