@@ -13,9 +13,12 @@
  *    build of the reference with amdflang, which is how it is built.
  * 2. Flat-argument wrappers around axh_upload_attenuation_cg4/_full (Fortran
  *    cannot easily build a struct of pointers to allocatable arrays).
- * 3. getenv for the run-mode switches of the host.
+ * 3. getenv for the run-mode switches of the host, and a file copy for the
+ *    record mode with several sources (the reference reads its source from
+ *    ./inparam_source, SOLVER/source.f90:51).
  */
 #include <ISO_Fortran_binding.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -95,4 +98,24 @@ int axh_glue_getenv(const char *name, char *val, int n) {
   if (l > n) l = n;
   memcpy(val, v, (size_t)l);
   return l;
+}
+
+/* Copies file `from` to `to` (both NUL-terminated); 0 on success. */
+int axh_glue_copy_file(const char *from, const char *to) {
+  FILE *a = fopen(from, "rb");
+  if (!a) return -1;
+  FILE *b = fopen(to, "wb");
+  if (!b) {
+    fclose(a);
+    return -1;
+  }
+  char buf[4096];
+  size_t n;
+  int rc = 0;
+  while ((n = fread(buf, 1, sizeof buf, a)) > 0)
+    if (fwrite(buf, 1, n, b) != n) rc = -1;
+  if (ferror(a)) rc = -1;
+  fclose(a);
+  if (fclose(b) != 0) rc = -1;
+  return rc;
 }

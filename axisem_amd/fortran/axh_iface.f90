@@ -199,6 +199,11 @@ module axh_iface
        character(kind=c_char), intent(out) :: val(*)
        integer(c_int), value :: n
      end function
+
+     integer(c_int) function axh_glue_copy_file(from, to) bind(C, name='axh_glue_copy_file')
+       import :: c_int, c_char
+       character(kind=c_char), intent(in) :: from(*), to(*)
+     end function
   end interface
 
 contains

@@ -16,6 +16,9 @@
 !! Environment switches of the host:
 !!   AXH_SAVE_STATE=<path>  write the uploads to a state file (axh_save_state)
 !!   AXH_NO_RUN=1           stop after prepare/upload (records states without a GPU)
+!!   AXH_EXTRA_SOURCES=<f>  with AXH_NO_RUN: after the first state, record one state per
+!!                          line of file f ("<inparam_source file> <state file>"), all
+!!                          on the mesh and model terms set up once (one source type)
 module time_evol_wave
 
   use global_parameters
@@ -58,6 +61,9 @@ subroutine prepare_waves
   use clocks_mod
   use commun
 
+  character(len=1024) :: path
+  integer(c_int)      :: lpath
+
   if (lpr) then
      write(6,*)''
      write(6,*)'++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++++'
@@ -94,6 +100,15 @@ subroutine prepare_waves
 
   call write_parameters
   call upload_all()
+
+  lpath = axh_glue_getenv('AXH_SAVE_STATE'//c_null_char, path, len(path))
+  if (lpath > 0) then
+     call axh_check(axh_save_state(ctx, trim(path(1:lpath))//c_null_char), 'save state')
+     if (lpr) write(6,*) '  wrote GPU state file ', trim(path(1:lpath))
+  endif
+  lpath = axh_glue_getenv('AXH_NO_RUN'//c_null_char, path, len(path))
+  no_run = lpath > 0 .and. path(1:1) /= '0'
+  if (no_run) call record_extra_sources()
 
   call barrier
   if (lpr) write(6,*) 'done preparing waves (GPU time loop).'
@@ -376,8 +391,6 @@ subroutine upload_all()
   real(kind=dp)                 :: tacc
   integer(c_int32_t)            :: nst
   integer                       :: i, iter, nextra, iel
-  character(len=1024)           :: path
-  integer(c_int)                :: lpath
 
   ! time scheme (read_inparam_basic lower-cases TIME_SCHEME,
   ! SOLVER/parameters.F90:471, so the McLachlan schemes arrive as ml_so*)
@@ -550,13 +563,6 @@ subroutine upload_all()
   ! layout checks of the library, before any device work
   call axh_check(axh_check_inputs(ctx), 'input checks')
 
-  lpath = axh_glue_getenv('AXH_SAVE_STATE'//c_null_char, path, len(path))
-  if (lpath > 0) then
-     call axh_check(axh_save_state(ctx, trim(path(1:lpath))//c_null_char), 'save state')
-     if (lpr) write(6,*) '  wrote GPU state file ', trim(path(1:lpath))
-  endif
-  lpath = axh_glue_getenv('AXH_NO_RUN'//c_null_char, path, len(path))
-  no_run = lpath > 0 .and. path(1:1) /= '0'
 
 contains
 
@@ -573,6 +579,76 @@ contains
   end subroutine
 
 end subroutine upload_all
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> Record mode with several sources of one source type (AXH_EXTRA_SOURCES): the mesh,
+!! the model and every precomputed term stay as set up for the first source; for each
+!! listed source the reference's own read_sourceparams, compute_stf and compute_src run
+!! again (on a copy of the listed file as ./inparam_source, where read_sourceparams
+!! reads it, SOLVER/source.f90:51) and the uploads are written to the listed state file.
+!! Each state equals the one a separate run of the host on that source records.
+subroutine record_extra_sources()
+
+  use source, only: read_sourceparams, compute_stf, compute_src
+
+  character(len=1024) :: list, line, srcfile, statefile
+  character(len=10)   :: type0
+  integer(c_int)      :: l
+  integer             :: u, ios, nrec, i
+
+  l = axh_glue_getenv('AXH_EXTRA_SOURCES'//c_null_char, list, len(list))
+  if (l <= 0) return
+  type0 = src_type(1)
+  nrec = nrec_all
+  open(newunit=u, file=list(1:l), status='old', action='read', iostat=ios)
+  if (ios /= 0) then
+     write(6,*) 'GPU time loop: cannot open the source list ', list(1:l)
+     stop 2
+  endif
+  do
+     read(u, '(a)', iostat=ios) line
+     if (ios /= 0) exit
+     if (len_trim(line) == 0) cycle
+     ! two blank-separated paths (not list-directed input: '/' would end the record)
+     line = adjustl(line)
+     i = index(trim(line), ' ')
+     if (i == 0) then
+        write(6,*) 'GPU time loop: bad line in the source list: ', trim(line)
+        stop 2
+     endif
+     srcfile = line(1:i-1)
+     statefile = adjustl(line(i+1:))
+     if (axh_glue_copy_file(trim(srcfile)//c_null_char, 'inparam_source'//c_null_char) /= 0) then
+        write(6,*) 'GPU time loop: cannot copy ', trim(srcfile), ' to inparam_source'
+        stop 2
+     endif
+     call read_sourceparams
+     if (src_type(1) /= type0 .or. rot_src) then
+        write(6,*) 'GPU time loop: the sources of one record run must be on the axis and ', &
+                   'of one source type (', trim(type0), '); ', trim(srcfile), ' is ', &
+                   trim(src_type(1))
+        stop 2
+     endif
+     if (allocated(stf)) deallocate(stf)
+     if (allocated(source_term_el)) deallocate(source_term_el)
+     if (allocated(source_term_fl)) deallocate(source_term_fl)
+     ielsrc = 0
+     call compute_stf
+     call compute_src
+     call axh_check(axh_destroy(ctx), 'destroy')
+     ctx = c_null_ptr
+     call upload_all()
+     if (nrec_all /= nrec) then
+        write(6,*) 'GPU time loop: receivers changed between sources'
+        stop 2
+     endif
+     call axh_check(axh_save_state(ctx, trim(statefile)//c_null_char), 'save state')
+     if (lpr) write(6,*) '  wrote GPU state file ', trim(statefile)
+  enddo
+  close(u)
+
+end subroutine record_extra_sources
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------

@@ -107,15 +107,67 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
             f.write("SNAPSHOT_DT         %.3f\n" % cfg.snapshot_dt)
             f.write("SNAPSHOTS_FORMAT    %s\n" % cfg.snapshots)
             f.write("XDMF_GLL_I       '3 0 2 4'\nXDMF_GLL_J       '3 0 2 4'\n")
-    with open(os.path.join(rundir, "inparam_source"), "w") as f:
-        f.write("SOURCE_TYPE       %s\n" % cfg.source_type)
-        f.write("SOURCE_DEPTH      %.3f\n" % cfg.source_depth_km)
-        f.write("SOURCE_LAT         90.0\nSOURCE_LON          0.0\n")
-        f.write("SOURCE_AMPLITUDE   %.6E\n" % cfg.source_amplitude)
+    write_source(os.path.join(rundir, "inparam_source"), cfg)
     with open(os.path.join(rundir, "STATIONS"), "w") as f:
         for k, dist in enumerate(cfg.stations_deg):
             f.write("R%03d  AX  %9.4f  0.0000  0.0  0.0\n" % (k, 90.0 - dist))
     return rundir
+
+
+def write_source(path: str, cfg: RunConfig) -> None:
+    """inparam_source of a source on the axis (SOLVER/source.f90:40-140)."""
+    with open(path, "w") as f:
+        f.write("SOURCE_TYPE       %s\n" % cfg.source_type)
+        f.write("SOURCE_DEPTH      %.3f\n" % cfg.source_depth_km)
+        f.write("SOURCE_LAT         90.0\nSOURCE_LON          0.0\n")
+        f.write("SOURCE_AMPLITUDE   %.6E\n" % cfg.source_amplitude)
+
+
+# source type (src_type(1)) of each SOURCE_TYPE, SOLVER/source.f90:66-74
+SOURCE_KIND = {"mrr": 0, "explosion": 0, "mtt_p_mpp": 0, "mpp_p_mtt": 0, "vertforce": 0,
+               "mtr": 1, "mpr": 1, "mrt": 1, "mrp": 1, "thetaforce": 1, "phiforce": 1,
+               "mtp": 2, "mpt": 2, "mtt_m_mpp": 2}
+
+
+def record_states(host_exe: str, rundir: str, mesh_dir: str, cfgs: list, state_paths: list,
+                  timeout: float = 3600) -> list:
+    """States of several sources of one source type on one mesh from one run of
+    the drop-in host (AXH_EXTRA_SOURCES): the reference setup of the mesh and
+    model runs once, then the source routines once per source.  The configs
+    may differ only in the source (type of one kind, depth, amplitude)."""
+    import subprocess
+    import resource
+
+    if len(cfgs) != len(state_paths) or not cfgs:
+        raise ValueError("one state path per config")
+    kinds = {SOURCE_KIND[c.source_type.lower()] for c in cfgs}
+    if len(kinds) != 1:
+        raise ValueError("record_states: sources of one source type only")
+    base = {k: v for k, v in cfgs[0].__dict__.items()
+            if k not in ("source_type", "source_depth_km", "source_amplitude")}
+    for c in cfgs[1:]:
+        if {k: v for k, v in c.__dict__.items() if k in base} != base:
+            raise ValueError("record_states: configs differ beyond the source")
+    write_run_dir(rundir, mesh_dir, cfgs[0])
+    lines = []
+    for k, (c, sp) in enumerate(zip(cfgs[1:], state_paths[1:]), start=1):
+        src = os.path.join(rundir, "inparam_source.%d" % k)
+        write_source(src, c)
+        lines.append("%s %s\n" % (os.path.abspath(src), os.path.abspath(sp)))
+    lst = os.path.join(rundir, "extra_sources")
+    with open(lst, "w") as f:
+        f.writelines(lines)
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, AXH_SAVE_STATE=os.path.abspath(state_paths[0]), AXH_NO_RUN="1",
+               AXH_EXTRA_SOURCES=os.path.abspath(lst))
+    with open(os.path.join(rundir, "OUTPUT_record"), "w") as log:
+        subprocess.run([os.path.abspath(host_exe)], cwd=rundir, stdout=log,
+                       stderr=subprocess.STDOUT, check=True, timeout=timeout,
+                       preexec_fn=unlimited_stack, env=env)
+    return list(state_paths)
 
 
 def record_state(host_exe: str, rundir: str, state_path: str, timeout: float = 1800) -> str:
