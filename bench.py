@@ -307,6 +307,130 @@ def copy_calibration(nbytes: int, reps: int = 5) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
+# The plain CPU-runnable configuration on the GPU: prem_iso 50 s, elastic,
+# newmark2, vertical surface force, 1800 s at 61 receivers 3 degrees apart.
+P50_MESH = os.path.join(REPO, "data", "meshes", "prem_iso_50s")
+P50_WORKLOAD = inputs.RunConfig(
+    source_type="vertforce", source_depth_km=0.0, source_amplitude=1e20,
+    time_scheme="newmark2", attenuation=False, seismogram_length_s=1800.0,
+    stations_deg=[3.0 * k for k in range(61)])
+
+
+def p50_leg(workdir: str, steps: int, warmup: int) -> dict:
+    """Step time of the 50 s configuration (6 050 elements: a few tiles per
+    CU, launch-latency bound)."""
+    from axisem_amd.solver import Solver
+    path = prepare_state(workdir, P50_WORKLOAD, P50_MESH)
+    st = read_dims(path)
+    gll = 25 * (st["nel_solid"] + st["nel_fluid"])
+    with Solver.from_state_file(path) as s:
+        s.run(warmup)
+        t0 = time.perf_counter()
+        s.run(steps)
+        wall = time.perf_counter() - t0
+        dev = s.last_run_ms()
+    return {"workload": "prem_iso 50 s, NPOL 4, elastic, newmark2, monopole vertforce 0 km, "
+                        "1800 s, 61 receivers",
+            "nel": st["nel_solid"] + st["nel_fluid"], "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "device_ms_per_step": dev / steps,
+            "value": gll * steps / wall, "unit": "GLL-point updates/s"}
+
+
+# Batched wavefields on one mesh: prem_iso 5 s, elastic, newmark2 (the time
+# scheme is not stated for this configuration; newmark2 is the reference's
+# default), the four moment-tensor runs at 100 km, and the same four at 10,
+# 100, 300 and 600 km (16 wavefields); 1800 s at 181 receivers.
+MT_MESH = os.path.join(REPO, "data", "meshes", "prem_iso_5s")
+MT_DEPTHS_KM = (10.0, 100.0, 300.0, 600.0)
+
+
+def mt_config(run: str, depth_km: float) -> inputs.RunConfig:
+    return inputs.RunConfig(source_type=run, source_depth_km=depth_km, source_amplitude=1e20,
+                            time_scheme="newmark2", attenuation=False, seismogram_length_s=1800.0,
+                            stations_deg=[float(k) for k in range(181)])
+
+
+def prepare_mt_states(workdir: str) -> dict:
+    """States of the 16 moment-tensor wavefields, (run, depth) -> path: one
+    record run of the drop-in host per source type (monopole: Mrr and
+    Mtt+Mpp; dipole: Mtr; quadrupole: Mtp), the three in parallel."""
+    if not os.path.exists(os.path.join(MT_MESH, "meshdb.dat0000")):
+        raise SystemExit("missing %s: run __graft_entry__.build() where the reference "
+                         "tree exists" % MT_MESH)
+    kinds = {"monopole": ("mrr", "mtt_p_mpp"), "dipole": ("mtr",), "quadpole": ("mtp",)}
+    jobs = []
+    for kind, runs in kinds.items():
+        keys = [(r, d) for r in runs for d in MT_DEPTHS_KM]
+        cfgs = [mt_config(*k) for k in keys]
+        paths = [os.path.join(workdir, "%s_%g.bin" % k) for k in keys]
+        jobs.append((kind, keys, cfgs, paths))
+    os.makedirs(workdir, exist_ok=True)
+    stamp = os.path.join(workdir, "stamp.json")
+    key = json.dumps({"jobs": [repr(j[2]) for j in jobs],
+                      "meshsize": os.path.getsize(os.path.join(MT_MESH, "meshdb.dat0000")),
+                      "host": os.path.getmtime(HOST)})
+    out = {k: p for _, keys, _, paths in jobs for k, p in zip(keys, paths)}
+    if (os.path.exists(stamp) and open(stamp).read() == key
+            and all(os.path.exists(p) for p in out.values())):
+        return out
+    t = time.time()
+
+    def rec(job):
+        kind, _, cfgs, paths = job
+        inputs.record_states(HOST, os.path.join(workdir, "run_" + kind), MT_MESH, cfgs, paths)
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        list(ex.map(rec, jobs))
+    log("recorded the 16 moment-tensor states in %.1fs" % (time.time() - t))
+    with open(stamp, "w") as f:
+        f.write(key)
+    return out
+
+
+def mt_leg(workdir: str, steps: int, warmup: int) -> dict:
+    """Config of batched wavefields: each moment-tensor run alone, the four
+    at 100 km as one batch of mixed source types, and all 16 as one batch;
+    vs_plain = per-wavefield gain = sum of the members' plain step times over
+    the batch's step time."""
+    from axisem_amd.solver import Batch, Solver
+    paths = prepare_mt_states(workdir)
+    d = read_dims(paths[("mrr", 100.0)])
+    gll = 25 * (d["nel_solid"] + d["nel_fluid"])
+
+    def timed(runner):
+        runner.run(warmup)
+        t0 = time.perf_counter()
+        runner.run(steps)
+        return (time.perf_counter() - t0) / steps * 1e3, runner.last_run_ms() / steps
+
+    plain = {}
+    for run in MT_RUNS:
+        with Solver.from_state_file(paths[(run, 100.0)]) as s:
+            plain[run] = timed(s)[0]
+    log("moment tensor, plain ms per step: " + json.dumps(plain))
+    out = {"workload": "prem_iso 5 s, NPOL 4, elastic, newmark2, moment-tensor runs mrr, "
+                       "mtt_p_mpp, mtr, mtp at 100 km (x4: at 10/100/300/600 km), 1800 s, "
+                       "181 receivers",
+           "nel_solid": d["nel_solid"], "nel_fluid": d["nel_fluid"],
+           "niter_full_run": d["niter"], "deltat": d["deltat"], "steps": steps,
+           "plain_ms_per_step": plain}
+    for name, keys in (("batch4", [(r, 100.0) for r in MT_RUNS]),
+                       ("batch16", [(r, dd) for dd in MT_DEPTHS_KM for r in MT_RUNS])):
+        members = [Solver.from_state_file(paths[k]) for k in keys]
+        try:
+            with Batch(members) as b:
+                ms, dev = timed(b)
+                mem = sum(m.device_bytes() for m in members)
+        finally:
+            for m in members:
+                m.close()
+        ref = sum(plain[r] for r, _ in keys)
+        out[name] = {"wavefields": len(keys), "ms_per_step": ms, "device_ms_per_step": dev,
+                     "value": len(keys) * gll / (ms * 1e-3), "unit": "GLL-point updates/s",
+                     "vs_plain": ref / ms}
+        log("moment tensor %s: %s" % (name, json.dumps(out[name])))
+    return out
+
+
 # Other source types and time schemes on the 10 s mesh (not the headline
 # workload): plain and 4-wavefield step times, stiffness launch per kernel.
 # name, source, scheme, attenuation, coarse-grained memory variables
@@ -375,6 +499,10 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=200, help="steps per batch chunk")
     ap.add_argument("--group-width", type=int, default=None,
                     help="wavefields per kernel instance in the batched leg (default: library)")
+    ap.add_argument("--no-p50", action="store_true", help="no 50 s configuration leg")
+    ap.add_argument("--p50-steps", type=int, default=1000, help="timed steps of the 50 s leg")
+    ap.add_argument("--no-mt", action="store_true",
+                    help="no moment-tensor batch leg (5 s mesh, 4 and 16 wavefields)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1),
                     help="independent reference runs of the all-cores CPU leg")
@@ -481,6 +609,17 @@ def main():
         }
         log("batched kernels: " + json.dumps(batched["kernels_us"]))
 
+    p50 = None
+    if not a.no_p50:
+        log("50 s configuration: %d steps" % a.p50_steps)
+        p50 = p50_leg(os.path.join(REPO, "data", "bench_50s"), a.p50_steps, 50)
+        log("50 s: " + json.dumps(p50))
+
+    mt = None
+    if not a.no_mt:
+        log("moment-tensor batches on the 5 s mesh")
+        mt = mt_leg(os.path.join(REPO, "data", "bench_5s"), max(a.steps, 20), max(a.warmup, 5))
+
     serving = None
     if a.serve_solves > 0:
         log("serving: %d solves, %.2f arrivals/s" % (a.serve_solves, a.serve_rate))
@@ -532,6 +671,8 @@ def main():
         "latency": latency,
         "kernels_us": kernels,
         "batched": batched,
+        "plain_50s": p50,
+        "moment_tensor_5s": mt,
         "serving": serving,
         "cpu_baseline": cpu,
     }
