@@ -13,6 +13,8 @@
   ``axh_ref_time_evol_wave`` / ``axh_ref_attenuation`` with every entity
   public (a rename of the module and of its default accessibility, written
   into the build directory only).
+* ``build_post(src)`` -- the reference's seismogram post-processing program
+  with its convolution replaced by the GPU's (``fortran/_build/xpost_gpu``).
 """
 from __future__ import annotations
 
@@ -44,11 +46,25 @@ def build_library(out: str = LIB) -> str:
     srcs = [os.path.join(PKG, "csrc", "axh_host.cpp"),
             os.path.join(PKG, "csrc", "axh_meshdb.cpp"),
             os.path.join(PKG, "csrc", "axh_device.hip")]
+    inc = ["-I", os.path.join(REPO, "include")]
+
+    def run(cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    # the post-processing kernels follow a program without the solver's
+    # flush-to-zero (SOLVER/ftz.c is linked into the SOLVER only): their
+    # object keeps FP32 denormals
+    post_obj = out + ".post.o"
+    post_flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fgpu-flush-denormals-to-zero")]
+    run([HIPCC, *post_flags, *inc, "-c", os.path.join(PKG, "csrc", "axh_post.hip"),
+         "-o", post_obj])
     tmp = out + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO, "include"), *srcs, "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    try:
+        run([HIPCC, *HIP_FLAGS, *inc, *srcs, "-x", "none", post_obj, "-o", tmp])
+    finally:
+        if os.path.exists(post_obj):
+            os.remove(post_obj)
     os.replace(tmp, out)
     return out
 
@@ -78,6 +94,52 @@ def public_module_copy(src: str, new_name: str, out_dir: str) -> str:
     with open(out, "w") as f:
         f.write(head + sep + rest)
     return out
+
+
+POST_EXE = os.path.join(HOST_DIR, "xpost_gpu")
+
+
+def routine_renamed(src: str, name: str, new_name: str, out_dir: str, out_name: str) -> str:
+    """The program in src with its subroutine `name` renamed `new_name`
+    (definition and end statement; the calls keep `name` and so reach a
+    replacement linked beside it), written to out_dir (the build directory);
+    the statements are otherwise the reference's, unchanged."""
+    txt = open(src).read()
+    txt, n = re.subn(r"^([ \t]*subroutine[ \t]+)%s\b" % name, r"\g<1>%s" % new_name, txt,
+                     flags=re.I | re.M)
+    txt, m = re.subn(r"^([ \t]*end[ \t]+subroutine[ \t]+)%s\b" % name, r"\g<1>%s" % new_name,
+                     txt, flags=re.I | re.M)
+    if n != 1 or m != 1:
+        raise RuntimeError("unexpected layout of " + src)
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, out_name)
+    with open(out, "w") as f:
+        f.write(txt)
+    return out
+
+
+def build_post(ref_root: str = "/root/reference", jobs: int = 8) -> str:
+    """The reference's seismogram post-processing program
+    (SOLVER/UTILS/post_processing.F90) with its convolve_with_stf replaced by
+    the GPU convolution (fortran/convolve_gpu.f90 -> axh_convolve_traces):
+    axisem_amd/fortran/_build/xpost_gpu."""
+    from .fortran import fortran_build as fb
+
+    utils = os.path.join(ref_root, "SOLVER", "UTILS")
+    if not os.path.isdir(utils):
+        raise FileNotFoundError(utils)
+    if not os.path.exists(LIB):
+        build_library()
+    gen = os.path.join(HOST_DIR, "src")
+    fdir = os.path.join(PKG, "fortran")
+    srcs = [os.path.join(utils, "nc_postroutines.F90"),
+            routine_renamed(os.path.join(utils, "post_processing.F90"), "convolve_with_stf",
+                            "ref_convolve_with_stf", gen, "post_processing_gpu.F90"),
+            os.path.join(fdir, "axh_iface.f90"), os.path.join(fdir, "convolve_gpu.f90")]
+    obj = os.path.join(HOST_DIR, "obj_post")
+    objs = fb.compile_all(srcs, obj, fflags=["-O2", "-cpp"], jobs=jobs)
+    fb.link(objs, POST_EXE, ldflags=["-L" + PKG, "-laxihip", "-Wl,-rpath,$ORIGIN/../.."])
+    return POST_EXE
 
 
 def build_host(ref_root: str = "/root/reference", jobs: int = 8) -> str:

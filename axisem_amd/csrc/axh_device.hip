@@ -236,6 +236,24 @@ __device__ __forceinline__ void stv_nt(float *p, const float (&o)[B]) {
 // kernel argument blocks: mesh part, the group's interleaved fields, and the
 // per-wavefield source / receiver data of its members
 // ---------------------------------------------------------------------------
+// The time step a launch belongs to, read on the device: wavefield b runs
+// step base[b] + koff.  The host sets base once per run (k_step_set) and the
+// launches of a step differ only in koff, so a captured graph of steps
+// replays with base advanced on the device (k_step_advance).
+struct StepRef {
+  const int *base;  // [B]
+  int koff;
+};
+__device__ __forceinline__ int step_of(const StepRef &r, int b) { return r.base[b] + r.koff; }
+
+// Source time function of wavefield b at the step of r and stage st (table
+// [niter][nstages], NULL: no source term in this kernel), SOLVER/time_evol_wave.F90:466,
+// 1082-1097.
+__device__ __forceinline__ float stf_at(const float *tab, const StepRef &r, int b, int nst,
+                                        int st) {
+  return tab ? tab[(size_t)(step_of(r, b) - 1) * nst + st] : 0.0f;
+}
+
 struct SolidArgs {
   int nel, nsls, nglob, nslot, ntile;
   const float *G, *coef, *m0;
@@ -273,7 +291,9 @@ struct FluidArgs {
   float *force;           // [nslot_f][B]
   const int *srcel[kGroup];
   const float *src_fl[kGroup];
-  float stf[kGroup];      // 0 when the wavefield has no fluid source this stage
+  const float *stf[kGroup];  // NULL when the wavefield has no fluid source
+  StepRef step;
+  int nstages, stage;
 };
 
 // ---------------------------------------------------------------------------
@@ -598,7 +618,7 @@ __device__ __forceinline__ void fluid_tile(const FluidArgs &A, int tile, float *
       if (axial) loc = loc + G.g0i * (w * V1[b]);
     }
     if (valid) {
-      const float stf = A.stf[b];
+      const float stf = stf_at(A.stf[b], A.step, b, A.nstages, A.stage);
       if (stf != 0.0f) {
         const int ks = A.srcel[b][e];
         if (ks >= 0) loc = loc - A.src_fl[b][ks * 25 + k] * stf;
@@ -1175,23 +1195,28 @@ void k_solid(SolidArgs A, FluidArgs FA) {
 }
 
 // receiver samples of compute_recfile_seis_bare (SOLVER/seismograms.f90:742),
-// blockIdx.y = wavefield of the group; members without a sample this step
-// have out NULL
+// blockIdx.y = wavefield of the group: at the steps with mod(iter, seis_it)
+// == 0, sample q = iter / seis_it goes to slot mod(q - 1, cap) of the
+// wavefield's staging ring
 struct RecArgs {
   int src, nglob;
   const float *disp;     // [3][nglob][B]
-  int nrec[kGroup];
+  int nrec[kGroup], seis_it[kGroup], cap[kGroup];
   const int *rec_g[kGroup];
-  float *out[kGroup];
+  float *ring[kGroup];   // [cap][nrec][3]
+  StepRef step;
 };
 
 template <int B>
 __device__ __forceinline__ void rec_sample(const RecArgs &A, int r, int b) {
-  if (!A.out[b] || r >= A.nrec[b]) return;
+  if (r >= A.nrec[b]) return;
+  const int iter = step_of(A.step, b);
+  if (iter % A.seis_it[b] != 0) return;
+  const int slot = (iter / A.seis_it[b] - 1) % A.cap[b];
   const int g = A.rec_g[b][r];
   const float d1 = A.disp[(size_t)g * B + b], d2 = A.disp[((size_t)A.nglob + g) * B + b],
               d3 = A.disp[(2 * (size_t)A.nglob + g) * B + b];
-  float *o = A.out[b] + 3 * (size_t)r;
+  float *o = A.ring[b] + ((size_t)slot * A.nrec[b] + r) * 3;
   if (A.src == AXH_MONOPOLE) { o[0] = d1; o[1] = d3; o[2] = 0.0f; }
   else if (A.src == AXH_DIPOLE) { o[0] = d1 + d2; o[1] = d1 - d2; o[2] = d3; }
   else { o[0] = d1; o[1] = d2; o[2] = d3; }
@@ -1682,7 +1707,7 @@ struct PredArgs {
   float *axb_val;        // unmasked prediction of the axb points [naxb][3][B]
   int *blow[kGroup];
   double thr[kGroup];
-  int iter[kGroup];      // each wavefield's current step
+  StepRef step;
 };
 
 template <int B>
@@ -1697,7 +1722,7 @@ __device__ __forceinline__ void predict_point(const PredArgs &A, int g) {
     if (A.check_blow && (fl & kFlagBlow)) {
 #pragma unroll
       for (int b = 0; b < B; b++)
-        if ((double)fabsf(u[b]) > A.thr[b]) atomicMin(A.blow[b], A.iter[b]);
+        if ((double)fabsf(u[b]) > A.thr[b]) atomicMin(A.blow[b], step_of(A.step, b));
     }
     if (A.newmark) {
       float a0[B];
@@ -1752,7 +1777,9 @@ struct AsmSolidArgs {
   int nsrc[kGroup];
   const int *src_g[kGroup];
   const float *src_val[kGroup];
-  float stf[kGroup];
+  const float *stf[kGroup];  // NULL when the source is not added here
+  StepRef step;
+  int nstages, stage;
   // symplectic: the drift of the next stage (or the final drift) right after
   // the kick, u = u + v*cd_next, masked when drift_mask is set
   int drift, drift_mask, naxb;
@@ -1847,10 +1874,13 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
           for (int b = 0; b < B; b++) s[c][b] = s[c][b] + x[w][c][b];
   }
   int si[B];
+  float stf[B];
 #pragma unroll
   for (int b = 0; b < B; b++) {
     si[b] = -1;
-    if ((fl & kFlagSource) && A.stf[b] != 0.0f) {
+    stf[b] = 0.0f;
+    if (fl & kFlagSource) stf[b] = stf_at(A.stf[b], A.step, b, A.nstages, A.stage);
+    if (stf[b] != 0.0f) {
       const int *sg = A.src_g[b];
       for (int k = 0; k < A.nsrc[b]; k++)
         if (sg[k] == g) { si[b] = k; break; }
@@ -1861,7 +1891,7 @@ __global__ __launch_bounds__(256) void k_asm_solid(AsmSolidArgs A) {
     if (!(A.comps & (1 << c))) continue;
 #pragma unroll
     for (int b = 0; b < B; b++)
-      if (si[b] >= 0) s[c][b] = s[c][b] - A.src_val[b][3 * si[b] + c] * A.stf[b];
+      if (si[b] >= 0) s[c][b] = s[c][b] - A.src_val[b][3 * si[b] + c] * stf[b];
     const size_t p = ((size_t)c * A.n + g) * B;
     if (A.newmark) {
       float a[B];
@@ -2028,6 +2058,27 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
 template <int B>
 __global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
   rec_sample<B>(A, blockIdx.x * 256 + threadIdx.x, blockIdx.y);
+}
+
+// Start of a run: each wavefield's first step and its blow-up flag cleared
+// (a kernel, not a memset: also safe where a caller captures the stream).
+struct StepSet {
+  int *base;
+  int first[kGroup];
+  int *blow[kGroup];
+};
+__global__ void k_step_set(StepSet A, int B) {
+  const int b = threadIdx.x;
+  if (b < B) {
+    A.base[b] = A.first[b];
+    *A.blow[b] = INT_MAX;
+  }
+}
+
+// End of a replayed graph of n steps: every wavefield n steps on.
+__global__ void k_step_advance(int *base, int B, int n) {
+  const int b = threadIdx.x;
+  if (b < B) base[b] += n;
 }
 
 // moving one wavefield's state between a group's interleaved arrays and its
@@ -2732,7 +2783,7 @@ struct Member {
   int seis_cap = 0;
   int *blow = nullptr;
   double blow_thr = 0.0;
-  std::vector<float> stf;     // [niter][nstages]
+  float *stf_d = nullptr;     // source time function [niter][nstages]
 };
 
 // State of B wavefields, interleaved (element x of wavefield b at x*B + b),
@@ -2750,6 +2801,7 @@ struct Fields {
   uint8_t *flags_s = nullptr;
   int nasm = 0;
   int *asm_pts = nullptr;
+  int *step_base = nullptr;   // [kGroup] step of each wavefield (StepRef)
   // solid tiles holding a source point of the group (at most kMaxSrcTiles)
   std::vector<int> src_tiles;
 };
@@ -2828,6 +2880,7 @@ static void fields_alloc(const MeshDev &M, Fields &F, int B) {
   DevAlloc &m = F.mem;
   F.flags_s = m.alloc<uint8_t>(M.nglob_s);
   F.asm_pts = m.alloc<int>(M.nglob_s);
+  F.step_base = m.zeros<int>(kGroup);
   const size_t b = B;
   F.disp = m.zeros<float>(3 * (size_t)M.nglob_s * b);
   F.velo = m.zeros<float>(3 * (size_t)M.nglob_s * b);
@@ -2908,7 +2961,7 @@ static void member_build(const axh_ctx *ctx, const MeshDev &M, Member &W) {
   // blow-up threshold of runtime_info: maxval(abs(disp(1,1,:,:))) >
   // 10*abs(magnitude), the real(4) maximum promoted to double
   W.blow_thr = 10.0 * std::fabs(d.magnitude);
-  W.stf = std::move(H.stf);
+  W.stf_d = m.upload(H.stf);
   W.src_g_h = std::move(H.src_g);
 }
 
@@ -2921,6 +2974,12 @@ struct Runner {
   std::vector<int> kev_class;
   double kms[KC_N] = {0};
   long kcount[KC_N] = {0};
+  // replayed graph of graph_steps steps of one group (0: launch directly)
+  int graph_steps = 0;
+  hipGraphExec_t gexec = nullptr;
+  const void *gfields = nullptr;
+  const int *gbase = nullptr;
+  int gsteps = 0;
   void init() {
     if (stream) return;
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -2930,6 +2989,7 @@ struct Runner {
   ~Runner() {
     // teardown: errors are not actionable here
     if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (auto e : kev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -2953,7 +3013,8 @@ struct BatchState {
   std::vector<std::unique_ptr<Fields>> groups;
   std::vector<std::vector<int>> index;          // members of each group
   std::vector<std::shared_ptr<MeshDev>> mesh;   // each group's mesh part
-  Runner run;
+  Runner run;                                   // attach / detach copies
+  std::vector<std::unique_ptr<Runner>> runs;    // each group's stream
 };
 
 void device_build(axh_ctx *ctx) {
@@ -2989,7 +3050,16 @@ bool kernel_stats(const Device *d, int k, double *ms, int64_t *n) {
 }
 
 bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n) {
-  return runner_stats(b && b->state ? &b->state->run : nullptr, k, ms, n);
+  if (!runner_stats(nullptr, k, ms, n)) return false;
+  if (!b || !b->state) return true;
+  for (const auto &r : b->state->runs) {
+    double m = 0.0;
+    int64_t c = 0;
+    runner_stats(r.get(), k, &m, &c);
+    *ms += m;
+    *n += c;
+  }
+  return true;
 }
 
 bool in_running_batch(const axh_ctx *ctx) { return ctx->dev && ctx->dev->live; }
@@ -3049,6 +3119,8 @@ void batch_attach(axh_batch *bt) {
       S->index.push_back(idx);
       S->groups.push_back(std::move(F));
       S->mesh.push_back(mesh);
+      S->runs.emplace_back(new Runner);
+      S->runs.back()->init();
     }
   }
   HIPCHECK(hipStreamSynchronize(S->run.stream));
@@ -3155,15 +3227,85 @@ static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A,
     hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), grid, dim3(256), lds, s, A, R);
 }
 
-// Advances each of the B wavefields of F (member data mb[b], dims of member
-// 0) by nsteps, wavefield b through its steps start[b] .. start[b]+nsteps-1,
-// on the stream of R; seis[b] (may be NULL) receives wavefield b's samples
-// and blow[b] its blow-up step (INT_MAX: none).  Returns the device time of
-// the run [ms].
+// Device buffers a run writes besides the fields: the step base of the
+// group, each wavefield's blow-up flag and its receiver-sample ring of cap
+// samples.  A context's own (run_buffers) or a fixed scratch area.
+struct RunBuffers {
+  int *step_base = nullptr;
+  int *blow[kGroup] = {};
+  float *ring[kGroup] = {};
+  int cap[kGroup] = {};
+};
+
+static RunBuffers run_buffers(Fields &F, Member *const *mb, int B) {
+  RunBuffers r;
+  r.step_base = F.step_base;
+  for (int b = 0; b < B; b++) {
+    r.blow[b] = mb[b]->blow;
+    r.ring[b] = mb[b]->seis_d;
+    r.cap[b] = std::max(mb[b]->seis_cap, 1);
+  }
+  return r;
+}
+
+// The receiver samples of a group's run, per wavefield: sample q (the step
+// q * seis_it) is staged in slot mod(q - 1, cap) of the wavefield's ring;
+// q0 is the last sample before the run, qdone the last one whose step is
+// queued, qout the last one queued for copying out.
+struct GroupTail {
+  int B = 0;
+  Member *mb[kGroup] = {};
+  float *seis[kGroup] = {};
+  RunBuffers rb;
+  int q0[kGroup] = {}, qdone[kGroup] = {}, qout[kGroup] = {};
+  // queue the copy of samples qout+1 .. qdone into seis (at most two pieces)
+  void copy_out(int b, hipStream_t s) {
+    const Member &w = *mb[b];
+    const int n = qdone[b] - qout[b];
+    if (n <= 0) return;
+    if (seis[b]) {
+      const size_t row = 3 * (size_t)w.nrec;
+      const int cap = rb.cap[b];
+      const int slot = qout[b] % cap;
+      const int first = std::min(n, cap - slot);
+      float *dst = seis[b] + (size_t)(qout[b] - q0[b]) * row;
+      HIPCHECK(hipMemcpyAsync(dst, rb.ring[b] + slot * row, first * row * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+      if (n > first)
+        HIPCHECK(hipMemcpyAsync(dst + first * row, rb.ring[b], (n - first) * row * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+    }
+    qout[b] = qdone[b];
+  }
+  // steps it+1 .. it+n of wavefield b are about to be queued: their samples
+  // must fit the ring beside those not yet copied out (stream order keeps a
+  // copy ahead of the kernels that refill its slots)
+  void reserve(int b, int it, int n, hipStream_t s) {
+    const Member &w = *mb[b];
+    if (w.nrec <= 0) return;
+    const int q1 = (it + n) / std::max(w.seis_it, 1);
+    if (q1 - qout[b] > rb.cap[b]) copy_out(b, s);
+    qdone[b] = q1;
+  }
+};
+
+// Queues on the stream of R the advance of each of the B wavefields of F
+// (member data mb[b], dims of member 0) by nsteps, wavefield b through its
+// steps start[b] .. start[b]+nsteps-1; seis[b] (may be NULL) is where
+// wavefield b's samples go.  The host waits only when a wavefield's sample
+// ring fills up inside the run; group_finish completes the run.  Groups on
+// different streams (a batch's source types) run concurrently.
+//
+// Every launch reads its step from the group's device step base (StepRef),
+// so the launches of one step are the same for all steps.  With
+// R.graph_steps = S > 0 (and no per-kernel profiling) the first S steps are
+// captured once into a graph of the group (ending with the advance of the
+// step base by S) that is replayed for every S steps of the run; the steps
+// left over are launched directly.  Both give the same results bit for bit.
 template <int B>
-static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
-                        const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
-                        float *const *seis, int *blow) {
+static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
+                      const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
+                      float *const *seis, const RunBuffers &RB, GroupTail &T) {
   if (F.B != B) fail(AXH_ERR_STATE, "group width mismatch");
   const axh_dims &d = ctx[0]->dims;
   const bool multipole = M.src != AXH_MONOPOLE;
@@ -3173,8 +3315,14 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   Launcher L{R};
   hipStream_t st_ = R.stream;
 
-  for (int b = 0; b < B; b++)
-    HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)mb[b]->blow, INT_MAX, 1, st_));
+  // the run's first steps and cleared blow-up flags
+  StepSet SS{};
+  SS.base = RB.step_base;
+  for (int b = 0; b < B; b++) {
+    SS.first[b] = start[b];
+    SS.blow[b] = RB.blow[b];
+  }
+  hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, st_, SS, B);
 
   SolidArgs SA{};
   SA.nel = M.nel_s; SA.nsls = M.nsls; SA.nglob = M.nglob_s; SA.nslot = (int)nslot_s;
@@ -3249,26 +3397,35 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   for (int b = 0; b < B; b++) {
     Member &w = *mb[b];
     FA.srcel[b] = w.srcel_f; FA.src_fl[b] = w.src_fl;
-    PS.blow[b] = w.blow; PS.thr[b] = w.blow_thr;
-    PF.blow[b] = w.blow; PF.thr[b] = w.blow_thr;
+    PS.blow[b] = RB.blow[b]; PS.thr[b] = w.blow_thr;
+    PF.blow[b] = RB.blow[b]; PF.thr[b] = w.blow_thr;
     AS.nsrc[b] = w.nsrc; AS.src_g[b] = w.src_g; AS.src_val[b] = w.src_val;
-    RA.nrec[b] = w.nrec; RA.rec_g[b] = w.rec_g;
+    RA.nrec[b] = w.nrec; RA.rec_g[b] = w.rec_g; RA.ring[b] = RB.ring[b];
+    RA.seis_it[b] = std::max(w.seis_it, 1); RA.cap[b] = RB.cap[b];
     maxrec = std::max(maxrec, w.nrec);
   }
 
   HIPCHECK(hipEventRecord(R.ev0, st_));
-  // samples staged on the device (isamp) and already copied out (nout)
-  int isamp[kGroup] = {0, 0, 0, 0}, nout[kGroup] = {0, 0, 0, 0};
-  auto copy_out = [&](int b) {
-    const Member &w = *mb[b];
-    if (seis && seis[b] && isamp[b] > 0)
-      HIPCHECK(hipMemcpyAsync(seis[b] + (size_t)nout[b] * 3 * w.nrec, w.seis_d,
-                              (size_t)isamp[b] * 3 * w.nrec * sizeof(float),
-                              hipMemcpyDeviceToHost, st_));
-    nout[b] += isamp[b];
-    isamp[b] = 0;
-  };
-  for (int step = 0; step < nsteps; step++) {
+  T.B = B;
+  T.rb = RB;
+  for (int b = 0; b < B; b++) {
+    T.mb[b] = mb[b];
+    T.seis[b] = seis ? seis[b] : nullptr;
+    const int si = std::max(mb[b]->seis_it, 1);
+    T.q0[b] = T.qdone[b] = T.qout[b] = (start[b] - 1) / si;
+  }
+  const StepRef sref{RB.step_base, 0};
+  PS.step = PF.step = FA.step = AS.step = RA.step = sref;
+  FA.nstages = AS.nstages = M.nstages;
+  for (int b = 0; b < B; b++) {
+    FA.stf[b] = (newmark && mb[b]->fluid_src) ? mb[b]->stf_d : nullptr;
+    AS.stf[b] = (newmark && mb[b]->fluid_src) ? nullptr : mb[b]->stf_d;
+  }
+  // the launches of the step koff steps past the step base; with sample_all
+  // the receiver kernels run every step (a graph's steps are all alike), else
+  // only where a wavefield takes a sample (it: the step on the host)
+  auto one_step = [&](int koff, bool sample_all, int step) {
+    PS.step.koff = PF.step.koff = FA.step.koff = AS.step.koff = RA.step.koff = koff;
     for (int st = 0; st < M.nstages; st++) {
       // predictor / first drift of the step; the symplectic drifts of the
       // later stages and the final drift are fused into the kicks below
@@ -3279,7 +3436,6 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         PF.apply_mask = multipole;
         PS.check_blow = 1;
         PF.check_blow = 0;
-        for (int b = 0; b < B; b++) PS.iter[b] = PF.iter[b] = start[b] + step;
         L(KC_PRED, [&] {
           const int nbs = nblk(PS.n), nbf = M.nel_f > 0 ? nblk(PF.n) : 0;
           hipLaunchKernelGGL(k_predict<B>, dim3(nbs + nbf), dim3(256), 0, st_, PS, PF, nbs);
@@ -3292,11 +3448,7 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       AS.cd_next = AF.cd_next = newmark ? 0.0 : M.coefd[st + 1];
       AS.drift_mask = !last;
       AF.drift_mask = !last && multipole;
-      for (int b = 0; b < B; b++) {
-        const float stf = mb[b]->stf[(size_t)(start[b] + step - 1) * M.nstages + st];
-        FA.stf[b] = (newmark && mb[b]->fluid_src) ? stf : 0.0f;
-        AS.stf[b] = (newmark && mb[b]->fluid_src) ? 0.0f : stf;
-      }
+      FA.stage = AS.stage = st;
       SA.direct = 1;
       SA.cv = newmark ? 0.0 : M.coefv[st];
       SA.cd_next = newmark ? 0.0 : M.coefd[st + 1];
@@ -3320,19 +3472,9 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
         hipLaunchKernelGGL(k_asm_solid<B>, dim3(nblk(AS.npts)), dim3(256), 0, st_, AS);
       });
     }
-    bool any = false;
-    for (int b = 0; b < B; b++) {
-      Member &w = *mb[b];
-      RA.out[b] = nullptr;
-      if (w.nrec > 0 && (start[b] + step) % w.seis_it == 0) {
-        // staging buffer full: its samples go out first (stream order keeps
-        // the copy ahead of the kernels that refill it)
-        if (isamp[b] == w.seis_cap) copy_out(b);
-        RA.out[b] = w.seis_d + (size_t)isamp[b] * 3 * w.nrec;
-        isamp[b]++;
-        any = true;
-      }
-    }
+    bool any = sample_all && maxrec > 0;
+    for (int b = 0; b < B && !any; b++)
+      any = mb[b]->nrec > 0 && (start[b] + step) % mb[b]->seis_it == 0;
     if (!newmark && M.anel_full)
       L(KC_MEMVAR, [&] { dispatch_anel_full<false, true, B>(M, st_, AN); });
     if (!newmark && M.anel) {
@@ -3345,14 +3487,59 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
       L(KC_REC, [&] {
         hipLaunchKernelGGL(k_receivers<B>, dim3(nblk(maxrec), B), dim3(256), 0, st_, RA);
       });
+  };
+  auto reserve = [&](int step, int n) {
+    for (int b = 0; b < B; b++) T.reserve(b, start[b] + step - 1, n, st_);
+  };
+
+  int step = 0;
+  const int S = R.graph_steps;
+  bool fits = S > 0 && !R.profile && nsteps >= S;
+  for (int b = 0; b < B && fits; b++)
+    fits = mb[b]->nrec <= 0 || S / std::max(mb[b]->seis_it, 1) + 1 <= RB.cap[b];
+  if (fits) {
+    if (!R.gexec || R.gfields != &F || R.gsteps != S || R.gbase != RB.step_base) {
+      if (R.gexec) HIPCHECK(hipGraphExecDestroy(R.gexec));
+      R.gexec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < S; k++) one_step(k, true, k);
+      hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st_, RB.step_base, B, S);
+      HIPCHECK(hipStreamEndCapture(st_, &g));
+      const hipError_t rc = hipGraphInstantiate(&R.gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (rc != hipSuccess) {
+        R.gexec = nullptr;
+        fail(AXH_ERR_HIP, std::string("graph of the step: ") + hipGetErrorString(rc));
+      }
+      R.gfields = &F;
+      R.gsteps = S;
+      R.gbase = RB.step_base;
+    }
+    for (; step + S <= nsteps; step += S) {
+      reserve(step, S);
+      HIPCHECK(hipGraphLaunch(R.gexec, st_));
+    }
+  }
+  // the rest, launched directly (koff counts from the current step base)
+  for (int k = 0; step < nsteps; step++, k++) {
+    reserve(step, 1);
+    one_step(k, false, step);
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(R.ev1, st_));
-  for (int b = 0; b < B; b++) {
-    HIPCHECK(hipMemcpyAsync(&blow[b], mb[b]->blow, sizeof(int), hipMemcpyDeviceToHost, st_));
-    copy_out(b);
+}
+
+// Completes a group's run: copies out the staged samples, waits for the
+// stream and collects the per-kernel times; blow[b] receives wavefield b's
+// blow-up step (INT_MAX: none).  Returns the device time of the run [ms].
+static double group_finish(Runner &R, GroupTail &T, int *blow) {
+  for (int b = 0; b < T.B; b++) {
+    HIPCHECK(hipMemcpyAsync(&blow[b], T.rb.blow[b], sizeof(int), hipMemcpyDeviceToHost,
+                            R.stream));
+    T.copy_out(b, R.stream);
   }
-  HIPCHECK(hipStreamSynchronize(st_));
+  HIPCHECK(hipStreamSynchronize(R.stream));
   float ms = 0.0f;
   HIPCHECK(hipEventElapsedTime(&ms, R.ev0, R.ev1));
   if (R.profile) {
@@ -3369,14 +3556,14 @@ static double run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *m
   return ms;
 }
 
-static double run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
-                      const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
-                      float *const *seis, int *blow) {
+static void run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
+                    const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
+                    float *const *seis, const RunBuffers &RB, GroupTail &T) {
   switch (F.B) {
-    case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, blow);
-    case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, blow);
-    case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, blow);
-    case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, blow);
+    case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
+    case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
+    case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
+    case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
     default: fail(AXH_ERR_STATE, "group width out of range");
   }
 }
@@ -3386,30 +3573,38 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
   V.run.init();
   V.run.profile = ctx->profile;
+  V.run.graph_steps = ctx->graph_steps;
   Member *mb[1] = {&V.member};
   const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
   int blow = INT_MAX;
   const int32_t start[1] = {it0};
-  ctx->last_run_ms = run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, &blow);
+  GroupTail T;
+  run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out,
+          run_buffers(V.own, mb, 1), T);
+  ctx->last_run_ms = group_finish(V.run, T, &blow);
   ctx->iter_done = it1;
   if (blow != INT_MAX)
     fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
 }
 
+// The groups of a batch are queued one after the other, each on its own
+// stream (groups of different source types, or more groups than one width
+// holds, run concurrently), then completed in order.
 void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
   BatchState &S = *bt->state;
-  S.run.profile = bt->profile;
-  double ms = 0.0;
   std::vector<int> blow(bt->members.size(), INT_MAX);
+  std::vector<GroupTail> tails(S.groups.size());
   for (size_t g = 0; g < S.groups.size(); g++) {
+    Runner &R = *S.runs[g];
+    R.profile = bt->profile;
+    R.graph_steps = bt->graph_steps;
     const std::vector<int> &idx = S.index[g];
     const MeshDev &M = *S.mesh[g];
     Member *mb[kGroup];
     const axh_ctx *cx[kGroup];
     float *out[kGroup];
     int32_t start[kGroup];
-    int bl[kGroup];
     for (size_t b = 0; b < idx.size(); b++) {
       axh_ctx *c = bt->members[idx[b]];
       if (c->dev->mesh.get() != &M) fail(AXH_ERR_STATE, "batch members on different meshes");
@@ -3418,13 +3613,27 @@ void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
       out[b] = seis ? seis[idx[b]] : nullptr;
       start[b] = c->iter_done + 1;
     }
-    ms += run_any(S.run, M, *S.groups[g], mb, cx, start, nsteps, out, bl);
-    for (size_t b = 0; b < idx.size(); b++) blow[idx[b]] = bl[b];
+    run_any(R, M, *S.groups[g], mb, cx, start, nsteps, out,
+            run_buffers(*S.groups[g], mb, (int)idx.size()), tails[g]);
   }
-  bt->last_run_ms = ms;
+  // device time of the batch: first start to last end over the streams
+  // (times relative to group 0's start)
+  double first = 0.0, last = 0.0;
+  for (size_t g = 0; g < S.groups.size(); g++) {
+    int bl[kGroup];
+    group_finish(*S.runs[g], tails[g], bl);
+    const std::vector<int> &idx = S.index[g];
+    for (size_t b = 0; b < idx.size(); b++) blow[idx[b]] = bl[b];
+    float t0 = 0.0f, t1 = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&t0, S.runs[0]->ev0, S.runs[g]->ev0));
+    HIPCHECK(hipEventElapsedTime(&t1, S.runs[0]->ev0, S.runs[g]->ev1));
+    first = std::min(first, (double)t0);
+    last = std::max(last, (double)t1);
+  }
+  bt->last_run_ms = last - first;
   for (axh_ctx *c : bt->members) {
     c->iter_done += nsteps;
-    c->last_run_ms = ms;
+    c->last_run_ms = bt->last_run_ms;
   }
   for (size_t b = 0; b < blow.size(); b++)
     if (blow[b] != INT_MAX)
@@ -3556,6 +3765,7 @@ void footprint(axh_ctx *ctx, size_t *mesh_bytes, size_t *wave_bytes) {
   if (nel_f) w += 3 * gf * 4 + nf * 4 + nel_f * 4 + 200 * 4;
   if (d.anel) w += nel_s * (24 * d.n_sls + 24 + 4 + 24) * 4;
   w += (size_t)std::max(d.nelsrc, 0) * 25 * 16 + (size_t)d.num_rec * 4 + 4;
+  if (const Array *a = ctx->get("stf")) w += a->count() * 4;   // device stf table
   if (d.num_rec > 0) w += (size_t)seis_staging_samples(d) * 3 * d.num_rec * 4;
   *mesh_bytes = m;
   *wave_bytes = w;

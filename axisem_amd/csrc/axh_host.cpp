@@ -419,6 +419,15 @@ int axh_set_profiling(axh_ctx *ctx, int on) {
   });
 }
 
+int axh_set_graph_steps(axh_ctx *ctx, int32_t steps) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (steps < 0 || steps > AXH_MAX_GRAPH_STEPS)
+      fail(AXH_ERR_ARG, "graph steps must be 0.." + std::to_string(AXH_MAX_GRAPH_STEPS));
+    ctx->graph_steps = steps;
+  });
+}
+
 const char *axh_kernel_name(int kclass) { return kernel_class_name(kclass); }
 
 int axh_kernel_stats(const axh_ctx *ctx, int kclass, double *total_ms,
@@ -570,6 +579,15 @@ int axh_batch_set_group_width(axh_batch *batch, int32_t width) {
   });
 }
 
+int axh_batch_set_graph_steps(axh_batch *batch, int32_t steps) {
+  return guarded([&] {
+    check_batch(batch);
+    if (steps < 0 || steps > AXH_MAX_GRAPH_STEPS)
+      fail(AXH_ERR_ARG, "graph steps must be 0.." + std::to_string(AXH_MAX_GRAPH_STEPS));
+    batch->graph_steps = steps;
+  });
+}
+
 int axh_batch_set_profiling(axh_batch *batch, int on) {
   return guarded([&] {
     check_batch(batch);
@@ -639,6 +657,19 @@ int axh_mesh_bytes(const axh_mesh *mesh, size_t *bytes) {
 
 int axh_mesh_release(axh_mesh *mesh) {
   return guarded([&] { mesh_release(mesh); });
+}
+
+int axh_convolve_traces(int32_t nt, int32_t ntraces, const float *seis, int32_t nj,
+                        const float *stf, double dt, double scale, float *out) {
+  return guarded([&] {
+    if (nt < 0 || ntraces < 0 || nj < 0) fail(AXH_ERR_ARG, "negative size");
+    if (nj > AXH_MAX_STF_TERMS)
+      fail(AXH_ERR_ARG, "at most " + std::to_string(AXH_MAX_STF_TERMS) +
+                            " source time function terms");
+    if ((size_t)nt * ntraces > 0 && (!seis || !out || (nj > 0 && !stf)))
+      fail(AXH_ERR_ARG, "null pointer");
+    convolve_traces(nt, ntraces, seis, nj, stf, dt, scale, out);
+  });
 }
 
 int axh_symplectic_coefficients(int32_t scheme, double deltat, double *coefd,

@@ -81,6 +81,7 @@ struct axh_ctx {
   int32_t iter_done = 0;  // last completed step
   double last_run_ms = 0.0;
   bool profile = false;
+  int graph_steps = 0;    // steps per replayed graph (0: direct launches)
   int batch_refs = 0;     // batches holding this context
   // device mesh part to use when the device part is built (axh_mesh_bind)
   std::shared_ptr<axh::MeshDev> bound_mesh;
@@ -106,6 +107,7 @@ struct axh_batch {
   std::unique_ptr<axh::BatchState, axh::BatchStateDeleter> state;  // NULL before the first run
   double last_run_ms = 0.0;
   bool profile = false;
+  int graph_steps = 0;              // steps per replayed graph (0: direct)
   int group_width = AXH_MAX_GROUP;  // wavefields per kernel instance
 };
 
@@ -118,6 +120,8 @@ extern const char *const kFluid3dNames[4];
 void device_build(axh_ctx *ctx);
 void device_destroy(Device *d);
 void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void convolve_traces(int nt, int ntraces, const float *seis, int nj, const float *stf,
+                     double dt, double scale, float *out);
 void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
                      float *dchi, float *ddchi);
 void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,

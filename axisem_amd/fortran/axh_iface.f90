@@ -1,6 +1,7 @@
 !=========================================================================================
 !> iso_c_binding interfaces of the GPU time-loop library (include/axihip.h) and of the
-!! host glue (axh_host_glue.c).  Used by the replacement time_evol_wave module.
+!! host glue (axh_host_glue.c).  Used by the replacement time_evol_wave module and by the
+!! GPU convolution of the post-processing program (convolve_gpu.f90).
 module axh_iface
 
   use, intrinsic :: iso_c_binding
@@ -159,6 +160,15 @@ module axh_iface
        real(c_double), value :: deltat
        real(c_double), intent(out) :: coefd(*), coefv(*), coeff(*)
        integer(c_int32_t), intent(out) :: nstages
+     end function
+
+     integer(c_int) function axh_convolve_traces(nt, ntraces, seis, nj, stf, dt, scale, &
+          out) bind(C, name='axh_convolve_traces')
+       import :: c_int, c_int32_t, c_float, c_double
+       integer(c_int32_t), value :: nt, ntraces, nj
+       real(c_float), intent(in) :: seis(*), stf(*)
+       real(c_double), value :: dt, scale
+       real(c_float), intent(out) :: out(*)
      end function
 
      ! host glue (axh_host_glue.c)

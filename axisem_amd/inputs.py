@@ -29,6 +29,7 @@ class RunConfig:
     snapshots: str = ""                 # "", "xdmf", "vtk" or "both" (SAVE_SNAPSHOTS)
     snapshot_dt: float = 20.0           # SNAPSHOT_DT, seconds
     energy: bool = False                # SAVE_ENERGY
+    simulation_type: str = "single"     # SIMULATION_TYPE: single, or moment (one MT run)
 
 
 def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
@@ -86,7 +87,7 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
     else:
         os.symlink(os.path.abspath(mesh_dir), mesh_link)
     with open(os.path.join(rundir, "inparam_basic"), "w") as f:
-        f.write("SIMULATION_TYPE   single\n")
+        f.write("SIMULATION_TYPE   %s\n" % cfg.simulation_type)
         f.write("SEISMOGRAM_LENGTH  %.1f\n" % cfg.seismogram_length_s)
         f.write("RECFILE_TYPE      stations\nMESHNAME           Mesh\n")
         f.write("LAT_HETEROGENEITY  false\n")

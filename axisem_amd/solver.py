@@ -22,7 +22,7 @@ from . import state as _state
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libaxihip.so")
 
-AXH_OK, AXH_ERR_BLOWUP = 0, -6
+AXH_OK, AXH_ERR_ARG, AXH_ERR_STATE, AXH_ERR_BLOWUP = 0, -1, -2, -6
 MAX_BATCH = 16
 
 SOLID_TERMS = ["M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M11z", "M21z",
@@ -109,6 +109,10 @@ def library():
         L.axh_kernel_name.restype = C.c_char_p
         L.axh_kernel_name.argtypes = [C.c_int]
         L.axh_set_profiling.argtypes = [P, C.c_int]
+        L.axh_set_graph_steps.argtypes = [P, C.c_int32]
+        L.axh_batch_set_graph_steps.argtypes = [P, C.c_int32]
+        L.axh_convolve_traces.argtypes = [C.c_int32, C.c_int32, P, C.c_int32, P, C.c_double,
+                                          C.c_double, P]
         L.axh_check_inputs.argtypes = [P]
         L.axh_global_numbering.argtypes = [P, C.c_int32, P, C.POINTER(C.c_int32)]
         L.axh_symplectic_coefficients.argtypes = [C.c_int32, C.c_double, P, P, P,
@@ -315,6 +319,11 @@ class Solver:
     def set_profiling(self, on: bool):
         _check(library().axh_set_profiling(self._h, 1 if on else 0))
 
+    def set_graph_steps(self, steps: int):
+        """Replay a captured HIP graph of `steps` time steps (0: launch every
+        kernel directly); results are identical either way."""
+        _check(library().axh_set_graph_steps(self._h, steps))
+
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""
         return _kernel_stats(library().axh_kernel_stats, self._h)
@@ -339,7 +348,7 @@ class Batch:
     reads the mesh terms once.  Each member's seismograms and fields are the
     ones it gets when run alone."""
 
-    def __init__(self, solvers, group_width: int | None = None):
+    def __init__(self, solvers, group_width: int | None = None, graph_steps: int = 0):
         self.solvers = list(solvers)
         n = len(self.solvers)
         arr = (C.c_void_p * n)(*[s._h for s in self.solvers])
@@ -347,6 +356,8 @@ class Batch:
         _check(library().axh_batch_create(arr, n, C.byref(self._h)))
         if group_width is not None:
             _check(library().axh_batch_set_group_width(self._h, group_width))
+        if graph_steps:
+            self.set_graph_steps(graph_steps)
 
     def close(self):
         if self._h:
@@ -414,6 +425,10 @@ class Batch:
 
     def set_profiling(self, on: bool):
         _check(library().axh_batch_set_profiling(self._h, 1 if on else 0))
+
+    def set_graph_steps(self, steps: int):
+        """Solver.set_graph_steps for every group of the batch."""
+        _check(library().axh_batch_set_graph_steps(self._h, steps))
 
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""

@@ -318,22 +318,31 @@ P50_WORKLOAD = inputs.RunConfig(
 
 def p50_leg(workdir: str, steps: int, warmup: int) -> dict:
     """Step time of the 50 s configuration (6 050 elements: a few tiles per
-    CU, launch-latency bound)."""
+    CU, launch-latency bound), every kernel launched directly and with the
+    autotuned replayed graph of steps."""
     from axisem_amd.solver import Solver
+    from axisem_amd.tune import tune_solo
     path = prepare_state(workdir, P50_WORKLOAD, P50_MESH)
     st = read_dims(path)
     gll = 25 * (st["nel_solid"] + st["nel_fluid"])
-    with Solver.from_state_file(path) as s:
-        s.run(warmup)
-        t0 = time.perf_counter()
-        s.run(steps)
-        wall = time.perf_counter() - t0
-        dev = s.last_run_ms()
-    return {"workload": "prem_iso 50 s, NPOL 4, elastic, newmark2, monopole vertforce 0 km, "
-                        "1800 s, 61 receivers",
-            "nel": st["nel_solid"] + st["nel_fluid"], "steps": steps,
-            "ms_per_step": wall / steps * 1e3, "device_ms_per_step": dev / steps,
-            "value": gll * steps / wall, "unit": "GLL-point updates/s"}
+    tuned = tune_solo(path)
+    out = {"workload": "prem_iso 50 s, NPOL 4, elastic, newmark2, monopole vertforce 0 km, "
+                       "1800 s, 61 receivers",
+           "nel": st["nel_solid"] + st["nel_fluid"], "steps": steps,
+           "tuned_graph_steps": tuned["graph_steps"],
+           "tuner_device_ms_per_step": tuned["ms_per_step"]}
+    for name, gs in (("direct", 0), ("graph", tuned["graph_steps"])):
+        with Solver.from_state_file(path) as s:
+            s.set_graph_steps(gs)
+            s.run(warmup)
+            t0 = time.perf_counter()
+            s.run(steps)
+            wall = time.perf_counter() - t0
+            dev = s.last_run_ms()
+        out[name] = {"graph_steps": gs, "ms_per_step": wall / steps * 1e3,
+                     "device_ms_per_step": dev / steps, "value": gll * steps / wall,
+                     "unit": "GLL-point updates/s"}
+    return out
 
 
 # Batched wavefields on one mesh: prem_iso 5 s, elastic, newmark2 (the time
@@ -396,7 +405,12 @@ def mt_leg(workdir: str, steps: int, warmup: int) -> dict:
     d = read_dims(paths[("mrr", 100.0)])
     gll = 25 * (d["nel_solid"] + d["nel_fluid"])
 
+    # graphs of 4 steps (captured in the warm-up); the tuner's run over 16
+    # states of this mesh would cost more than the leg itself
+    gs = 4 if warmup >= 4 else 0
+
     def timed(runner):
+        runner.set_graph_steps(gs)
         runner.run(warmup)
         t0 = time.perf_counter()
         runner.run(steps)
@@ -412,7 +426,7 @@ def mt_leg(workdir: str, steps: int, warmup: int) -> dict:
                        "181 receivers",
            "nel_solid": d["nel_solid"], "nel_fluid": d["nel_fluid"],
            "niter_full_run": d["niter"], "deltat": d["deltat"], "steps": steps,
-           "plain_ms_per_step": plain}
+           "graph_steps": gs, "plain_ms_per_step": plain}
     for name, keys in (("batch4", [(r, 100.0) for r in MT_RUNS]),
                        ("batch16", [(r, dd) for dd in MT_DEPTHS_KM for r in MT_RUNS])):
         members = [Solver.from_state_file(paths[k]) for k in keys]
@@ -498,7 +512,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.5, help="arrivals per second")
     ap.add_argument("--serve-chunk", type=int, default=200, help="steps per batch chunk")
     ap.add_argument("--group-width", type=int, default=None,
-                    help="wavefields per kernel instance in the batched leg (default: library)")
+                    help="wavefields per kernel instance in the batched leg (default: tuned)")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="steps per replayed HIP graph (default: tuned, at most --warmup)")
     ap.add_argument("--no-p50", action="store_true", help="no 50 s configuration leg")
     ap.add_argument("--p50-steps", type=int, default=1000, help="timed steps of the 50 s leg")
     ap.add_argument("--no-mt", action="store_true",
@@ -540,8 +556,17 @@ def main():
     if need > d["niter"]:
         raise SystemExit("warmup + steps exceed the %d steps of the workload" % d["niter"])
 
+    # ---- launch schedule: autotuned on throwaway contexts; graphs only as
+    # long as the warm-up (which captures them) ----
+    from axisem_amd.tune import GRAPH_CANDIDATES, tune_batch, tune_solo
+    cands = [c for c in GRAPH_CANDIDATES if c <= a.warmup]
+    tuned = tune_solo(state, candidates=cands) if a.graph_steps is None else \
+        {"graph_steps": a.graph_steps, "ms_per_step": {}}
+    log("plain: launch schedule %s" % json.dumps(tuned))
+
     # ---- plain path: one wavefield ----
     solver = Solver.from_state_file(state)
+    solver.set_graph_steps(tuned["graph_steps"])
     log("plain: warmup %d steps" % a.warmup)
     solver.run(a.warmup)
     log("plain: timed %d steps" % a.steps)
@@ -570,8 +595,15 @@ def main():
     # ---- batched path: nbatch wavefields on the same mesh ----
     batched = None
     if a.batch > 1:
+        if a.group_width is None or a.graph_steps is None:
+            btuned = tune_batch(states, candidates=cands)
+        else:
+            btuned = {"group_width": a.group_width, "graph_steps": a.graph_steps,
+                      "ms_per_step": {}}
+        log("batched: launch schedule %s" % json.dumps(btuned))
         members = [Solver.from_state_file(p) for p in states]
-        with Batch(members, group_width=a.group_width) as b:
+        with Batch(members, group_width=btuned["group_width"],
+                   graph_steps=btuned["graph_steps"]) as b:
             log("batched x%d: warmup %d steps" % (nbatch, a.warmup))
             b.run(a.warmup)
             log("batched x%d: timed %d steps" % (nbatch, a.steps))
@@ -591,7 +623,9 @@ def main():
         btotal = nbatch * gll * a.steps / bwall
         batched = {
             "wavefields": nbatch,
-            "group_width": a.group_width or 4,
+            "group_width": btuned["group_width"],
+            "graph_steps": btuned["graph_steps"],
+            "tuner_device_ms_per_step": btuned["ms_per_step"],
             "sources": "dipole thetaforce at %s km" % "/".join("%g" % x for x in
                                                                BATCH_DEPTHS_KM[:nbatch]),
             # GLL-point updates of all member wavefields per second, against
@@ -653,6 +687,8 @@ def main():
         "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3,
         "device_ms_per_step": dev_ms / a.steps,
+        "graph_steps": tuned["graph_steps"],
+        "tuner_device_ms_per_step": tuned["ms_per_step"],
         "time_steps_per_s": a.steps / wall,
         "higher_is_better": True,
         "scaling": "weak",

@@ -38,6 +38,7 @@ extern "C" {
 #define AXH_ABI_VERSION 1
 #define AXH_MAX_BATCH 16    /* wavefields per batch */
 #define AXH_MAX_GROUP 4     /* wavefields per kernel instance */
+#define AXH_MAX_GRAPH_STEPS 64  /* time steps per replayed HIP graph */
 
 enum axh_status {
   AXH_OK = 0,
@@ -260,6 +261,16 @@ const char *axh_kernel_name(int kclass);
 int axh_kernel_stats(const axh_ctx *ctx, int kclass, double *total_ms,
                      int64_t *launches);
 
+/* Launch mode of axh_run: steps = 0 launches every kernel directly (the
+ * default); steps = S > 0 captures S time steps of the context once into a
+ * HIP graph and replays it for every S steps of a run (the rest of a run,
+ * and runs with per-kernel profiling on, launch directly).  Each kernel reads
+ * its time step from device memory, so the replays need no host work per
+ * step; results are bit-identical either way.  Small meshes, whose steps are
+ * bound by launch latency, gain.  The reference has no counterpart (its time
+ * loop is a host loop, SOLVER/time_evol_wave.F90:353-500). */
+int axh_set_graph_steps(axh_ctx *ctx, int32_t steps);
+
 /* Device memory held by the context [bytes] (0 before the first run). */
 int axh_device_bytes(axh_ctx *ctx, size_t *bytes);
 
@@ -308,6 +319,8 @@ int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms);
  * lane.  Only before the first axh_batch_run. */
 int axh_batch_set_group_width(axh_batch *batch, int32_t width);
 int axh_batch_set_profiling(axh_batch *batch, int on);
+/* axh_set_graph_steps for the groups of a batch (one graph per group). */
+int axh_batch_set_graph_steps(axh_batch *batch, int32_t steps);
 int axh_batch_kernel_stats(const axh_batch *batch, int kclass,
                            double *total_ms, int64_t *launches);
 
@@ -392,6 +405,22 @@ int axh_meshdb_upload(const axh_meshdb *db, axh_ctx *ctx);
  * host (tests, benchmarks, the request loop). */
 int axh_save_state(const axh_ctx *ctx, const char *path);
 int axh_load_state(const char *path, axh_ctx **out);
+
+/* ---- seismogram post-processing ------------------------------------------ */
+
+/* The convolution of traces with a source time function of the reference's
+ * post-processing, convolve_with_stf (SOLVER/UTILS/post_processing.F90:
+ * 1014-1084), on the GPU: seis and out are ntraces traces of nt samples, one
+ * after the other; for sample i (1-based) of a trace,
+ *   out(i) = sum over j = 1 .. min(nj, i-1) of seis(i-j) * stf(j) * dt,
+ * each term the REAL(4) product seis*stf times dt in REAL(8), added in
+ * REAL(8) to the REAL(4) partial sum in ascending j, and finally
+ * out(i) = out(i) * scale in REAL(8): the reference's arithmetic bit for
+ * bit.  stf(1..nj) is the tabulated source time function (at most
+ * AXH_MAX_STF_TERMS terms).  Allocates its device buffers per call. */
+#define AXH_MAX_STF_TERMS 8192
+int axh_convolve_traces(int32_t nt, int32_t ntraces, const float *seis, int32_t nj,
+                        const float *stf, double dt, double scale, float *out);
 
 /* ---- helpers ------------------------------------------------------------- */
 

@@ -8,7 +8,9 @@ TEST INFRASTRUCTURE ONLY (the checker, never the product path).
 * ``build_reference(src)`` -- amdflang on the reference's own SOLVER and
   MESHER sources, compiled where they lie under ``src`` (nothing is copied),
   serial, no NetCDF, realkind = 4 as shipped -> ``oracle/_ref/axisem`` and
-  ``oracle/_ref/xmesh``.  Same flags as the drop-in host build.
+  ``oracle/_ref/xmesh``.  Same flags as the drop-in host build.  Also the
+  reference's seismogram post-processing program
+  (SOLVER/UTILS/post_processing.F90, no NetCDF) -> ``oracle/_ref/xpost``.
 * ``build_reference_fp64(src)`` -- the same programs in double precision,
   as the reference's own comment directs ("Choose solver precision here",
   SOLVER/global_parameters.f90:40, MESHER/global_parameters.f90:40, and
@@ -119,7 +121,12 @@ def build_reference(ref_root: str = "/root/reference", jobs: int = 8):
     xmesh = _fortran_program(msrc, [], os.path.join(REF_OUT, "obj_mesher"),
                              os.path.join(REF_OUT, "xmesh"), ["-O2", "-cpp", "-fopenmp"],
                              ["-fopenmp"], jobs)
-    return axisem, xmesh
+    utils = os.path.join(solver, "UTILS")
+    xpost = _fortran_program([os.path.join(utils, "nc_postroutines.F90"),
+                              os.path.join(utils, "post_processing.F90")], [],
+                             os.path.join(REF_OUT, "obj_post"), os.path.join(REF_OUT, "xpost"),
+                             ["-O2", "-cpp"], [], jobs)
+    return axisem, xmesh, xpost
 
 
 # (file relative to the reference root, [(pattern, replacement)]), each pattern
