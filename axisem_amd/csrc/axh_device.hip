@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3639,6 +3640,71 @@ void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
     if (blow[b] != INT_MAX)
       fail(AXH_ERR_BLOWUP, "batch member " + std::to_string(b) +
                                ": displacements blew up at time step " + std::to_string(blow[b]));
+}
+
+// ---------------------------------------------------------------------------
+// runs that allocate nothing (axh_run_noalloc)
+// ---------------------------------------------------------------------------
+// The device buffers a run writes besides the fields -- the step base, the
+// blow-up flag and the receiver-sample ring -- in one fixed array per GPU
+// instead of the context's own.  Every call on a GPU shares that GPU's array
+// and finds it as the last call left it (each run sets the step base and
+// clears the flag itself); calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)AXH_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchHead = 256;   // step base [kGroup], blow-up flag
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+
+// the array's device address on the current GPU (in host code `scratch` only
+// names a host-side stand-in), looked up once per GPU
+static unsigned char *scratch_ptr() {
+  static std::mutex mu;
+  static unsigned char *ptr[kMaxDevices] = {};
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(AXH_ERR_HIP, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+      fail(AXH_ERR_HIP, std::string("device address of the scratch array: ") +
+                            hipGetErrorString(rc));
+    ptr[dev] = (unsigned char *)p;
+  }
+  return ptr[dev];
+}
+
+int noalloc_max_receivers() {
+  return (int)((kScratchBytes - kScratchHead) / (3 * sizeof(float)));
+}
+
+void device_run_noalloc(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
+  Device &V = *ctx->dev;
+  if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
+  if (!V.run.stream)
+    fail(AXH_ERR_STATE, "axh_run_noalloc needs a context that has run (axh_run) before");
+  unsigned char *base = scratch_ptr();
+  Member *mb[1] = {&V.member};
+  RunBuffers RB;
+  RB.step_base = (int *)base;
+  RB.blow[0] = (int *)base + kGroup;
+  RB.ring[0] = (float *)(base + kScratchHead);
+  RB.cap[0] = (int)((kScratchBytes - kScratchHead) /
+                    (3 * sizeof(float) * (size_t)std::max(V.member.nrec, 1)));
+  // per-kernel profiling creates events: off for this entry point
+  V.run.profile = false;
+  V.run.graph_steps = ctx->graph_steps;
+  const axh_ctx *cx[1] = {ctx};
+  float *out[1] = {seis};
+  int blow = INT_MAX;
+  const int32_t start[1] = {it0};
+  GroupTail T;
+  run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, RB, T);
+  ctx->last_run_ms = group_finish(V.run, T, &blow);
+  ctx->iter_done = it1;
+  if (blow != INT_MAX)
+    fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
 }
 
 // The live copy of ctx's state: its own fields, or its slot in a batch group.

@@ -370,6 +370,24 @@ int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
   });
 }
 
+int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (ctx->dims.num_rec > noalloc_max_receivers())
+      fail(AXH_ERR_ARG, "axh_run_noalloc: " + std::to_string(ctx->dims.num_rec) +
+                            " receivers; the fixed scratch array holds the samples of at most " +
+                            std::to_string(noalloc_max_receivers()));
+    if (it_begin != ctx->iter_done + 1)
+      fail(AXH_ERR_ARG, "axh_run_noalloc must continue at step " +
+                            std::to_string(ctx->iter_done + 1));
+    if (it_end < it_begin - 1 || it_end > ctx->dims.niter)
+      fail(AXH_ERR_ARG, "step range outside 1..niter");
+    if (!ctx->dev)
+      fail(AXH_ERR_STATE, "axh_run_noalloc needs a context that has run (axh_run) before");
+    device_run_noalloc(ctx, it_begin, it_end, seis);
+  });
+}
+
 int axh_download_fields(axh_ctx *ctx, float *disp, float *velo, float *chi,
                         float *dchi, float *ddchi) {
   return guarded([&] {

@@ -120,8 +120,10 @@ extern const char *const kFluid3dNames[4];
 void device_build(axh_ctx *ctx);
 void device_destroy(Device *d);
 void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_run_noalloc(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
 void convolve_traces(int nt, int ntraces, const float *seis, int nj, const float *stf,
                      double dt, double scale, float *out);
+int noalloc_max_receivers();
 void device_download(axh_ctx *ctx, float *disp, float *velo, float *chi,
                      float *dchi, float *ddchi);
 void device_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,

@@ -111,6 +111,7 @@ def library():
         L.axh_set_profiling.argtypes = [P, C.c_int]
         L.axh_set_graph_steps.argtypes = [P, C.c_int32]
         L.axh_batch_set_graph_steps.argtypes = [P, C.c_int32]
+        L.axh_run_noalloc.argtypes = [P, C.c_int32, C.c_int32, P]
         L.axh_convolve_traces.argtypes = [C.c_int32, C.c_int32, P, C.c_int32, P, C.c_double,
                                           C.c_double, P]
         L.axh_check_inputs.argtypes = [P]

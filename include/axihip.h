@@ -235,6 +235,19 @@ int axh_global_numbering(axh_ctx *ctx, int32_t domain, int32_t *gid,
  * u_phi, u_z), k counting the sampled steps of this call from 0. */
 int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
 
+/* axh_run that allocates nothing: no hipMalloc, hipFree or other allocating
+ * call, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  No counterpart in the reference.
+ * The context must have run (axh_run) before, so that its device part exists;
+ * the buffers the run writes besides the fields -- the step counter, the
+ * blow-up flag and the staging of the receiver samples -- are a fixed array
+ * of AXH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, shared
+ * by every call on that GPU: calls that overlap on one GPU race.  At most
+ * (AXH_NOALLOC_SCRATCH_BYTES - 256) / 12 receivers (AXH_ERR_ARG beyond);
+ * no per-kernel profiling.  Results equal axh_run's bit for bit. */
+#define AXH_NOALLOC_SCRATCH_BYTES (1 << 20)
+int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
+
 /* Number of seismogram samples axh_run(ctx, it_begin, it_end) will write. */
 int axh_seis_samples(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
 
