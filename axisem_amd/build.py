@@ -26,6 +26,7 @@ import subprocess
 PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libaxihip.so")
+LIB_FMA = os.path.join(PKG, "libaxihip_fma.so")
 HOST_DIR = os.path.join(PKG, "fortran", "_build")
 HOST_EXE = os.path.join(HOST_DIR, "axisem_gpu")
 
@@ -42,7 +43,14 @@ HIP_FLAGS = [
 FFLAGS = ["-O2", "-cpp", "-Dserial", "-Dsolver"]
 
 
-def build_library(out: str = LIB) -> str:
+def build_library(out: str = LIB, fused: bool = False) -> str:
+    """The library; fused=True builds the variant whose kernels contract
+    a*b+c into fused multiply-adds (libaxihip_fma.so): not the reference's
+    arithmetic bit for bit, within its seismogram misfit."""
+    flags = HIP_FLAGS
+    if fused:
+        flags = [("-ffp-contract=fast" if f == "-ffp-contract=off" else f) for f in HIP_FLAGS]
+        flags.append("-DAXH_FUSED_MULTIPLY_ADD")
     srcs = [os.path.join(PKG, "csrc", "axh_host.cpp"),
             os.path.join(PKG, "csrc", "axh_meshdb.cpp"),
             os.path.join(PKG, "csrc", "axh_device.hip")]
@@ -61,7 +69,7 @@ def build_library(out: str = LIB) -> str:
          "-o", post_obj])
     tmp = out + ".tmp"
     try:
-        run([HIPCC, *HIP_FLAGS, *inc, *srcs, "-x", "none", post_obj, "-o", tmp])
+        run([HIPCC, *flags, *inc, *srcs, "-x", "none", post_obj, "-o", tmp])
     finally:
         if os.path.exists(post_obj):
             os.remove(post_obj)

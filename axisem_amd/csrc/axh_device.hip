@@ -26,7 +26,12 @@
 //  * Arithmetic order follows the reference expression by expression; the
 //    file is compiled with -ffp-contract=off (no FMA), and dp constants enter
 //    in double precision where the Fortran promotes to dp.  A wavefield's
-//    arithmetic does not depend on the other members of its group.
+//    arithmetic does not depend on the other members of its group.  A second
+//    build (AXH_FUSED_MULTIPLY_ADD, libaxihip_fma.so) contracts a*b+c into
+//    fused multiply-adds: one rounding instead of two per pair, so its
+//    results are not the reference's bit for bit but within its seismogram
+//    misfit budget, for the kernels whose issue rate bounds them (the
+//    stiffness of several wavefields per launch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,7 +45,11 @@
 
 #include "axh_internal.h"
 
+#ifdef AXH_FUSED_MULTIPLY_ADD
+#pragma clang fp contract(fast)
+#else
 #pragma clang fp contract(off)
+#endif
 
 namespace axh {
 

@@ -18,7 +18,8 @@
 !!   AXH_NO_RUN=1           stop after prepare/upload (records states without a GPU)
 !!   AXH_EXTRA_SOURCES=<f>  with AXH_NO_RUN: after the first state, record one state per
 !!                          line of file f ("<inparam_source file> <state file>"), all
-!!                          on the mesh and model terms set up once (one source type)
+!!                          on the mesh and model terms set up once (one source type);
+!!                          f may be a pipe that a request loop writes as requests come
 module time_evol_wave
 
   use global_parameters
@@ -104,7 +105,8 @@ subroutine prepare_waves
   lpath = axh_glue_getenv('AXH_SAVE_STATE'//c_null_char, path, len(path))
   if (lpath > 0) then
      call axh_check(axh_save_state(ctx, trim(path(1:lpath))//c_null_char), 'save state')
-     if (lpr) write(6,*) '  wrote GPU state file ', trim(path(1:lpath))
+     if (lpr) write(6,'(a,a)') ' wrote GPU state file ', trim(path(1:lpath))
+     call flush(6)
   endif
   lpath = axh_glue_getenv('AXH_NO_RUN'//c_null_char, path, len(path))
   no_run = lpath > 0 .and. path(1:1) /= '0'
@@ -644,7 +646,9 @@ subroutine record_extra_sources()
         stop 2
      endif
      call axh_check(axh_save_state(ctx, trim(statefile)//c_null_char), 'save state')
-     if (lpr) write(6,*) '  wrote GPU state file ', trim(statefile)
+     ! a reader waiting for this state (a pipe of requests) sees it at once
+     if (lpr) write(6,'(a,a)') ' wrote GPU state file ', trim(statefile)
+     call flush(6)
   enddo
   close(u)
 

@@ -60,7 +60,7 @@ def run_mesher(xmesh: str, outdir: str, period_s: float, model: str = "prem_iso"
     def unlimited_stack():
         resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
-    env = dict(os.environ, OMP_STACKSIZE="512M")
+    env = dict(os.environ, OMP_STACKSIZE="4G")
     env.setdefault("OMP_NUM_THREADS", "8")
     with open(os.path.join(outdir, "OUTPUT_MESH"), "w") as log:
         subprocess.run([os.path.abspath(xmesh)], cwd=outdir, stdout=log, stderr=subprocess.STDOUT,
@@ -198,3 +198,87 @@ def read_seismograms(data_dir: str, nrec: int, ncomp: int):
         a = np.loadtxt(os.path.join(data_dir, "R%03d_AX_disp.dat" % k), dtype=np.float64, ndmin=2)
         cols.append(a[:, :ncomp].astype(np.float32))
     return np.stack(cols, axis=1)
+
+
+class SourceRecorder:
+    """A drop-in host kept running in record mode for one mesh, model, time
+    scheme and source type: the reference setup runs once (for the first
+    source), then every further source is recorded on demand by the
+    reference's own source routines (AXH_EXTRA_SOURCES read from a pipe), in
+    seconds instead of a full setup.  record() is serialised per recorder."""
+
+    def __init__(self, host_exe: str, rundir: str, mesh_dir: str, first: RunConfig,
+                 first_state: str, timeout: float = 3600):
+        import resource
+        import subprocess
+        import threading
+
+        self.rundir = rundir
+        self.base = {k: v for k, v in first.__dict__.items()
+                     if k not in ("source_type", "source_depth_km", "source_amplitude")}
+        self.kind = SOURCE_KIND[first.source_type.lower()]
+        write_run_dir(rundir, mesh_dir, first)
+        self.fifo = os.path.join(rundir, "source_pipe")
+        if os.path.exists(self.fifo):
+            os.remove(self.fifo)
+        os.mkfifo(self.fifo)
+        self._n = 0
+        self._lock = threading.Lock()
+        self._first = os.path.abspath(first_state)
+
+        def unlimited_stack():
+            resource.setrlimit(resource.RLIMIT_STACK,
+                               (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+        env = dict(os.environ, AXH_SAVE_STATE=self._first, AXH_NO_RUN="1",
+                   AXH_EXTRA_SOURCES=os.path.abspath(self.fifo))
+        self.log = open(os.path.join(rundir, "OUTPUT_record"), "w")
+        self.proc = subprocess.Popen([os.path.abspath(host_exe)], cwd=rundir,
+                                     stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                     text=True, preexec_fn=unlimited_stack, env=env)
+        self.timeout = timeout
+        self._wait_for(self._first)
+        # the host opens the pipe for reading after the first state
+        self._pipe = open(self.fifo, "w")
+
+    def _wait_for(self, path: str) -> None:
+        want = "wrote GPU state file " + path
+        for line in self.proc.stdout:
+            self.log.write(line)
+            if line.strip() == want.strip():
+                self.log.flush()
+                return
+        self.log.flush()
+        raise RuntimeError("the drop-in host ended before writing %s (see %s)"
+                           % (path, self.log.name))
+
+    @property
+    def first_state(self) -> str:
+        return self._first
+
+    def record(self, cfg: RunConfig, state_path: str) -> str:
+        if SOURCE_KIND[cfg.source_type.lower()] != self.kind:
+            raise ValueError("a recorder serves one source type")
+        if {k: v for k, v in cfg.__dict__.items() if k in self.base} != self.base:
+            raise ValueError("a recorder serves configs that differ in the source only")
+        with self._lock:
+            self._n += 1
+            src = os.path.join(self.rundir, "inparam_source.%d" % self._n)
+            write_source(src, cfg)
+            path = os.path.abspath(state_path)
+            self._pipe.write("%s %s\n" % (os.path.abspath(src), path))
+            self._pipe.flush()
+            self._wait_for(path)
+            return path
+
+    def close(self) -> None:
+        if self._pipe is not None:
+            self._pipe.close()
+            self._pipe = None
+        if self.proc is not None:
+            try:
+                rest = self.proc.communicate(timeout=self.timeout)[0]
+                self.log.write(rest or "")
+            finally:
+                self.proc = None
+                self.log.close()

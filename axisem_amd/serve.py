@@ -1,21 +1,29 @@
 """Request loop over the GPU time loop: continuous batching of solves.
 
-Solves (one source on one mesh, prepared by the reference's setup and handed
-over as a state file) arrive over time.  The loop
+Solves (one source on one mesh) arrive over time, either as a state file
+already prepared by the reference's setup or as a run configuration that the
+loop prepares when the request arrives: one drop-in host per mesh, model,
+time scheme and source type stays running in record mode
+(axisem_amd.inputs.SourceRecorder), so only the first request of such a key
+pays the full setup and later ones only the reference's source routines.
+The loop
 
 * keeps meshes resident on the device (one device mesh part per distinct mesh
   fingerprint, shared by every request on it, kept while memory allows);
 * admits arrived requests in arrival order as device memory allows (the
   footprint of a request is its wavefield part, plus its mesh part when that
   mesh is not resident; idle meshes are evicted, least recently used first);
-* co-batches the admitted requests that share a mesh (same mesh, source type
-  and time scheme): they advance together, one kernel launch per step and
-  stage for all of them, each from its own step (axh_batch_step);
+* co-batches the admitted requests that share a mesh numbering and time
+  scheme (axh_batch_key), whatever their source types (the four runs of a
+  moment tensor share one batch, one group per type): they advance together,
+  one kernel launch per step and stage for each group, each member from its
+  own step (axh_batch_step);
 * re-forms a batch between chunks of steps when requests join or finish, so a
   request that arrives while others run starts at the next chunk boundary;
 * runs the batches of different meshes concurrently (one host thread and one
   HIP stream each), so their kernels overlap on the device;
-* records per request the time from arrival to its complete seismograms.
+* records per request the time from arrival to its complete seismograms
+  (including its preparation when it arrived as a configuration).
 
 Each request's seismograms are the ones it gets alone (the batched kernels
 are bit-identical per wavefield).  The reference has no counterpart: it runs
@@ -35,13 +43,18 @@ import numpy as np
 @dataclass
 class Request:
     rid: int
-    state_path: str
+    state_path: str | None
     arrival_s: float = 0.0          # seconds after the loop starts
+    mesh_dir: str | None = None     # prepared on arrival from (mesh_dir, cfg)
+    cfg: object = None
+    ready_s: float | None = None    # state prepared
     admitted_s: float | None = None
     done_s: float | None = None
     niter: int = 0
+    steps: int | None = None        # steps to run (None: the state's niter)
     gll_per_step: int = 0
-    key: tuple | None = None
+    key: tuple | None = None        # batch key
+    fp: tuple | None = None         # mesh fingerprint
     seis: np.ndarray | None = None  # (nsamples, num_rec, 3)
     error: str | None = None
     _sol: object = field(default=None, repr=False)   # loaded, not yet admitted
@@ -49,6 +62,10 @@ class Request:
     @property
     def latency_s(self):
         return None if self.done_s is None else self.done_s - self.arrival_s
+
+    @property
+    def setup_s(self):
+        return None if self.ready_s is None else self.ready_s - self.arrival_s
 
 
 class GpuBackend:
@@ -66,6 +83,12 @@ class GpuBackend:
         fp = (C.c_uint64 * 2)()
         self.S._check(self.S.library().axh_mesh_fingerprint(sol._h, fp))
         return (int(fp[0]), int(fp[1]))
+
+    def batch_key(self, sol):
+        import ctypes as C
+        k = (C.c_uint64 * 2)()
+        self.S._check(self.S.library().axh_batch_key(sol._h, k))
+        return (int(k[0]), int(k[1]))
 
     def footprint(self, sol):
         import ctypes as C
@@ -97,10 +120,83 @@ class _Active:
     chunks: list = field(default_factory=list)
 
 
+class Preparer:
+    """States of requests that arrive as (mesh, configuration): one recorder
+    (a drop-in host in record mode fed through a pipe) per mesh, model, time
+    scheme and source type, started by the first request of that key.
+
+    A recorder records every source for the longest duration it serves
+    (max_length_s); a request runs only the first niter steps its own
+    duration needs, ceiling((length + smallval_dble) / deltat) as the
+    reference counts them (SOLVER/parameters.F90:901).  Nothing in a step
+    depends on the run's length (the source time function and the receiver
+    sampling are functions of the step), so that prefix is the shorter run
+    bit for bit."""
+
+    def __init__(self, host_exe: str, workdir: str, max_length_s: float = 3600.0):
+        import os
+        self.host = host_exe
+        self.max_length = max_length_s
+        self.workdir = workdir
+        os.makedirs(workdir, exist_ok=True)
+        self._rec = {}
+        self._lock = threading.Lock()
+        self._n = 0
+
+    def _key(self, mesh_dir, cfg):
+        from .inputs import SOURCE_KIND
+        base = {k: v for k, v in cfg.__dict__.items()
+                if k not in ("source_type", "source_depth_km", "source_amplitude",
+                             "seismogram_length_s")}
+        return (mesh_dir, repr(sorted(base.items())), SOURCE_KIND[cfg.source_type.lower()])
+
+    @staticmethod
+    def steps_for(length_s: float, deltat: float) -> int:
+        """niter of SOLVER/parameters.F90:901 (smallval_dble is the REAL(4)
+        literal 1e-11 widened to REAL(8))."""
+        import math
+        return int(math.ceil((length_s + float(np.float32(1e-11))) / deltat))
+
+    def prepare(self, mesh_dir, cfg):
+        """(state path, steps of this request)."""
+        from .inputs import RunConfig
+        from .state import read_dims
+        if cfg.seismogram_length_s > self.max_length:
+            raise ValueError("longer than the recorders' %.0f s" % self.max_length)
+        rec_cfg = RunConfig(**{**cfg.__dict__, "seismogram_length_s": self.max_length})
+        path = self._record(mesh_dir, rec_cfg)
+        d = read_dims(path)
+        return path, min(d["niter"], self.steps_for(cfg.seismogram_length_s, d["deltat"]))
+
+    def _record(self, mesh_dir, cfg) -> str:
+        import os
+        from .inputs import SourceRecorder
+        key = self._key(mesh_dir, cfg)
+        with self._lock:
+            self._n += 1
+            n = self._n
+            entry = self._rec.get(key)
+            if entry is None:
+                entry = self._rec[key] = [threading.Lock(), None]
+        path = os.path.join(self.workdir, "req%04d.bin" % n)
+        with entry[0]:
+            if entry[1] is None:
+                entry[1] = SourceRecorder(self.host, os.path.join(self.workdir, "host%04d" % n),
+                                          mesh_dir, cfg, path)
+                return entry[1].first_state
+            return entry[1].record(cfg, path)
+
+    def close(self):
+        for lock, rec in self._rec.values():
+            if rec is not None:
+                rec.close()
+        self._rec.clear()
+
+
 class RequestLoop:
-    def __init__(self, hbm_budget_bytes: float = 200e9, max_members: int = 8,
+    def __init__(self, hbm_budget_bytes: float = 200e9, max_members: int = 16,
                  group_width: int = 4, chunk_steps: int = 100, backend=None,
-                 clock=time.perf_counter, log=None):
+                 clock=time.perf_counter, log=None, preparer: Preparer | None = None):
         self.budget = float(hbm_budget_bytes)
         self.max_members = max_members
         self.group_width = group_width
@@ -116,9 +212,22 @@ class RequestLoop:
         self._batches: dict = {}                  # key -> (batch, member ids)
         self.chunks_run = 0
         self.max_cobatched = 0
+        self.preparer = preparer
 
-    def submit(self, state_path: str, arrival_s: float = 0.0) -> Request:
-        r = Request(rid=len(self.requests), state_path=state_path, arrival_s=arrival_s)
+    def submit(self, state_path: str, arrival_s: float = 0.0, steps: int | None = None) -> Request:
+        """A request with its state prepared; steps (default: the state's niter)
+        is how many steps it runs."""
+        r = Request(rid=len(self.requests), state_path=state_path, arrival_s=arrival_s,
+                    steps=steps)
+        self.requests.append(r)
+        return r
+
+    def submit_config(self, mesh_dir: str, cfg, arrival_s: float = 0.0) -> Request:
+        """A request prepared by the loop's Preparer when it arrives."""
+        if self.preparer is None:
+            raise ValueError("the loop has no preparer")
+        r = Request(rid=len(self.requests), state_path=None, arrival_s=arrival_s,
+                    mesh_dir=mesh_dir, cfg=cfg)
         self.requests.append(r)
         return r
 
@@ -138,29 +247,31 @@ class RequestLoop:
         if req._sol is None:
             req._sol = self.be.load(req.state_path)
         sol = req._sol
-        key = self.be.key(sol)
+        fp = self.be.key(sol)
+        key = self.be.batch_key(sol)
         mesh_b, wave_b = self.be.footprint(sol)
-        resident = key in self._meshes
+        resident = fp in self._meshes
         if len(self._active.get(key, ())) >= self.max_members:
             return False
         need = wave_b + (0 if resident else mesh_b)
         if self.used + need > self.budget:
             self._evict_idle(need)
-            resident = key in self._meshes
+            resident = fp in self._meshes
             need = wave_b + (0 if resident else mesh_b)
             if self.used + need > self.budget:
                 return False
         req._sol = None
         if resident:
-            self.be.mesh_bind(sol, self._meshes[key][0])
-            self._meshes.move_to_end(key)
+            self.be.mesh_bind(sol, self._meshes[fp][0])
+            self._meshes.move_to_end(fp)
         else:
             h = self.be.mesh_get(sol)
-            self._meshes[key] = [h, mesh_b, 0]
-        self._meshes[key][2] += 1
+            self._meshes[fp] = [h, mesh_b, 0]
+        self._meshes[fp][2] += 1
         self.used += need
         self.peak_used = max(self.peak_used, self.used)
-        req.key, req.admitted_s, req.niter = key, now, sol.dims["niter"]
+        req.key, req.fp, req.admitted_s = key, fp, now
+        req.niter = sol.dims["niter"] if req.steps is None else min(req.steps, sol.dims["niter"])
         req.gll_per_step = 25 * (sol.dims["nel_solid"] + sol.dims["nel_fluid"])
         self._active[key].append(_Active(req, sol, wave_b))
         self.log("admitted request %d at %.3f s (key %x, %d on it)"
@@ -203,7 +314,7 @@ class RequestLoop:
                 a.req.done_s = now
                 a.sol.close()
                 self.used -= a.wave_bytes
-                self._meshes[key][2] -= 1
+                self._meshes[a.req.fp][2] -= 1
                 self.log("request %d done at %.3f s" % (a.req.rid, now))
             self._active[key] = [a for a in self._active[key] if a not in done]
             if not self._active[key]:
@@ -213,28 +324,50 @@ class RequestLoop:
         """Serve every submitted request; returns summary statistics."""
         order = sorted(self.requests, key=lambda r: (r.arrival_s, r.rid))
         pending = collections.deque(order)
+        preparing: list = []
         waiting: collections.deque = collections.deque()
         t0 = self.clock()
         lock = threading.Lock()
-        with ThreadPoolExecutor(max_workers=8) as pool:
-            while pending or waiting or self._active:
+        with ThreadPoolExecutor(max_workers=8) as pool, \
+                ThreadPoolExecutor(max_workers=8) as prep:
+            while pending or preparing or waiting or self._active:
                 now = self.clock() - t0
                 while pending and pending[0].arrival_s <= now:
-                    waiting.append(pending.popleft())
+                    req = pending.popleft()
+                    if req.state_path is None:
+                        preparing.append((req, prep.submit(self.preparer.prepare, req.mesh_dir,
+                                                           req.cfg)))
+                    else:
+                        req.ready_s = req.arrival_s
+                        waiting.append(req)
+                for item in list(preparing):
+                    req, fut = item
+                    if not fut.done():
+                        continue
+                    preparing.remove(item)
+                    try:
+                        req.state_path, req.steps = fut.result()
+                        req.ready_s = self.clock() - t0
+                        waiting.append(req)
+                        self.log("request %d prepared in %.2f s" % (req.rid, req.setup_s))
+                    except Exception as e:      # reported per request
+                        req.error = "preparation failed: %s" % e
                 # admission in arrival order; a request that does not fit waits
                 # without blocking smaller ones behind it
                 for req in list(waiting):
                     if self._admit(req, now):
                         waiting.remove(req)
                 if not self._active:
-                    if waiting and not pending:
+                    if waiting and not pending and not preparing:
                         req = waiting.popleft()
                         req.error = "does not fit the device memory budget"
                         if req._sol is not None:
                             req._sol.close()
                             req._sol = None
                         continue
-                    if pending:
+                    if preparing:
+                        time.sleep(0.005)
+                    elif pending:
                         time.sleep(max(0.0, pending[0].arrival_s - (self.clock() - t0)))
                     continue
                 keys = list(self._active)
@@ -257,8 +390,11 @@ class RequestLoop:
         done = [r for r in self.requests if r.done_s is not None]
         lat = np.array([r.latency_s for r in done]) if done else np.zeros(0)
         upd = sum(r.gll_per_step * r.niter for r in done)
+        setup = np.array([r.setup_s for r in done if r.setup_s is not None])
         return {
             "requests": len(self.requests), "completed": len(done),
+            "setup_p50_s": float(np.percentile(setup, 50)) if setup.size else None,
+            "setup_max_s": float(setup.max()) if setup.size else None,
             "rejected": sum(1 for r in self.requests if r.error),
             "makespan_s": makespan,
             "time_to_seismogram_p50_s": float(np.percentile(lat, 50)) if done else None,

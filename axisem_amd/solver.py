@@ -21,6 +21,9 @@ from . import state as _state
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libaxihip.so")
+# the build whose kernels contract a*b+c into fused multiply-adds: within the
+# reference's seismogram misfit, not bit for bit
+LIB_FMA_PATH = os.path.join(PKG, "libaxihip_fma.so")
 
 AXH_OK, AXH_ERR_ARG, AXH_ERR_STATE, AXH_ERR_BLOWUP = 0, -1, -2, -6
 MAX_BATCH = 16
@@ -71,16 +74,19 @@ class MeshDBInfo(C.Structure):
         [("bkgrdmodel", C.c_char * 64)]
 
 
-_lib = None
+_libs = {}
 
 
-def library():
-    """Load libaxihip.so (raises if it has not been built)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError("libaxihip.so is not built; run __graft_entry__.build()")
-        L = C.CDLL(LIB_PATH)
+def library(fused: bool = False):
+    """Load libaxihip.so, or with fused=True libaxihip_fma.so (raises if it
+    has not been built).  Contexts of one build are used with that build."""
+    L = _libs.get(fused)
+    if L is None:
+        path = LIB_FMA_PATH if fused else LIB_PATH
+        if not os.path.exists(path):
+            raise ImportError("%s is not built; run __graft_entry__.build()"
+                              % os.path.basename(path))
+        L = C.CDLL(path)
         P, I = C.c_void_p, C.c_int
         L.axh_last_error.restype = C.c_char_p
         L.axh_create.argtypes = [C.POINTER(Dims), C.POINTER(P)]
@@ -139,13 +145,13 @@ def library():
         L.axh_meshdb_get_info.argtypes = [P, C.POINTER(MeshDBInfo)]
         L.axh_meshdb_array.argtypes = [P, C.c_char_p, P, C.c_size_t]
         L.axh_meshdb_upload.argtypes = [P, P]
-        _lib = L
-    return _lib
+        _libs[fused] = L
+    return L
 
 
-def _check(rc):
+def _check(rc, L=None):
     if rc != AXH_OK:
-        msg = library().axh_last_error().decode(errors="replace")
+        msg = (L or library()).axh_last_error().decode(errors="replace")
         raise (BlowUp if rc == AXH_ERR_BLOWUP else AxhError)(rc, msg)
 
 
@@ -164,17 +170,23 @@ def symplectic_coefficients(scheme: int, dt: float):
 class Solver:
     """One wavefield on one GPU: prepared once, then stepped by ``run``."""
 
-    def __init__(self, handle, dims: dict):
+    def __init__(self, handle, dims: dict, fused: bool = False):
         self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
         self.dims = dims
         self.iter = 0
+        self.fused = fused
+        self._L = library(fused)
+
+    def _ok(self, rc):
+        _check(rc, self._L)
 
     # --- construction -----------------------------------------------------
     @classmethod
-    def from_state_file(cls, path: str) -> "Solver":
+    def from_state_file(cls, path: str, fused: bool = False) -> "Solver":
+        L = library(fused)
         h = C.c_void_p()
-        _check(library().axh_load_state(path.encode(), C.byref(h)))
-        return cls(h, _state.read_dims(path))
+        _check(L.axh_load_state(path.encode(), C.byref(h)), L)
+        return cls(h, _state.read_dims(path), fused)
 
     @classmethod
     def from_state(cls, st: _state.State) -> "Solver":
@@ -261,7 +273,7 @@ class Solver:
 
     def close(self):
         if self._h:
-            library().axh_destroy(self._h)
+            self._L.axh_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -282,11 +294,11 @@ class Solver:
         the sampled steps, shape (nsamples, num_rec, 3)."""
         it0 = self.iter + 1
         it1 = self.dims["niter"] if nsteps is None else min(self.dims["niter"], self.iter + nsteps)
-        n = library().axh_seis_samples(self._h, it0, it1)
+        n = self._L.axh_seis_samples(self._h, it0, it1)
         seis = np.zeros((max(n, 1), max(self.dims["num_rec"], 1), 3), np.float32)
-        rc = library().axh_run(self._h, it0, it1, _p(seis))
+        rc = self._L.axh_run(self._h, it0, it1, _p(seis))
         self.iter = it1 if rc in (AXH_OK, AXH_ERR_BLOWUP) else self.iter
-        _check(rc)
+        self._ok(rc)
         return seis[:n]
 
     def fields(self):
@@ -298,36 +310,36 @@ class Solver:
         chi = np.zeros(max(nf, 1), np.float32)
         dchi = np.zeros(max(nf, 1), np.float32)
         ddchi = np.zeros(max(nf, 1), np.float32)
-        _check(library().axh_download_fields(self._h, _p(disp), _p(velo), _p(chi), _p(dchi),
+        self._ok(self._L.axh_download_fields(self._h, _p(disp), _p(velo), _p(chi), _p(dchi),
                                              _p(ddchi)))
         return disp, velo, chi[:nf], dchi[:nf], ddchi[:nf]
 
     def set_fields(self, disp=None, velo=None, chi=None, dchi=None, ddchi=None):
         f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
         arrs = [f(disp), f(velo), f(chi), f(dchi), f(ddchi)]
-        _check(library().axh_upload_fields(self._h, *[_p(a) for a in arrs]))
+        self._ok(self._L.axh_upload_fields(self._h, *[_p(a) for a in arrs]))
 
     def last_run_ms(self) -> float:
         v = C.c_double()
-        _check(library().axh_last_run_ms(self._h, C.byref(v)))
+        self._ok(self._L.axh_last_run_ms(self._h, C.byref(v)))
         return v.value
 
     def device_bytes(self) -> int:
         v = C.c_size_t()
-        _check(library().axh_device_bytes(self._h, C.byref(v)))
+        self._ok(self._L.axh_device_bytes(self._h, C.byref(v)))
         return v.value
 
     def set_profiling(self, on: bool):
-        _check(library().axh_set_profiling(self._h, 1 if on else 0))
+        self._ok(self._L.axh_set_profiling(self._h, 1 if on else 0))
 
     def set_graph_steps(self, steps: int):
         """Replay a captured HIP graph of `steps` time steps (0: launch every
         kernel directly); results are identical either way."""
-        _check(library().axh_set_graph_steps(self._h, steps))
+        self._ok(self._L.axh_set_graph_steps(self._h, steps))
 
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""
-        return _kernel_stats(library().axh_kernel_stats, self._h)
+        return _kernel_stats(self._L.axh_kernel_stats, self._h)
 
 
 
@@ -352,17 +364,24 @@ class Batch:
     def __init__(self, solvers, group_width: int | None = None, graph_steps: int = 0):
         self.solvers = list(solvers)
         n = len(self.solvers)
+        if len({s.fused for s in self.solvers}) > 1:
+            raise ValueError("batch members must come from one build of the library")
+        self._L = self.solvers[0]._L
+        self.fused = self.solvers[0].fused
         arr = (C.c_void_p * n)(*[s._h for s in self.solvers])
         self._h = C.c_void_p()
-        _check(library().axh_batch_create(arr, n, C.byref(self._h)))
+        self._ok(self._L.axh_batch_create(arr, n, C.byref(self._h)))
         if group_width is not None:
-            _check(library().axh_batch_set_group_width(self._h, group_width))
+            self._ok(self._L.axh_batch_set_group_width(self._h, group_width))
         if graph_steps:
             self.set_graph_steps(graph_steps)
 
+    def _ok(self, rc):
+        _check(rc, self._L)
+
     def close(self):
         if self._h:
-            library().axh_batch_destroy(self._h)
+            self._L.axh_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -389,16 +408,16 @@ class Batch:
         it1 = top if nsteps is None else min(top, self.iter + nsteps)
         outs, ptrs = [], []
         for s in self.solvers:
-            n = library().axh_seis_samples(s._h, it0, it1)
+            n = s._L.axh_seis_samples(s._h, it0, it1)
             a = np.zeros((max(n, 1), max(s.dims["num_rec"], 1), 3), np.float32)
             outs.append((a, n))
             ptrs.append(a.ctypes.data_as(C.c_void_p))
         arr = (C.c_void_p * len(ptrs))(*ptrs)
-        rc = library().axh_batch_run(self._h, it0, it1, arr)
+        rc = self._L.axh_batch_run(self._h, it0, it1, arr)
         if rc in (AXH_OK, AXH_ERR_BLOWUP):
             for s in self.solvers:
                 s.iter = it1
-        _check(rc)
+        self._ok(rc)
         return [a[:n] for a, n in outs]
 
     def step(self, nsteps: int) -> list:
@@ -407,33 +426,33 @@ class Batch:
         member."""
         outs, ptrs = [], []
         for s in self.solvers:
-            n = library().axh_seis_samples(s._h, s.iter + 1, s.iter + nsteps)
+            n = s._L.axh_seis_samples(s._h, s.iter + 1, s.iter + nsteps)
             a = np.zeros((max(n, 1), max(s.dims["num_rec"], 1), 3), np.float32)
             outs.append((a, n))
             ptrs.append(a.ctypes.data_as(C.c_void_p))
         arr = (C.c_void_p * len(ptrs))(*ptrs)
-        rc = library().axh_batch_step(self._h, nsteps, arr)
+        rc = self._L.axh_batch_step(self._h, nsteps, arr)
         if rc in (AXH_OK, AXH_ERR_BLOWUP):
             for s in self.solvers:
                 s.iter += nsteps
-        _check(rc)
+        self._ok(rc)
         return [a[:n] for a, n in outs]
 
     def last_run_ms(self) -> float:
         v = C.c_double()
-        _check(library().axh_batch_last_run_ms(self._h, C.byref(v)))
+        self._ok(self._L.axh_batch_last_run_ms(self._h, C.byref(v)))
         return v.value
 
     def set_profiling(self, on: bool):
-        _check(library().axh_batch_set_profiling(self._h, 1 if on else 0))
+        self._ok(self._L.axh_batch_set_profiling(self._h, 1 if on else 0))
 
     def set_graph_steps(self, steps: int):
         """Solver.set_graph_steps for every group of the batch."""
-        _check(library().axh_batch_set_graph_steps(self._h, steps))
+        self._ok(self._L.axh_batch_set_graph_steps(self._h, steps))
 
     def kernel_stats(self) -> dict:
         """Per kernel class: (total ms, launches) since profiling was enabled."""
-        return _kernel_stats(library().axh_batch_kernel_stats, self._h)
+        return _kernel_stats(self._L.axh_batch_kernel_stats, self._h)
 
 
 # element counts of the arrays axh_meshdb_array returns: name -> (dtype, size)

@@ -152,23 +152,30 @@ def serving_stream(nsolves: int, rate_per_s: float, seed: int = 0) -> list:
 
 
 def serving_leg(workdir: str, nsolves: int, rate: float, chunk: int) -> dict:
-    """The request loop on the serving stream: states recorded first (in
-    parallel), then the requests submitted with their arrival times."""
-    from axisem_amd.serve import RequestLoop
+    """The request loop on the serving stream.  Requests arrive as run
+    configurations and are prepared by the loop on arrival (one drop-in host
+    kept running per mesh and source type), so time to seismogram includes
+    the setup; the first request of a mesh and source type pays the full
+    setup, later ones the source routines only."""
+    from axisem_amd.serve import Preparer, RequestLoop
     stream = serving_stream(nsolves, rate)
-    jobs = [(os.path.join(workdir, "req%02d" % i), cfg, os.path.join(REPO, "data", "meshes", m))
-            for i, (_, m, cfg) in enumerate(stream)]
-    with ThreadPoolExecutor(max_workers=8) as ex:
-        paths = list(ex.map(lambda j: prepare_state(*j), jobs))
-    loop = RequestLoop(hbm_budget_bytes=200e9, chunk_steps=chunk)
-    for (arr, _, _), p in zip(stream, paths):
-        loop.submit(p, arrival_s=arr)
-    st = loop.run()
+    if os.path.isdir(workdir):
+        shutil.rmtree(workdir)
+    prep = Preparer(HOST, os.path.join(workdir, "prep"))
+    loop = RequestLoop(hbm_budget_bytes=200e9, chunk_steps=chunk, preparer=prep)
+    try:
+        for arr, m, cfg in stream:
+            loop.submit_config(os.path.join(REPO, "data", "meshes", m), cfg, arrival_s=arr)
+        st = loop.run()
+    finally:
+        prep.close()
     st["solves"] = nsolves
     st["arrival_rate_per_s"] = rate
     st["chunk_steps"] = chunk
     st["mix"] = ["%s %s %.0fkm %.0fs" % (m, c.source_type, c.source_depth_km, c.seismogram_length_s)
                  for _, m, c in stream]
+    st["setup_s"] = [round(r.setup_s, 2) if r.setup_s is not None else None
+                     for r in loop.requests]
     return st
 
 
@@ -445,6 +452,79 @@ def mt_leg(workdir: str, steps: int, warmup: int) -> dict:
     return out
 
 
+# The large single-GPU configuration: prem_ani (transversely isotropic
+# PREM) at 2 s, anelastic 5 SLS, symplec4, the four moment-tensor runs at
+# 100 km batched, 1800 s at 181 receivers.  Opt-in (--config5): the mesh
+# (2.5 M elements) and the three setups (one per source type) take the host
+# about ten minutes.
+C5_MESH = os.path.join(REPO, "data", "meshes", "prem_ani_2s")
+
+
+def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
+    from axisem_amd.solver import Batch, Solver
+    t0 = time.time()
+    if not os.path.exists(os.path.join(C5_MESH, "meshdb.dat0000")):
+        inputs.run_mesher(os.path.join(REPO, "oracle", "_ref", "xmesh"), C5_MESH, 2.0,
+                          "prem_ani")
+    tmesh = time.time() - t0
+    log("config 5: mesh in %.0f s" % tmesh)
+
+    def cfg(run):
+        return inputs.RunConfig(source_type=run, source_depth_km=100.0, source_amplitude=1e20,
+                                time_scheme="symplec4", attenuation=True,
+                                seismogram_length_s=1800.0,
+                                stations_deg=[float(k) for k in range(181)])
+    kinds = {"monopole": ("mrr", "mtt_p_mpp"), "dipole": ("mtr",), "quadpole": ("mtp",)}
+    os.makedirs(workdir, exist_ok=True)
+    paths = {r: os.path.join(workdir, r + ".bin") for rs in kinds.values() for r in rs}
+    t1 = time.time()
+
+    def rec(item):
+        kind, runs = item
+        inputs.record_states(HOST, os.path.join(workdir, "run_" + kind), C5_MESH,
+                             [cfg(r) for r in runs], [paths[r] for r in runs])
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        list(ex.map(rec, kinds.items()))
+    tsetup = time.time() - t1
+    log("config 5: states in %.0f s" % tsetup)
+    d = read_dims(paths["mtr"])
+    gll = 25 * (d["nel_solid"] + d["nel_fluid"])
+    out = {"workload": "prem_ani 2 s, NPOL 4, anelastic 5 SLS (cg4), symplec4, moment-tensor "
+                       "runs mrr, mtt_p_mpp, mtr, mtp at 100 km, 1800 s, 181 receivers",
+           "nel_solid": d["nel_solid"], "nel_fluid": d["nel_fluid"],
+           "niter_full_run": d["niter"], "deltat": d["deltat"], "steps": steps,
+           "host_mesh_s": tmesh, "host_setup_s": tsetup}
+    import ctypes as C
+    from axisem_amd import solver as S
+    plain = {}
+    for r in ("mrr", "mtt_p_mpp", "mtr", "mtp"):
+        with Solver.from_state_file(paths[r]) as s:
+            mb, wb = C.c_size_t(), C.c_size_t()
+            S._check(S.library().axh_footprint(s._h, C.byref(mb), C.byref(wb)))
+            out.setdefault("footprint_bytes", {})[r] = {"mesh": mb.value, "wavefield": wb.value}
+            s.set_graph_steps(4)
+            s.run(warmup)
+            t = time.perf_counter()
+            s.run(steps)
+            plain[r] = (time.perf_counter() - t) / steps * 1e3
+            out.setdefault("device_bytes", {})[r] = s.device_bytes()
+    out["plain_ms_per_step"] = plain
+    members = [Solver.from_state_file(paths[r]) for r in ("mrr", "mtt_p_mpp", "mtr", "mtp")]
+    try:
+        with Batch(members, graph_steps=4) as b:
+            b.run(warmup)
+            t = time.perf_counter()
+            b.run(steps)
+            ms = (time.perf_counter() - t) / steps * 1e3
+            out["batch4_device_bytes"] = sum(m.device_bytes() for m in members)
+    finally:
+        for m in members:
+            m.close()
+    out["batch4"] = {"ms_per_step": ms, "value": 4 * gll / (ms * 1e-3),
+                     "unit": "GLL-point updates/s", "vs_plain": sum(plain.values()) / ms}
+    return out
+
+
 # Other source types and time schemes on the 10 s mesh (not the headline
 # workload): plain and 4-wavefield step times, stiffness launch per kernel.
 # name, source, scheme, attenuation, coarse-grained memory variables
@@ -516,9 +596,14 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per replayed HIP graph (default: tuned, at most --warmup)")
     ap.add_argument("--no-p50", action="store_true", help="no 50 s configuration leg")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="no legs with the fused multiply-add build")
     ap.add_argument("--p50-steps", type=int, default=1000, help="timed steps of the 50 s leg")
     ap.add_argument("--no-mt", action="store_true",
                     help="no moment-tensor batch leg (5 s mesh, 4 and 16 wavefields)")
+    ap.add_argument("--config5", action="store_true",
+                    help="also the large single-GPU configuration (prem_ani 2 s; ~10 min "
+                         "of host setup)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1),
                     help="independent reference runs of the all-cores CPU leg")
@@ -643,6 +728,56 @@ def main():
         }
         log("batched kernels: " + json.dumps(batched["kernels_us"]))
 
+    # ---- the same legs with the build that fuses multiply-adds (within the
+    # misfit budget, not bit for bit) ----
+    fused = None
+    if not a.no_fused and os.path.exists(os.path.join(REPO, "axisem_amd", "libaxihip_fma.so")):
+        import numpy as np
+        fused = {}
+        with Solver.from_state_file(state, fused=True) as s:
+            s.set_graph_steps(tuned["graph_steps"])
+            s.run(a.warmup)
+            t0 = time.perf_counter()
+            s.run(a.steps)
+            fwall = time.perf_counter() - t0
+            s.set_profiling(True)
+            s.run(a.profile_steps)
+            fk = kernel_table(s.kernel_stats())
+        fused["plain"] = {"value": gll * a.steps / fwall, "ms_per_step": fwall / a.steps * 1e3,
+                          "kernels_us": fk}
+        if batched is not None:
+            members = [Solver.from_state_file(p, fused=True) for p in states]
+            with Batch(members, group_width=btuned["group_width"],
+                       graph_steps=btuned["graph_steps"]) as b:
+                b.run(a.warmup)
+                t0 = time.perf_counter()
+                b.run(a.steps)
+                fbwall = time.perf_counter() - t0
+                b.set_profiling(True)
+                b.run(a.profile_steps)
+                fbk = kernel_table(b.kernel_stats())
+            for m in members:
+                m.close()
+            fbsk = fbk["stiffness"]["avg_us"] * 1e-6
+            fbtotal = nbatch * gll * a.steps / fbwall
+            fused["batched"] = {
+                "value": fbtotal, "vs_plain": fbtotal / value,
+                "ms_per_step": fbwall / a.steps * 1e3,
+                "roofline": {"kernel": "stiffness, %d wavefields per launch" % nbatch,
+                             "achieved": bsk_bytes / fbsk / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": bsk_bytes / fbsk / 1e9 / HBM_PEAK_GBS,
+                             "avg_launch_us": fbsk * 1e6},
+                "kernels_us": fbk}
+        # misfit of the fields after 100 steps against the exact build
+        disp = []
+        for fl in (False, True):
+            with Solver.from_state_file(state, fused=fl) as s:
+                s.run(100)
+                disp.append(s.fields()[0].astype(np.float64))
+        fused["disp_rel_l2_vs_exact_100_steps"] = float(
+            np.linalg.norm(disp[1] - disp[0]) / np.linalg.norm(disp[0]))
+        log("fused build: " + json.dumps({k: v for k, v in fused.items()}))
+
     p50 = None
     if not a.no_p50:
         log("50 s configuration: %d steps" % a.p50_steps)
@@ -653,6 +788,13 @@ def main():
     if not a.no_mt:
         log("moment-tensor batches on the 5 s mesh")
         mt = mt_leg(os.path.join(REPO, "data", "bench_5s"), max(a.steps, 20), max(a.warmup, 5))
+
+    c5 = None
+    if a.config5:
+        log("config 5: prem_ani 2 s")
+        c5 = config5_leg(os.path.join(REPO, "data", "bench_c5"), max(a.steps, 10),
+                         max(a.warmup, 5))
+        log("config 5: " + json.dumps(c5))
 
     serving = None
     if a.serve_solves > 0:
@@ -707,8 +849,10 @@ def main():
         "latency": latency,
         "kernels_us": kernels,
         "batched": batched,
+        "fused_multiply_add_build": fused,
         "plain_50s": p50,
         "moment_tensor_5s": mt,
+        "prem_ani_2s": c5,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -52,6 +52,11 @@ class FakeBackend:
     def key(self, sol):
         return (hash(sol.mesh) & 0xffffffff, 0)
 
+    def batch_key(self, sol):
+        # "<mesh>:<source type>": one device mesh part per type, one batch
+        # for every type on a mesh
+        return (hash(sol.mesh.split(":")[0]) & 0xffffffff, 1)
+
     def footprint(self, sol):
         return sol.meshb, sol.wave
 
@@ -120,3 +125,16 @@ def test_max_members_per_batch():
     st = loop.run()
     assert st["completed"] == 5 and st["max_cobatched"] == 2
     assert all(e[2] <= 2 for e in be.log if e[0] == "batch")
+
+
+def test_source_types_of_one_mesh_share_a_batch():
+    """The four runs of a moment tensor (different source types: separate
+    device mesh parts) are stepped as one batch."""
+    be = FakeBackend()
+    loop = RequestLoop(hbm_budget_bytes=1000, chunk_steps=10, backend=be)
+    for t in ("mono", "mono", "di", "quad"):
+        loop.submit("mesh=A:%s;niter=20;wave=10;meshb=100" % t, 0.0)
+    st = loop.run()
+    assert st["completed"] == 4 and st["max_cobatched"] == 4
+    assert ("batch", ("A:mono", "A:mono", "A:di", "A:quad"), 4) in be.log
+    assert sorted(x[1] for x in be.log if x[0] == "mesh_get") == ["A:di", "A:mono", "A:quad"]
