@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -1240,14 +1241,21 @@ __device__ __forceinline__ void rec_sample(const RecArgs &A, int r, int b) {
 // and sums the new values over the SLS (from zero, in SLS order) as it goes.
 // The products ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j of the element are
 // formed once per element, exactly as the per-entry expressions group them.
-// Dynamic LDS: the tile's memory variables, 10 * 24 * nsls * B floats.
+// Groups (B > 1): the lane's memory variables (NR entries of every SLS,
+// MS >= nsls) are requested into registers before anything else, so their
+// latency overlaps the displacement gather and the strain sources, and the
+// workgroup needs no LDS for them (more workgroups per CU).  One wavefield:
+// the tile's memory variables are staged in dynamic LDS (10 * 24 * nsls
+// floats); measured, this is the faster form at B = 1 (36 vs 43 us per step
+// on the 10 s mesh), the registers at B = 4 (127 vs 158 us).
 // Blocks past the solid tiles take the step's receiver samples (R; nothing
 // in the update writes the displacement they read), which saves the step a
 // launch.
-template <int SRC, int B>
+template <int SRC, int B, int MS>
 __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
   constexpr int NR = (24 * B + 24) / 25;  // entries per lane
+  constexpr bool REG = B > 1;             // memory variables in registers
   extern __shared__ __align__(16) float sMv[];
   __shared__ __align__(16) float sU[NCOMP * 256 * B];
   __shared__ __align__(16) float sG[105];
@@ -1267,10 +1275,24 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
   const int nsls = A.nsls;
   const int n = 24 * nsls * B;
   const int eb = le * 25 * B;
+  float *memvar = A.memvar + (size_t)(valid ? e : 0) * n;
+  float mvr[REG ? NR : 1][REG ? MS : 1];
+  if constexpr (REG) {
+    if (valid) {
+#pragma unroll
+      for (int w = 0; w < NR; w++) {
+        const int r = k + 25 * w;
+        if (r >= 24 * B) continue;
+#pragma unroll
+        for (int jj = 0; jj < MS; jj++)
+          if (jj < nsls) mvr[w][jj] = memvar[jj * 24 * B + r];
+      }
+    }
+  }
   stage_g(A.G, sG);
   gather_disp<NCOMP, B>(A, valid, valid ? A.gid[slot] : 0, t, sU);
-  float *mv = &sMv[le * n];
-  {
+  const float *mv = &sMv[le * n];
+  if constexpr (!REG) {
     // the tile's memory variables are one contiguous block (n is a multiple
     // of 24): copied by the whole workgroup in 16-byte pieces
     const int e0 = tile * kElemPerTile;
@@ -1298,7 +1320,6 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
   }
   __syncthreads();
   if (!valid) return;
-  float *memvar = A.memvar + (size_t)e * n;
   // per entry: its sources (converted once) and its running SLS sum; the SLS
   // loop is outside so that the lane's entries proceed side by side
   double sd[NR], od[NR], st[NR], ot[NR];
@@ -1318,7 +1339,9 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
       ot[w] = (double)old[q * B + b];
     }
   }
-  for (int jj = 0; jj < nsls; jj++) {
+#pragma unroll
+  for (int jj = 0; jj < MS; jj++) {
+    if (jj >= nsls) break;
     const double E = A.expw[jj];
     const double c0 = sCf[le][jj][0], c1 = sCf[le][jj][1], c2 = sCf[le][jj][2],
                  c3 = sCf[le][jj][3];
@@ -1329,11 +1352,11 @@ __global__ __launch_bounds__(256) void k_memvar(SolidArgs A, RecArgs R) {
       const int m = jj * 24 * B + r;
       if (!upd[w]) {
         // not updated for a monopole source: summed as they are
-        rs[w] = rs[w] + mv[m];
+        rs[w] = rs[w] + (REG ? mvr[REG ? w : 0][REG ? jj : 0] : mv[m]);
         continue;
       }
       const int c = (r / B) >> 2;
-      const double mvo = (double)mv[m];
+      const double mvo = (double)(REG ? mvr[REG ? w : 0][REG ? jj : 0] : mv[m]);
       const float db = (float)(c0 * sd[w] + c1 * od[w]);
       float nv;
       if (c < 3) {
@@ -2068,6 +2091,82 @@ __global__ __launch_bounds__(256) void k_asm_fluid(AsmFluidArgs A) {
 template <int B>
 __global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
   rec_sample<B>(A, blockIdx.x * 256 + threadIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel-wavefield snapshots, dump_type displ_only (dump_disp_global,
+// SOLVER/wavefields_io.f90:1019-1097, with kwf_mapping_sol / kwf_mapping_flu,
+// :743-781), one wavefield: at the steps with mod(iter, strain_it) == 0,
+// snapshot q = iter / strain_it goes to slot mod(q, cap) of the ring, as
+// [ncomp][npt] (monopole s, z; else s, phi, z).  A lane per point (grid
+// stride): solid point k reads the displacement of its global point (sol_g,
+// -1 for a point of a source element, zeroed), fluid point k the chi values
+// of its row and column in its element and forms the fluid displacement
+//   mxm1 = sum_k GT(i,k) chi(k,j)   (GT = G1T on axial elements, else G2T)
+//   mxm2 = sum_k chi(i,k) G2(k,j)   (both summed from zero in k order)
+//   s = inv_rho (dzdeta mxm1 + dzdxi mxm2), z = inv_rho (dsdeta mxm1 + dsdxi mxm2)
+// as axisym_gradient_fluid (SOLVER/pointwise_derivatives.f90:509-544) and
+// the product with inv_rho_fluid (wavefields_io.f90:1087-1089) evaluate it.
+// ---------------------------------------------------------------------------
+struct KwfArgs {
+  int nps, npt, ncomp, dipole, strain_it, cap, nglob_s;
+  const int *sol_g;      // [nps] global solid point, or -1 (zeroed)
+  const int *flu_g;      // [npf][10]: chi(0..4, j), then chi(i, 0..4)
+  const int *flu_ij;     // [npf]: i | j << 3 | axial << 6
+  const float *flu_c;    // [5][npf]: inv_rho, DsDeta, DzDeta, DsDxi, DzDxi (over J)
+  const float *G;        // G0 | G1 | G1T | G2 | G2T
+  const float *disp;     // [3][nglob_s]
+  const float *chi;      // [nglob_f]
+  float *ring;           // [cap][ncomp][npt]
+  StepRef step;
+};
+
+__global__ __launch_bounds__(256) void k_kwf(KwfArgs A) {
+  const int iter = step_of(A.step, 0);
+  if (iter % A.strain_it != 0) return;  // uniform: not a snapshot step
+  float *o = A.ring + (size_t)((iter / A.strain_it) % A.cap) * A.ncomp * A.npt;
+  const int npf = A.npt - A.nps;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < A.npt; k += gridDim.x * 256) {
+    float s, p, z;
+    if (k < A.nps) {
+      const int g = A.sol_g[k];
+      if (g < 0) {
+        s = p = z = 0.0f;
+      } else {
+        const float u1 = A.disp[g], u2 = A.disp[(size_t)A.nglob_s + g],
+                    u3 = A.disp[2 * (size_t)A.nglob_s + g];
+        if (A.dipole) { s = u1 + u2; p = u1 - u2; z = u3; }
+        else { s = u1; p = u2; z = u3; }
+      }
+    } else {
+      const int q = k - A.nps;
+      const int ij = A.flu_ij[q];
+      const int i = ij & 7, j = (ij >> 3) & 7;
+      const float *GT = (ij >> 6) ? A.G + 30 : A.G + 80;
+      const float *G2 = A.G + 55;
+      const int *id = A.flu_g + (size_t)q * 10;
+      float m1 = 0.0f, m2 = 0.0f;
+#pragma unroll
+      for (int kk = 0; kk < 5; kk++) m1 = m1 + GT[i + 5 * kk] * A.chi[id[kk]];
+#pragma unroll
+      for (int kk = 0; kk < 5; kk++) m2 = m2 + A.chi[id[5 + kk]] * G2[kk + 5 * j];
+      const float rho = A.flu_c[q], dsdeta = A.flu_c[(size_t)npf + q],
+                  dzdeta = A.flu_c[2 * (size_t)npf + q], dsdxi = A.flu_c[3 * (size_t)npf + q],
+                  dzdxi = A.flu_c[4 * (size_t)npf + q];
+      const float gs = dzdeta * m1 + dzdxi * m2;
+      const float gz = dsdeta * m1 + dsdxi * m2;
+      s = rho * gs;
+      p = 0.0f;
+      z = rho * gz;
+    }
+    o[k] = s;
+    if (A.ncomp == 3) {
+      o[(size_t)A.npt + k] = p;
+      o[2 * (size_t)A.npt + k] = z;
+    } else {
+      o[(size_t)A.npt + k] = z;
+    }
+  }
 }
 
 // Start of a run: each wavefield's first step and its blow-up flag cleared
@@ -2990,6 +3089,7 @@ struct Runner {
   const void *gfields = nullptr;
   const int *gbase = nullptr;
   int gsteps = 0;
+  uint64_t gkwf = 0;   // kernel-wavefield gather captured in the graph (0: none)
   void init() {
     if (stream) return;
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -3007,6 +3107,90 @@ struct Runner {
   }
 };
 
+// Device buffers a run writes besides the fields: the step base of the
+// group, each wavefield's blow-up flag and its receiver-sample ring of cap
+// samples.  A context's own (run_buffers) or a fixed scratch area.
+struct RunBuffers {
+  int *step_base = nullptr;
+  int *blow[kGroup] = {};
+  float *ring[kGroup] = {};
+  int cap[kGroup] = {};
+};
+
+static RunBuffers run_buffers(Fields &F, Member *const *mb, int B) {
+  RunBuffers r;
+  r.step_base = F.step_base;
+  for (int b = 0; b < B; b++) {
+    r.blow[b] = mb[b]->blow;
+    r.ring[b] = mb[b]->seis_d;
+    r.cap[b] = std::max(mb[b]->seis_cap, 1);
+  }
+  return r;
+}
+
+// The receiver samples of a group's run, per wavefield: sample q (the step
+// q * seis_it) is staged in slot mod(q - 1, cap) of the wavefield's ring;
+// q0 is the last sample before the run, qdone the last one whose step is
+// queued, qout the last one queued for copying out.
+struct GroupTail {
+  int B = 0;
+  Member *mb[kGroup] = {};
+  float *seis[kGroup] = {};
+  RunBuffers rb;
+  int q0[kGroup] = {}, qdone[kGroup] = {}, qout[kGroup] = {};
+  // queue the copy of samples qout+1 .. qdone into seis (at most two pieces)
+  void copy_out(int b, hipStream_t s) {
+    const Member &w = *mb[b];
+    const int n = qdone[b] - qout[b];
+    if (n <= 0) return;
+    if (seis[b]) {
+      const size_t row = 3 * (size_t)w.nrec;
+      const int cap = rb.cap[b];
+      const int slot = qout[b] % cap;
+      const int first = std::min(n, cap - slot);
+      float *dst = seis[b] + (size_t)(qout[b] - q0[b]) * row;
+      HIPCHECK(hipMemcpyAsync(dst, rb.ring[b] + slot * row, first * row * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+      if (n > first)
+        HIPCHECK(hipMemcpyAsync(dst + first * row, rb.ring[b], (n - first) * row * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+    }
+    qout[b] = qdone[b];
+  }
+  // steps it+1 .. it+n of wavefield b are about to be queued: their samples
+  // must fit the ring beside those not yet copied out (stream order keeps a
+  // copy ahead of the kernels that refill its slots)
+  void reserve(int b, int it, int n, hipStream_t s) {
+    const Member &w = *mb[b];
+    if (w.nrec <= 0) return;
+    const int q1 = (it + n) / std::max(w.seis_it, 1);
+    if (q1 - qout[b] > rb.cap[b]) copy_out(b, s);
+    qdone[b] = q1;
+  }
+};
+
+// The kernel-wavefield gather of a context (axh_run_kwf): the point tables of
+// k_kwf and its snapshot ring, for one grid, strain_it and source mask.
+struct KwfDev {
+  DevAlloc mem;
+  int nps = 0, npf = 0, ncomp = 2, strain_it = 0, src_mask = 0, ring_req = 0;
+  int cap = 2, half = 1;   // ring slots; snapshots per pinned staging half
+  uint64_t gen = 0;  // distinct for every build (keys the captured graph)
+  int *sol_g = nullptr, *flu_g = nullptr, *flu_ij = nullptr;
+  float *flu_c = nullptr, *ring = nullptr;
+  float *stage = nullptr;  // pinned host memory, 2 * half snapshots
+  hipEvent_t ev[2] = {};
+  KwfDev() = default;
+  KwfDev(const KwfDev &) = delete;
+  KwfDev &operator=(const KwfDev &) = delete;
+  ~KwfDev() {
+    // teardown: errors are not actionable here
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stage) (void)hipHostFree(stage);
+  }
+};
+
 // One context on the device: the (shared) mesh part, its source data, its own
 // fields (B = 1) and, while a batch steps it, where its state lives instead.
 struct Device {
@@ -3015,6 +3199,9 @@ struct Device {
   Fields own;
   Fields *live = nullptr;  // a batch group holding the state, or NULL
   int live_slot = 0;
+  GroupTail pending;       // a run queued by axh_run_begin, until axh_run_end
+  int32_t pending_it1 = 0;
+  std::unique_ptr<KwfDev> kwf;  // built by the first axh_run_kwf
   Runner run;  // declared last: its stream is drained before buffers are freed
 };
 
@@ -3043,7 +3230,9 @@ void BatchStateDeleter::operator()(BatchState *b) const { delete b; }
 void device_destroy(Device *d) { delete d; }
 
 size_t device_bytes(const Device *d) {
-  return d ? d->mesh->mem.bytes + d->member.mem.bytes + d->own.mem.bytes : 0;
+  return d ? d->mesh->mem.bytes + d->member.mem.bytes + d->own.mem.bytes +
+                 (d->kwf ? d->kwf->mem.bytes : 0)
+           : 0;
 }
 
 const char *kernel_class_name(int k) { return (k >= 0 && k < KC_N) ? kClassName[k] : nullptr; }
@@ -3224,80 +3413,27 @@ static void dispatch_anel_full(const MeshDev &M, hipStream_t s, const AnelFullAr
     hipLaunchKernelGGL((k_anel_full<AXH_QUADPOLE, STIFF, UPD, B>), grid, blk, 0, s, A);
 }
 
+// the SLS capacity of the kernel: 5 (the reference's default number of
+// linear solids) or kMaxSls
+template <int B, int MS>
+static void dispatch_memvar_ms(const MeshDev &M, hipStream_t s, const SolidArgs &A,
+                               const RecArgs &R, int nrec_blk) {
+  const dim3 grid(M.ntile_s + nrec_blk);
+  // one wavefield stages the tile's memory variables in dynamic LDS
+  const size_t lds = B > 1 ? 0 : sizeof(float) * kElemPerTile * 24 * M.nsls;
+  if (M.src == AXH_MONOPOLE)
+    hipLaunchKernelGGL((k_memvar<AXH_MONOPOLE, B, MS>), grid, dim3(256), lds, s, A, R);
+  else if (M.src == AXH_DIPOLE)
+    hipLaunchKernelGGL((k_memvar<AXH_DIPOLE, B, MS>), grid, dim3(256), lds, s, A, R);
+  else
+    hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B, MS>), grid, dim3(256), lds, s, A, R);
+}
 template <int B>
 static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A,
                             const RecArgs &R, int nrec_blk) {
-  const size_t lds = sizeof(float) * kElemPerTile * 24 * M.nsls * B;
-  const dim3 grid(M.ntile_s + nrec_blk);
-  if (M.src == AXH_MONOPOLE)
-    hipLaunchKernelGGL((k_memvar<AXH_MONOPOLE, B>), grid, dim3(256), lds, s, A, R);
-  else if (M.src == AXH_DIPOLE)
-    hipLaunchKernelGGL((k_memvar<AXH_DIPOLE, B>), grid, dim3(256), lds, s, A, R);
-  else
-    hipLaunchKernelGGL((k_memvar<AXH_QUADPOLE, B>), grid, dim3(256), lds, s, A, R);
+  if (M.nsls <= 5) dispatch_memvar_ms<B, 5>(M, s, A, R, nrec_blk);
+  else dispatch_memvar_ms<B, kMaxSls>(M, s, A, R, nrec_blk);
 }
-
-// Device buffers a run writes besides the fields: the step base of the
-// group, each wavefield's blow-up flag and its receiver-sample ring of cap
-// samples.  A context's own (run_buffers) or a fixed scratch area.
-struct RunBuffers {
-  int *step_base = nullptr;
-  int *blow[kGroup] = {};
-  float *ring[kGroup] = {};
-  int cap[kGroup] = {};
-};
-
-static RunBuffers run_buffers(Fields &F, Member *const *mb, int B) {
-  RunBuffers r;
-  r.step_base = F.step_base;
-  for (int b = 0; b < B; b++) {
-    r.blow[b] = mb[b]->blow;
-    r.ring[b] = mb[b]->seis_d;
-    r.cap[b] = std::max(mb[b]->seis_cap, 1);
-  }
-  return r;
-}
-
-// The receiver samples of a group's run, per wavefield: sample q (the step
-// q * seis_it) is staged in slot mod(q - 1, cap) of the wavefield's ring;
-// q0 is the last sample before the run, qdone the last one whose step is
-// queued, qout the last one queued for copying out.
-struct GroupTail {
-  int B = 0;
-  Member *mb[kGroup] = {};
-  float *seis[kGroup] = {};
-  RunBuffers rb;
-  int q0[kGroup] = {}, qdone[kGroup] = {}, qout[kGroup] = {};
-  // queue the copy of samples qout+1 .. qdone into seis (at most two pieces)
-  void copy_out(int b, hipStream_t s) {
-    const Member &w = *mb[b];
-    const int n = qdone[b] - qout[b];
-    if (n <= 0) return;
-    if (seis[b]) {
-      const size_t row = 3 * (size_t)w.nrec;
-      const int cap = rb.cap[b];
-      const int slot = qout[b] % cap;
-      const int first = std::min(n, cap - slot);
-      float *dst = seis[b] + (size_t)(qout[b] - q0[b]) * row;
-      HIPCHECK(hipMemcpyAsync(dst, rb.ring[b] + slot * row, first * row * sizeof(float),
-                              hipMemcpyDeviceToHost, s));
-      if (n > first)
-        HIPCHECK(hipMemcpyAsync(dst + first * row, rb.ring[b], (n - first) * row * sizeof(float),
-                                hipMemcpyDeviceToHost, s));
-    }
-    qout[b] = qdone[b];
-  }
-  // steps it+1 .. it+n of wavefield b are about to be queued: their samples
-  // must fit the ring beside those not yet copied out (stream order keeps a
-  // copy ahead of the kernels that refill its slots)
-  void reserve(int b, int it, int n, hipStream_t s) {
-    const Member &w = *mb[b];
-    if (w.nrec <= 0) return;
-    const int q1 = (it + n) / std::max(w.seis_it, 1);
-    if (q1 - qout[b] > rb.cap[b]) copy_out(b, s);
-    qdone[b] = q1;
-  }
-};
 
 // Queues on the stream of R the advance of each of the B wavefields of F
 // (member data mb[b], dims of member 0) by nsteps, wavefield b through its
@@ -3312,10 +3448,80 @@ struct GroupTail {
 // captured once into a graph of the group (ending with the advance of the
 // step base by S) that is replayed for every S steps of the run; the steps
 // left over are launched directly.  Both give the same results bit for bit.
+// Copy-out of the kernel-wavefield snapshots of a run.  Snapshot q sits in
+// ring slot q mod cap until its copy is queued; copies are queued in the run's
+// stream order (after the steps that wrote them, before those that reuse the
+// slots) into one of two pinned staging halves of H snapshots, from where the
+// host moves them to the caller's buffer while the GPU steps on.  A half is
+// reused once its previous copy has completed and been moved.
+struct KwfTail {
+  int s = 1, cap = 2, H = 1;
+  size_t snap = 0;              // floats per snapshot
+  const float *ring = nullptr;
+  float *stage[2] = {};
+  hipEvent_t ev[2] = {};
+  int pend_q[2] = {}, pend_n[2] = {};
+  int half = 0;
+  float *out = nullptr;         // the caller's buffer, from snapshot first_q on
+  int first_q = 0, qout = -1, qdone = -1;  // copy queued up to qout; steps up to qdone
+
+  void drain(int h) {
+    if (pend_n[h] == 0) return;
+    HIPCHECK(hipEventSynchronize(ev[h]));
+    std::memcpy(out + (size_t)(pend_q[h] - first_q) * snap, stage[h],
+                (size_t)pend_n[h] * snap * sizeof(float));
+    pend_n[h] = 0;
+  }
+  // queue the copy of snapshots qout+1 .. qout+n (n <= H) into the next half
+  void flush(int n, hipStream_t st) {
+    const int h = half;
+    half ^= 1;
+    drain(h);
+    const int q0 = qout + 1, slot = q0 % cap, n1 = std::min(n, cap - slot);
+    HIPCHECK(hipMemcpyAsync(stage[h], ring + (size_t)slot * snap, (size_t)n1 * snap * sizeof(float),
+                            hipMemcpyDeviceToHost, st));
+    if (n > n1)
+      HIPCHECK(hipMemcpyAsync(stage[h] + (size_t)n1 * snap, ring,
+                              (size_t)(n - n1) * snap * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipEventRecord(ev[h], st));
+    pend_q[h] = q0;
+    pend_n[h] = n;
+    qout += n;
+  }
+  // steps it+1 .. it+n are about to be queued: copy out every H snapshots,
+  // and whatever the launch would otherwise overwrite
+  void reserve(int it, int n, hipStream_t st) {
+    const int q1 = (it + n) / s;
+    while (qdone - qout >= H) flush(H, st);
+    while (q1 - qout > cap && qout < qdone) flush(std::min(H, qdone - qout), st);
+    if (q1 - qout > cap) fail(AXH_ERR_STATE, "kernel-wavefield ring too small for one launch");
+    qdone = q1;
+  }
+  // the copies of the rest, queued; complete() moves every pending half
+  void finish(hipStream_t st) {
+    while (qout < qdone) flush(std::min(H, qdone - qout), st);
+  }
+  void complete() {
+    drain(half);
+    drain(half ^ 1);
+  }
+};
+
+// A kernel-wavefield gather to run with the steps of one wavefield; gen
+// identifies its tables and ring (a replayed graph is captured for one gen).
+struct KwfLaunch {
+  KwfArgs args;
+  uint64_t gen = 0;
+  KwfTail *tail = nullptr;
+};
+constexpr int kKwfBlocks = 1024;   // grid of k_kwf (grid-stride over the points)
+
 template <int B>
 static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
                       const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
-                      float *const *seis, const RunBuffers &RB, GroupTail &T) {
+                      float *const *seis, const RunBuffers &RB, GroupTail &T,
+                      const KwfLaunch *kw = nullptr) {
+  if (kw && B != 1) fail(AXH_ERR_STATE, "kernel wavefields are gathered for one wavefield");
   if (F.B != B) fail(AXH_ERR_STATE, "group width mismatch");
   const axh_dims &d = ctx[0]->dims;
   const bool multipole = M.src != AXH_MONOPOLE;
@@ -3426,6 +3632,19 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
   }
   const StepRef sref{RB.step_base, 0};
   PS.step = PF.step = FA.step = AS.step = RA.step = sref;
+  KwfArgs KA{};
+  const int kwf_blocks = kw ? std::max(1, std::min(kKwfBlocks, nblk(kw->args.npt))) : 0;
+  if (kw) {
+    KA = kw->args;
+    KA.step = sref;
+    if (start[0] == 1 && KA.npt > 0) {
+      // the initial fields (iter 0), as dump_stuff(0, ...) before the loop
+      KA.step.koff = -1;
+      L(KC_REC, [&] {
+        hipLaunchKernelGGL(k_kwf, dim3(kwf_blocks), dim3(256), 0, st_, KA);
+      });
+    }
+  }
   FA.nstages = AS.nstages = M.nstages;
   for (int b = 0; b < B; b++) {
     FA.stf[b] = (newmark && mb[b]->fluid_src) ? mb[b]->stf_d : nullptr;
@@ -3497,9 +3716,18 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
       L(KC_REC, [&] {
         hipLaunchKernelGGL(k_receivers<B>, dim3(nblk(maxrec), B), dim3(256), 0, st_, RA);
       });
+    // kernel-wavefield snapshot of the step (a graph's steps all launch it;
+    // the kernel returns at once where the step is not a snapshot step)
+    if (kw && KA.npt > 0 && (sample_all || (start[0] + step) % KA.strain_it == 0)) {
+      KA.step.koff = koff;
+      L(KC_REC, [&] {
+        hipLaunchKernelGGL(k_kwf, dim3(kwf_blocks), dim3(256), 0, st_, KA);
+      });
+    }
   };
   auto reserve = [&](int step, int n) {
     for (int b = 0; b < B; b++) T.reserve(b, start[b] + step - 1, n, st_);
+    if (kw && kw->tail) kw->tail->reserve(start[0] + step - 1, n, st_);
   };
 
   int step = 0;
@@ -3507,8 +3735,11 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
   bool fits = S > 0 && !R.profile && nsteps >= S;
   for (int b = 0; b < B && fits; b++)
     fits = mb[b]->nrec <= 0 || S / std::max(mb[b]->seis_it, 1) + 1 <= RB.cap[b];
+  if (kw) fits = fits && S / kw->args.strain_it + 1 <= kw->args.cap;
   if (fits) {
-    if (!R.gexec || R.gfields != &F || R.gsteps != S || R.gbase != RB.step_base) {
+    const uint64_t kgen = kw ? kw->gen : 0;
+    if (!R.gexec || R.gfields != &F || R.gsteps != S || R.gbase != RB.step_base ||
+        R.gkwf != kgen) {
       if (R.gexec) HIPCHECK(hipGraphExecDestroy(R.gexec));
       R.gexec = nullptr;
       hipGraph_t g = nullptr;
@@ -3525,6 +3756,7 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
       R.gfields = &F;
       R.gsteps = S;
       R.gbase = RB.step_base;
+      R.gkwf = kgen;
     }
     for (; step + S <= nsteps; step += S) {
       reserve(step, S);
@@ -3568,9 +3800,10 @@ static double group_finish(Runner &R, GroupTail &T, int *blow) {
 
 static void run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
                     const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
-                    float *const *seis, const RunBuffers &RB, GroupTail &T) {
+                    float *const *seis, const RunBuffers &RB, GroupTail &T,
+                    const KwfLaunch *kw = nullptr) {
   switch (F.B) {
-    case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
+    case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, RB, T, kw);
     case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
     case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
     case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
@@ -3578,7 +3811,10 @@ static void run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
   }
 }
 
-void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
+// Queues steps it0..it1 of a context on its stream; device_run_end
+// completes them.  Between the two the host is free (e.g. to write the
+// previous run's samples) and seis must stay untouched.
+void device_run_begin(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   Device &V = *ctx->dev;
   if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
   V.run.init();
@@ -3587,12 +3823,161 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
   Member *mb[1] = {&V.member};
   const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
+  const int32_t start[1] = {it0};
+  V.pending = GroupTail();
+  run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out,
+          run_buffers(V.own, mb, 1), V.pending);
+  V.pending_it1 = it1;
+}
+
+void device_run_end(axh_ctx *ctx) {
+  Device &V = *ctx->dev;
   int blow = INT_MAX;
+  ctx->last_run_ms = group_finish(V.run, V.pending, &blow);
+  ctx->iter_done = V.pending_it1;
+  if (blow != INT_MAX)
+    fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
+}
+
+void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
+  device_run_begin(ctx, it0, it1, seis);
+  device_run_end(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// kernel-wavefield runs (axh_run_kwf)
+// ---------------------------------------------------------------------------
+constexpr size_t kKwfRingBytes = AXH_KWF_RING_BYTES;  // snapshot ring on the device
+constexpr size_t kKwfStageBytes = (size_t)64 << 20;   // one pinned staging half
+
+// The point tables of k_kwf from the context's grid, in point order: the
+// solid points' global ids (-1 in a source element when src_mask), the fluid
+// points' row and column ids with their element terms; and a ring of
+// min(snapshots of the whole run, kKwfRingBytes / snapshot) snapshots, or
+// the caller's ring size.
+static void kwf_build(axh_ctx *ctx, Device &V) {
+  const axh_dims &d = ctx->dims;
+  const MeshDev &M = *V.mesh;
+  const int32_t *np = ctx->need("kwf_npoints").as<int32_t>();
+  const int32_t *mapping = ctx->need("kwf_mapping").as<int32_t>();
+  const int32_t *mask = ctx->need("kwf_mask").as<int32_t>();
+  std::unique_ptr<KwfDev> K(new KwfDev);
+  K->nps = np[0];
+  K->npf = np[1];
+  K->ncomp = d.src_type == AXH_MONOPOLE ? 2 : 3;
+  K->strain_it = ctx->kwf_strain_it;
+  K->src_mask = ctx->kwf_src_mask;
+  const int npt = K->nps + K->npf;
+  // source elements zeroed (src_dump_type 'mask'): ielsrc holds solid
+  // elements unless the source sits in the fluid, where nothing is zeroed
+  std::vector<char> zero(d.nel_solid, 0);
+  if (K->src_mask && !d.fluid_src && d.nelsrc > 0) {
+    const int32_t *ielsrc = ctx->need("ielsrc").as<int32_t>();
+    for (int q = 0; q < d.nelsrc; q++)
+      if (ielsrc[q] >= 1 && ielsrc[q] <= d.nel_solid) zero[ielsrc[q] - 1] = 1;
+  }
+  std::vector<int> sol_g(std::max(K->nps, 1), -1), flu_g(10 * (size_t)std::max(K->npf, 1), 0),
+      flu_ij(std::max(K->npf, 1), 0);
+  std::vector<float> flu_c(5 * (size_t)std::max(K->npf, 1), 0.0f);
+  const float *fc[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (K->npf > 0)
+    for (int q = 0; q < 5; q++) fc[q] = ctx->need(kFluidGradNames[q]).as<float>();
+  const int32_t *axf = d.nel_fluid > 0 ? ctx->need("axis_fluid").as<int32_t>() : nullptr;
+  for (int e = 0; e < d.nel_solid + d.nel_fluid; e++)
+    for (int p = 0; p < kNpp; p++) {
+      const size_t x = (size_t)e * kNpp + p;
+      if (!mask[x]) continue;
+      const int k = mapping[x] - 1;
+      if (e < d.nel_solid) {
+        if (k < 0 || k >= K->nps) fail(AXH_ERR_STATE, "bad kernel-wavefield grid");
+        sol_g[k] = zero[e] ? -1 : M.gid_s_h[slot_of(e, p)];
+      } else {
+        const int ef = e - d.nel_solid, q = k - K->nps;
+        if (q < 0 || q >= K->npf) fail(AXH_ERR_STATE, "bad kernel-wavefield grid");
+        const int i = p % 5, j = p / 5;
+        for (int kk = 0; kk < 5; kk++) {
+          flu_g[(size_t)q * 10 + kk] = M.gid_f_h[slot_of(ef, kk + 5 * j)];
+          flu_g[(size_t)q * 10 + 5 + kk] = M.gid_f_h[slot_of(ef, i + 5 * kk)];
+        }
+        flu_ij[q] = i | (j << 3) | ((axf[ef] ? 1 : 0) << 6);
+        for (int c = 0; c < 5; c++) flu_c[(size_t)c * K->npf + q] = fc[c][(size_t)ef * kNpp + p];
+      }
+    }
+  const size_t snap = (size_t)K->ncomp * std::max(npt, 1);
+  const size_t snap_b = snap * sizeof(float);
+  const int total = d.niter / K->strain_it + 1;
+  // staging halves of up to 64 MiB; the ring holds two halves and the
+  // snapshots of one replayed graph (at least 2 slots: a run from step 1
+  // takes the initial snapshot and the first), at most 1 GiB
+  if (ctx->kwf_ring > 0) {
+    K->cap = ctx->kwf_ring;
+    K->half = (int)std::max<size_t>(1, std::min<size_t>(K->cap / 2, kKwfStageBytes / snap_b));
+  } else {
+    K->half = (int)std::max<size_t>(1, std::min<size_t>(total, kKwfStageBytes / snap_b));
+    const size_t want = 2 * (size_t)K->half + std::min(total, AXH_MAX_GRAPH_STEPS / K->strain_it + 2);
+    K->cap = (int)std::max<size_t>(2, std::min(want, std::max<size_t>(2, kKwfRingBytes / snap_b)));
+  }
+  K->ring_req = ctx->kwf_ring;
+  K->sol_g = K->mem.upload(sol_g);
+  K->flu_g = K->mem.upload(flu_g);
+  K->flu_ij = K->mem.upload(flu_ij);
+  K->flu_c = K->mem.upload(flu_c);
+  K->ring = K->mem.alloc<float>(snap * K->cap);
+  HIPCHECK(hipHostMalloc((void **)&K->stage, 2 * (size_t)K->half * snap_b, hipHostMallocDefault));
+  for (auto &e : K->ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  static std::atomic<uint64_t> builds{0};
+  K->gen = ++builds;
+  V.kwf = std::move(K);
+}
+
+// Steps it0..it1 with the kernel-wavefield snapshots of their steps (and of
+// the initial fields when it0 = 1) written to kwf in step order.  One queued
+// run: the snapshots are copied out through the pinned staging halves while
+// the following steps run (KwfTail).
+void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf) {
+  Device &V = *ctx->dev;
+  if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
+  if (!V.kwf || V.kwf->strain_it != ctx->kwf_strain_it || V.kwf->src_mask != ctx->kwf_src_mask ||
+      V.kwf->ring_req != ctx->kwf_ring)
+    kwf_build(ctx, V);
+  KwfDev &K = *V.kwf;
+  KwfLaunch kl;
+  kl.gen = K.gen;
+  KwfArgs &A = kl.args;
+  A.nps = K.nps; A.npt = K.nps + K.npf; A.ncomp = K.ncomp;
+  A.dipole = V.mesh->src == AXH_DIPOLE; A.strain_it = K.strain_it; A.cap = K.cap;
+  A.nglob_s = V.mesh->nglob_s;
+  A.sol_g = K.sol_g; A.flu_g = K.flu_g; A.flu_ij = K.flu_ij; A.flu_c = K.flu_c;
+  A.G = V.mesh->G; A.disp = V.own.disp; A.chi = V.own.chi; A.ring = K.ring;
+  const size_t snap = (size_t)K.ncomp * A.npt;
+  KwfTail kt;
+  kt.s = K.strain_it;
+  kt.cap = K.cap;
+  kt.H = K.half;
+  kt.snap = snap;
+  kt.ring = K.ring;
+  kt.stage[0] = K.stage;
+  kt.stage[1] = K.stage + (size_t)K.half * snap;
+  kt.ev[0] = K.ev[0];
+  kt.ev[1] = K.ev[1];
+  kt.out = kwf;
+  kt.first_q = it0 == 1 ? 0 : (it0 + K.strain_it - 1) / K.strain_it;
+  kt.qout = kt.qdone = kt.first_q - 1;
+  if (A.npt > 0) kl.tail = &kt;
+  V.run.init();
+  V.run.profile = ctx->profile;
+  V.run.graph_steps = ctx->graph_steps;
+  Member *mb[1] = {&V.member};
+  const axh_ctx *cx[1] = {ctx};
+  float *out[1] = {seis};
   const int32_t start[1] = {it0};
   GroupTail T;
-  run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out,
-          run_buffers(V.own, mb, 1), T);
+  run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, run_buffers(V.own, mb, 1),
+          T, &kl);
+  if (kl.tail) kt.finish(V.run.stream);
+  int blow = INT_MAX;
   ctx->last_run_ms = group_finish(V.run, T, &blow);
+  if (kl.tail) kt.complete();
   ctx->iter_done = it1;
   if (blow != INT_MAX)
     fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));

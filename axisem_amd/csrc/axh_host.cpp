@@ -19,6 +19,9 @@ const char *const kSolid3dNames[AXH_N_SOLID_3D] = {
     "M_w1", "M_w2", "M_w3", "M_w4", "M_w5"};
 const char *const kFluid3dNames[4] = {"M1chi_fl", "M2chi_fl", "M4chi_fl",
                                       "M_w_fl"};
+const char *const kFluidGradNames[5] = {"inv_rho_fluid", "DsDeta_over_J_flu",
+                                        "DzDeta_over_J_flu", "DsDxi_over_J_flu",
+                                        "DzDxi_over_J_flu"};
 
 [[noreturn]] void fail(int code, const std::string &msg) {
   throw Error{code, msg};
@@ -357,22 +360,53 @@ int axh_seis_samples(const axh_ctx *ctx, int32_t it_begin, int32_t it_end) {
   return it_end / s - (it_begin - 1) / s;
 }
 
+// the context has no run queued by axh_run_begin
+static void check_idle(const axh_ctx *ctx) {
+  if (ctx->running) fail(AXH_ERR_STATE, "a run of the context is queued: call axh_run_end");
+}
+
+static void check_range(const axh_ctx *ctx, int32_t it_begin, int32_t it_end, const char *fn) {
+  if (it_begin != ctx->iter_done + 1)
+    fail(AXH_ERR_ARG, std::string(fn) + " must continue at step " +
+                          std::to_string(ctx->iter_done + 1));
+  if (it_end < it_begin - 1 || it_end > ctx->dims.niter)
+    fail(AXH_ERR_ARG, "step range outside 1..niter");
+}
+
 int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
   return guarded([&] {
     check_ctx(ctx);
-    if (it_begin != ctx->iter_done + 1)
-      fail(AXH_ERR_ARG, "axh_run must continue at step " +
-                            std::to_string(ctx->iter_done + 1));
-    if (it_end < it_begin - 1 || it_end > ctx->dims.niter)
-      fail(AXH_ERR_ARG, "step range outside 1..niter");
+    check_idle(ctx);
+    check_range(ctx, it_begin, it_end, "axh_run");
     if (!ctx->dev) device_build(ctx);
     device_run(ctx, it_begin, it_end, seis);
+  });
+}
+
+int axh_run_begin(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_idle(ctx);
+    check_range(ctx, it_begin, it_end, "axh_run_begin");
+    if (!ctx->dev) device_build(ctx);
+    device_run_begin(ctx, it_begin, it_end, seis);
+    ctx->running = true;
+  });
+}
+
+int axh_run_end(axh_ctx *ctx) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!ctx->running) fail(AXH_ERR_STATE, "no run of the context is queued");
+    ctx->running = false;
+    device_run_end(ctx);
   });
 }
 
 int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis) {
   return guarded([&] {
     check_ctx(ctx);
+    check_idle(ctx);
     if (ctx->dims.num_rec > noalloc_max_receivers())
       fail(AXH_ERR_ARG, "axh_run_noalloc: " + std::to_string(ctx->dims.num_rec) +
                             " receivers; the fixed scratch array holds the samples of at most " +
@@ -392,6 +426,7 @@ int axh_download_fields(axh_ctx *ctx, float *disp, float *velo, float *chi,
                         float *dchi, float *ddchi) {
   return guarded([&] {
     check_ctx(ctx);
+    check_idle(ctx);
     if (!ctx->dev) device_build(ctx);
     device_download(ctx, disp, velo, chi, dchi, ddchi);
   });
@@ -401,6 +436,7 @@ int axh_upload_fields(axh_ctx *ctx, const float *disp, const float *velo,
                       const float *chi, const float *dchi, const float *ddchi) {
   return guarded([&] {
     check_ctx(ctx);
+    check_idle(ctx);
     if (!ctx->dev) device_build(ctx);
     device_upload_fields(ctx, disp, velo, chi, dchi, ddchi);
   });
@@ -524,6 +560,7 @@ int axh_batch_create(axh_ctx *const *members, int32_t n, axh_batch **out) {
     for (int i = 0; i < n; i++) {
       if (!members[i]) fail(AXH_ERR_ARG, "null batch member");
       if (members[i]->batch_refs > 0) fail(AXH_ERR_STATE, "a context is already in a batch");
+      check_idle(members[i]);
       for (int j = 0; j < i; j++)
         if (members[j] == members[i]) fail(AXH_ERR_ARG, "a context appears twice in a batch");
     }
@@ -675,6 +712,170 @@ int axh_mesh_bytes(const axh_mesh *mesh, size_t *bytes) {
 
 int axh_mesh_release(axh_mesh *mesh) {
   return guarded([&] { mesh_release(mesh); });
+}
+
+// ---- kernel wavefields ------------------------------------------------------
+
+int axh_upload_fluid_gradient(axh_ctx *ctx, const float *inv_rho_fluid,
+                              const float *DsDeta_over_J_flu, const float *DzDeta_over_J_flu,
+                              const float *DsDxi_over_J_flu, const float *DzDxi_over_J_flu) {
+  return guarded([&] {
+    check_ctx(ctx);
+    const uint64_t nel = (uint64_t)ctx->dims.nel_fluid;
+    if (nel == 0) return;
+    const float *src[5] = {inv_rho_fluid, DsDeta_over_J_flu, DzDeta_over_J_flu,
+                           DsDxi_over_J_flu, DzDxi_over_J_flu};
+    for (int q = 0; q < 5; q++) {
+      if (!src[q]) fail(AXH_ERR_ARG, "null fluid gradient array");
+      ctx->put(kFluidGradNames[q], kF32, {5, 5, nel}, src[q]);
+    }
+  });
+}
+
+// The numbering of build_kwf_grid (SOLVER/meshes_io.F90:489-610): global
+// point ids as the reference forms them (fluid points 1..nglob_fluid, solid
+// points after them), each numbered where it first appears; only the entry
+// that numbers a point is kept (mask 1).
+static void kwf_grid_build(const axh_ctx *ctx, int ibeg, int iend, int jbeg, int jend,
+                           const int32_t *emask, std::vector<int32_t> &mapping,
+                           std::vector<int32_t> &mask, int &nps, int &npf) {
+  const axh_dims &d = ctx->dims;
+  const size_t nel = (size_t)d.nel_solid + d.nel_fluid;
+  mapping.assign(nel * kNpp, 0);
+  mask.assign(nel * kNpp, 0);
+  std::vector<char> seen((size_t)d.nglob_solid + d.nglob_fluid, 0);
+  int ct = 0;
+  for (int dom = 0; dom < 2; dom++) {
+    const bool solid = dom == 0;
+    const int n = solid ? d.nel_solid : d.nel_fluid;
+    if (n == 0) continue;
+    const int32_t *ig = ctx->need(solid ? "igloc_solid" : "igloc_fluid").as<int32_t>();
+    const int32_t *ax = ctx->need(solid ? "axis_solid" : "axis_fluid").as<int32_t>();
+    const int goff = solid ? d.nglob_fluid : 0;
+    const size_t eoff = solid ? 0 : (size_t)d.nel_solid;
+    const int ng = solid ? d.nglob_solid : d.nglob_fluid;
+    for (int e = 0; e < n; e++) {
+      if (emask && !emask[eoff + e]) continue;
+      auto visit = [&](int i, int j) {
+        const int p = i + 5 * j;
+        const int g = ig[(size_t)e * kNpp + p] - 1;
+        if (g < 0 || g >= ng) fail(AXH_ERR_ARG, "igloc out of range");
+        if (!seen[(size_t)g + goff]) {
+          seen[(size_t)g + goff] = 1;
+          mapping[(eoff + e) * kNpp + p] = ++ct;
+          mask[(eoff + e) * kNpp + p] = 1;
+        }
+      };
+      for (int j = jbeg; j <= jend; j++) {
+        for (int i = ibeg; i <= iend; i++) visit(i, j);
+        if (ax[e]) visit(0, j);   // the axis point of the row
+      }
+    }
+    if (solid) nps = ct;
+  }
+  npf = ct - nps;
+}
+
+static void kwf_put_grid(axh_ctx *ctx, int nps, int npf, const std::vector<int32_t> &mapping,
+                         const std::vector<int32_t> &mask) {
+  const uint64_t n = (uint64_t)mapping.size();
+  const int32_t np[2] = {nps, npf};
+  ctx->put("kwf_mapping", kI32, {n}, mapping.data());
+  ctx->put("kwf_mask", kI32, {n}, mask.data());
+  ctx->put("kwf_npoints", kI32, {2}, np);
+}
+
+int axh_kwf_grid(axh_ctx *ctx, int32_t ibeg, int32_t iend, int32_t jbeg, int32_t jend,
+                 const int32_t *elem_mask, int32_t *npoint_solid, int32_t *npoint_fluid) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (ibeg < 0 || iend > kNpol || ibeg > iend || jbeg < 0 || jend > kNpol || jbeg > jend)
+      fail(AXH_ERR_ARG, "GLL index ranges must lie in 0..npol");
+    std::vector<int32_t> mapping, mask;
+    int nps = 0, npf = 0;
+    kwf_grid_build(ctx, ibeg, iend, jbeg, jend, elem_mask, mapping, mask, nps, npf);
+    kwf_put_grid(ctx, nps, npf, mapping, mask);
+    if (npoint_solid) *npoint_solid = nps;
+    if (npoint_fluid) *npoint_fluid = npf;
+  });
+}
+
+int axh_upload_kwf_grid(axh_ctx *ctx, int32_t npoint_solid, int32_t npoint_fluid,
+                        const int32_t *mapping_ijel_ikwf, const int32_t *kwf_mask) {
+  return guarded([&] {
+    check_ctx(ctx);
+    const axh_dims &d = ctx->dims;
+    const size_t nel = (size_t)d.nel_solid + d.nel_fluid;
+    if (!mapping_ijel_ikwf || !kwf_mask) fail(AXH_ERR_ARG, "null grid array");
+    if (npoint_solid < 0 || npoint_fluid < 0 || (d.nel_fluid == 0 && npoint_fluid > 0))
+      fail(AXH_ERR_ARG, "bad kernel-wavefield point counts");
+    const int npt = npoint_solid + npoint_fluid;
+    std::vector<int32_t> mapping(nel * kNpp, 0), mask(nel * kNpp, 0);
+    std::vector<char> hit((size_t)npt, 0);
+    for (size_t q = 0; q < nel * kNpp; q++) {
+      if (!kwf_mask[q]) continue;
+      const int m = mapping_ijel_ikwf[q];
+      const bool solid = q < (size_t)d.nel_solid * kNpp;
+      const bool ok = solid ? (m >= 1 && m <= npoint_solid)
+                            : (m > npoint_solid && m <= npt);
+      if (!ok || hit[m - 1]) fail(AXH_ERR_ARG, "kwf_mask / mapping_ijel_ikwf do not number "
+                                               "each point once");
+      hit[m - 1] = 1;
+      mapping[q] = m;
+      mask[q] = 1;
+    }
+    for (int k = 0; k < npt; k++)
+      if (!hit[k]) fail(AXH_ERR_ARG, "kernel-wavefield point " + std::to_string(k + 1) +
+                                         " has no masked entry");
+    kwf_put_grid(ctx, npoint_solid, npoint_fluid, mapping, mask);
+  });
+}
+
+int axh_kwf_get_grid(const axh_ctx *ctx, int32_t *mapping_ijel_ikwf, int32_t *kwf_mask,
+                     int32_t *npoint_solid, int32_t *npoint_fluid) {
+  return guarded([&] {
+    check_ctx(ctx);
+    const Array &np = ctx->need("kwf_npoints");
+    if (npoint_solid) *npoint_solid = np.as<int32_t>()[0];
+    if (npoint_fluid) *npoint_fluid = np.as<int32_t>()[1];
+    const Array &mp = ctx->need("kwf_mapping"), &mk = ctx->need("kwf_mask");
+    if (mapping_ijel_ikwf) std::memcpy(mapping_ijel_ikwf, mp.bytes.data(), mp.bytes.size());
+    if (kwf_mask) std::memcpy(kwf_mask, mk.bytes.data(), mk.bytes.size());
+  });
+}
+
+int axh_set_kwf(axh_ctx *ctx, int32_t strain_it, int32_t src_mask, int32_t ring_snapshots) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (strain_it < 1) fail(AXH_ERR_ARG, "strain_it must be >= 1");
+    if (src_mask != 0 && src_mask != 1) fail(AXH_ERR_ARG, "src_mask must be 0 or 1");
+    if (ring_snapshots < 0 || ring_snapshots == 1)
+      fail(AXH_ERR_ARG, "ring_snapshots must be 0 or at least 2");
+    ctx->kwf_strain_it = strain_it;
+    ctx->kwf_src_mask = src_mask;
+    ctx->kwf_ring = ring_snapshots;
+  });
+}
+
+int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end) {
+  if (!ctx || ctx->kwf_strain_it < 1 || it_begin < 1 || it_end < it_begin) return 0;
+  const int s = ctx->kwf_strain_it;
+  return it_end / s - (it_begin - 1) / s + (it_begin == 1 ? 1 : 0);
+}
+
+int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_idle(ctx);
+    check_range(ctx, it_begin, it_end, "axh_run_kwf");
+    if (ctx->kwf_strain_it < 1) fail(AXH_ERR_STATE, "axh_run_kwf needs axh_set_kwf first");
+    ctx->need("kwf_npoints");
+    if (!kwf && axh_kwf_snapshots(ctx, it_begin, it_end) > 0 &&
+        ctx->need("kwf_npoints").as<int32_t>()[0] + ctx->need("kwf_npoints").as<int32_t>()[1] > 0)
+      fail(AXH_ERR_ARG, "null kwf buffer");
+    if (!ctx->dev) device_build(ctx);
+    device_run_kwf(ctx, it_begin, it_end, seis, kwf);
+  });
 }
 
 int axh_convolve_traces(int32_t nt, int32_t ntraces, const float *seis, int32_t nj,

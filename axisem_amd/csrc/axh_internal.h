@@ -82,6 +82,11 @@ struct axh_ctx {
   double last_run_ms = 0.0;
   bool profile = false;
   int graph_steps = 0;    // steps per replayed graph (0: direct launches)
+  bool running = false;   // between axh_run_begin and axh_run_end
+  // kernel-wavefield snapshots of axh_run_kwf (axh_set_kwf)
+  int kwf_strain_it = 0;
+  int kwf_src_mask = 1;
+  int kwf_ring = 0;       // device ring capacity in snapshots (0: automatic)
   int batch_refs = 0;     // batches holding this context
   // device mesh part to use when the device part is built (axh_mesh_bind)
   std::shared_ptr<axh::MeshDev> bound_mesh;
@@ -115,12 +120,17 @@ namespace axh {
 // names of the solid 3-D terms in enum axh_solid_term order
 extern const char *const kSolid3dNames[AXH_N_SOLID_3D];
 extern const char *const kFluid3dNames[4];
+// inv_rho_fluid and the fluid's derivative terms over J (axh_upload_fluid_gradient)
+extern const char *const kFluidGradNames[5];
 
 // device lifecycle (axh_device.hip)
 void device_build(axh_ctx *ctx);
 void device_destroy(Device *d);
 void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_run_begin(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_run_end(axh_ctx *ctx);
 void device_run_noalloc(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
+void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf);
 void convolve_traces(int nt, int ntraces, const float *seis, int nj, const float *stf,
                      double dt, double scale, float *out);
 int noalloc_max_receivers();

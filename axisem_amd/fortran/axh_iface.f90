@@ -135,6 +135,56 @@ module axh_iface
        real(c_float), intent(out) :: seis(*)
      end function
 
+     integer(c_int) function axh_run_begin(ctx, it_begin, it_end, seis) &
+          bind(C, name='axh_run_begin')
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: it_begin, it_end
+       real(c_float) :: seis(*)
+     end function
+
+     integer(c_int) function axh_run_end(ctx) bind(C, name='axh_run_end')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+
+     integer(c_int) function axh_upload_fluid_gradient(ctx, inv_rho, dsdeta, dzdeta, &
+          dsdxi, dzdxi) bind(C, name='axh_upload_fluid_gradient')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: inv_rho(*), dsdeta(*), dzdeta(*), dsdxi(*), dzdxi(*)
+     end function
+
+     integer(c_int) function axh_upload_kwf_grid(ctx, npoint_solid, npoint_fluid, &
+          mapping, mask) bind(C, name='axh_upload_kwf_grid')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: npoint_solid, npoint_fluid
+       integer(c_int32_t), intent(in) :: mapping(*), mask(*)
+     end function
+
+     integer(c_int) function axh_set_kwf(ctx, strain_it, src_mask, ring_snapshots) &
+          bind(C, name='axh_set_kwf')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: strain_it, src_mask, ring_snapshots
+     end function
+
+     integer(c_int) function axh_kwf_snapshots(ctx, it_begin, it_end) &
+          bind(C, name='axh_kwf_snapshots')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: it_begin, it_end
+     end function
+
+     integer(c_int) function axh_run_kwf(ctx, it_begin, it_end, seis, kwf) &
+          bind(C, name='axh_run_kwf')
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: it_begin, it_end
+       real(c_float), intent(out) :: seis(*), kwf(*)
+     end function
+
      integer(c_int) function axh_download_fields(ctx, disp, velo, chi, dchi, ddchi) &
           bind(C, name='axh_download_fields')
        import :: c_int, c_ptr, c_float

@@ -20,6 +20,12 @@
 !!                          line of file f ("<inparam_source file> <state file>"), all
 !!                          on the mesh and model terms set up once (one source type);
 !!                          f may be a pipe that a request loop writes as requests come
+!!   AXH_KWF_STRAIN_IT=<n>  kernel-wavefield dumps (dump_type displ_only) every n steps on
+!!                          the grid of the reference's build_kwf_grid: the values the
+!!                          NetCDF variables disp_s, disp_p, disp_z of a snapshot hold
+!!                          (SOLVER/nc_routines.F90:945-970), appended per snapshot to
+!!                          Data/kwf_disp_<s|p|z>.bin (float32, npoint_kwf per snapshot;
+!!                          no NetCDF library here, so no axisem_output.nc4)
 module time_evol_wave
 
   use global_parameters
@@ -47,6 +53,7 @@ module time_evol_wave
   integer,     save :: iepi = 0, ianti = 0, iequ = 0   !< rows of the generic traces
   logical,     save :: no_run = .false.
   integer,     save :: isnap = 0             !< snapshots written (dump_stuff's isnap)
+  integer,     save :: kwf_it = 0            !< kernel-wavefield interval (0: no dumps)
 
 contains
 
@@ -73,6 +80,12 @@ subroutine prepare_waves
   endif
 
   call check_supported_pre()
+
+  ! kernel-wavefield dumps (AXH_KWF_STRAIN_IT) need the fluid displacement, as
+  ! readin_parameters decides for dump_type displ_only (SOLVER/parameters.F90:214-221):
+  ! read_model_compute_terms then fills inv_rho_fluid
+  lpath = axh_glue_getenv('AXH_KWF_STRAIN_IT'//c_null_char, path, len(path))
+  if (lpath > 0) need_fluid_displ = .true.
 
   call read_sourceparams
   if (rot_src) call def_rot_matrix
@@ -127,8 +140,10 @@ subroutine time_loop
   use data_io,    only: datapath, lfdata
 
   integer                          :: it0, it1, nsamp, k, ierr, chunk, isamp_iter
-  real(kind=realkind), allocatable :: seis(:)
+  integer                          :: nk, ncomp, npt, c
+  real(kind=realkind), allocatable :: seis(:), kwf(:)
   real(kind=dp)                    :: tloc
+  character(len=1)                 :: kname(3)
 
   if (no_run) then
      if (lpr) write(6,*) 'AXH_NO_RUN set: skipping the time loop.'
@@ -156,20 +171,56 @@ subroutine time_loop
   ! step with SAVE_ENERGY)
   chunk = max(1, min(niter, 512 * seis_it))
   if (dump_energy) chunk = 1
+  if (kwf_it > 0) then
+     ! kernel wavefields: chunks of at most 16 snapshots, one file per variable
+     chunk = min(chunk, 16 * kwf_it)
+     npt = npoint_solid_kwf + npoint_fluid_kwf
+     if (src_type(1) == 'monopole') then
+        ncomp = 2
+        kname(1:2) = ['s', 'z']
+     else
+        ncomp = 3
+        kname = ['s', 'p', 'z']
+     endif
+     allocate(kwf(max(1, (chunk / kwf_it + 2) * ncomp * npt)))
+     do c = 1, ncomp
+        open(13200 + c, file=datapath(1:lfdata)//'/kwf_disp_'//kname(c)//'.bin', &
+             access='stream', form='unformatted', status='replace')
+     enddo
+  endif
   allocate(seis(3 * max(nrec_all, 1) * (chunk / seis_it + 1)))
   tloc = zero
   isamp_iter = 0
   it0 = 1
+  if (.not. (dump_vtk .or. dump_xdmf .or. dump_energy) .and. kwf_it == 0) then
+     ! nothing needs the fields between chunks: the next chunk runs on the GPU
+     ! while this one's samples are written
+     call time_loop_overlapped(seis, chunk, tloc)
+     it0 = niter + 1
+  endif
   do while (it0 <= niter)
      it1 = min(niter, it0 + chunk - 1)
      if (dump_vtk .or. dump_xdmf) it1 = min(it1, (it0 + snap_it - 1) / snap_it * snap_it)
-     ierr = axh_run(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis)
+     if (kwf_it > 0) then
+        nk = axh_kwf_snapshots(ctx, int(it0, c_int32_t), int(it1, c_int32_t))
+        ierr = axh_run_kwf(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis, kwf)
+     else
+        ierr = axh_run(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis)
+     endif
      if (ierr == AXH_ERR_BLOWUP) then
         write(6,*) procstrg, '!!!!!!!!!!!!!!! DISPLACEMENTS BLEW UP !!!!!!!!!!!!!!!'
         write(6,*) procstrg, trim(axh_error_string())
         stop
      endif
      call axh_check(ierr, 'axh_run')
+     ! the chunk's kernel-wavefield snapshots, variable by variable
+     if (kwf_it > 0) then
+        do k = 1, nk
+           do c = 1, ncomp
+              write(13200 + c) kwf(1 + ((k - 1) * ncomp + c - 1) * npt:((k - 1) * ncomp + c) * npt)
+           enddo
+        enddo
+     endif
      ! replay the reference's time accumulation t = t + deltat for the sampled steps
      nsamp = 0
      do k = it0, it1
@@ -192,6 +243,11 @@ subroutine time_loop
      it0 = it1 + 1
   enddo
   t = tloc
+  if (kwf_it > 0) then
+     do c = 1, ncomp
+        close(13200 + c)
+     enddo
+  endif
 
   iclockold = tick(id=idold, since=iclockold)
 
@@ -354,6 +410,65 @@ end subroutine write_samples
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------
+!> The chunks of the time loop with two sample buffers: chunk k + 1 is queued on the GPU
+!! (axh_run_begin) before the samples of chunk k are written, and collected
+!! (axh_run_end) after, so the writes of the reference's seismogram files overlap the
+!! time steps.  Same files as one chunk after the other.
+subroutine time_loop_overlapped(seis, chunk, tloc)
+
+  real(kind=realkind), allocatable, intent(inout) :: seis(:)
+  integer, intent(in)                             :: chunk
+  real(kind=dp), intent(inout)                    :: tloc
+
+  real(kind=realkind), allocatable :: next(:), tmp(:)
+  integer                          :: it0, it1, n0, n1, k, nsamp
+  integer(c_int)                   :: ierr
+
+  allocate(next(size(seis)))
+  it0 = 1
+  it1 = min(niter, chunk)
+  call axh_check(axh_run_begin(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis), &
+                 'axh_run_begin')
+  do
+     ierr = axh_run_end(ctx)
+     if (ierr == AXH_ERR_BLOWUP) then
+        write(6,*) procstrg, '!!!!!!!!!!!!!!! DISPLACEMENTS BLEW UP !!!!!!!!!!!!!!!'
+        write(6,*) procstrg, trim(axh_error_string())
+        stop
+     endif
+     call axh_check(ierr, 'axh_run_end')
+     n0 = it1 + 1
+     n1 = min(niter, it1 + chunk)
+     if (n0 <= niter) &
+        call axh_check(axh_run_begin(ctx, int(n0, c_int32_t), int(n1, c_int32_t), next), &
+                       'axh_run_begin')
+     ! the samples of steps it0..it1, with the reference's time accumulation
+     nsamp = 0
+     do k = it0, it1
+        tloc = tloc + deltat
+        if (mod(k, seis_it) == 0 .and. nrec_all > 0) then
+           nsamp = nsamp + 1
+           call write_samples(seis(1 + 3 * nrec_all * (nsamp - 1):), 1, tloc)
+        endif
+     enddo
+     if (lpr) then
+        write(6,13) it1, tloc, real(it1) / real(niter) * 100.
+        call flush(6)
+     endif
+13   format('  time step:',i6,'; t=',f8.2,' s (',f5.1,'%)')
+     if (n0 > niter) exit
+     call move_alloc(seis, tmp)
+     call move_alloc(next, seis)
+     call move_alloc(tmp, next)
+     it0 = n0
+     it1 = n1
+  enddo
+  deallocate(next)
+
+end subroutine time_loop_overlapped
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
 !> Outputs and options the GPU loop does not produce stop here, before any work.
 subroutine check_supported_pre()
 
@@ -363,7 +478,7 @@ subroutine check_supported_pre()
   why = ' '
   if (nproc > 1)        why = 'more than one mesh slice (NTHETA/NRADIAL_SLICES > 1)'
   if (npol /= 4)        why = 'npol /= 4'
-  if (dump_wavefields)  why = 'KERNEL_WAVEFIELDS'
+  if (dump_wavefields)  why = 'KERNEL_WAVEFIELDS (NetCDF; AXH_KWF_STRAIN_IT writes displ_only)'
   if (use_netcdf)       why = 'USE_NETCDF (ASCII seismograms only)'
   if (why /= ' ') then
      write(6,*) 'GPU time loop: unsupported option: ', trim(why)
@@ -381,7 +496,7 @@ subroutine upload_all()
   use data_matr
   use data_spec
   use data_pointwise
-  use data_io,        only: datapath, lfdata
+  use data_io,        only: datapath, lfdata, need_fluid_displ
   use attenuation,    only: n_sls_attenuation, att_coarse_grained
   use source,         only: compute_stf_t
 
@@ -562,6 +677,14 @@ subroutine upload_all()
 
   if (nrec_all > 0) call axh_check(axh_upload_receivers(ctx, rec), 'receivers')
 
+  ! the fluid terms of the kernel-wavefield dumps (dump_disp_global), where the
+  ! reference computes inv_rho_fluid (SOLVER/def_precomp_terms.f90:803-805)
+  if (have_fluid .and. need_fluid_displ .and. allocated(inv_rho_fluid) .and. &
+      allocated(DsDeta_over_J_flu)) &
+     call axh_check(axh_upload_fluid_gradient(ctx, inv_rho_fluid, DsDeta_over_J_flu, &
+                    DzDeta_over_J_flu, DsDxi_over_J_flu, DzDxi_over_J_flu), 'fluid gradient')
+  call upload_kwf()
+
   ! layout checks of the library, before any device work
   call axh_check(axh_check_inputs(ctx), 'input checks')
 
@@ -581,6 +704,40 @@ contains
   end subroutine
 
 end subroutine upload_all
+!-----------------------------------------------------------------------------------------
+
+!-----------------------------------------------------------------------------------------
+!> With AXH_KWF_STRAIN_IT set: the kernel-wavefield grid of the reference's own
+!! build_kwf_grid (SOLVER/meshes_io.F90:489-778; ibeg..jend and the KERNEL_* limits of
+!! inparam_advanced) and the dump interval, handed to the library.
+subroutine upload_kwf()
+
+  use meshes_io, only: build_kwf_grid
+
+  character(len=64)               :: val
+  integer(c_int)                  :: lval
+  integer(c_int32_t), allocatable :: kmask(:,:,:)
+  integer                         :: ios
+
+  lval = axh_glue_getenv('AXH_KWF_STRAIN_IT'//c_null_char, val, len(val))
+  if (lval <= 0) return
+  read(val(1:lval), *, iostat=ios) kwf_it
+  if (ios /= 0 .or. kwf_it < 1) then
+     write(6,*) 'AXH_KWF_STRAIN_IT must be a positive step count: ', val(1:lval)
+     stop 2
+  endif
+  call build_kwf_grid()
+  allocate(kmask(0:npol, 0:npol, nel_solid + nel_fluid))
+  kmask = merge(1_c_int32_t, 0_c_int32_t, kwf_mask(:, :, 1:nel_solid + nel_fluid))
+  call axh_check(axh_upload_kwf_grid(ctx, int(npoint_solid_kwf, c_int32_t), &
+                 int(npoint_fluid_kwf, c_int32_t), mapping_ijel_ikwf, kmask), 'kwf grid')
+  call axh_check(axh_set_kwf(ctx, int(kwf_it, c_int32_t), &
+                 merge(1_c_int32_t, 0_c_int32_t, trim(src_dump_type) == 'mask'), 0_c_int32_t), &
+                 'kwf')
+  if (lpr) write(6,'(a,i9,a,i9,a,i6,a)') '  kernel wavefields: ', npoint_solid_kwf, &
+     ' solid +', npoint_fluid_kwf, ' fluid points every', kwf_it, ' steps'
+
+end subroutine upload_kwf
 !-----------------------------------------------------------------------------------------
 
 !-----------------------------------------------------------------------------------------

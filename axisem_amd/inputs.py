@@ -30,6 +30,9 @@ class RunConfig:
     snapshot_dt: float = 20.0           # SNAPSHOT_DT, seconds
     energy: bool = False                # SAVE_ENERGY
     simulation_type: str = "single"     # SIMULATION_TYPE: single, or moment (one MT run)
+    xdmf_gll_i: str = "3 0 2 4"         # XDMF_GLL_I: count, then the ipol indices
+    xdmf_gll_j: str = "3 0 2 4"         # XDMF_GLL_J
+    extra_advanced: str = ""            # more inparam_advanced lines (e.g. KERNEL_IBEG 0)
 
 
 def write_mesh_inparam(path: str, period_s: float, model: str = "prem_iso",
@@ -107,7 +110,10 @@ def write_run_dir(rundir: str, mesh_dir: str, cfg: RunConfig, copy_mesh: bool = 
         if cfg.snapshots:
             f.write("SNAPSHOT_DT         %.3f\n" % cfg.snapshot_dt)
             f.write("SNAPSHOTS_FORMAT    %s\n" % cfg.snapshots)
-            f.write("XDMF_GLL_I       '3 0 2 4'\nXDMF_GLL_J       '3 0 2 4'\n")
+            f.write("XDMF_GLL_I       '%s'\nXDMF_GLL_J       '%s'\n"
+                    % (cfg.xdmf_gll_i, cfg.xdmf_gll_j))
+        if cfg.extra_advanced:
+            f.write(cfg.extra_advanced.rstrip("\n") + "\n")
     write_source(os.path.join(rundir, "inparam_source"), cfg)
     with open(os.path.join(rundir, "STATIONS"), "w") as f:
         for k, dist in enumerate(cfg.stations_deg):
@@ -171,9 +177,11 @@ def record_states(host_exe: str, rundir: str, mesh_dir: str, cfgs: list, state_p
     return list(state_paths)
 
 
-def record_state(host_exe: str, rundir: str, state_path: str, timeout: float = 1800) -> str:
+def record_state(host_exe: str, rundir: str, state_path: str, timeout: float = 1800,
+                 kwf_strain_it: int = 0) -> str:
     """Run the drop-in host in record mode (no GPU): prepare_waves + uploads,
-    written to state_path; the time loop is skipped."""
+    written to state_path; the time loop is skipped.  kwf_strain_it > 0 also
+    records the reference's kernel-wavefield grid (AXH_KWF_STRAIN_IT)."""
     import subprocess
     import resource
 
@@ -181,6 +189,8 @@ def record_state(host_exe: str, rundir: str, state_path: str, timeout: float = 1
         resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
     env = dict(os.environ, AXH_SAVE_STATE=os.path.abspath(state_path), AXH_NO_RUN="1")
+    if kwf_strain_it > 0:
+        env["AXH_KWF_STRAIN_IT"] = str(int(kwf_strain_it))
     with open(os.path.join(rundir, "OUTPUT_record"), "w") as log:
         subprocess.run([os.path.abspath(host_exe)], cwd=rundir, stdout=log,
                        stderr=subprocess.STDOUT, check=True, timeout=timeout,

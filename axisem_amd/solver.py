@@ -33,6 +33,9 @@ SOLID_TERMS = ["M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M11z", "M21z",
                "M_1", "M_2", "M_3", "M_4", "M_5", "M_6", "M_7", "M_8",
                "M_w1", "M_w2", "M_w3", "M_w4", "M_w5"]
 FLUID_TERMS = ["M1chi_fl", "M2chi_fl", "M4chi_fl", "M_w_fl"]
+# arguments of axh_upload_fluid_gradient, in order
+FLUID_GRADIENT = ["inv_rho_fluid", "DsDeta_over_J_flu", "DzDeta_over_J_flu",
+                  "DsDxi_over_J_flu", "DzDxi_over_J_flu"]
 
 
 class AxhError(RuntimeError):
@@ -106,6 +109,8 @@ def library(fused: bool = False):
         L.axh_upload_attenuation_full.argtypes = [P, C.POINTER(AttFull)]
         L.axh_upload_receivers.argtypes = [P, P]
         L.axh_run.argtypes = [P, C.c_int32, C.c_int32, P]
+        L.axh_run_begin.argtypes = [P, C.c_int32, C.c_int32, P]
+        L.axh_run_end.argtypes = [P]
         L.axh_seis_samples.argtypes = [P, C.c_int32, C.c_int32]
         L.axh_download_fields.argtypes = [P, P, P, P, P, P]
         L.axh_upload_fields.argtypes = [P, P, P, P, P, P]
@@ -145,6 +150,14 @@ def library(fused: bool = False):
         L.axh_meshdb_get_info.argtypes = [P, C.POINTER(MeshDBInfo)]
         L.axh_meshdb_array.argtypes = [P, C.c_char_p, P, C.c_size_t]
         L.axh_meshdb_upload.argtypes = [P, P]
+        L.axh_upload_fluid_gradient.argtypes = [P] * 6
+        I32, PI32 = C.c_int32, C.POINTER(C.c_int32)
+        L.axh_kwf_grid.argtypes = [P, I32, I32, I32, I32, P, PI32, PI32]
+        L.axh_upload_kwf_grid.argtypes = [P, I32, I32, P, P]
+        L.axh_kwf_get_grid.argtypes = [P, P, P, PI32, PI32]
+        L.axh_set_kwf.argtypes = [P, I32, I32, I32]
+        L.axh_kwf_snapshots.argtypes = [P, I32, I32]
+        L.axh_run_kwf.argtypes = [P, I32, I32, P, P]
         _libs[fused] = L
     return L
 
@@ -269,6 +282,12 @@ class Solver:
             _check(L.axh_upload_attenuation_cg4(h, C.byref(a)))
         if d["num_rec"] > 0:
             _check(L.axh_upload_receivers(h, K(i32("recfile_el"))))
+        if "inv_rho_fluid" in st.arrays:
+            _check(L.axh_upload_fluid_gradient(h, *[K(f32(n)) for n in FLUID_GRADIENT]))
+        if "kwf_npoints" in st.arrays:
+            nps, npf = (int(x) for x in st["kwf_npoints"])
+            _check(L.axh_upload_kwf_grid(h, nps, npf, K(i32("kwf_mapping")),
+                                         K(i32("kwf_mask"))))
         return self
 
     def close(self):
@@ -300,6 +319,72 @@ class Solver:
         self.iter = it1 if rc in (AXH_OK, AXH_ERR_BLOWUP) else self.iter
         self._ok(rc)
         return seis[:n]
+
+    def run_begin(self, nsteps: int) -> np.ndarray:
+        """Queue nsteps on the GPU and return at once; the returned array holds
+        the samples after run_end()."""
+        it0 = self.iter + 1
+        it1 = min(self.dims["niter"], self.iter + nsteps)
+        n = self._L.axh_seis_samples(self._h, it0, it1)
+        self._pending = (np.zeros((max(n, 1), max(self.dims["num_rec"], 1), 3), np.float32),
+                         n, it1)
+        self._ok(self._L.axh_run_begin(self._h, it0, it1, _p(self._pending[0])))
+        return self._pending[0][:n]
+
+    def run_end(self):
+        seis, n, it1 = self._pending
+        self._pending = None
+        rc = self._L.axh_run_end(self._h)
+        if rc in (AXH_OK, AXH_ERR_BLOWUP):
+            self.iter = it1
+        self._ok(rc)
+
+    # --- kernel wavefields (displ_only) ---------------------------------------
+    def kwf_grid(self, ibeg=1, iend=3, jbeg=1, jend=3, elem_mask=None):
+        """Build the kernel-wavefield grid (build_kwf_grid; the reference's
+        defaults select GLL points 1..3 x 1..3 of every element plus the axis
+        points); returns (npoint_solid, npoint_fluid).  Before the first run."""
+        em = None if elem_mask is None else np.ascontiguousarray(elem_mask, np.int32)
+        nps, npf = C.c_int32(0), C.c_int32(0)
+        self._ok(self._L.axh_kwf_grid(self._h, ibeg, iend, jbeg, jend, _p(em), C.byref(nps),
+                                      C.byref(npf)))
+        return nps.value, npf.value
+
+    def kwf_get_grid(self):
+        """(mapping_ijel_ikwf, kwf_mask, npoint_solid, npoint_fluid); the arrays
+        [nel_solid + nel_fluid][25], mapping 1-based at the masked entries."""
+        n = 25 * (self.dims["nel_solid"] + self.dims["nel_fluid"])
+        mapping = np.zeros(n, np.int32)
+        mask = np.zeros(n, np.int32)
+        nps, npf = C.c_int32(0), C.c_int32(0)
+        self._ok(self._L.axh_kwf_get_grid(self._h, _p(mapping), _p(mask), C.byref(nps),
+                                          C.byref(npf)))
+        return mapping, mask, nps.value, npf.value
+
+    def set_kwf(self, strain_it: int, src_mask: bool = True, ring_snapshots: int = 0):
+        """Snapshots every strain_it steps (and of the initial fields);
+        src_mask zeroes the source elements (src_dump_type 'mask');
+        ring_snapshots bounds the device ring (0: automatic)."""
+        self._ok(self._L.axh_set_kwf(self._h, int(strain_it), int(bool(src_mask)),
+                                     int(ring_snapshots)))
+
+    def run_kwf(self, nsteps: int | None = None):
+        """run(), plus the kernel-wavefield snapshots of the steps it covers:
+        (seis, kwf) with kwf of shape (nsnap, ncomp, npoint), components s, z
+        (monopole) or s, phi, z."""
+        it0 = self.iter + 1
+        it1 = self.dims["niter"] if nsteps is None else min(self.dims["niter"], self.iter + nsteps)
+        n = self._L.axh_seis_samples(self._h, it0, it1)
+        seis = np.zeros((max(n, 1), max(self.dims["num_rec"], 1), 3), np.float32)
+        nsnap = self._L.axh_kwf_snapshots(self._h, it0, it1)
+        nps, npf = C.c_int32(0), C.c_int32(0)
+        self._ok(self._L.axh_kwf_get_grid(self._h, None, None, C.byref(nps), C.byref(npf)))
+        ncomp = 2 if self.dims["src_type"] == 0 else 3
+        kwf = np.zeros((max(nsnap, 1), ncomp, max(nps.value + npf.value, 1)), np.float32)
+        rc = self._L.axh_run_kwf(self._h, it0, it1, _p(seis), _p(kwf))
+        self.iter = it1 if rc in (AXH_OK, AXH_ERR_BLOWUP) else self.iter
+        self._ok(rc)
+        return seis[:n], kwf[:nsnap, :, :nps.value + npf.value]
 
     def fields(self):
         """(disp, velo, chi, dchi, ddchi) in the reference's element-local layout."""

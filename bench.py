@@ -352,6 +352,70 @@ def p50_leg(workdir: str, steps: int, warmup: int) -> dict:
     return out
 
 
+def kwf_leg(workdir: str, steps: int, warmup: int) -> dict:
+    """Kernel-wavefield dumps (displ_only) on the plain workload: the step time
+    with and without the snapshots (gathered on the GPU every strain_it steps
+    and copied to host memory), and the gather kernel's bytes and time.
+    strain_it follows the reference's rule for a 10 s source period with the
+    default 8 samples per period, floor(period / 8 / deltat)
+    (SOLVER/parameters.F90:1016-1022)."""
+    import math
+    from axisem_amd.solver import Solver
+    os.makedirs(workdir, exist_ok=True)
+    rundir = os.path.join(workdir, "run_gpu")
+    state = os.path.join(workdir, "state.bin")
+    stamp = os.path.join(workdir, "stamp.json")
+    key = json.dumps({"cfg": repr(WORKLOAD), "host": os.path.getmtime(HOST), "kwf": 1})
+    if not (os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key):
+        inputs.write_run_dir(rundir, MESH, WORKLOAD)
+        t = time.time()
+        inputs.record_state(HOST, rundir, state, kwf_strain_it=1)
+        log("recorded the kernel-wavefield state in %.1fs" % (time.time() - t))
+        with open(stamp, "w") as f:
+            f.write(key)
+    st = read_state(state)
+    d = st.dims
+    gll = 25 * (d["nel_solid"] + d["nel_fluid"])
+    strain_it = max(1, math.floor(10.0 / 8 / d["deltat"]))
+    nps, npf = (int(x) for x in st["kwf_npoints"])
+    from axisem_amd.tune import tune_solo
+    gs = tune_solo(state)["graph_steps"]
+    out = {"workload": WORKLOAD_NAME, "strain_it": strain_it, "graph_steps": gs,
+           "npoint_solid": nps, "npoint_fluid": npf, "steps": steps}
+    with Solver.from_state_file(state) as s:
+        s.set_graph_steps(gs)
+        s.run(warmup)
+        t0 = time.perf_counter()
+        s.run(steps)
+        plain = time.perf_counter() - t0
+    with Solver.from_state_file(state) as s:
+        s.set_graph_steps(gs)
+        s.set_kwf(strain_it)
+        s.run_kwf(warmup)
+        t0 = time.perf_counter()
+        _, kwf = s.run_kwf(steps)
+        wall = time.perf_counter() - t0
+        s.set_profiling(True)
+        s.run_kwf(strain_it * 8)
+        ks = s.kernel_stats()
+    kms, kn = ks["receivers"]      # the gather's launches (the receivers ride in k_memvar)
+    snap_bytes = kwf[0].nbytes
+    # algorithmic bytes of one gather: solid point 4 B index + 12 B displacement +
+    # 12 B written; fluid point 40 B row/column ids + 4 B (i, j) + 20 B terms + 12 B
+    # written; chi once
+    nbytes = nps * 28 + npf * 76 + 4 * d["nglob_fluid"]
+    avg_s = kms / max(kn, 1) * 1e-3
+    out.update({
+        "snapshots": int(kwf.shape[0]), "snapshot_bytes": snap_bytes,
+        "ms_per_step_plain": plain / steps * 1e3, "ms_per_step_kwf": wall / steps * 1e3,
+        "overhead": wall / plain - 1.0, "value_kwf": gll * steps / wall,
+        "kernel": {"name": "k_kwf", "launches": kn, "avg_us": avg_s * 1e6,
+                   "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / avg_s / 1e9,
+                   "frac": nbytes / avg_s / 1e9 / HBM_PEAK_GBS},
+        "snapshot_copy_GBps": snap_bytes * kwf.shape[0] / max(wall - plain, 1e-9) / 1e9})
+    return out
+
+
 # Batched wavefields on one mesh: prem_iso 5 s, elastic, newmark2 (the time
 # scheme is not stated for this configuration; newmark2 is the reference's
 # default), the four moment-tensor runs at 100 km, and the same four at 10,
@@ -599,6 +663,8 @@ def main():
     ap.add_argument("--no-fused", action="store_true",
                     help="no legs with the fused multiply-add build")
     ap.add_argument("--p50-steps", type=int, default=1000, help="timed steps of the 50 s leg")
+    ap.add_argument("--no-kwf", action="store_true",
+                    help="no kernel-wavefield (displ_only) leg")
     ap.add_argument("--no-mt", action="store_true",
                     help="no moment-tensor batch leg (5 s mesh, 4 and 16 wavefields)")
     ap.add_argument("--config5", action="store_true",
@@ -784,6 +850,12 @@ def main():
         p50 = p50_leg(os.path.join(REPO, "data", "bench_50s"), a.p50_steps, 50)
         log("50 s: " + json.dumps(p50))
 
+    kwf = None
+    if not a.no_kwf:
+        log("kernel-wavefield dumps on the plain workload")
+        kwf = kwf_leg(os.path.join(a.workdir, "kwf"), a.steps, a.warmup)
+        log("kernel wavefields: " + json.dumps(kwf))
+
     mt = None
     if not a.no_mt:
         log("moment-tensor batches on the 5 s mesh")
@@ -851,6 +923,7 @@ def main():
         "batched": batched,
         "fused_multiply_add_build": fused,
         "plain_50s": p50,
+        "kernel_wavefields": kwf,
         "moment_tensor_5s": mt,
         "prem_ani_2s": c5,
         "serving": serving,

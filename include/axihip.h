@@ -248,6 +248,15 @@ int axh_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
 #define AXH_NOALLOC_SCRATCH_BYTES (1 << 20)
 int axh_run_noalloc(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
 
+/* axh_run in two halves, for a host that works while the GPU runs (the
+ * drop-in host writes one chunk's samples while the next chunk runs):
+ * axh_run_begin queues steps it_begin..it_end on the context's stream and
+ * returns; axh_run_end waits for them and writes seis (which must stay valid
+ * and untouched in between), with axh_run's results and errors.  Between the
+ * two the context takes no other call but axh_destroy. */
+int axh_run_begin(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis);
+int axh_run_end(axh_ctx *ctx);
+
 /* Number of seismogram samples axh_run(ctx, it_begin, it_end) will write. */
 int axh_seis_samples(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
 
@@ -418,6 +427,74 @@ int axh_meshdb_upload(const axh_meshdb *db, axh_ctx *ctx);
  * host (tests, benchmarks, the request loop). */
 int axh_save_state(const axh_ctx *ctx, const char *path);
 int axh_load_state(const char *path, axh_ctx **out);
+
+/* ---- kernel wavefields (Instaseis databases, dump_type displ_only) ------- */
+
+/* The arrays the fluid displacement of the dumps needs, each
+ * (0:npol,0:npol,nel_fluid): inv_rho_fluid and the fluid's pointwise
+ * derivative terms over J (SOLVER/def_precomp_terms.f90:148-304,631), read by
+ * dump_disp_global through axisym_gradient_fluid
+ * (SOLVER/wavefields_io.f90:1082-1090, pointwise_derivatives.f90:509-544).
+ * An upload (before the first run). */
+int axh_upload_fluid_gradient(axh_ctx *ctx, const float *inv_rho_fluid,
+                              const float *DsDeta_over_J_flu, const float *DzDeta_over_J_flu,
+                              const float *DsDxi_over_J_flu, const float *DzDxi_over_J_flu);
+
+/* build_kwf_grid (SOLVER/meshes_io.F90:489-610): the kernel-wavefield points
+ * are the GLL points ibeg..iend x jbeg..jend (0..4) of the selected elements
+ * plus the axis point ipol = 0 of every row of an axial element, numbered
+ * 1, 2, ... by the first appearance of their global point, the solid elements
+ * first (jpol outer, ipol inner, the axis point after each row), then the
+ * fluid elements; the solid points come first.  elem_mask holds
+ * nel_solid + nel_fluid flags, solid elements first (the reference's
+ * mask_tp_elem, which selects by the radius and colatitude of the element
+ * corners); NULL selects every element, as the reference's defaults
+ * (KERNEL_RMIN 0, KERNEL_RMAX 7000 km, colatitudes 0..180) do.  Returns the
+ * point counts npoint_solid_kwf and npoint_fluid_kwf.  An upload. */
+int axh_kwf_grid(axh_ctx *ctx, int32_t ibeg, int32_t iend, int32_t jbeg, int32_t jend,
+                 const int32_t *elem_mask, int32_t *npoint_solid, int32_t *npoint_fluid);
+
+/* The grid as the reference's build_kwf_grid leaves it in module data_mesh:
+ * mapping_ijel_ikwf and kwf_mask (0 / 1), both (0:npol,0:npol,nel_solid +
+ * nel_fluid) with the solid elements first, and npoint_solid_kwf,
+ * npoint_fluid_kwf.  Checked: the masked entries number every point exactly
+ * once, solid points 1..npoint_solid, fluid points after them; unmasked
+ * entries are not read.  An upload. */
+int axh_upload_kwf_grid(axh_ctx *ctx, int32_t npoint_solid, int32_t npoint_fluid,
+                        const int32_t *mapping_ijel_ikwf, const int32_t *kwf_mask);
+
+/* The context's grid (either of the two above): the point counts, and, for
+ * non-NULL arrays, mapping_ijel_ikwf (0 at unmasked entries) and kwf_mask. */
+int axh_kwf_get_grid(const axh_ctx *ctx, int32_t *mapping_ijel_ikwf, int32_t *kwf_mask,
+                     int32_t *npoint_solid, int32_t *npoint_fluid);
+
+/* Snapshot interval of axh_run_kwf: the steps iter with mod(iter, strain_it)
+ * == 0, including the initial fields (iter 0), as dump_stuff
+ * (SOLVER/time_evol_wave.F90:1200-1217) calls dump_disp_global; src_mask 1
+ * zeroes the solid displacement of the source elements (src_dump_type 'mask',
+ * the reference's default; eradicate_src_elem_vec_values,
+ * SOLVER/wavefields_io.f90:1123-1135), 0 keeps it ('none').  ring_snapshots
+ * sets the device ring the snapshots are gathered in (at least 2; 0: chosen
+ * by the library, at most AXH_KWF_RING_BYTES); they are copied out half a
+ * ring (at most 64 MiB) at a time.  Any time. */
+#define AXH_KWF_RING_BYTES ((size_t)1 << 30)
+int axh_set_kwf(axh_ctx *ctx, int32_t strain_it, int32_t src_mask, int32_t ring_snapshots);
+
+/* Number of snapshots axh_run_kwf(ctx, it_begin, it_end, ...) writes. */
+int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
+
+/* axh_run, and the kernel-wavefield snapshots of the steps it covers written
+ * to kwf: per snapshot, per component, npoint_solid + npoint_fluid floats
+ * (point k + 1 of the grid at index k), components s, z for a monopole source
+ * and s, phi, z otherwise; the values dump_disp_global gathers with
+ * kwf_mapping_sol / kwf_mapping_flu (SOLVER/wavefields_io.f90:743-781,
+ * 1040-1097) for the NetCDF variables disp_sol_* / disp_flu_* of a snapshot
+ * (SOLVER/nc_routines.F90:945-970): the solid displacement (dipole: u1 + u2,
+ * u1 - u2, u3), and in the fluid inv_rho_fluid times the gradient of chi
+ * (phi: 0).  The snapshots are gathered on the GPU into a device ring and
+ * copied to host memory through pinned staging buffers while the following
+ * steps run; kwf is complete when the call returns. */
+int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf);
 
 /* ---- seismogram post-processing ------------------------------------------ */
 

@@ -28,7 +28,12 @@
  * add_source_fl (a source in the fluid), which the reference program cannot
  * place on its meshes (SOLVER/source.f90:531), is pinned by a closed form:
  * with every operator zeroed, a Newmark step from rest is a point-mass update
- * of the source elements' points (tests/test_oracle_math.py).
+ * of the source elements' points (tests/test_oracle_math.py).  The
+ * kernel-wavefield restatement (orc_kwf_grid, orc_kwf_sample), whose NetCDF
+ * output the reference cannot write here, is pinned against the reference's
+ * own build_kwf_grid (run by the drop-in host) and against the xdmf
+ * snapshots of the reference program, which hold the same solid and fluid
+ * displacements at the same GLL points (tests/test_kwf.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -1466,6 +1471,7 @@ orc_t *orc_load(const char *path) {
 int orc_nel_solid(orc_t *o) { return o->nel_s; }
 int orc_nel_fluid(orc_t *o) { return o->nel_f; }
 int orc_iter(orc_t *o) { return o->iter; }
+int orc_src_type(orc_t *o) { return o->src_type; }
 
 /* Run steps it0..it1 (1-based, it0 == iter+1); write receiver samples for
  * every step with mod(iter, seis_it) == 0.  Returns the number of samples,
@@ -1547,3 +1553,142 @@ int orc_symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
   return symp_coeffs(scheme, dt, cd, cv, cf);
 }
 void orc_mxm(const float *a, const float *b, float *c) { mxm(a, b, c); }
+
+/* ------------------------------------------------------------------------ */
+/* kernel wavefields, displ_only (SOLVER/meshes_io.F90, wavefields_io.f90)   */
+/* ------------------------------------------------------------------------ */
+/* build_kwf_grid (SOLVER/meshes_io.F90:489-610): the points ibeg..iend x
+ * jbeg..jend of the elements selected by elem_mask (solid elements 1..nel_s,
+ * then fluid elements as nel_s + 1..; NULL selects all), plus the axis point
+ * ipol = 0 of each row of an axial element, numbered by the first appearance
+ * of their global point: solid elements first, jpol outer, ipol inner, the
+ * axis point after each row; then the fluid elements.  mapping[p + 25 e]
+ * (e over solid then fluid elements) receives the 1-based number (0 where
+ * the entry is not a kernel-wavefield point) and mask[p + 25 e] 1 at the
+ * entry that numbered its point.  Returns the solid point count through
+ * nps and the fluid count through npf. */
+void orc_kwf_grid(orc_t *o, int ibeg, int iend, int jbeg, int jend,
+                  const int32_t *elem_mask, int32_t *mapping, int32_t *mask,
+                  int32_t *nps, int32_t *npf) {
+  const int nel = o->nel_s + o->nel_f;
+  const size_t ng = (size_t)o->nglob_s + o->nglob_f;
+  int32_t *num = calloc(ng ? ng : 1, sizeof(int32_t)); /* 0: unseen */
+  memset(mapping, 0, (size_t)nel * NPP * sizeof(int32_t));
+  memset(mask, 0, (size_t)nel * NPP * sizeof(int32_t));
+  int ct = 0;
+  for (int dom = 0; dom < 2; dom++) {
+    const int n = dom == 0 ? o->nel_s : o->nel_f;
+    const int32_t *ig = dom == 0 ? o->igloc_s : o->igloc_f;
+    const int32_t *ax = dom == 0 ? o->axis_s : o->axis_f;
+    const int off = dom == 0 ? o->nglob_f : 0; /* idest: fluid ids first */
+    const int eoff = dom == 0 ? 0 : o->nel_s;
+    for (int e = 0; e < n; e++) {
+      if (elem_mask && !elem_mask[eoff + e]) continue;
+      for (int j = jbeg; j <= jend; j++) {
+        for (int pass = 0; pass < 2; pass++) {
+          const int i0 = pass == 0 ? ibeg : 0, i1 = pass == 0 ? iend : 0;
+          if (pass == 1 && !ax[e]) break;
+          for (int i = i0; i <= i1; i++) {
+            const int p = i + NP * j;
+            const int g = ig[(size_t)e * NPP + p] - 1 + off;
+            if (!num[g]) {
+              num[g] = ++ct;
+              mask[(size_t)(eoff + e) * NPP + p] = 1;
+            }
+            mapping[(size_t)(eoff + e) * NPP + p] = num[g];
+          }
+        }
+      }
+    }
+    if (dom == 0) *nps = ct;
+  }
+  *npf = ct - *nps;
+  free(num);
+}
+
+/* axisym_gradient_fluid (SOLVER/pointwise_derivatives.f90:509-544) of one
+ * fluid element: grad(:,:,1) = dzdeta mxm1 + dzdxi mxm2, grad(:,:,2) =
+ * dsdeta mxm1 + dsdxi mxm2 with mxm1 = G1T f (axial) or G2T f, mxm2 = f G2;
+ * every term an over-J array of the element. */
+static void grad_fluid(orc_t *o, const float *f, int e, const float *dsdeta,
+                       const float *dzdeta, const float *dsdxi,
+                       const float *dzdxi, float *gs, float *gz) {
+  float m1[NPP], m2[NPP];
+  mxm(o->axis_f[e] ? o->G1T : o->G2T, f, m1);
+  mxm(f, o->G2, m2);
+  const size_t b = (size_t)e * NPP;
+  for (int p = 0; p < NPP; p++) {
+    gs[p] = dzdeta[b + p] * m1[p] + dzdxi[b + p] * m2[p];
+    gz[p] = dsdeta[b + p] * m1[p] + dsdxi[b + p] * m2[p];
+  }
+}
+
+/* dump_disp_global with dump_type displ_only (SOLVER/wavefields_io.f90:
+ * 1019-1097) for the current fields: the solid displacement (dipole: s =
+ * u1 + u2, phi = u1 - u2, z = u3; else u as it is), with every entry of the
+ * source elements ielsrc(1:nelsrc) zeroed when src_mask (src_dump_type
+ * 'mask', eradicate_src_elem_vec_values :1123-1135), gathered by
+ * kwf_mapping_sol (:743-760); the fluid displacement inv_rho_fluid * grad
+ * chi (s, 0, z) gathered by kwf_mapping_flu (:764-781).  out[c][k] holds
+ * component c (monopole: s, z; else s, phi, z) of kernel-wavefield point
+ * k + 1 (solid points first).  Returns -1 when the fluid arrays are missing
+ * from the state. */
+int orc_kwf_sample(orc_t *o, const int32_t *mapping, const int32_t *mask,
+                   int nps, int npf, int src_mask, float *out) {
+  enable_ftz();
+  const int npt = nps + npf;
+  const int mono = o->src_type == MONO;
+  const int nc = mono ? 2 : 3;
+  const size_t ns = (size_t)o->nel_s * NPP;
+  float *f = malloc(3 * (ns ? ns : 1) * sizeof(float));
+  for (size_t q = 0; q < ns; q++) {
+    const float u1 = o->disp[q], u2 = o->disp[ns + q], u3 = o->disp[2 * ns + q];
+    if (o->src_type == DI) {
+      f[q] = u1 + u2; f[ns + q] = u1 - u2; f[2 * ns + q] = u3;
+    } else {
+      f[q] = u1; f[ns + q] = u2; f[2 * ns + q] = u3;
+    }
+  }
+  if (src_mask)
+    for (int k = 0; k < o->nelsrc; k++) {
+      const size_t b = (size_t)(o->ielsrc[k] - 1) * NPP;
+      for (int c = 0; c < 3; c++)
+        for (int p = 0; p < NPP; p++) f[c * ns + b + p] = 0.0f;
+    }
+  /* solid: components 1, (2), 3 of f */
+  for (int e = 0; e < o->nel_s; e++)
+    for (int p = 0; p < NPP; p++) {
+      const size_t q = (size_t)e * NPP + p;
+      if (!mask[q]) continue;
+      const int k = mapping[q] - 1;
+      for (int c = 0, oc = 0; c < 3; c++) {
+        if (mono && c == 1) continue;
+        out[(size_t)oc++ * npt + k] = f[c * ns + q];
+      }
+    }
+  free(f);
+  if (o->nel_f > 0 && npf > 0) {
+    const float *rho = find(o, "inv_rho_fluid"), *dsdeta = find(o, "DsDeta_over_J_flu"),
+                *dzdeta = find(o, "DzDeta_over_J_flu"), *dsdxi = find(o, "DsDxi_over_J_flu"),
+                *dzdxi = find(o, "DzDxi_over_J_flu");
+    if (!rho || !dsdeta || !dzdeta || !dsdxi || !dzdxi) return -1;
+    for (int e = 0; e < o->nel_f; e++) {
+      const int32_t *mk = mask + (size_t)(o->nel_s + e) * NPP;
+      const int32_t *mp = mapping + (size_t)(o->nel_s + e) * NPP;
+      int any = 0;
+      for (int p = 0; p < NPP; p++) any |= mk[p];
+      if (!any) continue;
+      float gs[NPP], gz[NPP];
+      grad_fluid(o, o->chi + (size_t)e * NPP, e, dsdeta, dzdeta, dsdxi, dzdxi, gs, gz);
+      for (int p = 0; p < NPP; p++) {
+        if (!mk[p]) continue;
+        const int k = mp[p] - 1; /* the fluid points follow the solid ones */
+        const float r = rho[(size_t)e * NPP + p];
+        out[k] = r * gs[p];
+        if (!mono) out[(size_t)npt + k] = 0.0f;
+        out[(size_t)(nc - 1) * npt + k] = r * gz[p];
+      }
+    }
+  }
+  return 0;
+}

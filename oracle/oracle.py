@@ -29,7 +29,7 @@ def lib():
         L.orc_free.argtypes = [P]
         L.orc_run.restype = I
         L.orc_run.argtypes = [P, I, I, P]
-        for f in ("orc_nel_solid", "orc_nel_fluid", "orc_iter"):
+        for f in ("orc_nel_solid", "orc_nel_fluid", "orc_iter", "orc_src_type"):
             getattr(L, f).restype = I
             getattr(L, f).argtypes = [P]
         L.orc_get_fields.argtypes = [P, P, P, P, P, P]
@@ -45,6 +45,9 @@ def lib():
         L.orc_symplectic_coefficients.argtypes = [I, D, P, P, P]
         L.orc_mxm.argtypes = [P, P, P]
         L.orc_strain_cg4.argtypes = [P, P, I, P]
+        L.orc_kwf_grid.argtypes = [P, I, I, I, I, P, P, P, P, P]
+        L.orc_kwf_sample.restype = I
+        L.orc_kwf_sample.argtypes = [P, P, P, I, I, I, P]
         _lib = L
     return _lib
 
@@ -110,6 +113,29 @@ class Oracle:
         lib().orc_fluid_stiffness(self.h, _p(chi), _p(out))
         return out
 
+    def kwf_grid(self, ibeg=1, iend=3, jbeg=1, jend=3, elem_mask=None):
+        """build_kwf_grid: (mapping, mask, npoint_solid, npoint_fluid), the
+        first two [nel_solid + nel_fluid][25] (1-based numbers, 0: none)."""
+        n = (self.nel_s + self.nel_f) * 25
+        mapping = np.zeros(max(n, 1), np.int32)
+        mask = np.zeros(max(n, 1), np.int32)
+        em = None if elem_mask is None else np.ascontiguousarray(elem_mask, np.int32)
+        nps, npf = C.c_int32(0), C.c_int32(0)
+        lib().orc_kwf_grid(self.h, ibeg, iend, jbeg, jend, _p(em), _p(mapping), _p(mask),
+                           C.byref(nps), C.byref(npf))
+        return mapping[:n], mask[:n], nps.value, npf.value
+
+    def kwf_sample(self, grid, src_mask=True):
+        """The displ_only kernel-wavefield sample of the current fields:
+        [ncomp][npoint] (monopole: s, z; else s, phi, z)."""
+        mapping, mask, nps, npf = grid
+        nc = 2 if lib_src_is_mono(self) else 3
+        out = np.zeros((nc, max(nps + npf, 1)), np.float32)
+        if lib().orc_kwf_sample(self.h, _p(mapping), _p(mask), nps, npf, int(src_mask),
+                                _p(out)) != 0:
+            raise ValueError("the state holds no fluid gradient arrays")
+        return out[:, :nps + npf]
+
     def assemble_solid(self, vec, nc=3):
         vec = np.array(vec, np.float32, copy=True)
         lib().orc_assemble_solid(self.h, _p(vec), nc)
@@ -119,6 +145,10 @@ class Oracle:
         vec = np.array(vec, np.float32, copy=True)
         lib().orc_assemble_fluid(self.h, _p(vec))
         return vec
+
+
+def lib_src_is_mono(o: Oracle) -> bool:
+    return lib().orc_src_type(o.h) == 0
 
 
 def symplectic_coefficients(scheme: int, dt: float):

@@ -169,7 +169,23 @@ ORACLE_CASES = {
                                                 500.0, STATIONS),
     "quad_mtp_mlso4m5_anel": RunConfig("mtp", 300.0, 1e20, "ML_SO4m5", True, 500.0, STATIONS),
 }
-ALL_CONFIGS = {**CASES, **BATCH_MEMBERS, **ORACLE_CASES}
+# Kernel wavefields (displ_only).  The reference cannot write them without
+# NetCDF, but its xdmf snapshots hold the same values: the fluid displacement
+# inv_rho_fluid * grad chi and the solid displacement in (s, phi, z), at the
+# GLL points XDMF_GLL_I x XDMF_GLL_J; with ipol 0..3 and jpol 1..3 they cover
+# every point of the default kernel-wavefield grid (1..3 x 1..3 and the axis
+# points).  tests/golden/<case>/ref_xdmf.npz holds one such snapshot.
+KWF_CASES = {
+    "kwf_dipole_thetaforce_newmark": RunConfig("thetaforce", 50.0, 1e20, "newmark2", False,
+                                               720.0, STATIONS, snapshots="xdmf",
+                                               snapshot_dt=350.0, xdmf_gll_i="4 0 1 2 3",
+                                               xdmf_gll_j="3 1 2 3"),
+    "kwf_mono_explosion_symplec4": RunConfig("explosion", 600.0, 1e20, "symplec4", False,
+                                             720.0, STATIONS, snapshots="xdmf",
+                                             snapshot_dt=350.0, xdmf_gll_i="4 0 1 2 3",
+                                             xdmf_gll_j="3 1 2 3"),
+}
+ALL_CONFIGS = {**CASES, **BATCH_MEMBERS, **ORACLE_CASES, **KWF_CASES}
 
 NCOMP = {"vertforce": 2, "explosion": 2, "mrr": 2, "mtt_p_mpp": 2,
          "thetaforce": 3, "phiforce": 3, "mtr": 3, "mpr": 3, "mtp": 3, "mtt_m_mpp": 3}

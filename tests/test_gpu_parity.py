@@ -170,3 +170,22 @@ def test_state_roundtrip_on_device(state_cache, tmp_path):
     with Solver.from_state(st) as a, Solver.from_state_file(st_path) as b:
         np.testing.assert_array_equal(a.run(30), b.run(30))
         assert a.device_bytes() > 0
+
+
+def test_run_begin_end_equals_run(state_cache):
+    """axh_run_begin / axh_run_end (the drop-in host's overlapped chunks) give
+    axh_run's samples and fields; the context refuses other calls while a run
+    is queued."""
+    from axisem_amd import solver as S
+    _, st = state_cache("dipole_phiforce_symplec4_anel")
+    with Solver.from_state_file(st) as a, Solver.from_state_file(st) as b:
+        ref = a.run(61)
+        first = b.run_begin(25)
+        disp = np.zeros(75 * b.dims["nel_solid"], np.float32)
+        rc = b._L.axh_download_fields(b._h, S._p(disp), None, None, None, None)
+        assert rc == S.AXH_ERR_STATE
+        b.run_end()
+        second = b.run_begin(36)
+        b.run_end()
+        np.testing.assert_array_equal(np.concatenate([first, second]), ref)
+        _compare_fields(b.fields(), a.fields())
