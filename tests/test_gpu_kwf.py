@@ -86,7 +86,9 @@ def _gpu_snapshots(stp, pieces, strain_it, src_mask, graph=0, ring=0):
     return np.concatenate(snaps), np.concatenate(seis)
 
 
-CASES = ("dipole_thetaforce_newmark_anel", "mono_explosion_symplec4_anel", "quad_mtp_newmark")
+CASES = ("dipole_thetaforce_newmark_anel", "mono_explosion_symplec4_anel", "quad_mtp_newmark",
+         # regional mesh with absorbing boundaries; transversely isotropic PREM
+         "reg_dipole_thetaforce_symplec4_anel", "ani_dipole_mtr_symplec4_anel")
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -127,6 +129,26 @@ def test_gpu_snapshot_runs_split_and_replayed(mode, kwf_states):
     np.testing.assert_array_equal(seis, plain_seis)
     with Solver.from_state_file(stp) as s:
         np.testing.assert_array_equal(s.run(steps), plain_seis)
+
+
+def test_gpu_snapshots_all_points_radius_limited(kwf_states):
+    """The reference's grid over all 25 points of the elements between 2000 and
+    5000 km radius (KERNEL_IBEG..JEND 0..4, KERNEL_RMIN/RMAX): points shared by
+    several elements, solid and fluid, take the value of the element that
+    numbers them."""
+    name = "dipole_thetaforce_newmark_anel"
+    cfg = inputs.RunConfig(**{**cases.ALL_CONFIGS[name].__dict__, "extra_advanced":
+                              "KERNEL_IBEG 0\nKERNEL_IEND 4\nKERNEL_JBEG 0\nKERNEL_JEND 4\n"
+                              "KERNEL_RMIN 2000\nKERNEL_RMAX 5000\n"})
+    stp = kwf_states("all_points_r2000_5000", cases.mesh_of(name), cfg)
+    st = read_state(stp)
+    nps, npf = (int(x) for x in st["kwf_npoints"])
+    assert nps > 0 and npf > 0
+    steps = 50
+    print("kernel wavefields, all points of a radius range", flush=True)
+    ref = _oracle_snapshots(stp, steps, STRAIN_IT, True)
+    got, _ = _gpu_snapshots(stp, [steps], STRAIN_IT, True, graph=8)
+    np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
 
 
 def test_gpu_snapshots_50s_mesh(kwf_states):
