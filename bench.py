@@ -378,8 +378,9 @@ def kwf_leg(workdir: str, steps: int, warmup: int) -> dict:
     gll = 25 * (d["nel_solid"] + d["nel_fluid"])
     strain_it = max(1, math.floor(10.0 / 8 / d["deltat"]))
     nps, npf = (int(x) for x in st["kwf_npoints"])
-    from axisem_amd.tune import tune_solo
-    gs = tune_solo(state)["graph_steps"]
+    from axisem_amd.tune import GRAPH_CANDIDATES, tune_solo
+    # graphs no longer than the warm-up, which captures them
+    gs = tune_solo(state, candidates=[c for c in GRAPH_CANDIDATES if c <= warmup])["graph_steps"]
     out = {"workload": WORKLOAD_NAME, "strain_it": strain_it, "graph_steps": gs,
            "npoint_solid": nps, "npoint_fluid": npf, "steps": steps}
     with Solver.from_state_file(state) as s:
@@ -853,7 +854,9 @@ def main():
     kwf = None
     if not a.no_kwf:
         log("kernel-wavefield dumps on the plain workload")
-        kwf = kwf_leg(os.path.join(a.workdir, "kwf"), a.steps, a.warmup)
+        # at least 40 snapshots, whatever --steps is: the copies overlap the
+        # steps, which a run of a few snapshots cannot show
+        kwf = kwf_leg(os.path.join(a.workdir, "kwf"), max(a.steps, 520), max(a.warmup, 20))
         log("kernel wavefields: " + json.dumps(kwf))
 
     mt = None

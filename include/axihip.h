@@ -493,7 +493,8 @@ int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
  * u1 - u2, u3), and in the fluid inv_rho_fluid times the gradient of chi
  * (phi: 0).  The snapshots are gathered on the GPU into a device ring and
  * copied to host memory through pinned staging buffers while the following
- * steps run; kwf is complete when the call returns. */
+ * steps run; kwf is complete when the call returns.  Batches (axh_batch_*)
+ * gather no kernel wavefields. */
 int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf);
 
 /* ---- seismogram post-processing ------------------------------------------ */
