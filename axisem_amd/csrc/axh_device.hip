@@ -652,7 +652,10 @@ __device__ __forceinline__ void fluid_tile(const FluidArgs &A, int tile, float *
 // (occupancy hint: 5 workgroups per CU for a single wavefield, which fits the
 // register budget without spilling; wider groups take what their registers
 // allow)
-template <int SRC, int MODE, bool ANEL, int B>
+// MS: the SLS capacity of the Newmark memory-variable update (5, the
+// reference's default number of linear solids, or kMaxSls); only MODE 0
+// with attenuation uses it.
+template <int SRC, int MODE, bool ANEL, int B, int MS = kMaxSls>
 __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2)))
 void k_solid(SolidArgs A, FluidArgs FA) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
@@ -669,10 +672,18 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   static_assert(kElemPerTile * 24 <= 256, "SLS sums must fit a product buffer");
   float *const sR = &sS[0][0];
   float *const sSa = &sS[1][0];
-  __shared__ __align__(16) float sSrc[MV ? kElemPerTile * 28 * B : 1];
-  __shared__ __align__(16) float sOld[MV ? kElemPerTile * 28 * B : 1];
+  // the strain sources of the Newmark memory-variable update: for 6 product
+  // buffers they go over buffers 2..5, dead once the second products are
+  // summed (the SLS sums use buffers 0 and 1); a dipole group of 4 then fits
+  // 4 workgroups per CU instead of 3
+  constexpr bool SRC_IN_S = MV && NS == 6;
+  static_assert(2 * kElemPerTile * 28 <= 4 * 256, "sources must fit four product buffers");
+  __shared__ __align__(16) float sSrcOwn[(MV && !SRC_IN_S) ? kElemPerTile * 28 * B : 1];
+  __shared__ __align__(16) float sOldOwn[(MV && !SRC_IN_S) ? kElemPerTile * 28 * B : 1];
+  float *const sSrc = SRC_IN_S ? &sS[2][0] : sSrcOwn;
+  float *const sOld = SRC_IN_S ? &sS[2][0] + kElemPerTile * 28 * B : sOldOwn;
   // ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j per element (mu, then kappa)
-  __shared__ double sCf[MV ? kElemPerTile : 1][MV ? kMaxSls : 1][4];
+  __shared__ double sCf[MV ? kElemPerTile : 1][MV ? MS : 1][4];
   __shared__ __align__(16) float sG[105];
 
   if ((int)blockIdx.x >= A.ntile) {  // a fluid tile, in the same LDS
@@ -725,11 +736,11 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   }
   // newmark2 anelastic: lane k < 24 owns entries (k, b) of every SLS, read
   // now; lane 24 forms the element's SLS coefficient products
-  float mvr[MV ? kMaxSls : 1][B];
+  float mvr[MV ? MS : 1][B];
   if (MV && valid && k < 24) {
     const float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
 #pragma unroll
-    for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++)
+    for (int jj = 0; jj < (MV ? MS : 1); jj++)
       if (jj < A.nsls) ldv<B>(mv + (jj * 24 + k) * B, mvr[jj]);
   }
   if (MV && valid && k == 24) {
@@ -992,7 +1003,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
         for (int b = 0; b < B; b++) {
           float s = 0.0f;
 #pragma unroll
-          for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++)
+          for (int jj = 0; jj < (MV ? MS : 1); jj++)
             if (jj < A.nsls) s = s + mvr[jj][b];
           R[k * B + b] = s;
         }
@@ -1137,7 +1148,7 @@ void k_solid(SolidArgs A, FluidArgs FA) {
           ot[b] = (double)old[q * B + b];
         }
 #pragma unroll
-        for (int jj = 0; jj < (MV ? kMaxSls : 1); jj++) {
+        for (int jj = 0; jj < (MV ? MS : 1); jj++) {
           if (jj >= A.nsls) break;
           const double E = A.expw[jj];
           const double *cf = sCf[MV ? le : 0][MV ? jj : 0];
@@ -3378,12 +3389,16 @@ struct Launcher {
   }
 };
 
-// one launch: the solid tiles, then the fluid tiles
+// one launch: the solid tiles, then the fluid tiles (the Newmark
+// memory-variable update sized for 5 linear solids where that suffices)
 template <int SRC, int MODE, bool ANEL, int B>
 static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
                          const FluidArgs &F) {
   const int nblocks = M.ntile_s + (M.nel_f > 0 ? M.ntile_f : 0);
-  hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B>), dim3(nblocks), dim3(256), 0, s, A, F);
+  if (MODE == 0 && ANEL && M.nsls <= 5)
+    hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B, 5>), dim3(nblocks), dim3(256), 0, s, A, F);
+  else
+    hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B>), dim3(nblocks), dim3(256), 0, s, A, F);
 }
 
 template <int B>

@@ -615,7 +615,12 @@ def sweep(workdir: str, steps: int = 300, only=None) -> dict:
                                    seismogram_length_s=1800.0,
                                    stations_deg=[float(k) for k in range(181)])
             paths.append(prepare_state(os.path.join(workdir, "%s_%g" % (name, depth)), cfg))
+        from axisem_amd.tune import GRAPH_CANDIDATES, tune_batch, tune_solo
+        cands = [c for c in GRAPH_CANDIDATES if c <= 50]
+        tuned = tune_solo(paths[0], candidates=cands)
+        btuned = tune_batch(paths, candidates=cands)
         s = Solver.from_state_file(paths[0])
+        s.set_graph_steps(tuned["graph_steps"])
         s.run(50)
         t0 = time.perf_counter()
         s.run(steps)
@@ -625,7 +630,8 @@ def sweep(workdir: str, steps: int = 300, only=None) -> dict:
         ks = kernel_table(s.kernel_stats())
         s.close()
         members = [Solver.from_state_file(p) for p in paths]
-        with Batch(members) as b:
+        with Batch(members, group_width=btuned["group_width"],
+                   graph_steps=btuned["graph_steps"]) as b:
             b.run(50)
             t0 = time.perf_counter()
             b.run(steps)
@@ -637,6 +643,9 @@ def sweep(workdir: str, steps: int = 300, only=None) -> dict:
             m.close()
         out[name] = {"plain_ms_per_step": plain, "batch4_ms_per_step": bat,
                      "batch4_vs_plain": 4 * plain / bat,
+                     "plain_graph_steps": tuned["graph_steps"],
+                     "batch4_schedule": [btuned["group_width"], btuned["graph_steps"]],
+                     "kernels_batch4": kb,
                      "stiffness_us": [ks["stiffness"]["avg_us"], kb["stiffness"]["avg_us"]],
                      "kernels_plain": ks}
         log("sweep %s: %s" % (name, json.dumps(out[name])))
