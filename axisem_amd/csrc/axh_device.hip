@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -2107,45 +2108,53 @@ __global__ __launch_bounds__(256) void k_receivers(RecArgs A) {
 // ---------------------------------------------------------------------------
 // Kernel-wavefield snapshots, dump_type displ_only (dump_disp_global,
 // SOLVER/wavefields_io.f90:1019-1097, with kwf_mapping_sol / kwf_mapping_flu,
-// :743-781), one wavefield: at the steps with mod(iter, strain_it) == 0,
-// snapshot q = iter / strain_it goes to slot mod(q, cap) of the ring, as
-// [ncomp][npt] (monopole s, z; else s, phi, z).  A lane per point (grid
-// stride): solid point k reads the displacement of its global point (sol_g,
-// -1 for a point of a source element, zeroed), fluid point k the chi values
-// of its row and column in its element and forms the fluid displacement
+// :743-781), blockIdx.y = wavefield b of the group: at the steps with
+// mod(iter, strain_it) == 0, snapshot q = iter / strain_it goes to slot
+// mod(q, cap) of wavefield b's ring, as [ncomp][npt] (monopole s, z; else s,
+// phi, z).  A lane per point (grid stride): solid point k reads the
+// displacement of its global point (sol_g, -1 for a point of one of the
+// wavefield's source elements, zeroed), fluid point k the chi values of its
+// row and column in its element and forms the fluid displacement
 //   mxm1 = sum_k GT(i,k) chi(k,j)   (GT = G1T on axial elements, else G2T)
 //   mxm2 = sum_k chi(i,k) G2(k,j)   (both summed from zero in k order)
 //   s = inv_rho (dzdeta mxm1 + dzdxi mxm2), z = inv_rho (dsdeta mxm1 + dsdxi mxm2)
 // as axisym_gradient_fluid (SOLVER/pointwise_derivatives.f90:509-544) and
 // the product with inv_rho_fluid (wavefields_io.f90:1087-1089) evaluate it.
+// The launch for the initial fields (init_only) writes only the wavefields
+// at step 0.
 // ---------------------------------------------------------------------------
 struct KwfArgs {
-  int nps, npt, ncomp, dipole, strain_it, cap, nglob_s;
-  const int *sol_g;      // [nps] global solid point, or -1 (zeroed)
+  int nps, npt, ncomp, dipole, strain_it, cap, nglob_s, init_only;
+  const int *sol_g[kGroup];  // per wavefield: [nps] global solid point, or -1
   const int *flu_g;      // [npf][10]: chi(0..4, j), then chi(i, 0..4)
   const int *flu_ij;     // [npf]: i | j << 3 | axial << 6
   const float *flu_c;    // [5][npf]: inv_rho, DsDeta, DzDeta, DsDxi, DzDxi (over J)
   const float *G;        // G0 | G1 | G1T | G2 | G2T
-  const float *disp;     // [3][nglob_s]
-  const float *chi;      // [nglob_f]
-  float *ring;           // [cap][ncomp][npt]
+  const float *disp;     // [3][nglob_s][B]
+  const float *chi;      // [nglob_f][B]
+  float *ring[kGroup];   // per wavefield: [cap][ncomp][npt]
   StepRef step;
 };
 
+template <int B>
 __global__ __launch_bounds__(256) void k_kwf(KwfArgs A) {
-  const int iter = step_of(A.step, 0);
-  if (iter % A.strain_it != 0) return;  // uniform: not a snapshot step
-  float *o = A.ring + (size_t)((iter / A.strain_it) % A.cap) * A.ncomp * A.npt;
+  const int b = blockIdx.y;
+  const int iter = step_of(A.step, b);
+  // uniform per block: not a snapshot step of this wavefield
+  if (iter % A.strain_it != 0 || (A.init_only && iter != 0)) return;
+  float *o = A.ring[b] + (size_t)((iter / A.strain_it) % A.cap) * A.ncomp * A.npt;
   const int npf = A.npt - A.nps;
+  const int *sol_g = A.sol_g[b];
   for (int k = blockIdx.x * 256 + threadIdx.x; k < A.npt; k += gridDim.x * 256) {
     float s, p, z;
     if (k < A.nps) {
-      const int g = A.sol_g[k];
+      const int g = sol_g[k];
       if (g < 0) {
         s = p = z = 0.0f;
       } else {
-        const float u1 = A.disp[g], u2 = A.disp[(size_t)A.nglob_s + g],
-                    u3 = A.disp[2 * (size_t)A.nglob_s + g];
+        const float u1 = A.disp[(size_t)g * B + b],
+                    u2 = A.disp[((size_t)A.nglob_s + g) * B + b],
+                    u3 = A.disp[(2 * (size_t)A.nglob_s + g) * B + b];
         if (A.dipole) { s = u1 + u2; p = u1 - u2; z = u3; }
         else { s = u1; p = u2; z = u3; }
       }
@@ -2158,9 +2167,10 @@ __global__ __launch_bounds__(256) void k_kwf(KwfArgs A) {
       const int *id = A.flu_g + (size_t)q * 10;
       float m1 = 0.0f, m2 = 0.0f;
 #pragma unroll
-      for (int kk = 0; kk < 5; kk++) m1 = m1 + GT[i + 5 * kk] * A.chi[id[kk]];
+      for (int kk = 0; kk < 5; kk++) m1 = m1 + GT[i + 5 * kk] * A.chi[(size_t)id[kk] * B + b];
 #pragma unroll
-      for (int kk = 0; kk < 5; kk++) m2 = m2 + A.chi[id[5 + kk]] * G2[kk + 5 * j];
+      for (int kk = 0; kk < 5; kk++)
+        m2 = m2 + A.chi[(size_t)id[5 + kk] * B + b] * G2[kk + 5 * j];
       const float rho = A.flu_c[q], dsdeta = A.flu_c[(size_t)npf + q],
                   dzdeta = A.flu_c[2 * (size_t)npf + q], dsdxi = A.flu_c[3 * (size_t)npf + q],
                   dzdxi = A.flu_c[4 * (size_t)npf + q];
@@ -3522,12 +3532,13 @@ struct KwfTail {
   }
 };
 
-// A kernel-wavefield gather to run with the steps of one wavefield; gen
-// identifies its tables and ring (a replayed graph is captured for one gen).
+// A kernel-wavefield gather to run with the steps of a group's wavefields;
+// gen identifies their tables and rings (a replayed graph is captured for one
+// gen); tail[b] copies wavefield b's snapshots out.
 struct KwfLaunch {
   KwfArgs args;
   uint64_t gen = 0;
-  KwfTail *tail = nullptr;
+  KwfTail *tail[kGroup] = {};
 };
 constexpr int kKwfBlocks = 1024;   // grid of k_kwf (grid-stride over the points)
 
@@ -3536,7 +3547,6 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
                       const axh_ctx *const *ctx, const int32_t *start, int32_t nsteps,
                       float *const *seis, const RunBuffers &RB, GroupTail &T,
                       const KwfLaunch *kw = nullptr) {
-  if (kw && B != 1) fail(AXH_ERR_STATE, "kernel wavefields are gathered for one wavefield");
   if (F.B != B) fail(AXH_ERR_STATE, "group width mismatch");
   const axh_dims &d = ctx[0]->dims;
   const bool multipole = M.src != AXH_MONOPOLE;
@@ -3652,12 +3662,17 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
   if (kw) {
     KA = kw->args;
     KA.step = sref;
-    if (start[0] == 1 && KA.npt > 0) {
-      // the initial fields (iter 0), as dump_stuff(0, ...) before the loop
+    bool any0 = false;
+    for (int b = 0; b < B; b++) any0 = any0 || start[b] == 1;
+    if (any0 && KA.npt > 0) {
+      // the initial fields (iter 0) of the wavefields at their first step, as
+      // dump_stuff(0, ...) before the loop
       KA.step.koff = -1;
+      KA.init_only = 1;
       L(KC_REC, [&] {
-        hipLaunchKernelGGL(k_kwf, dim3(kwf_blocks), dim3(256), 0, st_, KA);
+        hipLaunchKernelGGL(k_kwf<B>, dim3(kwf_blocks, B), dim3(256), 0, st_, KA);
       });
+      KA.init_only = 0;
     }
   }
   FA.nstages = AS.nstages = M.nstages;
@@ -3733,16 +3748,20 @@ static void run_group(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
       });
     // kernel-wavefield snapshot of the step (a graph's steps all launch it;
     // the kernel returns at once where the step is not a snapshot step)
-    if (kw && KA.npt > 0 && (sample_all || (start[0] + step) % KA.strain_it == 0)) {
+    bool snap = kw && KA.npt > 0 && sample_all;
+    for (int b = 0; b < B && kw && KA.npt > 0 && !snap; b++)
+      snap = (start[b] + step) % KA.strain_it == 0;
+    if (snap) {
       KA.step.koff = koff;
       L(KC_REC, [&] {
-        hipLaunchKernelGGL(k_kwf, dim3(kwf_blocks), dim3(256), 0, st_, KA);
+        hipLaunchKernelGGL(k_kwf<B>, dim3(kwf_blocks, B), dim3(256), 0, st_, KA);
       });
     }
   };
   auto reserve = [&](int step, int n) {
     for (int b = 0; b < B; b++) T.reserve(b, start[b] + step - 1, n, st_);
-    if (kw && kw->tail) kw->tail->reserve(start[0] + step - 1, n, st_);
+    for (int b = 0; b < B && kw; b++)
+      if (kw->tail[b]) kw->tail[b]->reserve(start[b] + step - 1, n, st_);
   };
 
   int step = 0;
@@ -3819,9 +3838,9 @@ static void run_any(Runner &R, const MeshDev &M, Fields &F, Member *const *mb,
                     const KwfLaunch *kw = nullptr) {
   switch (F.B) {
     case 1: return run_group<1>(R, M, F, mb, ctx, start, nsteps, seis, RB, T, kw);
-    case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
-    case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
-    case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, RB, T);
+    case 2: return run_group<2>(R, M, F, mb, ctx, start, nsteps, seis, RB, T, kw);
+    case 3: return run_group<3>(R, M, F, mb, ctx, start, nsteps, seis, RB, T, kw);
+    case 4: return run_group<4>(R, M, F, mb, ctx, start, nsteps, seis, RB, T, kw);
     default: fail(AXH_ERR_STATE, "group width out of range");
   }
 }
@@ -3863,7 +3882,7 @@ void device_run(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis) {
 // kernel-wavefield runs (axh_run_kwf)
 // ---------------------------------------------------------------------------
 constexpr size_t kKwfRingBytes = AXH_KWF_RING_BYTES;  // snapshot ring on the device
-constexpr size_t kKwfStageBytes = (size_t)64 << 20;   // one pinned staging half
+constexpr size_t kKwfStageBytes = (size_t)16 << 20;   // one pinned staging half
 
 // The point tables of k_kwf from the context's grid, in point order: the
 // solid points' global ids (-1 in a source element when src_mask), the fluid
@@ -3921,7 +3940,7 @@ static void kwf_build(axh_ctx *ctx, Device &V) {
   const size_t snap = (size_t)K->ncomp * std::max(npt, 1);
   const size_t snap_b = snap * sizeof(float);
   const int total = d.niter / K->strain_it + 1;
-  // staging halves of up to 64 MiB; the ring holds two halves and the
+  // staging halves of up to 16 MiB (at least one snapshot); the ring holds two halves and the
   // snapshots of one replayed graph (at least 2 slots: a run from step 1
   // takes the initial snapshot and the first), at most 1 GiB
   if (ctx->kwf_ring > 0) {
@@ -3945,6 +3964,58 @@ static void kwf_build(axh_ctx *ctx, Device &V) {
   V.kwf = std::move(K);
 }
 
+// The context's gather tables and ring for its current settings (built on
+// first use and when axh_set_kwf changed them).
+static KwfDev &kwf_of(axh_ctx *ctx) {
+  Device &V = *ctx->dev;
+  if (!V.kwf || V.kwf->strain_it != ctx->kwf_strain_it || V.kwf->src_mask != ctx->kwf_src_mask ||
+      V.kwf->ring_req != ctx->kwf_ring)
+    kwf_build(ctx, V);
+  return *V.kwf;
+}
+
+// The gather of a group of B wavefields with tables K[b] on mesh part M and
+// fields F, their snapshots copied to out[b] from step start[b] on.  The
+// wavefields must share the grid, strain_it and ring size.
+static void kwf_group(KwfDev *const *K, int B, const MeshDev &M, const Fields &F,
+                      const int32_t *start, float *const *out, KwfLaunch &kl, KwfTail *kt) {
+  const KwfDev &K0 = *K[0];
+  for (int b = 1; b < B; b++)
+    if (K[b]->nps != K0.nps || K[b]->npf != K0.npf || K[b]->ncomp != K0.ncomp ||
+        K[b]->strain_it != K0.strain_it || K[b]->cap != K0.cap || K[b]->half != K0.half)
+      fail(AXH_ERR_STATE, "the kernel-wavefield grids or settings of a group's members differ");
+  KwfArgs &A = kl.args;
+  A = KwfArgs{};
+  A.nps = K0.nps; A.npt = K0.nps + K0.npf; A.ncomp = K0.ncomp;
+  A.dipole = M.src == AXH_DIPOLE; A.strain_it = K0.strain_it; A.cap = K0.cap;
+  A.nglob_s = M.nglob_s;
+  A.flu_g = K0.flu_g; A.flu_ij = K0.flu_ij; A.flu_c = K0.flu_c;
+  A.G = M.G; A.disp = F.disp; A.chi = F.chi;
+  kl.gen = 0;
+  const size_t snap = (size_t)K0.ncomp * A.npt;
+  for (int b = 0; b < B; b++) {
+    const KwfDev &Kb = *K[b];
+    A.sol_g[b] = Kb.sol_g;
+    A.ring[b] = Kb.ring;
+    kl.gen = kl.gen * 0x9E3779B97F4A7C15ull + Kb.gen;
+    KwfTail &t = kt[b];
+    t = KwfTail();
+    t.s = Kb.strain_it;
+    t.cap = Kb.cap;
+    t.H = Kb.half;
+    t.snap = snap;
+    t.ring = Kb.ring;
+    t.stage[0] = Kb.stage;
+    t.stage[1] = Kb.stage + (size_t)Kb.half * snap;
+    t.ev[0] = Kb.ev[0];
+    t.ev[1] = Kb.ev[1];
+    t.out = out[b];
+    t.first_q = start[b] == 1 ? 0 : (start[b] + Kb.strain_it - 1) / Kb.strain_it;
+    t.qout = t.qdone = t.first_q - 1;
+    kl.tail[b] = A.npt > 0 ? &t : nullptr;
+  }
+}
+
 // Steps it0..it1 with the kernel-wavefield snapshots of their steps (and of
 // the initial fields when it0 = 1) written to kwf in step order.  One queued
 // run: the snapshots are copied out through the pinned staging halves while
@@ -3952,47 +4023,25 @@ static void kwf_build(axh_ctx *ctx, Device &V) {
 void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf) {
   Device &V = *ctx->dev;
   if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
-  if (!V.kwf || V.kwf->strain_it != ctx->kwf_strain_it || V.kwf->src_mask != ctx->kwf_src_mask ||
-      V.kwf->ring_req != ctx->kwf_ring)
-    kwf_build(ctx, V);
-  KwfDev &K = *V.kwf;
+  KwfDev *K[1] = {&kwf_of(ctx)};
   KwfLaunch kl;
-  kl.gen = K.gen;
-  KwfArgs &A = kl.args;
-  A.nps = K.nps; A.npt = K.nps + K.npf; A.ncomp = K.ncomp;
-  A.dipole = V.mesh->src == AXH_DIPOLE; A.strain_it = K.strain_it; A.cap = K.cap;
-  A.nglob_s = V.mesh->nglob_s;
-  A.sol_g = K.sol_g; A.flu_g = K.flu_g; A.flu_ij = K.flu_ij; A.flu_c = K.flu_c;
-  A.G = V.mesh->G; A.disp = V.own.disp; A.chi = V.own.chi; A.ring = K.ring;
-  const size_t snap = (size_t)K.ncomp * A.npt;
-  KwfTail kt;
-  kt.s = K.strain_it;
-  kt.cap = K.cap;
-  kt.H = K.half;
-  kt.snap = snap;
-  kt.ring = K.ring;
-  kt.stage[0] = K.stage;
-  kt.stage[1] = K.stage + (size_t)K.half * snap;
-  kt.ev[0] = K.ev[0];
-  kt.ev[1] = K.ev[1];
-  kt.out = kwf;
-  kt.first_q = it0 == 1 ? 0 : (it0 + K.strain_it - 1) / K.strain_it;
-  kt.qout = kt.qdone = kt.first_q - 1;
-  if (A.npt > 0) kl.tail = &kt;
+  KwfTail kt[1];
+  float *kout[1] = {kwf};
+  const int32_t start[1] = {it0};
+  kwf_group(K, 1, *V.mesh, V.own, start, kout, kl, kt);
   V.run.init();
   V.run.profile = ctx->profile;
   V.run.graph_steps = ctx->graph_steps;
   Member *mb[1] = {&V.member};
   const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
-  const int32_t start[1] = {it0};
   GroupTail T;
   run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, run_buffers(V.own, mb, 1),
           T, &kl);
-  if (kl.tail) kt.finish(V.run.stream);
+  if (kl.tail[0]) kt[0].finish(V.run.stream);
   int blow = INT_MAX;
   ctx->last_run_ms = group_finish(V.run, T, &blow);
-  if (kl.tail) kt.complete();
+  if (kl.tail[0]) kt[0].complete();
   ctx->iter_done = it1;
   if (blow != INT_MAX)
     fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
@@ -4000,11 +4049,14 @@ void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *
 
 // The groups of a batch are queued one after the other, each on its own
 // stream (groups of different source types, or more groups than one width
-// holds, run concurrently), then completed in order.
-void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
+// holds, run concurrently), then completed in order.  With kwf, every
+// member's kernel-wavefield snapshots of the steps go to kwf[member].
+void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis, float *const *kwf) {
   BatchState &S = *bt->state;
   std::vector<int> blow(bt->members.size(), INT_MAX);
   std::vector<GroupTail> tails(S.groups.size());
+  std::vector<KwfLaunch> klaunch(kwf ? S.groups.size() : 0);
+  std::vector<std::array<KwfTail, kGroup>> ktails(kwf ? S.groups.size() : 0);
   for (size_t g = 0; g < S.groups.size(); g++) {
     Runner &R = *S.runs[g];
     R.profile = bt->profile;
@@ -4023,8 +4075,21 @@ void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
       out[b] = seis ? seis[idx[b]] : nullptr;
       start[b] = c->iter_done + 1;
     }
+    KwfLaunch *kl = nullptr;
+    if (kwf) {
+      KwfDev *K[kGroup];
+      float *kout[kGroup];
+      for (size_t b = 0; b < idx.size(); b++) {
+        K[b] = &kwf_of(bt->members[idx[b]]);
+        kout[b] = kwf[idx[b]];
+      }
+      kwf_group(K, (int)idx.size(), M, *S.groups[g], start, kout, klaunch[g], ktails[g].data());
+      kl = &klaunch[g];
+    }
     run_any(R, M, *S.groups[g], mb, cx, start, nsteps, out,
-            run_buffers(*S.groups[g], mb, (int)idx.size()), tails[g]);
+            run_buffers(*S.groups[g], mb, (int)idx.size()), tails[g], kl);
+    for (size_t b = 0; kl && b < idx.size(); b++)
+      if (kl->tail[b]) kl->tail[b]->finish(R.stream);
   }
   // device time of the batch: first start to last end over the streams
   // (times relative to group 0's start)
@@ -4033,6 +4098,8 @@ void batch_run(axh_batch *bt, int32_t nsteps, float *const *seis) {
     int bl[kGroup];
     group_finish(*S.runs[g], tails[g], bl);
     const std::vector<int> &idx = S.index[g];
+    for (size_t b = 0; kwf && b < idx.size(); b++)
+      if (klaunch[g].tail[b]) klaunch[g].tail[b]->complete();
     for (size_t b = 0; b < idx.size(); b++) blow[idx[b]] = bl[b];
     float t0 = 0.0f, t1 = 0.0f;
     HIPCHECK(hipEventElapsedTime(&t0, S.runs[0]->ev0, S.runs[g]->ev0));

@@ -616,6 +616,28 @@ int axh_batch_step(axh_batch *batch, int32_t nsteps, float *const *seis) {
   });
 }
 
+int axh_batch_step_kwf(axh_batch *batch, int32_t nsteps, float *const *seis,
+                       float *const *kwf) {
+  return guarded([&] {
+    check_batch(batch);
+    if (nsteps < 0) fail(AXH_ERR_ARG, "negative step count");
+    if (!kwf) fail(AXH_ERR_ARG, "null kwf array");
+    for (size_t i = 0; i < batch->members.size(); i++) {
+      axh_ctx *c = batch->members[i];
+      if (c->iter_done + nsteps > c->dims.niter)
+        fail(AXH_ERR_ARG, "a member has fewer than nsteps steps left");
+      if (c->kwf_strain_it < 1)
+        fail(AXH_ERR_STATE, "batch member " + std::to_string(i) + " has no axh_set_kwf");
+      const int32_t *np = c->need("kwf_npoints").as<int32_t>();
+      if (!kwf[i] && np[0] + np[1] > 0 &&
+          axh_kwf_snapshots(c, c->iter_done + 1, c->iter_done + nsteps) > 0)
+        fail(AXH_ERR_ARG, "null kwf buffer of batch member " + std::to_string(i));
+    }
+    batch_attach(batch);
+    batch_run(batch, nsteps, seis, kwf);
+  });
+}
+
 int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms) {
   return guarded([&] {
     check_batch(batch);

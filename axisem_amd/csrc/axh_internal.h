@@ -159,7 +159,7 @@ void mesh_bind(axh_ctx *ctx, const axh_mesh *m);
 size_t mesh_bytes(const axh_mesh *m);
 void mesh_release(axh_mesh *m);
 // advance every member nsteps from its own last completed step
-void batch_run(axh_batch *b, int32_t nsteps, float *const *seis);
+void batch_run(axh_batch *b, int32_t nsteps, float *const *seis, float *const *kwf = nullptr);
 bool batch_kernel_stats(const axh_batch *b, int k, double *ms, int64_t *n);
 int symplectic_coefficients(int scheme, double dt, double *cd, double *cv,
                             double *cf);

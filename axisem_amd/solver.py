@@ -158,6 +158,7 @@ def library(fused: bool = False):
         L.axh_set_kwf.argtypes = [P, I32, I32, I32]
         L.axh_kwf_snapshots.argtypes = [P, I32, I32]
         L.axh_run_kwf.argtypes = [P, I32, I32, P, P]
+        L.axh_batch_step_kwf.argtypes = [P, I32, C.POINTER(P), C.POINTER(P)]
         _libs[fused] = L
     return L
 
@@ -522,6 +523,33 @@ class Batch:
                 s.iter += nsteps
         self._ok(rc)
         return [a[:n] for a, n in outs]
+
+    def step_kwf(self, nsteps: int):
+        """step(), plus every member's kernel-wavefield snapshots of its steps
+        (each member's set_kwf and grid): (seis list, kwf list), kwf of shape
+        (nsnap, ncomp, npoint) per member."""
+        outs, ptrs, kouts, kptrs = [], [], [], []
+        for s in self.solvers:
+            n = s._L.axh_seis_samples(s._h, s.iter + 1, s.iter + nsteps)
+            a = np.zeros((max(n, 1), max(s.dims["num_rec"], 1), 3), np.float32)
+            outs.append((a, n))
+            ptrs.append(a.ctypes.data_as(C.c_void_p))
+            nsnap = s._L.axh_kwf_snapshots(s._h, s.iter + 1, s.iter + nsteps)
+            nps, npf = C.c_int32(0), C.c_int32(0)
+            s._ok(s._L.axh_kwf_get_grid(s._h, None, None, C.byref(nps), C.byref(npf)))
+            npt = nps.value + npf.value
+            ncomp = 2 if s.dims["src_type"] == 0 else 3
+            k = np.zeros((max(nsnap, 1), ncomp, max(npt, 1)), np.float32)
+            kouts.append((k, nsnap, npt))
+            kptrs.append(k.ctypes.data_as(C.c_void_p))
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        karr = (C.c_void_p * len(kptrs))(*kptrs)
+        rc = self._L.axh_batch_step_kwf(self._h, nsteps, arr, karr)
+        if rc in (AXH_OK, AXH_ERR_BLOWUP):
+            for s in self.solvers:
+                s.iter += nsteps
+        self._ok(rc)
+        return [a[:n] for a, n in outs], [k[:m, :, :npt] for k, m, npt in kouts]
 
     def last_run_ms(self) -> float:
         v = C.c_double()

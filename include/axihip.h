@@ -332,6 +332,13 @@ int axh_batch_run(axh_batch *batch, int32_t it_begin, int32_t it_end,
  * must have nsteps steps left.  seis as for axh_batch_run, counted over each
  * member's own steps (axh_seis_samples(member, done + 1, done + nsteps)). */
 int axh_batch_step(axh_batch *batch, int32_t nsteps, float *const *seis);
+/* axh_batch_step, and every member's kernel-wavefield snapshots of the steps
+ * it takes written to kwf[i] as axh_run_kwf writes them (the member's
+ * axh_set_kwf and grid; the members of one source type must share the grid,
+ * strain_it and ring size).  The moment-tensor runs of an Instaseis database
+ * stepped as one batch, each with its own dumps. */
+int axh_batch_step_kwf(axh_batch *batch, int32_t nsteps, float *const *seis,
+                       float *const *kwf);
 int axh_batch_last_run_ms(const axh_batch *batch, double *total_ms);
 
 /* Wavefields advanced by one kernel instance (1..AXH_MAX_GROUP, default
@@ -476,7 +483,7 @@ int axh_kwf_get_grid(const axh_ctx *ctx, int32_t *mapping_ijel_ikwf, int32_t *kw
  * SOLVER/wavefields_io.f90:1123-1135), 0 keeps it ('none').  ring_snapshots
  * sets the device ring the snapshots are gathered in (at least 2; 0: chosen
  * by the library, at most AXH_KWF_RING_BYTES); they are copied out half a
- * ring (at most 64 MiB) at a time.  Any time. */
+ * ring (at most 16 MiB, at least one snapshot) at a time.  Any time. */
 #define AXH_KWF_RING_BYTES ((size_t)1 << 30)
 int axh_set_kwf(axh_ctx *ctx, int32_t strain_it, int32_t src_mask, int32_t ring_snapshots);
 
@@ -493,8 +500,8 @@ int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
  * u1 - u2, u3), and in the fluid inv_rho_fluid times the gradient of chi
  * (phi: 0).  The snapshots are gathered on the GPU into a device ring and
  * copied to host memory through pinned staging buffers while the following
- * steps run; kwf is complete when the call returns.  Batches (axh_batch_*)
- * gather no kernel wavefields. */
+ * steps run; kwf is complete when the call returns.  Batches gather them
+ * with axh_batch_step_kwf. */
 int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf);
 
 /* ---- seismogram post-processing ------------------------------------------ */

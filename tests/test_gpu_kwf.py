@@ -195,3 +195,69 @@ def test_dropin_host_writes_snapshots(host_exe, tmp_path):
     ref = _oracle_snapshots(stp, (n - 1) * kit, kit, True)
     np.testing.assert_array_equal(got[:n].view(np.int32), ref.view(np.int32))
     assert np.abs(got[-1]).max() > 0
+
+
+def _solo_snapshots(stp, pieces, strain_it, graph=0):
+    with Solver.from_state_file(stp) as s:
+        s.set_kwf(strain_it)
+        if graph:
+            s.set_graph_steps(graph)
+        return np.concatenate([s.run_kwf(n)[1] for n in pieces])
+
+
+@pytest.mark.parametrize("batch", ["mt4_symplec4_anel", "dipole_symplec4_anel", "mt16_newmark"])
+def test_batch_snapshots_equal_solo_runs(batch, kwf_states):
+    """axh_batch_step_kwf: the moment tensor's four runs (mixed source types),
+    a group of four of one type, and the 16 runs at four depths, stepped as one
+    batch in several calls with graphs: every member's snapshots equal its solo
+    run's bit for bit (and, for the first member, the oracle's)."""
+    from axisem_amd.solver import Batch
+    names = cases.BATCHES[batch]
+    paths = [kwf_states(n) for n in names]
+    pieces = [23, 31]
+    print("kernel wavefields of batch", batch, flush=True)
+    members = [Solver.from_state_file(p) for p in paths]
+    try:
+        for m in members:
+            m.set_kwf(STRAIN_IT)
+        got = [[] for _ in members]
+        with Batch(members, graph_steps=8) as b:
+            for n in pieces:
+                _, ks = b.step_kwf(n)
+                for i, k in enumerate(ks):
+                    got[i].append(k)
+    finally:
+        for m in members:
+            m.close()
+    for i, (nm, p) in enumerate(zip(names, paths)):
+        g = np.concatenate(got[i])
+        ref = _solo_snapshots(p, [sum(pieces)], STRAIN_IT)
+        assert g.shape == ref.shape, nm
+        np.testing.assert_array_equal(g.view(np.int32), ref.view(np.int32), err_msg=nm)
+    ref0 = _oracle_snapshots(paths[0], sum(pieces), STRAIN_IT, True)
+    np.testing.assert_array_equal(np.concatenate(got[0]).view(np.int32), ref0.view(np.int32))
+
+
+def test_batch_snapshots_members_at_different_steps(kwf_states):
+    """A member that joins the batch 10 steps ahead of the others: each
+    member's snapshots continue its solo run's."""
+    from axisem_amd.solver import Batch
+    names = cases.BATCHES["dipole_symplec4_anel"]
+    paths = [kwf_states(n) for n in names]
+    members = [Solver.from_state_file(p) for p in paths]
+    try:
+        for m in members:
+            m.set_kwf(STRAIN_IT)
+        first = members[0].run_kwf(10)[1]
+        with Batch(members) as b:
+            _, ks = b.step_kwf(30)
+    finally:
+        for m in members:
+            m.close()
+    ref0 = _solo_snapshots(paths[0], [10, 30], STRAIN_IT)
+    np.testing.assert_array_equal(np.concatenate([first, ks[0]]).view(np.int32),
+                                  ref0.view(np.int32))
+    for i in range(1, len(names)):
+        ref = _solo_snapshots(paths[i], [30], STRAIN_IT)
+        np.testing.assert_array_equal(ks[i].view(np.int32), ref.view(np.int32),
+                                      err_msg=names[i])
