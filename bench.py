@@ -361,18 +361,26 @@ def kwf_leg(workdir: str, steps: int, warmup: int) -> dict:
     (SOLVER/parameters.F90:1016-1022)."""
     import math
     from axisem_amd.solver import Solver
-    os.makedirs(workdir, exist_ok=True)
-    rundir = os.path.join(workdir, "run_gpu")
-    state = os.path.join(workdir, "state.bin")
-    stamp = os.path.join(workdir, "stamp.json")
-    key = json.dumps({"cfg": repr(WORKLOAD), "host": os.path.getmtime(HOST), "kwf": 1})
-    if not (os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key):
-        inputs.write_run_dir(rundir, MESH, WORKLOAD)
-        t = time.time()
-        inputs.record_state(HOST, rundir, state, kwf_strain_it=1)
-        log("recorded the kernel-wavefield state in %.1fs" % (time.time() - t))
-        with open(stamp, "w") as f:
-            f.write(key)
+    from axisem_amd.solver import Batch
+
+    def record(sub, cfg):
+        wd = os.path.join(workdir, sub)
+        os.makedirs(wd, exist_ok=True)
+        rundir, state = os.path.join(wd, "run_gpu"), os.path.join(wd, "state.bin")
+        stamp = os.path.join(wd, "stamp.json")
+        key = json.dumps({"cfg": repr(cfg), "host": os.path.getmtime(HOST), "kwf": 1})
+        if not (os.path.exists(state) and os.path.exists(stamp) and open(stamp).read() == key):
+            inputs.write_run_dir(rundir, MESH, cfg)
+            inputs.record_state(HOST, rundir, state, kwf_strain_it=1)
+            with open(stamp, "w") as f:
+                f.write(key)
+        return state
+    t = time.time()
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        states = list(ex.map(lambda k: record("d%g" % BATCH_DEPTHS_KM[k], member_config(
+            BATCH_DEPTHS_KM[k])), range(len(BATCH_DEPTHS_KM))))
+    log("kernel-wavefield states in %.1fs" % (time.time() - t))
+    state = states[0]
     st = read_state(state)
     d = st.dims
     gll = 25 * (d["nel_solid"] + d["nel_fluid"])
@@ -414,6 +422,30 @@ def kwf_leg(workdir: str, steps: int, warmup: int) -> dict:
                    "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / avg_s / 1e9,
                    "frac": nbytes / avg_s / 1e9 / HBM_PEAK_GBS},
         "snapshot_copy_GBps": snap_bytes * kwf.shape[0] / max(wall - plain, 1e-9) / 1e9})
+    # the same workload at its four depths as one batch, with and without dumps
+    from axisem_amd.tune import tune_batch
+    bt = tune_batch(states, candidates=[c for c in GRAPH_CANDIDATES if c <= warmup])
+    times = {}
+    for dumps in (False, True):
+        members = [Solver.from_state_file(p) for p in states]
+        try:
+            for m in members:
+                m.set_kwf(strain_it)
+            with Batch(members, group_width=bt["group_width"], graph_steps=bt["graph_steps"]) as b:
+                (b.step_kwf if dumps else b.step)(warmup)
+                t0 = time.perf_counter()
+                r = (b.step_kwf if dumps else b.step)(steps)
+                times[dumps] = time.perf_counter() - t0
+                if dumps:
+                    nsnap4 = sum(k.shape[0] for k in r[1])
+        finally:
+            for m in members:
+                m.close()
+    out["batch4"] = {"schedule": [bt["group_width"], bt["graph_steps"]],
+                     "ms_per_step": times[False] / steps * 1e3,
+                     "ms_per_step_kwf": times[True] / steps * 1e3,
+                     "overhead": times[True] / times[False] - 1.0, "snapshots": nsnap4,
+                     "value_kwf": 4 * gll * steps / times[True]}
     return out
 
 
