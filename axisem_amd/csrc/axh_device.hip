@@ -3190,6 +3190,65 @@ struct GroupTail {
   }
 };
 
+// Copy-out of the kernel-wavefield snapshots of a run.  Snapshot q sits in
+// ring slot q mod cap until its copy is queued; copies are queued in the run's
+// stream order (after the steps that wrote them, before those that reuse the
+// slots) into one of two pinned staging halves of H snapshots, from where the
+// host moves them to the caller's buffer while the GPU steps on.  A half is
+// reused once its previous copy has completed and been moved.
+struct KwfTail {
+  int s = 1, cap = 2, H = 1;
+  size_t snap = 0;              // floats per snapshot
+  const float *ring = nullptr;
+  float *stage[2] = {};
+  hipEvent_t ev[2] = {};
+  int pend_q[2] = {}, pend_n[2] = {};
+  int half = 0;
+  float *out = nullptr;         // the caller's buffer, from snapshot first_q on
+  int first_q = 0, qout = -1, qdone = -1;  // copy queued up to qout; steps up to qdone
+
+  void drain(int h) {
+    if (pend_n[h] == 0) return;
+    HIPCHECK(hipEventSynchronize(ev[h]));
+    std::memcpy(out + (size_t)(pend_q[h] - first_q) * snap, stage[h],
+                (size_t)pend_n[h] * snap * sizeof(float));
+    pend_n[h] = 0;
+  }
+  // queue the copy of snapshots qout+1 .. qout+n (n <= H) into the next half
+  void flush(int n, hipStream_t st) {
+    const int h = half;
+    half ^= 1;
+    drain(h);
+    const int q0 = qout + 1, slot = q0 % cap, n1 = std::min(n, cap - slot);
+    HIPCHECK(hipMemcpyAsync(stage[h], ring + (size_t)slot * snap, (size_t)n1 * snap * sizeof(float),
+                            hipMemcpyDeviceToHost, st));
+    if (n > n1)
+      HIPCHECK(hipMemcpyAsync(stage[h] + (size_t)n1 * snap, ring,
+                              (size_t)(n - n1) * snap * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipEventRecord(ev[h], st));
+    pend_q[h] = q0;
+    pend_n[h] = n;
+    qout += n;
+  }
+  // steps it+1 .. it+n are about to be queued: copy out every H snapshots,
+  // and whatever the launch would otherwise overwrite
+  void reserve(int it, int n, hipStream_t st) {
+    const int q1 = (it + n) / s;
+    while (qdone - qout >= H) flush(H, st);
+    while (q1 - qout > cap && qout < qdone) flush(std::min(H, qdone - qout), st);
+    if (q1 - qout > cap) fail(AXH_ERR_STATE, "kernel-wavefield ring too small for one launch");
+    qdone = q1;
+  }
+  // the copies of the rest, queued; complete() moves every pending half
+  void finish(hipStream_t st) {
+    while (qout < qdone) flush(std::min(H, qdone - qout), st);
+  }
+  void complete() {
+    drain(half);
+    drain(half ^ 1);
+  }
+};
+
 // The kernel-wavefield gather of a context (axh_run_kwf): the point tables of
 // k_kwf and its snapshot ring, for one grid, strain_it and source mask.
 struct KwfDev {
@@ -3222,6 +3281,8 @@ struct Device {
   int live_slot = 0;
   GroupTail pending;       // a run queued by axh_run_begin, until axh_run_end
   int32_t pending_it1 = 0;
+  KwfTail pending_kwf;     // its kernel-wavefield copies (axh_run_kwf_begin)
+  bool pending_has_kwf = false;
   std::unique_ptr<KwfDev> kwf;  // built by the first axh_run_kwf
   Runner run;  // declared last: its stream is drained before buffers are freed
 };
@@ -3473,65 +3534,6 @@ static void dispatch_memvar(const MeshDev &M, hipStream_t s, const SolidArgs &A,
 // captured once into a graph of the group (ending with the advance of the
 // step base by S) that is replayed for every S steps of the run; the steps
 // left over are launched directly.  Both give the same results bit for bit.
-// Copy-out of the kernel-wavefield snapshots of a run.  Snapshot q sits in
-// ring slot q mod cap until its copy is queued; copies are queued in the run's
-// stream order (after the steps that wrote them, before those that reuse the
-// slots) into one of two pinned staging halves of H snapshots, from where the
-// host moves them to the caller's buffer while the GPU steps on.  A half is
-// reused once its previous copy has completed and been moved.
-struct KwfTail {
-  int s = 1, cap = 2, H = 1;
-  size_t snap = 0;              // floats per snapshot
-  const float *ring = nullptr;
-  float *stage[2] = {};
-  hipEvent_t ev[2] = {};
-  int pend_q[2] = {}, pend_n[2] = {};
-  int half = 0;
-  float *out = nullptr;         // the caller's buffer, from snapshot first_q on
-  int first_q = 0, qout = -1, qdone = -1;  // copy queued up to qout; steps up to qdone
-
-  void drain(int h) {
-    if (pend_n[h] == 0) return;
-    HIPCHECK(hipEventSynchronize(ev[h]));
-    std::memcpy(out + (size_t)(pend_q[h] - first_q) * snap, stage[h],
-                (size_t)pend_n[h] * snap * sizeof(float));
-    pend_n[h] = 0;
-  }
-  // queue the copy of snapshots qout+1 .. qout+n (n <= H) into the next half
-  void flush(int n, hipStream_t st) {
-    const int h = half;
-    half ^= 1;
-    drain(h);
-    const int q0 = qout + 1, slot = q0 % cap, n1 = std::min(n, cap - slot);
-    HIPCHECK(hipMemcpyAsync(stage[h], ring + (size_t)slot * snap, (size_t)n1 * snap * sizeof(float),
-                            hipMemcpyDeviceToHost, st));
-    if (n > n1)
-      HIPCHECK(hipMemcpyAsync(stage[h] + (size_t)n1 * snap, ring,
-                              (size_t)(n - n1) * snap * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipEventRecord(ev[h], st));
-    pend_q[h] = q0;
-    pend_n[h] = n;
-    qout += n;
-  }
-  // steps it+1 .. it+n are about to be queued: copy out every H snapshots,
-  // and whatever the launch would otherwise overwrite
-  void reserve(int it, int n, hipStream_t st) {
-    const int q1 = (it + n) / s;
-    while (qdone - qout >= H) flush(H, st);
-    while (q1 - qout > cap && qout < qdone) flush(std::min(H, qdone - qout), st);
-    if (q1 - qout > cap) fail(AXH_ERR_STATE, "kernel-wavefield ring too small for one launch");
-    qdone = q1;
-  }
-  // the copies of the rest, queued; complete() moves every pending half
-  void finish(hipStream_t st) {
-    while (qout < qdone) flush(std::min(H, qdone - qout), st);
-  }
-  void complete() {
-    drain(half);
-    drain(half ^ 1);
-  }
-};
-
 // A kernel-wavefield gather to run with the steps of a group's wavefields;
 // gen identifies their tables and rings (a replayed graph is captured for one
 // gen); tail[b] copies wavefield b's snapshots out.
@@ -3868,6 +3870,10 @@ void device_run_end(axh_ctx *ctx) {
   Device &V = *ctx->dev;
   int blow = INT_MAX;
   ctx->last_run_ms = group_finish(V.run, V.pending, &blow);
+  if (V.pending_has_kwf) {
+    V.pending_has_kwf = false;
+    V.pending_kwf.complete();
+  }
   ctx->iter_done = V.pending_it1;
   if (blow != INT_MAX)
     fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
@@ -4016,35 +4022,36 @@ static void kwf_group(KwfDev *const *K, int B, const MeshDev &M, const Fields &F
   }
 }
 
-// Steps it0..it1 with the kernel-wavefield snapshots of their steps (and of
-// the initial fields when it0 = 1) written to kwf in step order.  One queued
-// run: the snapshots are copied out through the pinned staging halves while
-// the following steps run (KwfTail).
-void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf) {
+// Queues steps it0..it1 with the kernel-wavefield snapshots of their steps
+// (and of the initial fields when it0 = 1), written to kwf in step order;
+// device_run_end completes them.  The snapshots are copied out through the
+// pinned staging halves while the following steps run (KwfTail); between the
+// two calls the host is free, and seis and kwf must stay untouched.
+void device_run_kwf_begin(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf) {
   Device &V = *ctx->dev;
   if (V.live) fail(AXH_ERR_STATE, "the context is stepped by a batch");
   KwfDev *K[1] = {&kwf_of(ctx)};
   KwfLaunch kl;
-  KwfTail kt[1];
   float *kout[1] = {kwf};
   const int32_t start[1] = {it0};
-  kwf_group(K, 1, *V.mesh, V.own, start, kout, kl, kt);
+  kwf_group(K, 1, *V.mesh, V.own, start, kout, kl, &V.pending_kwf);
+  V.pending_has_kwf = kl.tail[0] != nullptr;
   V.run.init();
   V.run.profile = ctx->profile;
   V.run.graph_steps = ctx->graph_steps;
   Member *mb[1] = {&V.member};
   const axh_ctx *cx[1] = {ctx};
   float *out[1] = {seis};
-  GroupTail T;
+  V.pending = GroupTail();
   run_any(V.run, *V.mesh, V.own, mb, cx, start, it1 - it0 + 1, out, run_buffers(V.own, mb, 1),
-          T, &kl);
-  if (kl.tail[0]) kt[0].finish(V.run.stream);
-  int blow = INT_MAX;
-  ctx->last_run_ms = group_finish(V.run, T, &blow);
-  if (kl.tail[0]) kt[0].complete();
-  ctx->iter_done = it1;
-  if (blow != INT_MAX)
-    fail(AXH_ERR_BLOWUP, "displacements blew up at time step " + std::to_string(blow));
+          V.pending, &kl);
+  if (V.pending_has_kwf) V.pending_kwf.finish(V.run.stream);
+  V.pending_it1 = it1;
+}
+
+void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf) {
+  device_run_kwf_begin(ctx, it0, it1, seis, kwf);
+  device_run_end(ctx);
 }
 
 // The groups of a batch are queued one after the other, each on its own

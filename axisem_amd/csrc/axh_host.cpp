@@ -885,18 +885,32 @@ int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end) {
   return it_end / s - (it_begin - 1) / s + (it_begin == 1 ? 1 : 0);
 }
 
+static void check_kwf_run(axh_ctx *ctx, int32_t it_begin, int32_t it_end, const float *kwf,
+                          const char *fn) {
+  check_ctx(ctx);
+  check_idle(ctx);
+  check_range(ctx, it_begin, it_end, fn);
+  if (ctx->kwf_strain_it < 1) fail(AXH_ERR_STATE, std::string(fn) + " needs axh_set_kwf first");
+  const int32_t *np = ctx->need("kwf_npoints").as<int32_t>();
+  if (!kwf && axh_kwf_snapshots(ctx, it_begin, it_end) > 0 && np[0] + np[1] > 0)
+    fail(AXH_ERR_ARG, "null kwf buffer");
+}
+
 int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf) {
   return guarded([&] {
-    check_ctx(ctx);
-    check_idle(ctx);
-    check_range(ctx, it_begin, it_end, "axh_run_kwf");
-    if (ctx->kwf_strain_it < 1) fail(AXH_ERR_STATE, "axh_run_kwf needs axh_set_kwf first");
-    ctx->need("kwf_npoints");
-    if (!kwf && axh_kwf_snapshots(ctx, it_begin, it_end) > 0 &&
-        ctx->need("kwf_npoints").as<int32_t>()[0] + ctx->need("kwf_npoints").as<int32_t>()[1] > 0)
-      fail(AXH_ERR_ARG, "null kwf buffer");
+    check_kwf_run(ctx, it_begin, it_end, kwf, "axh_run_kwf");
     if (!ctx->dev) device_build(ctx);
     device_run_kwf(ctx, it_begin, it_end, seis, kwf);
+  });
+}
+
+int axh_run_kwf_begin(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis,
+                      float *kwf) {
+  return guarded([&] {
+    check_kwf_run(ctx, it_begin, it_end, kwf, "axh_run_kwf_begin");
+    if (!ctx->dev) device_build(ctx);
+    device_run_kwf_begin(ctx, it_begin, it_end, seis, kwf);
+    ctx->running = true;
   });
 }
 

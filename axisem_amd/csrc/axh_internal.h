@@ -131,6 +131,7 @@ void device_run_begin(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
 void device_run_end(axh_ctx *ctx);
 void device_run_noalloc(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis);
 void device_run_kwf(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf);
+void device_run_kwf_begin(axh_ctx *ctx, int32_t it0, int32_t it1, float *seis, float *kwf);
 void convolve_traces(int nt, int ntraces, const float *seis, int nj, const float *stf,
                      double dt, double scale, float *out);
 int noalloc_max_receivers();

@@ -185,6 +185,14 @@ module axh_iface
        real(c_float), intent(out) :: seis(*), kwf(*)
      end function
 
+     integer(c_int) function axh_run_kwf_begin(ctx, it_begin, it_end, seis, kwf) &
+          bind(C, name='axh_run_kwf_begin')
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: it_begin, it_end
+       real(c_float) :: seis(*), kwf(*)
+     end function
+
      integer(c_int) function axh_download_fields(ctx, disp, velo, chi, dchi, ddchi) &
           bind(C, name='axh_download_fields')
        import :: c_int, c_ptr, c_float

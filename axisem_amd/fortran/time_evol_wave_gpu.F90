@@ -192,10 +192,15 @@ subroutine time_loop
   tloc = zero
   isamp_iter = 0
   it0 = 1
-  if (.not. (dump_vtk .or. dump_xdmf .or. dump_energy) .and. kwf_it == 0) then
+  if (.not. (dump_vtk .or. dump_xdmf .or. dump_energy)) then
      ! nothing needs the fields between chunks: the next chunk runs on the GPU
-     ! while this one's samples are written
-     call time_loop_overlapped(seis, chunk, tloc)
+     ! while this one's samples (and kernel-wavefield snapshots) are written
+     if (kwf_it == 0) then
+        ncomp = 0
+        npt = 0
+        allocate(kwf(1))
+     endif
+     call time_loop_overlapped(seis, kwf, chunk, tloc, ncomp, npt)
      it0 = niter + 1
   endif
   do while (it0 <= niter)
@@ -414,21 +419,20 @@ end subroutine write_samples
 !! (axh_run_begin) before the samples of chunk k are written, and collected
 !! (axh_run_end) after, so the writes of the reference's seismogram files overlap the
 !! time steps.  Same files as one chunk after the other.
-subroutine time_loop_overlapped(seis, chunk, tloc)
+subroutine time_loop_overlapped(seis, kwf, chunk, tloc, ncomp, npt)
 
-  real(kind=realkind), allocatable, intent(inout) :: seis(:)
-  integer, intent(in)                             :: chunk
+  real(kind=realkind), allocatable, intent(inout) :: seis(:), kwf(:)
+  integer, intent(in)                             :: chunk, ncomp, npt
   real(kind=dp), intent(inout)                    :: tloc
 
-  real(kind=realkind), allocatable :: next(:), tmp(:)
-  integer                          :: it0, it1, n0, n1, k, nsamp
+  real(kind=realkind), allocatable :: next(:), tmp(:), knext(:)
+  integer                          :: it0, it1, n0, n1, k, nsamp, c
   integer(c_int)                   :: ierr
 
-  allocate(next(size(seis)))
+  allocate(next(size(seis)), knext(size(kwf)))
   it0 = 1
   it1 = min(niter, chunk)
-  call axh_check(axh_run_begin(ctx, int(it0, c_int32_t), int(it1, c_int32_t), seis), &
-                 'axh_run_begin')
+  call queue_chunk(it0, it1, seis, kwf)
   do
      ierr = axh_run_end(ctx)
      if (ierr == AXH_ERR_BLOWUP) then
@@ -439,9 +443,15 @@ subroutine time_loop_overlapped(seis, chunk, tloc)
      call axh_check(ierr, 'axh_run_end')
      n0 = it1 + 1
      n1 = min(niter, it1 + chunk)
-     if (n0 <= niter) &
-        call axh_check(axh_run_begin(ctx, int(n0, c_int32_t), int(n1, c_int32_t), next), &
-                       'axh_run_begin')
+     if (n0 <= niter) call queue_chunk(n0, n1, next, knext)
+     ! the kernel-wavefield snapshots of steps it0..it1, variable by variable
+     if (kwf_it > 0) then
+        do k = 1, axh_kwf_snapshots(ctx, int(it0, c_int32_t), int(it1, c_int32_t))
+           do c = 1, ncomp
+              write(13200 + c) kwf(1 + ((k - 1) * ncomp + c - 1) * npt:((k - 1) * ncomp + c) * npt)
+           enddo
+        enddo
+     endif
      ! the samples of steps it0..it1, with the reference's time accumulation
      nsamp = 0
      do k = it0, it1
@@ -460,10 +470,27 @@ subroutine time_loop_overlapped(seis, chunk, tloc)
      call move_alloc(seis, tmp)
      call move_alloc(next, seis)
      call move_alloc(tmp, next)
+     call move_alloc(kwf, tmp)
+     call move_alloc(knext, kwf)
+     call move_alloc(tmp, knext)
      it0 = n0
      it1 = n1
   enddo
-  deallocate(next)
+  deallocate(next, knext)
+
+contains
+
+  subroutine queue_chunk(a, b, s, kb)
+    integer, intent(in)                :: a, b
+    real(kind=realkind), intent(inout) :: s(:), kb(:)
+    if (kwf_it > 0) then
+       call axh_check(axh_run_kwf_begin(ctx, int(a, c_int32_t), int(b, c_int32_t), s, kb), &
+                      'axh_run_kwf_begin')
+    else
+       call axh_check(axh_run_begin(ctx, int(a, c_int32_t), int(b, c_int32_t), s), &
+                      'axh_run_begin')
+    endif
+  end subroutine queue_chunk
 
 end subroutine time_loop_overlapped
 !-----------------------------------------------------------------------------------------

@@ -159,6 +159,7 @@ def library(fused: bool = False):
         L.axh_kwf_snapshots.argtypes = [P, I32, I32]
         L.axh_run_kwf.argtypes = [P, I32, I32, P, P]
         L.axh_batch_step_kwf.argtypes = [P, I32, C.POINTER(P), C.POINTER(P)]
+        L.axh_run_kwf_begin.argtypes = [P, I32, I32, P, P]
         _libs[fused] = L
     return L
 
@@ -333,7 +334,7 @@ class Solver:
         return self._pending[0][:n]
 
     def run_end(self):
-        seis, n, it1 = self._pending
+        it1 = self._pending[2]
         self._pending = None
         rc = self._L.axh_run_end(self._h)
         if rc in (AXH_OK, AXH_ERR_BLOWUP):
@@ -368,6 +369,22 @@ class Solver:
         ring_snapshots bounds the device ring (0: automatic)."""
         self._ok(self._L.axh_set_kwf(self._h, int(strain_it), int(bool(src_mask)),
                                      int(ring_snapshots)))
+
+    def run_kwf_begin(self, nsteps: int):
+        """run_kwf queued: returns (seis, kwf) at once, filled after run_end()."""
+        it0 = self.iter + 1
+        it1 = min(self.dims["niter"], self.iter + nsteps)
+        n = self._L.axh_seis_samples(self._h, it0, it1)
+        seis = np.zeros((max(n, 1), max(self.dims["num_rec"], 1), 3), np.float32)
+        nsnap = self._L.axh_kwf_snapshots(self._h, it0, it1)
+        nps, npf = C.c_int32(0), C.c_int32(0)
+        self._ok(self._L.axh_kwf_get_grid(self._h, None, None, C.byref(nps), C.byref(npf)))
+        ncomp = 2 if self.dims["src_type"] == 0 else 3
+        npt = nps.value + npf.value
+        kwf = np.zeros((max(nsnap, 1), ncomp, max(npt, 1)), np.float32)
+        self._pending = (seis, n, it1, kwf)
+        self._ok(self._L.axh_run_kwf_begin(self._h, it0, it1, _p(seis), _p(kwf)))
+        return seis[:n], kwf[:nsnap, :, :npt]
 
     def run_kwf(self, nsteps: int | None = None):
         """run(), plus the kernel-wavefield snapshots of the steps it covers:

@@ -503,6 +503,11 @@ int axh_kwf_snapshots(const axh_ctx *ctx, int32_t it_begin, int32_t it_end);
  * steps run; kwf is complete when the call returns.  Batches gather them
  * with axh_batch_step_kwf. */
 int axh_run_kwf(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis, float *kwf);
+/* The same queued (as axh_run_begin): axh_run_end completes it; until then the
+ * host is free (e.g. to write the previous chunk's snapshots) and seis and kwf
+ * must stay untouched. */
+int axh_run_kwf_begin(axh_ctx *ctx, int32_t it_begin, int32_t it_end, float *seis,
+                      float *kwf);
 
 /* ---- seismogram post-processing ------------------------------------------ */
 

@@ -131,6 +131,24 @@ def test_gpu_snapshot_runs_split_and_replayed(mode, kwf_states):
         np.testing.assert_array_equal(s.run(steps), plain_seis)
 
 
+def test_queued_kwf_runs_equal_one_run(kwf_states):
+    """axh_run_kwf_begin / axh_run_end in chunks (the host free between them)
+    give the snapshots and samples of one axh_run_kwf."""
+    stp = kwf_states("dipole_thetaforce_newmark_anel")
+    ref, ref_seis = _gpu_snapshots(stp, [75], STRAIN_IT, True)
+    snaps, seis = [], []
+    with Solver.from_state_file(stp) as s:
+        s.set_kwf(STRAIN_IT)
+        s.set_graph_steps(8)
+        for n in (30, 45):
+            a, k = s.run_kwf_begin(n)
+            s.run_end()
+            seis.append(a)
+            snaps.append(k)
+    np.testing.assert_array_equal(np.concatenate(snaps).view(np.int32), ref.view(np.int32))
+    np.testing.assert_array_equal(np.concatenate(seis), ref_seis)
+
+
 def test_gpu_snapshots_all_points_radius_limited(kwf_states):
     """The reference's grid over all 25 points of the elements between 2000 and
     5000 km radius (KERNEL_IBEG..JEND 0..4, KERNEL_RMIN/RMAX): points shared by
