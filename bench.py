@@ -553,12 +553,23 @@ def mt_leg(workdir: str, steps: int, warmup: int) -> dict:
 # PREM) at 2 s, anelastic 5 SLS, symplec4, the four moment-tensor runs at
 # 100 km batched, 1800 s at 181 receivers.  Opt-in (--config5): the mesh
 # (2.5 M elements) and the three setups (one per source type) take the host
-# about ten minutes.
+# about ten minutes; --config5-prepare does only that host part (no GPU), so
+# that it can run beside other work, and --config5-only then times the GPU
+# part on the states it left.
 C5_MESH = os.path.join(REPO, "data", "meshes", "prem_ani_2s")
+C5_KINDS = {"monopole": ("mrr", "mtt_p_mpp"), "dipole": ("mtr",), "quadpole": ("mtp",)}
+C5_RUNS = ("mrr", "mtt_p_mpp", "mtr", "mtp")
 
 
-def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
-    from axisem_amd.solver import Batch, Solver
+def config5_states(workdir: str) -> dict:
+    """The mesh and the four runs' states (host only).  Returns the state
+    paths and the host times; states a previous call completed are reused."""
+    done = os.path.join(workdir, "states.json")
+    if os.path.exists(done):
+        with open(done) as f:
+            info = json.load(f)
+        if all(os.path.exists(p) for p in info["paths"].values()):
+            return info
     t0 = time.time()
     if not os.path.exists(os.path.join(C5_MESH, "meshdb.dat0000")):
         inputs.run_mesher(os.path.join(REPO, "oracle", "_ref", "xmesh"), C5_MESH, 2.0,
@@ -571,31 +582,46 @@ def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
                                 time_scheme="symplec4", attenuation=True,
                                 seismogram_length_s=1800.0,
                                 stations_deg=[float(k) for k in range(181)])
-    kinds = {"monopole": ("mrr", "mtt_p_mpp"), "dipole": ("mtr",), "quadpole": ("mtp",)}
     os.makedirs(workdir, exist_ok=True)
-    paths = {r: os.path.join(workdir, r + ".bin") for rs in kinds.values() for r in rs}
+    paths = {r: os.path.join(workdir, r + ".bin") for rs in C5_KINDS.values() for r in rs}
     t1 = time.time()
 
     def rec(item):
         kind, runs = item
+        t = time.time()
         inputs.record_states(HOST, os.path.join(workdir, "run_" + kind), C5_MESH,
                              [cfg(r) for r in runs], [paths[r] for r in runs])
-    with ThreadPoolExecutor(max_workers=3) as ex:
-        list(ex.map(rec, kinds.items()))
+        log("config 5: %s state(s) %s in %.0f s" % (kind, ",".join(runs), time.time() - t))
+    with ThreadPoolExecutor(max_workers=len(C5_KINDS)) as ex:
+        list(ex.map(rec, C5_KINDS.items()))
     tsetup = time.time() - t1
     log("config 5: states in %.0f s" % tsetup)
+    info = {"paths": paths, "host_mesh_s": tmesh, "host_setup_s": tsetup,
+            "state_bytes": {r: os.path.getsize(p) for r, p in paths.items()}}
+    with open(done, "w") as f:
+        json.dump(info, f)
+    return info
+
+
+def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
+    from axisem_amd.solver import Batch, Solver
+    info = config5_states(workdir)
+    paths = info["paths"]
     d = read_dims(paths["mtr"])
     gll = 25 * (d["nel_solid"] + d["nel_fluid"])
     out = {"workload": "prem_ani 2 s, NPOL 4, anelastic 5 SLS (cg4), symplec4, moment-tensor "
                        "runs mrr, mtt_p_mpp, mtr, mtp at 100 km, 1800 s, 181 receivers",
            "nel_solid": d["nel_solid"], "nel_fluid": d["nel_fluid"],
            "niter_full_run": d["niter"], "deltat": d["deltat"], "steps": steps,
-           "host_mesh_s": tmesh, "host_setup_s": tsetup}
+           "host_mesh_s": info["host_mesh_s"], "host_setup_s": info["host_setup_s"],
+           "state_bytes": info["state_bytes"]}
     import ctypes as C
     from axisem_amd import solver as S
     plain = {}
-    for r in ("mrr", "mtt_p_mpp", "mtr", "mtp"):
+    for r in C5_RUNS:
+        t = time.time()
         with Solver.from_state_file(paths[r]) as s:
+            log("config 5: %s uploaded in %.0f s" % (r, time.time() - t))
             mb, wb = C.c_size_t(), C.c_size_t()
             S._check(S.library().axh_footprint(s._h, C.byref(mb), C.byref(wb)))
             out.setdefault("footprint_bytes", {})[r] = {"mesh": mb.value, "wavefield": wb.value}
@@ -605,8 +631,9 @@ def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
             s.run(steps)
             plain[r] = (time.perf_counter() - t) / steps * 1e3
             out.setdefault("device_bytes", {})[r] = s.device_bytes()
+        log("config 5: %s alone %.3f ms per step" % (r, plain[r]))
     out["plain_ms_per_step"] = plain
-    members = [Solver.from_state_file(paths[r]) for r in ("mrr", "mtt_p_mpp", "mtr", "mtp")]
+    members = [Solver.from_state_file(paths[r]) for r in C5_RUNS]
     try:
         with Batch(members, graph_steps=4) as b:
             b.run(warmup)
@@ -619,6 +646,7 @@ def config5_leg(workdir: str, steps: int, warmup: int) -> dict:
             m.close()
     out["batch4"] = {"ms_per_step": ms, "value": 4 * gll / (ms * 1e-3),
                      "unit": "GLL-point updates/s", "vs_plain": sum(plain.values()) / ms}
+    out["full_run_estimate_s"] = d["niter"] * ms * 1e-3
     return out
 
 
@@ -712,6 +740,11 @@ def main():
     ap.add_argument("--config5", action="store_true",
                     help="also the large single-GPU configuration (prem_ani 2 s; ~10 min "
                          "of host setup)")
+    ap.add_argument("--config5-prepare", action="store_true",
+                    help="only the host part of the large configuration: mesh and states "
+                         "(no GPU; one JSON line)")
+    ap.add_argument("--config5-only", action="store_true",
+                    help="only the large configuration (one JSON line)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1),
                     help="independent reference runs of the all-cores CPU leg")
@@ -731,6 +764,14 @@ def main():
         only = [x for x in a.sweep_only.split(",") if x]
         print(json.dumps({"sweep": sweep(os.path.join(REPO, "data", "bench_sweep"),
                                          only=only)}))
+        return
+    if a.config5_prepare:
+        print(json.dumps({"config5_states": config5_states(os.path.join(REPO, "data",
+                                                                        "bench_c5"))}))
+        return
+    if a.config5_only:
+        print(json.dumps({"config5": config5_leg(os.path.join(REPO, "data", "bench_c5"),
+                                                 a.steps, a.warmup)}))
         return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the time loop keeps the whole mesh resident on "
