@@ -519,15 +519,25 @@ __device__ __forceinline__ void memvar_sources(const float *cu, int q, const flo
   }
 }
 
+// Position of deviator component c among the sources of an element: all 6
+// (NSRC = 28), or for a monopole 0, 1, 2 and 4 (NSRC = 20; its components 3
+// and 5 are not updated and take no slot: -1).
+template <int NSRC>
+__device__ __forceinline__ constexpr int src_slot(int c) {
+  return NSRC == 28 ? c : (c < 3 ? c : (c == 4 ? 3 : -1));
+}
+
 // Strain sources of the memory-variable update at cg point q of element e
 // for wavefield b of the group (lanes k < 4 of the element, one cg point
 // each).  The sources of the previous update go to old, the new ones to src
-// and to the t-1 store; src/old are [28][B] per element.
-template <int SRC, int B>
+// and to the t-1 store; src/old are [NSRC][B] per element: the trace at the
+// 4 cg points, then the deviator components in src_slot order.
+template <int SRC, int B, int NSRC = 28>
 __device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *sG, bool axial,
                                                int b, int e, int q, const float *U1,
                                                const float *U2, const float *U3, float *src,
                                                float *old) {
+  static_assert(NSRC == 28 || SRC == AXH_MONOPOLE, "compact sources are the monopole's");
   const float *cu = A.cg_upd + (size_t)e * (kCgUpd * 4);
   float g[6], str, sdev[6];
   strain_cg4<SRC, B>(sG, axial, cu, q, U1 + b, U2 + b, U3 + b, g);
@@ -536,10 +546,13 @@ __device__ __forceinline__ void memvar_prepare(const SolidArgs &A, const float *
   float *devm1 = A.devm1 + (size_t)e * 24 * B;
   old[q * B + b] = trm1[q * B + b];
 #pragma unroll
-  for (int c = 0; c < 6; c++) old[(4 + 4 * c + q) * B + b] = devm1[(4 * c + q) * B + b];
+  for (int c = 0; c < 6; c++)
+    if (src_slot<NSRC>(c) >= 0)
+      old[(4 + 4 * src_slot<NSRC>(c) + q) * B + b] = devm1[(4 * c + q) * B + b];
   src[q * B + b] = str;
 #pragma unroll
-  for (int c = 0; c < 6; c++) src[(4 + 4 * c + q) * B + b] = sdev[c];
+  for (int c = 0; c < 6; c++)
+    if (src_slot<NSRC>(c) >= 0) src[(4 + 4 * src_slot<NSRC>(c) + q) * B + b] = sdev[c];
   trm1[q * B + b] = str;
 #pragma unroll
   for (int c = 0; c < 6; c++) devm1[(4 * c + q) * B + b] = sdev[c];
@@ -673,16 +686,16 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   static_assert(kElemPerTile * 24 <= 256, "SLS sums must fit a product buffer");
   float *const sR = &sS[0][0];
   float *const sSa = &sS[1][0];
-  // the strain sources of the Newmark memory-variable update: for 6 product
-  // buffers they go over buffers 2..5, dead once the second products are
-  // summed (the SLS sums use buffers 0 and 1); a dipole group of 4 then fits
-  // 4 workgroups per CU instead of 3
-  constexpr bool SRC_IN_S = MV && NS == 6;
-  static_assert(2 * kElemPerTile * 28 <= 4 * 256, "sources must fit four product buffers");
-  __shared__ __align__(16) float sSrcOwn[(MV && !SRC_IN_S) ? kElemPerTile * 28 * B : 1];
-  __shared__ __align__(16) float sOldOwn[(MV && !SRC_IN_S) ? kElemPerTile * 28 * B : 1];
-  float *const sSrc = SRC_IN_S ? &sS[2][0] : sSrcOwn;
-  float *const sOld = SRC_IN_S ? &sS[2][0] + kElemPerTile * 28 * B : sOldOwn;
+  // the strain sources of the Newmark memory-variable update go over the
+  // product buffers from 2 on, dead once the second products are summed (the
+  // SLS sums use buffers 0 and 1): a dipole group of 4 then fits 4 workgroups
+  // per CU instead of 3.  The monopole has 4 buffers and keeps NSRC = 20
+  // sources per element and wavefield (its components 3 and 5 are not
+  // updated: src_slot), which fit the 2 left
+  constexpr int NSRC = (SRC == AXH_MONOPOLE) ? 20 : 28;
+  static_assert(2 * kElemPerTile * NSRC <= (NS - 2) * 256, "sources must fit the free buffers");
+  float *const sSrc = &sS[2][0];
+  float *const sOld = &sS[2][0] + kElemPerTile * NSRC * B;
   // ts_fac_t(jj) a_j and ts_fac_tm1(jj) a_j per element (mu, then kappa)
   __shared__ double sCf[MV ? kElemPerTile : 1][MV ? MS : 1][4];
   __shared__ __align__(16) float sG[105];
@@ -1127,11 +1140,11 @@ void k_solid(SolidArgs A, FluidArgs FA) {
     if (MODE == 0) {
       // time_step_memvars_cg4 with the masked predicted displacement; lane
       // k: cg point k % 4 of wavefield k / 4
-      float *src = &sSrc[le * 28 * B], *old = &sOld[le * 28 * B];
+      float *src = &sSrc[le * NSRC * B], *old = &sOld[le * NSRC * B];
       const float *V1 = U0, *V2 = (NCOMP == 2 ? U0 : U1), *V3 = (NCOMP == 2 ? U1 : U2);
       if (valid && k < 4 * B) {
         // lane k: cg point k % 4 of wavefield k / 4
-        memvar_prepare<SRC, B>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
+        memvar_prepare<SRC, B, NSRC>(A, sG, axial, k >> 2, e, k & 3, V1, V2, V3, src, old);
       }
       __syncthreads();
       // time_step_memvars_cg4's update of the entries (k, b) this lane read
@@ -1140,11 +1153,12 @@ void k_solid(SolidArgs A, FluidArgs FA) {
       const int c = k >> 2, q = k & 3;
       if (valid && k < 24 && !(SRC == AXH_MONOPOLE && (c == 3 || c == 5))) {
         float *mv = A.memvar + (size_t)e * 24 * A.nsls * B;
+        const int sk = 4 + 4 * src_slot<NSRC>(c) + q;
         double sd[B], od[B], st[B], ot[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
-          sd[b] = (double)src[(4 + k) * B + b];
-          od[b] = (double)old[(4 + k) * B + b];
+          sd[b] = (double)src[sk * B + b];
+          od[b] = (double)old[sk * B + b];
           st[b] = (double)src[q * B + b];
           ot[b] = (double)old[q * B + b];
         }
