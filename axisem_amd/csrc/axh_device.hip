@@ -64,6 +64,11 @@ namespace axh {
 
 // number of packed element terms per source type (order: see pack_terms)
 constexpr int kNcMono = 15, kNcDi = 24, kNcQuad = 26;
+// the packed term that an isotropic model leaves +0 at every point: M_w2 of
+// a dipole, M_w3 of a quadrupole (-1: none)
+constexpr int zero_term_of(int src) {
+  return src == AXH_DIPOLE ? 9 : (src == AXH_QUADPOLE ? 10 : -1);
+}
 // tiles of a group whose points include a source point of the group, passed
 // to the stiffness kernel by value.  Every element holding a copy of a source
 // point is a source element (the wavefield layout rejects a nonzero term
@@ -148,6 +153,7 @@ struct MeshDev {
   float *G = nullptr;  // G0 | G1 | G1T | G2 | G2T
   // solid
   float *coef_s = nullptr, *m0_s = nullptr, *minv_s = nullptr;
+  bool zero_term = false;  // the term zero_term_of(src) is +0 at every point
   int *einfo_s = nullptr, *gid_s = nullptr, *gstart_s = nullptr, *gslot_s = nullptr;
   // fluid
   float *coef_f = nullptr, *m0_f = nullptr, *minv_f = nullptr;
@@ -669,7 +675,9 @@ __device__ __forceinline__ void fluid_tile(const FluidArgs &A, int tile, float *
 // MS: the SLS capacity of the Newmark memory-variable update (5, the
 // reference's default number of linear solids, or kMaxSls); only MODE 0
 // with attenuation uses it.
-template <int SRC, int MODE, bool ANEL, int B, int MS = kMaxSls>
+// ZT: the term zero_term_of(SRC) is +0 at every point of the mesh and is not
+// read (the kernel multiplies by the same +0: the same IEEE operations).
+template <int SRC, int MODE, bool ANEL, int B, int MS = kMaxSls, bool ZT = false>
 __global__ __launch_bounds__(256, (B == 1 ? 5 : (B == 2 ? 3 : 2)))
 void k_solid(SolidArgs A, FluidArgs FA) {
   constexpr int NCOMP = (SRC == AXH_MONOPOLE) ? 2 : 3;
@@ -724,7 +732,8 @@ void k_solid(SolidArgs A, FluidArgs FA) {
   {
     const float *coef = A.coef + (size_t)tile * NC * 256 + t;
 #pragma unroll
-    for (int a = 0; a < NC; a++) cf[a] = __builtin_nontemporal_load(coef + a * 256);
+    for (int a = 0; a < NC; a++)
+      cf[a] = (ZT && a == zero_term_of(SRC)) ? 0.0f : __builtin_nontemporal_load(coef + a * 256);
   }
   gather_disp<NCOMP, B>(A, valid, gp, t, sU);
   // a single-copy point is kicked here at the end: its velocity (and, for
@@ -2337,6 +2346,7 @@ struct MeshHost {
       cpl_ss, cpl_gs, cpl_sf, cpl_gf, cpl_axb, axb_g, cpl_slot_s, cpl_slot_f;
   std::vector<uint8_t> flags_s, flags_f;
   std::vector<double> aj_mu, aj_k;
+  bool zero_term = false;
   // memory variables at all points: packed [tile][kAfTerms][256] terms and
   // the axial vectors [nel][5][5]
   std::vector<float> af_terms, af_axis;
@@ -2460,6 +2470,12 @@ static void build_mesh_layout(const axh_ctx *ctx, MeshHost &H) {
       S(t);
   }
   pack_terms(ctx, names, H.nel_s, H.coef_s);
+  if (zero_term_of(src) >= 0 && H.nel_s > 0) {
+    const uint32_t *v = ctx->need(names[zero_term_of(src)]).as<uint32_t>();
+    bool zero = true;
+    for (size_t x = 0; x < (size_t)H.nel_s * kNpp && zero; x++) zero = v[x] == 0u;
+    H.zero_term = zero;
+  }
 
   // axial terms M0_w1..10 of the axial elements, the ones the source type
   // reads (monopole w1-3, dipole w1-4 and w6-10, quadrupole w1-6); the rest
@@ -2854,6 +2870,7 @@ static std::shared_ptr<MeshDev> mesh_build(const axh_ctx *ctx) {
   DevAlloc &m = V.mem;
   V.G = m.upload(H.G);
   V.coef_s = m.upload(H.coef_s);
+  V.zero_term = H.zero_term;
   V.einfo_s = m.upload(H.einfo_s);
   V.m0_s = m.upload(H.m0_s);
   V.gid_s = m.upload(H.gid_s);
@@ -3480,6 +3497,19 @@ template <int SRC, int MODE, bool ANEL, int B>
 static void launch_solid(const MeshDev &M, hipStream_t s, const SolidArgs &A,
                          const FluidArgs &F) {
   const int nblocks = M.ntile_s + (M.nel_f > 0 ? M.ntile_f : 0);
+  if constexpr (B == 1 && zero_term_of(SRC) >= 0) {
+    // one wavefield of an isotropic model: the +0 term is not read (a group's
+    // launch is bound by instruction issue, not by the term bytes)
+    if (M.zero_term) {
+      if (MODE == 0 && ANEL && M.nsls <= 5)
+        hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, 1, 5, true>), dim3(nblocks), dim3(256), 0,
+                           s, A, F);
+      else
+        hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, 1, kMaxSls, true>), dim3(nblocks),
+                           dim3(256), 0, s, A, F);
+      return;
+    }
+  }
   if (MODE == 0 && ANEL && M.nsls <= 5)
     hipLaunchKernelGGL((k_solid<SRC, MODE, ANEL, B, 5>), dim3(nblocks), dim3(256), 0, s, A, F);
   else

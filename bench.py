@@ -179,10 +179,37 @@ def serving_leg(workdir: str, nsolves: int, rate: float, chunk: int) -> dict:
     return st
 
 
+# the stiffness terms the library packs per element point, per source type
+PACKED_TERMS = {
+    0: ("M_1", "M_2", "M_3", "M_4", "M_w1", "M11s", "M21s", "M41s", "M12s", "M22s", "M32s",
+        "M42s", "M11z", "M21z", "M41z"),
+    1: ("M_1", "M_2", "M_3", "M_4", "M_5", "M_6", "M_7", "M_8", "M_w1", "M_w2", "M_w3",
+        "M11s", "M21s", "M41s", "M12s", "M22s", "M42s", "M13s", "M32s", "M33s", "M43s",
+        "M11z", "M21z", "M41z"),
+    2: ("M_1", "M_2", "M_3", "M_4", "M_5", "M_6", "M_7", "M_8", "M_w1", "M_w2", "M_w3",
+        "M_w4", "M_w5", "M11s", "M21s", "M41s", "M12s", "M22s", "M32s", "M42s", "M11z",
+        "M21z", "M41z", "M1phi", "M2phi", "M4phi"),
+}
+ZERO_TERM = {1: "M_w2", 2: "M_w3"}
+
+
+def read_terms(st, nwave: int = 1) -> int:
+    """Packed stiffness terms the stiffness kernel reads: with one wavefield,
+    the term an isotropic model leaves +0 at every point (M_w2 of a dipole,
+    M_w3 of a quadrupole) is not read; a group reads them all."""
+    import numpy as np
+    src = st.dims["src_type"]
+    n = len(PACKED_TERMS[src])
+    z = ZERO_TERM.get(src)
+    if nwave == 1 and z and not np.asarray(st[z]).view(np.uint32).any():
+        n -= 1
+    return n
+
+
 def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
     """Algorithmic HBM bytes of one launch of the stiffness kernel (solid and
     fluid tiles) in a symplectic stage advancing nwave wavefields: every
-    precomputed term of
+    precomputed term the kernel reads (read_terms) at
     every element-local point once, the element->global index, the axial
     index, the axial terms and the stage cg4 terms once (shared by the
     wavefields); per wavefield the displacement of every global point once,
@@ -194,7 +221,7 @@ def solid_kernel_bytes(st, nwave: int = 1, ndirect: int = 0) -> int:
     d = st.dims
     nel, nglob = d["nel_solid"], d["nglob_solid"]
     ncomp = 2 if d["src_type"] == 0 else 3
-    nterms = {0: 15, 1: 24, 2: 26}[d["src_type"]]
+    nterms = read_terms(st, nwave)
     naxel = int(st["axis_solid"].sum())
     mesh = 4 * 25 * nel * nterms       # stiffness terms
     mesh += 4 * 25 * nel               # element -> global index
